@@ -1,0 +1,227 @@
+"""dmrg_amd: MI355X-native H_eff hot path of DMRG++ (matvec, environment updates, eigensolver).
+
+Python mirror of the C ABI (include/dmrg_amd.h), used by the tests and the benchmark.
+Names follow the reference's C++ API (tools::common::contraction::matrix_vector_product,
+EnvBase::build_block, MatVecMPOS<T>). Tensors are numpy arrays with the reference's
+index order and column-major (Fortran) memory layout:
+    mps (d, chiL, chiR), env (chi, chi, D), mpo (DL, DR, d, d).
+All compute runs in libdmrg_amd.so on the GPU; nothing here computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import DMRG_CX32, DMRG_CX64, DMRG_FP32, DMRG_FP64, DmrgError, call, load
+
+__all__ = [
+    "DeviceArray", "MatVecMPOS", "matrix_vector_product", "env_build_block", "expectation_value",
+    "dot", "nrm2", "device_info", "DmrgError", "dtype_code",
+]
+
+_CODE = {np.dtype(np.float64): DMRG_FP64, np.dtype(np.complex128): DMRG_CX64,
+         np.dtype(np.float32): DMRG_FP32, np.dtype(np.complex64): DMRG_CX32}
+_NP = {v: k for k, v in _CODE.items()}
+
+
+def dtype_code(dt) -> int:
+    return _CODE[np.dtype(dt)]
+
+
+def _dims(shape) -> C.Array:
+    return (C.c_int64 * len(shape))(*[int(s) for s in shape])
+
+
+class DeviceArray:
+    """A column-major tensor in device memory (owned; freed on close/GC)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = C.c_void_p()
+        call("dmrg_malloc", C.byref(p), max(self.nbytes, 0))
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.asfortranarray(a)
+        d = cls(a.shape, a.dtype)
+        if d.nbytes:
+            call("dmrg_memcpy_htod", d.ptr, a.ctypes.data_as(C.c_void_p), d.nbytes)
+        return d
+
+    def to_numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype, order="F")
+        if self.nbytes:
+            call("dmrg_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes)
+        return out
+
+    def zero(self) -> None:
+        if self.nbytes:
+            call("dmrg_memset", self.ptr, 0, self.nbytes)
+
+    @property
+    def code(self) -> int:
+        return _CODE[self.dtype]
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            try:
+                call("dmrg_free", self.ptr)
+            finally:
+                self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _dev(a) -> DeviceArray:
+    return a if isinstance(a, DeviceArray) else DeviceArray.from_numpy(np.asarray(a))
+
+
+def device_info() -> dict:
+    name = C.create_string_buffer(256)
+    cus = C.c_int()
+    mem = C.c_size_t()
+    call("dmrg_device_info", name, 256, C.byref(cus), C.byref(mem))
+    return {"name": name.value.decode(), "cu_count": cus.value, "hbm_bytes": mem.value}
+
+
+def matrix_vector_product(mps, mpo, envL, envR, stream=None):
+    """out = H_eff mps (tools::common::contraction::matrix_vector_product, contraction_tblis.cpp:213-292).
+
+    numpy inputs -> numpy output; DeviceArray inputs -> DeviceArray output."""
+    on_dev = isinstance(mps, DeviceArray)
+    m, w, l, r = _dev(mps), _dev(mpo), _dev(envL), _dev(envR)
+    out = DeviceArray(m.shape, m.dtype)
+    call("dmrg_matrix_vector_product", m.code, out.ptr, m.ptr, _dims(m.shape), w.ptr, _dims(w.shape), l.ptr,
+         _dims(l.shape), r.ptr, _dims(r.shape), stream)
+    if on_dev:
+        return out
+    res = out.to_numpy()
+    out.close()
+    return res
+
+
+def env_build_block(side: str, otherblock, mps, mpo, stream=None):
+    """EnvBase::build_block (EnvBase.cpp:67-151): the new L (side 'L') or R (side 'R') block."""
+    on_dev = isinstance(mps, DeviceArray)
+    o, m, w = _dev(otherblock), _dev(mps), _dev(mpo)
+    if side == "L":
+        shape = (m.shape[2], m.shape[2], w.shape[1])
+    elif side == "R":
+        shape = (m.shape[1], m.shape[1], w.shape[0])
+    else:
+        raise ValueError("side must be 'L' or 'R'")
+    out = DeviceArray(shape, m.dtype)
+    call("dmrg_env_build_block", m.code, C.c_char(side.encode()), out.ptr, o.ptr, _dims(o.shape), m.ptr,
+         _dims(m.shape), w.ptr, _dims(w.shape), stream)
+    if on_dev:
+        return out
+    res = out.to_numpy()
+    out.close()
+    return res
+
+
+def expectation_value(mps, mpo, envL, envR, stream=None) -> complex:
+    """<mps|H_eff|mps> (contraction_tblis.cpp:30-97)."""
+    m, w, l, r = _dev(mps), _dev(mpo), _dev(envL), _dev(envR)
+    res = (C.c_double * 2)()
+    call("dmrg_expectation_value", m.code, m.ptr, _dims(m.shape), w.ptr, _dims(w.shape), l.ptr, _dims(l.shape),
+         r.ptr, _dims(r.shape), res, stream)
+    return complex(res[0], res[1]) if np.iscomplexobj(np.empty(0, m.dtype)) else res[0]
+
+
+def dot(x, y, stream=None):
+    """sum conj(x) y (contract_mps_mps_overlap, contraction_tblis.cpp:518-533)."""
+    a, b = _dev(x), _dev(y)
+    res = (C.c_double * 2)()
+    n = int(np.prod(a.shape))
+    call("dmrg_dot", a.code, n, a.ptr, b.ptr, res, stream)
+    return complex(res[0], res[1]) if a.dtype.kind == "c" else res[0]
+
+
+def nrm2(x, stream=None) -> float:
+    a = _dev(x)
+    res = C.c_double()
+    call("dmrg_nrm2", a.code, int(np.prod(a.shape)), a.ptr, C.byref(res), stream)
+    return res.value
+
+
+class MatVecMPOS:
+    """Device-resident effective Hamiltonian (MatVecMPOS<T>, matvec_mpos.h:23-180).
+
+    mpos: list of (DL, DR, d, d) cx64/fp64 arrays (merged when there are several);
+    envL (chiL, chiL, DL), envR (chiR, chiR, DR); dtype np.float64 or np.complex128."""
+
+    def __init__(self, mpos, envL, envR, dtype=np.complex128):
+        self.dtype = np.dtype(dtype)
+        mpos = [np.asfortranarray(np.asarray(m, dtype=np.complex128)) for m in mpos]
+        envL = np.asfortranarray(np.asarray(envL, dtype=np.complex128))
+        envR = np.asfortranarray(np.asarray(envR, dtype=np.complex128))
+        self._keep = (mpos, envL, envR)
+        ptrs = (C.c_void_p * len(mpos))(*[m.ctypes.data for m in mpos])
+        dims = (C.c_int64 * (4 * len(mpos)))(*[int(x) for m in mpos for x in m.shape])
+        h = C.c_void_p()
+        call("dmrg_matvec_create", C.byref(h), _CODE[self.dtype], len(mpos), ptrs, dims, envL.ctypes.data_as(C.c_void_p),
+             _dims(envL.shape), envR.ctypes.data_as(C.c_void_p), _dims(envR.shape))
+        self.h = h
+        shape = (C.c_int64 * 3)()
+        call("dmrg_matvec_shape_mps", self.h, shape)
+        self.shape_mps = tuple(shape)
+        self._keep = None
+
+    def rows(self) -> int:
+        return int(load().dmrg_matvec_rows(self.h))
+
+    @property
+    def num_mv(self) -> int:
+        return int(load().dmrg_matvec_num_mv(self.h))
+
+    def MultAx(self, x: np.ndarray) -> np.ndarray:
+        """Host path (the PRIMME/ARPACK callback): x is (n,) or (n, nvec) column-major."""
+        x = np.asfortranarray(np.asarray(x, dtype=self.dtype))
+        x2 = x.reshape(self.rows(), -1, order="F")
+        y = np.empty_like(x2, order="F")
+        call("dmrg_matvec_multax", self.h, x2.ctypes.data_as(C.c_void_p), x2.shape[0], y.ctypes.data_as(C.c_void_p),
+             y.shape[0], x2.shape[1])
+        return y.reshape(x.shape, order="F")
+
+    def MultAx_device(self, dx: DeviceArray, dy: DeviceArray, nvec: int = 1, ld: int | None = None, stream=None,
+                      fp32: bool = False) -> None:
+        ld = ld or self.rows()
+        fn = "dmrg_matvec_multax_device_fp32" if fp32 else "dmrg_matvec_multax_device"
+        call(fn, self.h, dx.ptr, ld, dy.ptr, ld, nvec, stream)
+
+    def set_shift(self, shift: complex) -> None:
+        call("dmrg_matvec_set_shift", self.h, float(np.real(shift)), float(np.imag(shift)))
+
+    def get_diagonal(self) -> np.ndarray:
+        out = DeviceArray(self.shape_mps, self.dtype)
+        call("dmrg_matvec_get_diagonal", self.h, out.ptr, None)
+        res = out.to_numpy()
+        out.close()
+        return res
+
+    def time_legs(self, dx: DeviceArray, dy: DeviceArray, nrep: int, fp32: bool = False, stream=None):
+        """Average duration (ms) of each of the three contraction legs over nrep device matvecs."""
+        ms = (C.c_float * 3)()
+        call("dmrg_matvec_time_legs", self.h, dx.ptr, dy.ptr, nrep, int(fp32), ms, stream)
+        return [float(v) for v in ms]
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            call("dmrg_matvec_destroy", self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

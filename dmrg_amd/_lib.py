@@ -1,0 +1,108 @@
+"""ctypes binding of libdmrg_amd.so (the C ABI declared in include/dmrg_amd.h).
+
+The shared library is built in-tree by dmrg_amd.build. Loading fails loudly when it is
+missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libdmrg_amd.so"
+
+DMRG_FP64, DMRG_CX64, DMRG_FP32, DMRG_CX32 = 0, 1, 2, 3
+
+STATUS = {
+    0: "DMRG_OK",
+    1: "DMRG_ERR_DIMS",
+    2: "DMRG_ERR_ARG",
+    3: "DMRG_ERR_OOM",
+    4: "DMRG_ERR_HIP",
+    5: "DMRG_ERR_NAN",
+    6: "DMRG_ERR_NOT_CONVERGED",
+    7: "DMRG_ERR_UNSUPPORTED",
+}
+
+vp = C.c_void_p
+i64 = C.c_int64
+i64p = C.POINTER(C.c_int64)
+dp = C.POINTER(C.c_double)
+st = C.c_int
+
+# name -> (restype, argtypes)
+SIGNATURES: dict[str, tuple] = {
+    "dmrg_last_error": (C.c_char_p, []),
+    "dmrg_version": (C.c_int, []),
+    "dmrg_device_count": (st, [C.POINTER(C.c_int)]),
+    "dmrg_set_device": (st, [C.c_int]),
+    "dmrg_device_info": (st, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]),
+    "dmrg_malloc": (st, [C.POINTER(vp), C.c_size_t]),
+    "dmrg_free": (st, [vp]),
+    "dmrg_memcpy_htod": (st, [vp, vp, C.c_size_t]),
+    "dmrg_memcpy_dtoh": (st, [vp, vp, C.c_size_t]),
+    "dmrg_memcpy_dtod": (st, [vp, vp, C.c_size_t]),
+    "dmrg_memset": (st, [vp, C.c_int, C.c_size_t]),
+    "dmrg_synchronize": (st, []),
+    "dmrg_stream_create": (st, [C.POINTER(vp)]),
+    "dmrg_stream_destroy": (st, [vp]),
+    "dmrg_stream_synchronize": (st, [vp]),
+    "dmrg_event_create": (st, [C.POINTER(vp)]),
+    "dmrg_event_destroy": (st, [vp]),
+    "dmrg_event_record": (st, [vp, vp]),
+    "dmrg_event_elapsed_ms": (st, [C.POINTER(C.c_float), vp, vp]),
+    "dmrg_matrix_vector_product": (st, [C.c_int, vp, vp, i64p, vp, i64p, vp, i64p, vp, i64p, vp]),
+    "dmrg_env_build_block": (st, [C.c_int, C.c_char, vp, vp, i64p, vp, i64p, vp, i64p, vp]),
+    "dmrg_expectation_value": (st, [C.c_int, vp, i64p, vp, i64p, vp, i64p, vp, i64p, dp, vp]),
+    "dmrg_dot": (st, [C.c_int, i64, vp, vp, dp, vp]),
+    "dmrg_nrm2": (st, [C.c_int, i64, vp, dp, vp]),
+    "dmrg_matvec_create": (st, [C.POINTER(vp), C.c_int, C.c_int, C.POINTER(vp), i64p, vp, i64p, vp, i64p]),
+    "dmrg_matvec_destroy": (st, [vp]),
+    "dmrg_matvec_rows": (i64, [vp]),
+    "dmrg_matvec_shape_mps": (st, [vp, i64p]),
+    "dmrg_matvec_num_mv": (i64, [vp]),
+    "dmrg_matvec_multax": (st, [vp, vp, i64, vp, i64, C.c_int]),
+    "dmrg_matvec_multax_device": (st, [vp, vp, i64, vp, i64, C.c_int, vp]),
+    "dmrg_matvec_multax_device_fp32": (st, [vp, vp, i64, vp, i64, C.c_int, vp]),
+    "dmrg_matvec_set_shift": (st, [vp, C.c_double, C.c_double]),
+    "dmrg_matvec_get_diagonal": (st, [vp, vp, vp]),
+    "dmrg_matvec_time_legs": (st, [vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_float), vp]),
+}
+
+_lib = None
+
+
+class DmrgError(RuntimeError):
+    def __init__(self, status: int, fn: str, msg: str):
+        super().__init__(f"{fn} -> {STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+def load() -> C.CDLL:
+    """Load libdmrg_amd.so (built in-tree); raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (or python -m dmrg_amd.build) first")
+    lib = C.CDLL(str(LIB_PATH), mode=os.RTLD_LOCAL | getattr(os, "RTLD_NOW", 2))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int, fn: str = "") -> None:
+    if status != 0:
+        msg = load().dmrg_last_error().decode(errors="replace")
+        raise DmrgError(status, fn, msg)
+
+
+def call(name: str, *args):
+    """Call a status-returning entry point and raise DmrgError on failure."""
+    fn = getattr(load(), name)
+    r = fn(*args)
+    check(r, name)
+    return r

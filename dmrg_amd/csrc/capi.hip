@@ -1,0 +1,322 @@
+// C-ABI entry points: runtime helpers, stateless contractions and the MatVecMPOS operator.
+// No C++ exception crosses this boundary: each entry point converts errors to a dmrg_status
+// and records the message for dmrg_last_error().
+#include "contract.hpp"
+#include "matvec_op.hpp"
+
+#include <complex>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+namespace dmrg {
+
+static thread_local std::string g_last_error;
+void                            set_last_error(const std::string &msg) { g_last_error = msg; }
+
+template <typename F>
+static dmrg_status guard(F &&f) {
+    try {
+        f();
+        return DMRG_OK;
+    } catch(const Error &e) {
+        set_last_error(e.what());
+        return e.status;
+    } catch(const std::bad_alloc &) {
+        set_last_error("host allocation failed");
+        return DMRG_ERR_OOM;
+    } catch(const std::exception &e) {
+        set_last_error(e.what());
+        return DMRG_ERR_ARG;
+    } catch(...) {
+        set_last_error("unknown error");
+        return DMRG_ERR_ARG;
+    }
+}
+
+// Serialises the stateless entry points that share the per-device workspace.
+static std::mutex g_stateless_mu;
+
+// Small per-device buffer for reduction results.
+static double *reduction_buffer() {
+    static std::mutex            mu;
+    static std::vector<double *> bufs;
+    std::lock_guard<std::mutex>  lock(mu);
+    int                          dev = 0;
+    DMRG_HIP_CHECK(hipGetDevice(&dev));
+    if(static_cast<int>(bufs.size()) <= dev) bufs.resize(dev + 1, nullptr);
+    if(!bufs[dev]) {
+        void *p = nullptr;
+        if(hipMalloc(&p, DOT_BUFFER_DOUBLES * sizeof(double)) != hipSuccess) throw Error(DMRG_ERR_OOM, "reduction buffer");
+        bufs[dev] = static_cast<double *>(p);
+    }
+    return bufs[dev];
+}
+
+} // namespace dmrg
+
+using namespace dmrg;
+
+extern "C" {
+
+const char *dmrg_last_error(void) { return g_last_error.c_str(); }
+int         dmrg_version(void) { return 100; }
+
+dmrg_status dmrg_device_count(int *count) {
+    return guard([&] {
+        DMRG_REQUIRE(count, DMRG_ERR_ARG, "null count");
+        int n = 0;
+        if(hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        *count = n;
+    });
+}
+
+dmrg_status dmrg_set_device(int device) { return guard([&] { DMRG_HIP_CHECK(hipSetDevice(device)); }); }
+
+dmrg_status dmrg_device_info(char *name, size_t name_len, int *cu_count, size_t *hbm_bytes) {
+    return guard([&] {
+        int dev = 0;
+        DMRG_HIP_CHECK(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        DMRG_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+        if(name && name_len) {
+            std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+        }
+        if(cu_count) *cu_count = prop.multiProcessorCount;
+        if(hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+    });
+}
+
+dmrg_status dmrg_malloc(void **dptr, size_t bytes) {
+    return guard([&] {
+        DMRG_REQUIRE(dptr, DMRG_ERR_ARG, "null pointer");
+        *dptr = nullptr;
+        if(bytes == 0) return;
+        if(hipMalloc(dptr, bytes) != hipSuccess) {
+            (void) hipGetLastError();
+            throw Error(DMRG_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        }
+    });
+}
+dmrg_status dmrg_free(void *dptr) {
+    return guard([&] {
+        if(dptr) DMRG_HIP_CHECK(hipFree(dptr));
+    });
+}
+dmrg_status dmrg_memcpy_htod(void *dst, const void *src, size_t bytes) {
+    return guard([&] {
+        if(bytes) DMRG_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    });
+}
+dmrg_status dmrg_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+    return guard([&] {
+        if(bytes) DMRG_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    });
+}
+dmrg_status dmrg_memcpy_dtod(void *dst, const void *src, size_t bytes) {
+    return guard([&] {
+        if(bytes) DMRG_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    });
+}
+dmrg_status dmrg_memset(void *dptr, int value, size_t bytes) {
+    return guard([&] {
+        if(bytes) DMRG_HIP_CHECK(hipMemset(dptr, value, bytes));
+    });
+}
+dmrg_status dmrg_synchronize(void) { return guard([&] { DMRG_HIP_CHECK(hipDeviceSynchronize()); }); }
+
+dmrg_status dmrg_stream_create(void **stream) {
+    return guard([&] {
+        hipStream_t s = nullptr;
+        DMRG_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        *stream = s;
+    });
+}
+dmrg_status dmrg_stream_destroy(void *stream) {
+    return guard([&] {
+        if(stream) DMRG_HIP_CHECK(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+    });
+}
+dmrg_status dmrg_stream_synchronize(void *stream) {
+    return guard([&] { DMRG_HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(stream))); });
+}
+dmrg_status dmrg_event_create(void **event) {
+    return guard([&] {
+        hipEvent_t e = nullptr;
+        DMRG_HIP_CHECK(hipEventCreate(&e));
+        *event = e;
+    });
+}
+dmrg_status dmrg_event_destroy(void *event) {
+    return guard([&] {
+        if(event) DMRG_HIP_CHECK(hipEventDestroy(static_cast<hipEvent_t>(event)));
+    });
+}
+dmrg_status dmrg_event_record(void *event, void *stream) {
+    return guard([&] { DMRG_HIP_CHECK(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream))); });
+}
+dmrg_status dmrg_event_elapsed_ms(float *ms, void *start, void *stop) {
+    return guard([&] {
+        DMRG_HIP_CHECK(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
+        DMRG_HIP_CHECK(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+    });
+}
+
+// ------------------------------------------------------------------------------- contractions
+
+dmrg_status dmrg_matrix_vector_product(dmrg_dtype dtype, void *res, const void *mps, const int64_t mps_dims[3],
+                                       const void *mpo, const int64_t mpo_dims[4], const void *envL,
+                                       const int64_t envL_dims[3], const void *envR, const int64_t envR_dims[3],
+                                       void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(res && mps && mpo && envL && envR && mps_dims && mpo_dims && envL_dims && envR_dims, DMRG_ERR_ARG,
+                     "null argument");
+        const Dims                  D = check_matvec_dims(mps_dims, mpo_dims, envL_dims, envR_dims);
+        std::lock_guard<std::mutex> lock(g_stateless_mu);
+        void *ws = scratch_workspace(static_cast<size_t>(matvec_ws_elems(D)) * dtype_size(dtype));
+        matvec(dtype, D, mps, D.n(), res, D.n(), 1, mpo, envL, envR, ws, static_cast<hipStream_t>(stream));
+    });
+}
+
+dmrg_status dmrg_env_build_block(dmrg_dtype dtype, char side, void *block_out, const void *otherblock,
+                                 const int64_t other_dims[3], const void *mps, const int64_t mps_dims[3],
+                                 const void *mpo, const int64_t mpo_dims[4], void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(block_out && otherblock && mps && mpo && other_dims && mps_dims && mpo_dims, DMRG_ERR_ARG, "null argument");
+        DMRG_REQUIRE(side == 'L' || side == 'R', DMRG_ERR_ARG, "side must be 'L' or 'R'");
+        DMRG_REQUIRE(mps_dims[0] == mpo_dims[2] && mpo_dims[2] == mpo_dims[3], DMRG_ERR_DIMS, "env: mps dim 0 != mpo dim 2");
+        DMRG_REQUIRE(other_dims[0] == other_dims[1], DMRG_ERR_DIMS, "env: block must be square in its bond indices");
+        Dims D;
+        D.d    = mps_dims[0];
+        D.chiL = mps_dims[1];
+        D.chiR = mps_dims[2];
+        D.DL   = mpo_dims[0];
+        D.DR   = mpo_dims[1];
+        if(side == 'L') {
+            // EnvBase.cpp:100-108
+            DMRG_REQUIRE(mps_dims[1] == other_dims[0], DMRG_ERR_DIMS, "envL: mps dim 1 != left-block dim 0");
+            DMRG_REQUIRE(mpo_dims[0] == other_dims[2], DMRG_ERR_DIMS, "envL: mpo dim 0 != left-block dim 2");
+        } else {
+            // EnvBase.cpp:137-145
+            DMRG_REQUIRE(mps_dims[2] == other_dims[0], DMRG_ERR_DIMS, "envR: mps dim 2 != right-block dim 0");
+            DMRG_REQUIRE(mpo_dims[1] == other_dims[2], DMRG_ERR_DIMS, "envR: mpo dim 1 != right-block dim 2");
+        }
+        std::lock_guard<std::mutex> lock(g_stateless_mu);
+        void *ws = scratch_workspace(static_cast<size_t>(env_ws_elems(D)) * dtype_size(dtype));
+        env_build_block(dtype, side, D, block_out, otherblock, mps, mpo, ws, static_cast<hipStream_t>(stream));
+    });
+}
+
+dmrg_status dmrg_dot(dmrg_dtype dtype, int64_t n, const void *x, const void *y, double result[2], void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(result && (n == 0 || (x && y)) && n >= 0, DMRG_ERR_ARG, "bad argument");
+        std::lock_guard<std::mutex> lock(g_stateless_mu);
+        double *buf = reduction_buffer();
+        auto    s   = static_cast<hipStream_t>(stream);
+        dot_device(dtype, n, x, y, buf, s);
+        DMRG_HIP_CHECK(hipMemcpyAsync(result, buf, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        DMRG_HIP_CHECK(hipStreamSynchronize(s));
+        if(!dtype_is_complex(dtype)) result[1] = 0.0;
+    });
+}
+
+dmrg_status dmrg_nrm2(dmrg_dtype dtype, int64_t n, const void *x, double *result, void *stream) {
+    return guard([&] {
+        double r[2];
+        std::lock_guard<std::mutex> lock(g_stateless_mu);
+        double *buf = reduction_buffer();
+        auto    s   = static_cast<hipStream_t>(stream);
+        nrm2sq_device(dtype, n, x, buf, s);
+        DMRG_HIP_CHECK(hipMemcpyAsync(r, buf, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        DMRG_HIP_CHECK(hipStreamSynchronize(s));
+        *result = std::sqrt(std::max(0.0, r[0]));
+    });
+}
+
+dmrg_status dmrg_expectation_value(dmrg_dtype dtype, const void *mps, const int64_t mps_dims[3], const void *mpo,
+                                   const int64_t mpo_dims[4], const void *envL, const int64_t envL_dims[3],
+                                   const void *envR, const int64_t envR_dims[3], double result[2], void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(result && mps && mpo && envL && envR, DMRG_ERR_ARG, "null argument");
+        const Dims                  D = check_matvec_dims(mps_dims, mpo_dims, envL_dims, envR_dims);
+        std::lock_guard<std::mutex> lock(g_stateless_mu);
+        const size_t es = dtype_size(dtype);
+        char *ws = static_cast<char *>(scratch_workspace(static_cast<size_t>(matvec_ws_elems(D) + D.n()) * es));
+        void *hv = ws + static_cast<size_t>(matvec_ws_elems(D)) * es;
+        auto  s  = static_cast<hipStream_t>(stream);
+        matvec(dtype, D, mps, D.n(), hv, D.n(), 1, mpo, envL, envR, ws, s);
+        double *buf = reduction_buffer();
+        dot_device(dtype, D.n(), mps, hv, buf, s);
+        DMRG_HIP_CHECK(hipMemcpyAsync(result, buf, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        DMRG_HIP_CHECK(hipStreamSynchronize(s));
+        if(!dtype_is_complex(dtype)) result[1] = 0.0;
+    });
+}
+
+// ------------------------------------------------------------------------------- MatVecMPOS
+
+dmrg_status dmrg_matvec_create(dmrg_matvec **op, dmrg_dtype dtype, int nmpo, const void *const *mpos,
+                               const int64_t *mpo_dims, const void *envL, const int64_t envL_dims[3],
+                               const void *envR, const int64_t envR_dims[3]) {
+    return guard([&] {
+        DMRG_REQUIRE(op, DMRG_ERR_ARG, "null op");
+        *op = nullptr;
+        DMRG_REQUIRE(dtype == DMRG_FP64 || dtype == DMRG_CX64, DMRG_ERR_ARG, "operator dtype must be FP64 or CX64");
+        *op = reinterpret_cast<dmrg_matvec *>(new MatVecOp(dtype, nmpo, mpos, mpo_dims, envL, envL_dims, envR, envR_dims));
+    });
+}
+dmrg_status dmrg_matvec_destroy(dmrg_matvec *op) {
+    return guard([&] { delete reinterpret_cast<MatVecOp *>(op); });
+}
+int64_t dmrg_matvec_rows(const dmrg_matvec *op) { return op ? reinterpret_cast<const MatVecOp *>(op)->D.n() : -1; }
+dmrg_status dmrg_matvec_shape_mps(const dmrg_matvec *op, int64_t shape[3]) {
+    return guard([&] {
+        DMRG_REQUIRE(op && shape, DMRG_ERR_ARG, "null argument");
+        const auto *o = reinterpret_cast<const MatVecOp *>(op);
+        shape[0] = o->D.d, shape[1] = o->D.chiL, shape[2] = o->D.chiR;
+    });
+}
+int64_t dmrg_matvec_num_mv(const dmrg_matvec *op) { return op ? reinterpret_cast<const MatVecOp *>(op)->num_mv : -1; }
+
+dmrg_status dmrg_matvec_multax(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec) {
+    return guard([&] {
+        DMRG_REQUIRE(op && x && y && nvec >= 0, DMRG_ERR_ARG, "bad argument");
+        reinterpret_cast<MatVecOp *>(op)->multax_host(x, ldx, y, ldy, nvec);
+    });
+}
+dmrg_status dmrg_matvec_multax_device(dmrg_matvec *op, const void *dx, int64_t ldx, void *dy, int64_t ldy, int nvec,
+                                      void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(op && dx && dy && nvec >= 0, DMRG_ERR_ARG, "bad argument");
+        reinterpret_cast<MatVecOp *>(op)->multax_device(dx, ldx, dy, ldy, nvec, static_cast<hipStream_t>(stream), false);
+    });
+}
+dmrg_status dmrg_matvec_multax_device_fp32(dmrg_matvec *op, const void *dx, int64_t ldx, void *dy, int64_t ldy,
+                                           int nvec, void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(op && dx && dy && nvec >= 0, DMRG_ERR_ARG, "bad argument");
+        reinterpret_cast<MatVecOp *>(op)->multax_device(dx, ldx, dy, ldy, nvec, static_cast<hipStream_t>(stream), true);
+    });
+}
+dmrg_status dmrg_matvec_set_shift(dmrg_matvec *op, double re, double im) {
+    return guard([&] {
+        DMRG_REQUIRE(op, DMRG_ERR_ARG, "null op");
+        reinterpret_cast<MatVecOp *>(op)->set_shift(std::complex<double>(re, im));
+    });
+}
+dmrg_status dmrg_matvec_get_diagonal(dmrg_matvec *op, void *ddiag, void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(op && ddiag, DMRG_ERR_ARG, "null argument");
+        reinterpret_cast<MatVecOp *>(op)->diagonal(ddiag, static_cast<hipStream_t>(stream));
+    });
+}
+
+dmrg_status dmrg_matvec_time_legs(dmrg_matvec *op, const void *dx, void *dy, int nrep, int fp32, float ms_per_leg[3],
+                                  void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(op && dx && dy && ms_per_leg && nrep > 0, DMRG_ERR_ARG, "bad argument");
+        reinterpret_cast<MatVecOp *>(op)->time_legs(dx, dy, nrep, fp32 != 0, ms_per_leg, static_cast<hipStream_t>(stream));
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,99 @@
+// Shared host/device definitions for the dmrg_amd HIP library (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/dmrg_amd.h"
+
+namespace dmrg {
+
+// Complex element stored interleaved (re, im), the layout of std::complex<T>.
+template <typename R>
+struct alignas(2 * sizeof(R)) cpx {
+    R re, im;
+};
+
+// Error carrying a dmrg_status across the C++ layers. Never crosses the C-ABI:
+// every extern "C" entry point catches it and returns the status.
+struct Error : std::runtime_error {
+    dmrg_status status;
+    Error(dmrg_status s, const std::string &msg) : std::runtime_error(msg), status(s) {}
+};
+
+void set_last_error(const std::string &msg);
+
+#define DMRG_HIP_CHECK(expr)                                                                                   \
+    do {                                                                                                       \
+        hipError_t e_ = (expr);                                                                                \
+        if(e_ != hipSuccess)                                                                                   \
+            throw ::dmrg::Error(DMRG_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_));      \
+    } while(0)
+
+#define DMRG_REQUIRE(cond, status, msg)                                                                        \
+    do {                                                                                                       \
+        if(!(cond)) throw ::dmrg::Error((status), (msg));                                                      \
+    } while(0)
+
+inline size_t dtype_size(dmrg_dtype t) {
+    switch(t) {
+        case DMRG_FP64: return 8;
+        case DMRG_CX64: return 16;
+        case DMRG_FP32: return 4;
+        case DMRG_CX32: return 8;
+    }
+    throw Error(DMRG_ERR_ARG, "unknown dtype");
+}
+inline bool dtype_is_complex(dmrg_dtype t) { return t == DMRG_CX64 || t == DMRG_CX32; }
+inline bool dtype_is_single(dmrg_dtype t) { return t == DMRG_FP32 || t == DMRG_CX32; }
+inline dmrg_dtype dtype_single_of(dmrg_dtype t) { return dtype_is_complex(t) ? DMRG_CX32 : DMRG_FP32; }
+inline dmrg_dtype dtype_double_of(dmrg_dtype t) { return dtype_is_complex(t) ? DMRG_CX64 : DMRG_FP64; }
+
+// Two-level strided index: off(i) = (i % c) * s0 + (i / c) * s1.
+// A plain stride is {s, 0, INT32_MAX}. Lets one GEMM read the reference's
+// column-major tensors in place when a GEMM dimension fuses two tensor indices.
+struct Idx2 {
+    int64_t  s0 = 0;
+    int64_t  s1 = 0;
+    uint32_t c  = 0x7fffffffu;
+};
+inline Idx2 plain(int64_t s) { return Idx2{s, 0, 0x7fffffffu}; }
+inline Idx2 fused(int64_t s0, int64_t s1, int64_t c) { return Idx2{s0, s1, static_cast<uint32_t>(c)}; }
+
+__host__ __device__ inline int64_t idx_off(const Idx2 &x, uint32_t i) {
+    return static_cast<int64_t>(i % x.c) * x.s0 + static_cast<int64_t>(i / x.c) * x.s1;
+}
+
+// One batched GEMM C[b][m,n] (+)= sum_k op(A)[b][m,k] * op(B)[b][k,n].
+struct GemmDesc {
+    dmrg_dtype  dtype = DMRG_FP64;
+    int32_t     M = 0, N = 0, K = 0, batch = 1;
+    const void *A = nullptr;
+    Idx2        am, ak;
+    int64_t     a_batch = 0;
+    int32_t     conjA   = 0;
+    const void *B       = nullptr;
+    Idx2        bk, bn;
+    int64_t     b_batch = 0;
+    int32_t     conjB   = 0;
+    void       *C       = nullptr;
+    Idx2        cm, cn;
+    int64_t     c_batch    = 0;
+    int32_t     accumulate = 0;
+    // Kernel-side loading hints, filled by the launcher.
+    int32_t a_fast_m = 1; // 1: consecutive threads walk M when loading A, 0: walk K
+    int32_t b_fast_k = 1; // 1: consecutive threads walk K when loading B, 0: walk N
+    int32_t xcd_remap = 1;
+};
+
+// Tile configuration id (see gemm.hip for the table).
+enum class TileCfg : int { T64x64 = 0, T32x64 = 1, T64x32 = 2, T16x128 = 3, T128x64 = 4, T32x32 = 5, AUTO = -1 };
+
+void gemm(const GemmDesc &g, hipStream_t stream, TileCfg cfg = TileCfg::AUTO);
+int  device_cu_count();
+
+} // namespace dmrg

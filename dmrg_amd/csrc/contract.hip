@@ -1,0 +1,198 @@
+// Contraction plans for the H_eff matvec and the environment updates.
+//
+// Each contraction is three GEMMs over the reference's column-major tensors:
+//   two O(chi^3 d D) legs that run on the matrix cores, and a middle "W leg" that
+//   contracts the MPO over (physical, MPO-bond) pairs (K = d*D, memory bound).
+// Index maps are written out per leg; m/k/n name the GEMM dimensions and the
+// tensor index each one fuses.
+#include "contract.hpp"
+
+#include <mutex>
+#include <vector>
+
+namespace dmrg {
+
+Dims check_matvec_dims(const int64_t mps[3], const int64_t mpo[4], const int64_t envL[3], const int64_t envR[3]) {
+    // contraction_tblis.cpp:230-234
+    DMRG_REQUIRE(mps[1] == envL[0], DMRG_ERR_DIMS, "Dimension mismatch mps and envL");
+    DMRG_REQUIRE(mps[2] == envR[0], DMRG_ERR_DIMS, "Dimension mismatch mps and envR");
+    DMRG_REQUIRE(mps[0] == mpo[2], DMRG_ERR_DIMS, "Dimension mismatch mps and mpo");
+    DMRG_REQUIRE(envL[2] == mpo[0], DMRG_ERR_DIMS, "Dimension mismatch envL and mpo");
+    DMRG_REQUIRE(envR[2] == mpo[1], DMRG_ERR_DIMS, "Dimension mismatch envR and mpo");
+    DMRG_REQUIRE(envL[0] == envL[1] && envR[0] == envR[1], DMRG_ERR_DIMS, "env blocks must be square in the bond indices");
+    DMRG_REQUIRE(mpo[2] == mpo[3], DMRG_ERR_DIMS, "mpo must have equal physical dims");
+    for(int i = 0; i < 3; ++i) DMRG_REQUIRE(mps[i] > 0 && envL[i] > 0 && envR[i] > 0, DMRG_ERR_DIMS, "zero-sized dimension");
+    Dims D;
+    D.d    = mps[0];
+    D.chiL = mps[1];
+    D.chiR = mps[2];
+    D.DL   = mpo[0];
+    D.DR   = mpo[1];
+    DMRG_REQUIRE(D.DL > 0 && D.DR > 0, DMRG_ERR_DIMS, "zero-sized mpo");
+    return D;
+}
+
+int64_t matvec_ws_elems(const Dims &D) { return D.d * D.chiL * D.chiR * (D.DL + D.DR); }
+
+static GemmDesc make(dmrg_dtype t, int64_t M, int64_t N, int64_t K, int nvec) {
+    DMRG_REQUIRE(M < (int64_t(1) << 31) && N < (int64_t(1) << 31) && K < (int64_t(1) << 31), DMRG_ERR_DIMS,
+                 "contraction too large");
+    GemmDesc g;
+    g.dtype = t;
+    g.M     = static_cast<int32_t>(M);
+    g.N     = static_cast<int32_t>(N);
+    g.K     = static_cast<int32_t>(K);
+    g.batch = nvec;
+    return g;
+}
+
+void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *out, int64_t ldy, int nvec,
+            const void *mpo, const void *envL, const void *envR, void *ws, hipStream_t s, hipEvent_t *ev) {
+    const int64_t d = D.d, cL = D.chiL, cR = D.chiR, DL = D.DL, DR = D.DR;
+    const size_t  es  = dtype_size(t);
+    auto          mark = [&](int i) {
+        if(ev) DMRG_HIP_CHECK(hipEventRecord(ev[i], s));
+    };
+    mark(0);
+    char         *w0  = static_cast<char *>(ws);
+    const int64_t nT1 = (cL >= cR) ? d * cR * cL * DL : d * cL * cR * DR; // first intermediate per vector
+    const int64_t nT2 = (cL >= cR) ? cR * DR * d * cL : cL * DL * cR * d; // second intermediate per vector
+    void         *T1  = w0;
+    void         *T2  = w0 + static_cast<size_t>(nvec) * nT1 * es;
+
+    if(cL >= cR) {
+        // Order of contraction_tblis.cpp:279-283 (theta.L first).
+        // Leg 1: T1[s,b,a',w] = sum_a theta[s,a,b] L[a,a',w]           m=(s,b) k=a n=(a',w)
+        GemmDesc g1 = make(t, d * cR, cL * DL, cL, nvec);
+        g1.A = theta, g1.am = fused(1, d * cL, d), g1.ak = plain(d), g1.a_batch = ldx;
+        g1.B = envL, g1.bk = plain(1), g1.bn = plain(cL);
+        g1.C = T1, g1.cm = plain(1), g1.cn = plain(d * cR), g1.c_batch = nT1;
+        gemm(g1, s);
+        mark(1);
+        // Leg 2: T2[b,w',s',a'] = sum_{s,w} W[w,w',s,s'] T1[s,b,a',w]  m=(w',s') k=(s,w) n=(b,a')
+        GemmDesc g2 = make(t, DR * d, cR * cL, d * DL, nvec);
+        g2.A = mpo, g2.am = fused(DL, DL * DR * d, DR), g2.ak = fused(DL * DR, 1, d);
+        g2.B = T1, g2.bk = fused(1, d * cR * cL, d), g2.bn = fused(d, d * cR, cR), g2.b_batch = nT1;
+        g2.C = T2, g2.cm = fused(cR, cR * DR, DR), g2.cn = fused(1, cR * DR * d, cR), g2.c_batch = nT2;
+        gemm(g2, s);
+        mark(2);
+        // Leg 3: out[s',a',b'] = sum_{b,w'} T2[b,w',s',a'] R[b,b',w']   m=(s',a') k=(b,w') n=b'
+        GemmDesc g3 = make(t, d * cL, cR, cR * DR, nvec);
+        g3.A = T2, g3.am = plain(cR * DR), g3.ak = plain(1), g3.a_batch = nT2;
+        g3.B = envR, g3.bk = fused(1, cR * cR, cR), g3.bn = plain(cR);
+        g3.C = out, g3.cm = plain(1), g3.cn = plain(d * cL), g3.c_batch = ldy;
+        gemm(g3, s);
+        mark(3);
+    } else {
+        // Order of contraction_tblis.cpp:285-289 (theta.R first).
+        // Leg 1: U[s,a,b',w'] = sum_b theta[s,a,b] R[b,b',w']            m=(s,a) k=b n=(b',w')
+        GemmDesc g1 = make(t, d * cL, cR * DR, cR, nvec);
+        g1.A = theta, g1.am = plain(1), g1.ak = plain(d * cL), g1.a_batch = ldx;
+        g1.B = envR, g1.bk = plain(1), g1.bn = plain(cR);
+        g1.C = T1, g1.cm = plain(1), g1.cn = plain(d * cL), g1.c_batch = nT1;
+        gemm(g1, s);
+        mark(1);
+        // Leg 2: V[a,w,b',s'] = sum_{s,w'} W[w,w',s,s'] U[s,a,b',w']    m=(w,s') k=(s,w') n=(a,b')
+        GemmDesc g2 = make(t, DL * d, cL * cR, d * DR, nvec);
+        g2.A = mpo, g2.am = fused(1, DL * DR * d, DL), g2.ak = fused(DL * DR, DL, d);
+        g2.B = T1, g2.bk = fused(1, d * cL * cR, d), g2.bn = fused(d, d * cL, cL), g2.b_batch = nT1;
+        g2.C = T2, g2.cm = fused(cL, cL * DL * cR, DL), g2.cn = fused(1, cL * DL, cL), g2.c_batch = nT2;
+        gemm(g2, s);
+        mark(2);
+        // Leg 3: out[s',a',b'] = sum_{a,w} L[a,a',w] V[a,w,b',s']       m=a' k=(a,w) n=(b',s')
+        GemmDesc g3 = make(t, cL, cR * d, cL * DL, nvec);
+        g3.A = envL, g3.am = plain(cL), g3.ak = fused(1, cL * cL, cL);
+        g3.B = T2, g3.bk = plain(1), g3.bn = plain(cL * DL), g3.b_batch = nT2;
+        g3.C = out, g3.cm = plain(d), g3.cn = fused(d * cL, 1, cR), g3.c_batch = ldy;
+        gemm(g3, s);
+        mark(3);
+    }
+}
+
+int64_t env_ws_elems(const Dims &D) { return D.d * D.chiL * D.chiR * (D.DL + D.DR); }
+
+void env_build_block(dmrg_dtype t, char side, const Dims &D, void *out, const void *other, const void *theta,
+                     const void *mpo, void *ws, hipStream_t s) {
+    const int64_t d = D.d, cL = D.chiL, cR = D.chiR, DL = D.DL, DR = D.DR;
+    const size_t  es = dtype_size(t);
+    char         *w0 = static_cast<char *>(ws);
+    if(side == 'L') {
+        // L'[b,b',w'] = sum theta[s,a,b] L[a,a',w] W[w,w',s,s'] conj theta[s',a',b']   (EnvBase.cpp:109-112)
+        void *X = w0;                                       // X[a,w,s',b']  (chiL*DL) x (d*chiR)
+        void *Y = w0 + static_cast<size_t>(cL * DL * d * cR) * es; // Y[s,a,b',w']  (d*chiL) x (chiR*DR)
+        // X[(a,w),(s',b')] = sum_a' L[a,a',w] conj theta[s',a',b']       m=(a,w) k=a' n=(s',b')
+        GemmDesc e1 = make(t, cL * DL, d * cR, cL, 1);
+        e1.A = other, e1.am = fused(1, cL * cL, cL), e1.ak = plain(cL);
+        e1.B = theta, e1.bk = plain(d), e1.bn = fused(1, d * cL, d), e1.conjB = 1;
+        e1.C = X, e1.cm = plain(1), e1.cn = plain(cL * DL);
+        gemm(e1, s);
+        // Y[(s,a),(b',w')] = sum_{s',w} W[w,w',s,s'] X[a,w,s',b']        m=(s,w') k=(s',w) n=(a,b')
+        GemmDesc e2 = make(t, d * DR, cL * cR, d * DL, 1);
+        e2.A = mpo, e2.am = fused(DL * DR, DL, d), e2.ak = fused(DL * DR * d, 1, d);
+        e2.B = X, e2.bk = fused(cL * DL, cL, d), e2.bn = fused(1, cL * DL * d, cL);
+        e2.C = Y, e2.cm = fused(1, d * cL * cR, d), e2.cn = fused(d, d * cL, cL);
+        gemm(e2, s);
+        // L'[b,(b',w')] = sum_{(s,a)} theta[s,a,b] Y[(s,a),(b',w')]        m=b k=(s,a) n=(b',w')
+        GemmDesc e3 = make(t, cR, cR * DR, d * cL, 1);
+        e3.A = theta, e3.am = plain(d * cL), e3.ak = plain(1);
+        e3.B = Y, e3.bk = plain(1), e3.bn = plain(d * cL);
+        e3.C = out, e3.cm = plain(1), e3.cn = plain(cR);
+        gemm(e3, s);
+    } else if(side == 'R') {
+        // R'[a,a',w] = sum theta[s,a,b] R[b,b',w'] W[w,w',s,s'] conj theta[s',a',b']   (EnvBase.cpp:146-149)
+        void *X = w0;                                       // X[s',a',b,w']  (d*chiL) x (chiR*DR)
+        void *Y = w0 + static_cast<size_t>(d * cL * cR * DR) * es; // Y[s,b,a',w]   (d*chiR) x (chiL*DL)
+        // X[(s',a'),(b,w')] = sum_b' conj theta[s',a',b'] R[b,b',w']      m=(s',a') k=b' n=(b,w')
+        GemmDesc e1 = make(t, d * cL, cR * DR, cR, 1);
+        e1.A = theta, e1.am = plain(1), e1.ak = plain(d * cL), e1.conjA = 1;
+        e1.B = other, e1.bk = plain(cR), e1.bn = fused(1, cR * cR, cR);
+        e1.C = X, e1.cm = plain(1), e1.cn = plain(d * cL);
+        gemm(e1, s);
+        // Y[(s,b),(a',w)] = sum_{s',w'} W[w,w',s,s'] X[s',a',b,w']        m=(s,w) k=(s',w') n=(b,a')
+        GemmDesc e2 = make(t, d * DL, cR * cL, d * DR, 1);
+        e2.A = mpo, e2.am = fused(DL * DR, 1, d), e2.ak = fused(DL * DR * d, DL, d);
+        e2.B = X, e2.bk = fused(1, d * cL * cR, d), e2.bn = fused(d * cL, d, cR);
+        e2.C = Y, e2.cm = fused(1, d * cR * cL, d), e2.cn = fused(d, d * cR, cR);
+        gemm(e2, s);
+        // R'[a,(a',w)] = sum_{(s,b)} theta[s,a,b] Y[(s,b),(a',w)]          m=a k=(s,b) n=(a',w)
+        GemmDesc e3 = make(t, cL, cL * DL, d * cR, 1);
+        e3.A = theta, e3.am = plain(d), e3.ak = fused(1, d * cL, d);
+        e3.B = Y, e3.bk = plain(1), e3.bn = plain(d * cR);
+        e3.C = out, e3.cm = plain(1), e3.cn = plain(cL);
+        gemm(e3, s);
+    } else {
+        throw Error(DMRG_ERR_ARG, "env side must be 'L' or 'R'");
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Stateless workspace
+
+void *scratch_workspace(size_t bytes) {
+    static std::mutex          mu;
+    static std::vector<void *> bufs;
+    static std::vector<size_t> sizes;
+    std::lock_guard<std::mutex> lock(mu);
+    int                         dev = 0;
+    DMRG_HIP_CHECK(hipGetDevice(&dev));
+    if(static_cast<int>(bufs.size()) <= dev) {
+        bufs.resize(dev + 1, nullptr);
+        sizes.resize(dev + 1, 0);
+    }
+    if(sizes[dev] < bytes) {
+        if(bufs[dev]) {
+            // Other streams may still be reading the old buffer.
+            DMRG_HIP_CHECK(hipDeviceSynchronize());
+            DMRG_HIP_CHECK(hipFree(bufs[dev]));
+            bufs[dev]  = nullptr;
+            sizes[dev] = 0;
+        }
+        void *p = nullptr;
+        if(hipMalloc(&p, bytes) != hipSuccess) throw Error(DMRG_ERR_OOM, "workspace allocation failed");
+        bufs[dev]  = p;
+        sizes[dev] = bytes;
+    }
+    return bufs[dev];
+}
+
+} // namespace dmrg

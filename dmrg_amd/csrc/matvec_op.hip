@@ -1,0 +1,168 @@
+// MatVecOp: device-resident H_eff operator (MatVecMPOS<T>, matvec_mpos.cpp:40-130, 963-984, 1433-1472).
+#include "matvec_op.hpp"
+
+#include <cstring>
+
+namespace dmrg {
+
+using cx = std::complex<double>;
+
+std::vector<cx> merge_mpos(int nmpo, const cx *const *mpos, const int64_t *dims, int64_t out[4]) {
+    DMRG_REQUIRE(nmpo >= 1 && mpos && dims, DMRG_ERR_ARG, "need at least one MPO");
+    for(int i = 0; i < nmpo; ++i) {
+        DMRG_REQUIRE(mpos[i], DMRG_ERR_ARG, "null MPO");
+        DMRG_REQUIRE(dims[4 * i] > 0 && dims[4 * i + 1] > 0 && dims[4 * i + 2] > 0, DMRG_ERR_DIMS, "zero-sized MPO");
+        DMRG_REQUIRE(dims[4 * i + 2] == dims[4 * i + 3], DMRG_ERR_DIMS, "MPO has different spin dimensions up and down");
+        if(i > 0) DMRG_REQUIRE(dims[4 * i] == dims[4 * (i - 1) + 1], DMRG_ERR_DIMS, "MPO chain bond mismatch");
+    }
+    int64_t         DL = dims[0], DR = dims[1], d = dims[2];
+    std::vector<cx> cur(mpos[0], mpos[0] + DL * DR * d * d);
+    for(int i = 1; i < nmpo; ++i) {
+        const int64_t DRn = dims[4 * i + 1], dn = dims[4 * i + 2], Dm = DR;
+        const cx     *B   = mpos[i];
+        const int64_t dd  = d * dn;
+        std::vector<cx> nxt(static_cast<size_t>(DL * DRn * dd * dd), cx(0.0, 0.0));
+        // new[wL,wR,(s1,s2),(s1',s2')] = sum_w A[wL,w,s1,s1'] B[w,wR,s2,s2'], s1 fastest in the fused index.
+        for(int64_t s2p = 0; s2p < dn; ++s2p)
+            for(int64_t s1p = 0; s1p < d; ++s1p)
+                for(int64_t s2 = 0; s2 < dn; ++s2)
+                    for(int64_t s1 = 0; s1 < d; ++s1)
+                        for(int64_t wR = 0; wR < DRn; ++wR)
+                            for(int64_t wL = 0; wL < DL; ++wL) {
+                                cx acc(0.0, 0.0);
+                                for(int64_t w = 0; w < Dm; ++w)
+                                    acc += cur[wL + DL * (w + Dm * (s1 + d * s1p))] * B[w + Dm * (wR + DRn * (s2 + dn * s2p))];
+                                nxt[wL + DL * (wR + DRn * ((s1 + d * s2) + dd * (s1p + d * s2p)))] = acc;
+                            }
+        cur = std::move(nxt);
+        DR  = DRn;
+        d   = dd;
+    }
+    out[0] = DL, out[1] = DR, out[2] = d, out[3] = d;
+    return cur;
+}
+
+template <typename T>
+static void upload_as(DevBuf &buf, const cx *src, int64_t n, dmrg_dtype t) {
+    const size_t es = dtype_size(t);
+    buf.reserve(static_cast<size_t>(n) * es);
+    if(t == DMRG_CX64) {
+        DMRG_HIP_CHECK(hipMemcpy(buf.p, src, static_cast<size_t>(n) * es, hipMemcpyHostToDevice));
+    } else {
+        std::vector<double> re(static_cast<size_t>(n));
+        for(int64_t i = 0; i < n; ++i) re[i] = src[i].real();
+        DMRG_HIP_CHECK(hipMemcpy(buf.p, re.data(), static_cast<size_t>(n) * es, hipMemcpyHostToDevice));
+    }
+}
+
+MatVecOp::MatVecOp(dmrg_dtype t, int nmpo, const void *const *mpos, const int64_t *mpo_dims, const void *envL_h,
+                   const int64_t envL_dims[3], const void *envR_h, const int64_t envR_dims[3])
+    : dtype(t), nmpo_in(nmpo) {
+    DMRG_REQUIRE(envL_h && envR_h && envL_dims && envR_dims, DMRG_ERR_ARG, "null env");
+    int64_t md[4];
+    mpo_host = merge_mpos(nmpo, reinterpret_cast<const cx *const *>(mpos), mpo_dims, md);
+    const int64_t mps_dims[3] = {md[2], envL_dims[0], envR_dims[0]};
+    D                         = check_matvec_dims(mps_dims, md, envL_dims, envR_dims);
+    if(t == DMRG_FP64) {
+        // MatVecMPOS<fp64> requires a real MPO (matvec_mpos.cpp:59); env blocks are cast with .real().
+        for(const auto &v : mpo_host) DMRG_REQUIRE(v.imag() == 0.0, DMRG_ERR_ARG, "mpo is not real");
+    }
+    upload_mpo();
+    upload_as<double>(envL, static_cast<const cx *>(envL_h), D.chiL * D.chiL * D.DL, t);
+    upload_as<double>(envR, static_cast<const cx *>(envR_h), D.chiR * D.chiR * D.DR, t);
+}
+
+void MatVecOp::upload_mpo() {
+    upload_as<double>(mpo, mpo_host.data(), static_cast<int64_t>(mpo_host.size()), dtype);
+    have32 = false;
+}
+
+void MatVecOp::ensure_fp32(hipStream_t s) {
+    if(have32) return;
+    const int64_t cf = dtype_is_complex(dtype) ? 2 : 1;
+    const int64_t nm = D.DL * D.DR * D.d * D.d * cf, nl = D.chiL * D.chiL * D.DL * cf, nr = D.chiR * D.chiR * D.DR * cf;
+    mpo32.reserve(nm * sizeof(float));
+    envL32.reserve(nl * sizeof(float));
+    envR32.reserve(nr * sizeof(float));
+    convert_f64_to_f32(nm, mpo.as<double>(), mpo32.as<float>(), s);
+    convert_f64_to_f32(nl, envL.as<double>(), envL32.as<float>(), s);
+    convert_f64_to_f32(nr, envR.as<double>(), envR32.as<float>(), s);
+    have32 = true;
+}
+
+void MatVecOp::multax_device(const void *dx, int64_t ldx, void *dy, int64_t ldy, int nvec, hipStream_t s, bool fp32) {
+    if(nvec == 0) return;
+    const int64_t n = D.n();
+    DMRG_REQUIRE(ldx >= n && ldy >= n, DMRG_ERR_ARG, "leading dimension smaller than rows()");
+    if(fp32) {
+        ensure_fp32(s);
+        const dmrg_dtype t32 = dtype_single_of(dtype);
+        void *w = ws32.reserve(static_cast<size_t>(nvec) * matvec_ws_elems(D) * dtype_size(t32));
+        matvec(t32, D, dx, ldx, dy, ldy, nvec, mpo32.p, envL32.p, envR32.p, w, s);
+    } else {
+        void *w = ws.reserve(static_cast<size_t>(nvec) * matvec_ws_elems(D) * dtype_size(dtype));
+        matvec(dtype, D, dx, ldx, dy, ldy, nvec, mpo.p, envL.p, envR.p, w, s);
+    }
+    num_mv += nvec;
+}
+
+void MatVecOp::multax_host(const void *x, int64_t ldx, void *y, int64_t ldy, int nvec) {
+    if(nvec == 0) return;
+    const int64_t n  = D.n();
+    const size_t  es = dtype_size(dtype);
+    DMRG_REQUIRE(ldx >= n && ldy >= n, DMRG_ERR_ARG, "leading dimension smaller than rows()");
+    xbuf.reserve(static_cast<size_t>(nvec) * n * es);
+    ybuf.reserve(static_cast<size_t>(nvec) * n * es);
+    DMRG_HIP_CHECK(hipMemcpy2D(xbuf.p, n * es, x, ldx * es, n * es, nvec, hipMemcpyHostToDevice));
+    multax_device(xbuf.p, n, ybuf.p, n, nvec, nullptr, false);
+    DMRG_HIP_CHECK(hipMemcpy2D(y, ldy * es, ybuf.p, n * es, n * es, nvec, hipMemcpyDeviceToHost));
+}
+
+void MatVecOp::set_shift(cx shift) {
+    // matvec_mpos.cpp:1437-1471: applied once (readyShift), on the merged MPO's bottom-left corner.
+    if(ready_shift) return;
+    if(sigma == shift) return;
+    const cx delta = sigma - shift; // one merged MPO: shift per MPO = shift / 1
+    for(int64_t s = 0; s < D.d; ++s) {
+        cx &v = mpo_host[(D.DL - 1) + D.DL * (0 + D.DR * (s + D.d * s))];
+        v += (dtype == DMRG_FP64) ? cx(delta.real(), 0.0) : delta;
+    }
+    sigma = shift;
+    upload_mpo();
+    ready_shift = true;
+}
+
+void MatVecOp::diagonal(void *out, hipStream_t s) {
+    void *w = diagws.reserve(static_cast<size_t>(D.chiL * D.DR * D.d) * dtype_size(dtype));
+    heff_diagonal(dtype, D, mpo.p, envL.p, envR.p, w, out, s);
+}
+
+void MatVecOp::time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms[3], hipStream_t s) {
+    hipEvent_t ev[4];
+    for(auto &e : ev) DMRG_HIP_CHECK(hipEventCreate(&e));
+    double acc[3] = {0, 0, 0};
+    try {
+        if(fp32) ensure_fp32(s);
+        const dmrg_dtype t  = fp32 ? dtype_single_of(dtype) : dtype;
+        DevBuf          &wb = fp32 ? ws32 : ws;
+        void            *w  = wb.reserve(static_cast<size_t>(matvec_ws_elems(D)) * dtype_size(t));
+        for(int r = 0; r < nrep; ++r) {
+            matvec(t, D, dx, D.n(), dy, D.n(), 1, fp32 ? mpo32.p : mpo.p, fp32 ? envL32.p : envL.p,
+                   fp32 ? envR32.p : envR.p, w, s, ev);
+            DMRG_HIP_CHECK(hipEventSynchronize(ev[3]));
+            for(int i = 0; i < 3; ++i) {
+                float t_ms = 0;
+                DMRG_HIP_CHECK(hipEventElapsedTime(&t_ms, ev[i], ev[i + 1]));
+                acc[i] += t_ms;
+            }
+        }
+        num_mv += nrep;
+    } catch(...) {
+        for(auto &e : ev) (void) hipEventDestroy(e);
+        throw;
+    }
+    for(auto &e : ev) (void) hipEventDestroy(e);
+    for(int i = 0; i < 3; ++i) ms[i] = static_cast<float>(acc[i] / nrep);
+}
+
+} // namespace dmrg

@@ -1,0 +1,76 @@
+// Device-resident H_eff operator: the GPU counterpart of MatVecMPOS<T> (matvec_mpos.h:23-180).
+#pragma once
+#include "contract.hpp"
+
+#include <complex>
+#include <vector>
+
+namespace dmrg {
+
+// Owning device buffer (grow-only).
+struct DevBuf {
+    void  *p     = nullptr;
+    size_t bytes = 0;
+    DevBuf()     = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr, o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if(p) (void) hipFree(p);
+        p = nullptr, bytes = 0;
+    }
+    void *reserve(size_t b) {
+        if(b > bytes) {
+            release();
+            if(hipMalloc(&p, b) != hipSuccess) {
+                (void) hipGetLastError();
+                p = nullptr;
+                throw Error(DMRG_ERR_OOM, "device allocation of " + std::to_string(b) + " bytes failed");
+            }
+            bytes = b;
+        }
+        return p;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+// Merge a chain of MPOs (cx64, each (DL_i, DR_i, d_i, d_i)) into one (DL_0, DR_last, prod d, prod d) with the
+// first site's physical index fastest (matvec_mpos.cpp:104-119: contract {1},{0}, shuffle {0,3,1,4,2,5}).
+std::vector<std::complex<double>> merge_mpos(int nmpo, const std::complex<double> *const *mpos, const int64_t *dims,
+                                             int64_t out_dims[4]);
+
+struct MatVecOp {
+    dmrg_dtype                         dtype;
+    Dims                               D;
+    int                                nmpo_in = 1;
+    std::vector<std::complex<double>>  mpo_host; // merged MPO in cx64 (shift applied), source of the device copies
+    DevBuf                             mpo, envL, envR;       // operator dtype
+    DevBuf                             mpo32, envL32, envR32; // FP32 shadows for the mixed path
+    bool                               have32 = false;
+    DevBuf                             ws, ws32, xbuf, ybuf, diagws;
+    std::complex<double>               sigma{0.0, 0.0};
+    bool                               ready_shift = false;
+    int64_t                            num_mv      = 0;
+
+    MatVecOp(dmrg_dtype t, int nmpo, const void *const *mpos, const int64_t *mpo_dims, const void *envL_h,
+             const int64_t envL_dims[3], const void *envR_h, const int64_t envR_dims[3]);
+
+    void upload_mpo();
+    void ensure_fp32(hipStream_t s);
+    void multax_device(const void *dx, int64_t ldx, void *dy, int64_t ldy, int nvec, hipStream_t s, bool fp32);
+    void multax_host(const void *x, int64_t ldx, void *y, int64_t ldy, int nvec);
+    void set_shift(std::complex<double> shift);
+    void diagonal(void *out, hipStream_t s);
+    void time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms[3], hipStream_t s);
+};
+
+} // namespace dmrg

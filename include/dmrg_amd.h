@@ -1,0 +1,148 @@
+/*
+ * dmrg_amd — MI355X (gfx950) implementation of the DMRG++ effective-Hamiltonian hot path.
+ *
+ * C ABI. Plain pointers and sizes only; no exception crosses this boundary. Every call
+ * returns a dmrg_status; on failure dmrg_last_error() gives a message (thread-local).
+ *
+ * Tensor layouts are those of the reference (Eigen column-major, complex interleaved
+ * as std::complex<double>):
+ *   mps / theta  (d, chiL, chiR)        element [s,a,b]   at s + d*(a + chiL*b)
+ *   env block    (chi, chi, D)          element [a,a',w]  at a + chi*(a' + chi*w)   (index 0 = ket)
+ *   mpo          (DL, DR, d, d)         element [w,w',s,s'] at w + DL*(w' + DR*(s + d*s'))
+ * Pointers named d* or documented "device" are device (HBM) pointers; "host" are host.
+ * A NULL stream means the default stream.
+ */
+#ifndef DMRG_AMD_H
+#define DMRG_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dmrg_status {
+    DMRG_OK               = 0,
+    DMRG_ERR_DIMS         = 1, /* dimension mismatch (reference: except::runtime_error, contraction_tblis.cpp:230-234) */
+    DMRG_ERR_ARG          = 2, /* invalid argument */
+    DMRG_ERR_OOM          = 3, /* device allocation failed */
+    DMRG_ERR_HIP          = 4, /* HIP runtime error */
+    DMRG_ERR_NAN          = 5, /* NaN/Inf detected */
+    DMRG_ERR_NOT_CONVERGED = 6, /* eigensolver hit its iteration or time limit */
+    DMRG_ERR_UNSUPPORTED  = 7
+} dmrg_status;
+
+typedef enum dmrg_dtype {
+    DMRG_FP64 = 0, /* double                                      (reference fp64) */
+    DMRG_CX64 = 1, /* std::complex<double>, interleaved            (reference cx64) */
+    DMRG_FP32 = 2, /* float: inner precision of the mixed path */
+    DMRG_CX32 = 3  /* std::complex<float>: inner precision of the mixed path */
+} dmrg_dtype;
+
+/* ---------------------------------------------------------------- runtime */
+const char *dmrg_last_error(void);
+int         dmrg_version(void);
+dmrg_status dmrg_device_count(int *count);
+dmrg_status dmrg_set_device(int device);
+dmrg_status dmrg_device_info(char *name, size_t name_len, int *cu_count, size_t *hbm_bytes);
+dmrg_status dmrg_malloc(void **dptr, size_t bytes);
+dmrg_status dmrg_free(void *dptr);
+dmrg_status dmrg_memcpy_htod(void *dst, const void *src, size_t bytes);
+dmrg_status dmrg_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+dmrg_status dmrg_memcpy_dtod(void *dst, const void *src, size_t bytes);
+dmrg_status dmrg_memset(void *dptr, int value, size_t bytes);
+dmrg_status dmrg_synchronize(void);
+
+/* Stream and event helpers (default priority streams only). */
+dmrg_status dmrg_stream_create(void **stream);
+dmrg_status dmrg_stream_destroy(void *stream);
+dmrg_status dmrg_stream_synchronize(void *stream);
+dmrg_status dmrg_event_create(void **event);
+dmrg_status dmrg_event_destroy(void *event);
+dmrg_status dmrg_event_record(void *event, void *stream);
+dmrg_status dmrg_event_elapsed_ms(float *ms, void *start, void *stop);
+
+/* ---------------------------------------------------------------- contractions (device pointers) */
+
+/* out[s',a',b'] = sum L[a,a',w] W[w,w',s,s'] R[b,b',w'] theta[s,a,b].
+ * Replaces tools::common::contraction::matrix_vector_product<Scalar>(res, mps, mps_dims, mpo, mpo_dims,
+ * envL, envL_dims, envR, envR_dims)  (contraction.h:31-36; contraction_tblis.cpp:213-292), Scalar in
+ * {fp64, cx64}; FP32/CX32 run the same contraction on FP32 matrix cores. Same dimension checks
+ * (contraction_tblis.cpp:230-234) -> DMRG_ERR_DIMS. Workspace is held by the library. */
+dmrg_status dmrg_matrix_vector_product(dmrg_dtype dtype, void *res, const void *mps, const int64_t mps_dims[3],
+                                       const void *mpo, const int64_t mpo_dims[4], const void *envL,
+                                       const int64_t envL_dims[3], const void *envR, const int64_t envR_dims[3],
+                                       void *stream);
+
+/* Environment update: side 'L': block[b,b',w'] = sum theta[s,a,b] other[a,a',w] W[w,w',s,s'] conj(theta[s',a',b']);
+ *                     side 'R': block[a,a',w]  = sum theta[s,a,b] other[b,b',w'] W[w,w',s,s'] conj(theta[s',a',b']).
+ * Replaces EnvBase::build_block(otherblock, mps, mpo) (EnvBase.cpp:67-151); the reference is always cx64
+ * (EnvBase.h:22-26); DMRG_FP64 is provided for real problems. block_out has shape (chiR,chiR,DR) for 'L'
+ * and (chiL,chiL,DL) for 'R'. Dimension checks of EnvBase.cpp:100-108,137-145 -> DMRG_ERR_DIMS. */
+dmrg_status dmrg_env_build_block(dmrg_dtype dtype, char side, void *block_out, const void *otherblock,
+                                 const int64_t other_dims[3], const void *mps, const int64_t mps_dims[3],
+                                 const void *mpo, const int64_t mpo_dims[4], void *stream);
+
+/* <theta|H_eff|theta> = sum conj(theta) . (H_eff theta).
+ * Replaces tools::common::contraction::expectation_value (contraction_tblis.cpp:30-97). result[0..1] = re, im. */
+dmrg_status dmrg_expectation_value(dmrg_dtype dtype, const void *mps, const int64_t mps_dims[3], const void *mpo,
+                                   const int64_t mpo_dims[4], const void *envL, const int64_t envL_dims[3],
+                                   const void *envR, const int64_t envR_dims[3], double result[2], void *stream);
+
+/* sum_i conj(x_i) y_i. Replaces contract_mps_mps_overlap / contract_mps_overlap (contraction_tblis.cpp:518-533). */
+dmrg_status dmrg_dot(dmrg_dtype dtype, int64_t n, const void *x, const void *y, double result[2], void *stream);
+/* sqrt(sum |x_i|^2). Replaces contract_mps_norm (contraction.h:284-301). */
+dmrg_status dmrg_nrm2(dmrg_dtype dtype, int64_t n, const void *x, double *result, void *stream);
+
+/* ---------------------------------------------------------------- MatVecMPOS operator (device resident) */
+
+typedef struct dmrg_matvec dmrg_matvec;
+
+/* Builds the operator from host data in the reference's storage: nmpo MPOs (cx64, each (DL,DR,d,d), dims in
+ * mpo_dims[4*i..4*i+3]) and the env blocks (cx64, (chiL,chiL,DL) and (chiR,chiR,DR)). With dtype DMRG_FP64 the
+ * data are cast to real, as MatVecMPOS<fp64> does (matvec_mpos.cpp:54-65); DMRG_CX64 keeps them complex.
+ * Up to 5 MPOs are pre-merged into one (DL, DR, prod d, prod d) MPO (matvec_mpos.cpp:104-119); longer chains
+ * are applied one MPO at a time (contraction_tblis.cpp:307-420).
+ * Replaces MatVecMPOS<T>::MatVecMPOS(mpos, envs) (matvec_mpos.cpp:40-130). Data are copied to the device once. */
+dmrg_status dmrg_matvec_create(dmrg_matvec **op, dmrg_dtype dtype, int nmpo, const void *const *mpos,
+                               const int64_t *mpo_dims, const void *envL, const int64_t envL_dims[3],
+                               const void *envR, const int64_t envR_dims[3]);
+dmrg_status dmrg_matvec_destroy(dmrg_matvec *op);
+/* MatVecMPOS::rows(): n = prod(d) * chiL * chiR. */
+int64_t     dmrg_matvec_rows(const dmrg_matvec *op);
+/* MatVecMPOS::get_shape_mps(): {prod d, chiL, chiR}. */
+dmrg_status dmrg_matvec_shape_mps(const dmrg_matvec *op, int64_t shape[3]);
+/* MatVecMPOS::num_mv counter (matvec_mpos.cpp:973). */
+int64_t     dmrg_matvec_num_mv(const dmrg_matvec *op);
+
+/* y[:,i] = H_eff x[:,i], i < nvec, columns at stride ldx / ldy elements, HOST buffers.
+ * Replaces MatVecMPOS<T>::MultAx(void *x, int *ldx, void *y, int *ldy, int *blockSize, primme_params*, int *err)
+ * (matvec_mpos.cpp:976-984) and MultAx(T*, T*) (matvec_mpos.cpp:963-974): the PRIMME / ARPACK callback path. */
+dmrg_status dmrg_matvec_multax(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec);
+/* Same with DEVICE buffers, stream ordered; nvec vectors are applied in one batched launch per leg. */
+dmrg_status dmrg_matvec_multax_device(dmrg_matvec *op, const void *dx, int64_t ldx, void *dy, int64_t ldy, int nvec,
+                                      void *stream);
+/* Mixed-precision inner product: DEVICE FP32 vectors (DMRG_FP32 / DMRG_CX32 of the operator's kind) through
+ * FP32 matrix cores with FP32 accumulation. No counterpart in the reference (it declares T32 but never uses
+ * it, matvec_mpos.h:29,35). */
+dmrg_status dmrg_matvec_multax_device_fp32(dmrg_matvec *op, const void *dx, int64_t ldx, void *dy, int64_t ldy,
+                                           int nvec, void *stream);
+/* Replaces MatVecMPOS<T>::set_shift(cx64 sigma) (matvec_mpos.cpp:1433-1472): adds (sigma_old - sigma)/nmpo * I to
+ * the bottom-left corner W[DL-1,0,:,:] of every MPO of the operator. */
+dmrg_status dmrg_matvec_set_shift(dmrg_matvec *op, double re, double im);
+/* Diagonal of H_eff (length rows), DEVICE output, operator dtype. Replaces MatVecMPOS::get_diagonal_new
+ * (matvec_mpos.cpp:399-407); feeds the Jacobi preconditioner (matvec_mpos.cpp:1333-1416, jcb = 1). */
+dmrg_status dmrg_matvec_get_diagonal(dmrg_matvec *op, void *ddiag, void *stream);
+
+/* Profiling: runs nrep device matvecs (one vector, x -> y) on stream and returns the average duration of each
+ * of the three contraction legs in ms_per_leg[0..2], measured with HIP events on that stream. No counterpart in
+ * the reference (its per-operator timers t_multAx, matvec_mpos.h:174-179, time the whole call). */
+dmrg_status dmrg_matvec_time_legs(dmrg_matvec *op, const void *dx, void *dy, int nrep, int fp32, float ms_per_leg[3],
+                                  void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMRG_AMD_H */

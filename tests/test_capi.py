@@ -1,0 +1,54 @@
+"""CPU tests of the C-ABI boundary: the library loads and exports every declared symbol.
+
+No compute call is made here (no GPU on the build host)."""
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "dmrg_amd.h"
+LIB = ROOT / "dmrg_amd" / "libdmrg_amd.so"
+
+
+def declared_symbols():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dmrg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    for name in ("dmrg_matrix_vector_product", "dmrg_env_build_block", "dmrg_matvec_multax", "dmrg_matvec_create"):
+        assert name in syms
+
+
+def test_library_exports_all_declared_symbols():
+    assert LIB.exists(), "build the library first (__graft_entry__.build())"
+    lib = ctypes.CDLL(str(LIB))
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, f"missing exports: {missing}"
+
+
+def test_python_binding_covers_header():
+    from dmrg_amd import _lib
+    missing = [s for s in declared_symbols() if s not in _lib.SIGNATURES]
+    assert not missing, f"ctypes signatures missing for: {missing}"
+
+
+def test_no_device_needed_for_error_path():
+    """Argument validation runs on the host: a null-argument call fails with DMRG_ERR_ARG, no GPU touched."""
+    from dmrg_amd import _lib
+    lib = _lib.load()
+    assert lib.dmrg_matvec_rows(None) == -1
+    r = lib.dmrg_matvec_set_shift(None, 0.0, 0.0)
+    assert r == 2
+    assert b"null" in lib.dmrg_last_error()
+
+
+def test_exports_are_c_symbols():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(LIB)], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (dmrg_\w+)", out))
+    assert set(declared_symbols()) <= exported

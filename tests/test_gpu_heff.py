@@ -1,0 +1,205 @@
+"""GPU parity of the H_eff hot path against the oracle, through the C ABI.
+
+Tolerances: FP64 (real and complex) relative 2-norm error <= 1e-12 against the oracle;
+the FP32 matrix-core path (FP32 inputs, FP32 accumulation) <= 5e-5 relative.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import dmrg_amd as G
+from dmrg_amd import DmrgError, _lib
+from dmrg_amd.workloads import random_heff_problem
+from oracle import heff as O
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 5e-5
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    n = C.c_int()
+    _lib.call("dmrg_device_count", C.byref(n))
+    assert n.value > 0, "no GPU visible"
+    info = G.device_info()
+    print(f"[gpu] {info}", file=sys.stderr, flush=True)
+    return info
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
+
+
+def general(rng, d, cL, cR, DL, DR, complex_):
+    f = (lambda *s: rng.standard_normal(s) + 1j * rng.standard_normal(s)) if complex_ else (lambda *s: rng.standard_normal(s))
+    return [np.asfortranarray(x) for x in (f(d, cL, cR), f(DL, DR, d, d), f(cL, cL, DL), f(cR, cR, DR))]
+
+
+SHAPES = [
+    (1, 1, 1, 1, 1), (2, 1, 8, 1, 3), (2, 8, 1, 3, 1), (2, 7, 5, 3, 3), (2, 16, 16, 5, 5), (2, 33, 17, 5, 5),
+    (2, 17, 33, 5, 5), (1, 9, 13, 2, 4), (4, 20, 31, 8, 8), (3, 50, 70, 4, 6), (2, 64, 64, 3, 3),
+    (2, 100, 93, 9, 9), (2, 128, 128, 5, 5), (4, 64, 48, 8, 8), (2, 40, 40, 25, 25),
+]
+
+
+@pytest.mark.parametrize("complex_", [False, True], ids=["fp64", "cx64"])
+@pytest.mark.parametrize("shape", SHAPES, ids=[str(s) for s in SHAPES])
+def test_matvec_parity(shape, complex_):
+    rng = np.random.default_rng(hash(shape) % 2**32)
+    th, W, L, R = general(rng, *shape, complex_)
+    ref = O.matrix_vector_product(th, W, L, R)
+    out = G.matrix_vector_product(th, W, L, R)
+    assert out.dtype == th.dtype
+    assert rel(out, ref) <= TOL64, rel(out, ref)
+
+
+@pytest.mark.parametrize("chi", [128, 256, 512])
+def test_matvec_parity_isolated_workload(chi):
+    """The isolated single-site workload: theta (2, chi, chi) cx64, Hermitian env blocks (chi, chi, 5), seed 1."""
+    th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=1)
+    ref = O.matrix_vector_product(th, W, L, R)
+    out = G.matrix_vector_product(th, W, L, R)
+    assert rel(out, ref) <= TOL64
+
+
+@pytest.mark.parametrize("complex_", [False, True], ids=["fp32", "cx32"])
+@pytest.mark.parametrize("shape", [(2, 7, 5, 3, 3), (2, 64, 64, 5, 5), (2, 130, 120, 5, 5), (2, 48, 48, 25, 25)])
+def test_matvec_fp32_parity(shape, complex_):
+    rng = np.random.default_rng(7)
+    th, W, L, R = general(rng, *shape, complex_)
+    ref = O.matrix_vector_product(th, W, L, R)
+    lo = np.complex64 if complex_ else np.float32
+    out = G.matrix_vector_product(*(x.astype(lo) for x in (th, W, L, R)))
+    assert out.dtype == np.dtype(lo)
+    assert rel(out, ref) <= TOL32
+
+
+def test_matvec_dims_mismatch_raises():
+    rng = np.random.default_rng(1)
+    th, W, L, R = general(rng, 2, 4, 5, 3, 3, True)
+    with pytest.raises(DmrgError) as e:
+        G.matrix_vector_product(th, W, L[:3, :3], R)
+    assert e.value.status == 1
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("complex_", [False, True], ids=["fp64", "cx64"])
+@pytest.mark.parametrize("shape", [(2, 1, 1, 1, 3), (2, 5, 7, 3, 3), (2, 16, 16, 5, 5), (2, 37, 29, 3, 3),
+                                   (4, 24, 40, 8, 8), (2, 64, 64, 9, 9), (2, 128, 96, 5, 5)])
+def test_env_parity(shape, side, complex_):
+    d, cL, cR, DL, DR = shape
+    rng = np.random.default_rng(17)
+    th, W, L, R = general(rng, d, cL, cR, DL, DR, complex_)
+    other = L if side == "L" else R
+    ref = O.env_build_block(side, other, th, W)
+    out = G.env_build_block(side, other, th, W)
+    assert out.shape == ref.shape
+    assert rel(out, ref) <= TOL64
+
+
+def test_env_dims_mismatch_raises():
+    rng = np.random.default_rng(1)
+    th, W, L, R = general(rng, 2, 4, 5, 3, 3, True)
+    with pytest.raises(DmrgError) as e:
+        G.env_build_block("L", R, th, W)
+    assert e.value.status == 1
+
+
+@pytest.mark.parametrize("complex_", [False, True])
+def test_expectation_value_and_dots(complex_):
+    rng = np.random.default_rng(5)
+    th, W, L, R = general(rng, 2, 30, 20, 5, 5, complex_)
+    ev = G.expectation_value(th, W, L, R)
+    ref = O.expectation_value(th, W, L, R)
+    assert abs(ev - ref) <= TOL64 * abs(ref) * 10
+    y = rng.standard_normal(th.shape) + (1j * rng.standard_normal(th.shape) if complex_ else 0)
+    y = np.asfortranarray(y.astype(th.dtype))
+    assert abs(G.dot(th, y) - np.vdot(th, y)) <= 1e-13 * np.linalg.norm(th) * np.linalg.norm(y)
+    assert abs(G.nrm2(th) - np.linalg.norm(th)) <= 1e-13 * np.linalg.norm(th)
+
+
+def test_dot_large_and_empty():
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal(3_000_001) + 1j * rng.standard_normal(3_000_001)
+    y = rng.standard_normal(3_000_001) + 1j * rng.standard_normal(3_000_001)
+    assert abs(G.dot(x, y) - np.vdot(x, y)) <= 1e-12 * np.linalg.norm(x) * np.linalg.norm(y)
+    # deterministic: same result twice
+    assert G.dot(x, y) == G.dot(x, y)
+
+
+# ------------------------------------------------------------------------------ MatVecMPOS
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128])
+def test_matvec_op_two_site_merge(dtype):
+    rng = np.random.default_rng(31)
+    cplx = dtype == np.complex128
+    W1 = rng.standard_normal((3, 4, 2, 2)) + (1j * rng.standard_normal((3, 4, 2, 2)) if cplx else 0)
+    W2 = rng.standard_normal((4, 5, 2, 2)) + (1j * rng.standard_normal((4, 5, 2, 2)) if cplx else 0)
+    L = rng.standard_normal((12, 12, 3)) + (1j * rng.standard_normal((12, 12, 3)) if cplx else 0)
+    R = rng.standard_normal((9, 9, 5)) + (1j * rng.standard_normal((9, 9, 5)) if cplx else 0)
+    op = G.MatVecMPOS([W1, W2], L, R, dtype=dtype)
+    assert op.shape_mps == (4, 12, 9) and op.rows() == 4 * 12 * 9
+    Wm = O.merge_mpos([W1.astype(complex), W2.astype(complex)])
+    x = rng.standard_normal((op.rows(), 3)) + (1j * rng.standard_normal((op.rows(), 3)) if cplx else 0)
+    y = op.MultAx(x.astype(dtype))
+    for j in range(3):
+        th = x[:, j].reshape(op.shape_mps, order="F").astype(dtype)
+        ref = O.matrix_vector_product(th, Wm.astype(dtype), L.astype(dtype), R.astype(dtype))
+        assert rel(y[:, j], ref.reshape(-1, order="F")) <= TOL64
+    assert op.num_mv == 3
+    op.close()
+
+
+def test_matvec_op_real_rejects_complex_mpo():
+    W = np.zeros((1, 1, 2, 2), dtype=complex)
+    W[0, 0] = O.SY
+    with pytest.raises(DmrgError):
+        G.MatVecMPOS([W], np.ones((3, 3, 1)), np.ones((3, 3, 1)), dtype=np.float64)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128])
+def test_matvec_op_set_shift_and_diagonal(dtype):
+    th, W, L, R = random_heff_problem(2, 20, 24, 4, 3, seed=3, complex_=(dtype == np.complex128))
+    op = G.MatVecMPOS([W], L, R, dtype=dtype)
+    diag = op.get_diagonal()
+    H = O.dense_heff(W, L, R)
+    np.testing.assert_allclose(diag.reshape(-1, order="F"), np.diag(H), rtol=1e-12, atol=1e-12 * np.abs(H).max())
+    sigma = 0.37
+    op.set_shift(sigma)
+    W2 = W.copy()
+    for s in range(W.shape[2]):
+        W2[-1, 0, s, s] -= sigma  # (sigma_old - sigma_new)/1 on the bottom-left corner
+    ref = O.matrix_vector_product(th, W2, L, R)
+    out = op.MultAx(th.reshape(-1, order="F")).reshape(th.shape, order="F")
+    assert rel(out, ref) <= TOL64
+    op.set_shift(1.5)  # readyShift: a second shift is ignored (matvec_mpos.cpp:1437)
+    out2 = op.MultAx(th.reshape(-1, order="F")).reshape(th.shape, order="F")
+    assert rel(out2, ref) <= TOL64
+    op.close()
+
+
+def test_matvec_op_device_block_and_fp32():
+    th, W, L, R = random_heff_problem(2, 96, 80, 5, 5, seed=4)
+    op = G.MatVecMPOS([W], L, R, dtype=np.complex128)
+    n = op.rows()
+    rng = np.random.default_rng(4)
+    X = np.asfortranarray(rng.standard_normal((n, 4)) + 1j * rng.standard_normal((n, 4)))
+    dx = G.DeviceArray.from_numpy(X)
+    dy = G.DeviceArray((n, 4), np.complex128)
+    op.MultAx_device(dx, dy, nvec=4)
+    Y = dy.to_numpy()
+    for j in range(4):
+        ref = O.matrix_vector_product(X[:, j].reshape(th.shape, order="F"), W, L, R)
+        assert rel(Y[:, j], ref.reshape(-1, order="F")) <= TOL64
+    dx32 = G.DeviceArray.from_numpy(X.astype(np.complex64))
+    dy32 = G.DeviceArray((n, 4), np.complex64)
+    op.MultAx_device(dx32, dy32, nvec=4, fp32=True)
+    Y32 = dy32.to_numpy()
+    for j in range(4):
+        ref = O.matrix_vector_product(X[:, j].reshape(th.shape, order="F"), W, L, R)
+        assert rel(Y32[:, j], ref.reshape(-1, order="F")) <= TOL32
+    op.close()
