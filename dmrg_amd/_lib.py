@@ -66,6 +66,10 @@ SIGNATURES: dict[str, tuple] = {
     "dmrg_matvec_multax_device_fp32": (st, [vp, vp, i64, vp, i64, C.c_int, vp]),
     "dmrg_matvec_set_shift": (st, [vp, C.c_double, C.c_double]),
     "dmrg_matvec_get_diagonal": (st, [vp, vp, vp]),
+    "dmrg_autotune_enable": (st, [C.c_int]),
+    "dmrg_autotune_reset": (st, []),
+    "dmrg_autotune_entries": (C.c_int, []),
+    "dmrg_gemm_plan_override": (st, [C.c_int, C.c_int]),
     "dmrg_matvec_time_legs": (st, [vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_float), vp]),
 }
 

@@ -21,7 +21,10 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # Packed FP32 VALU ops are disabled for all device code; the host pass warns that it
 # does not know the feature, which is harmless.
-DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                # Keep MFMA sources and accumulators in architectural VGPRs: avoids per-iteration
+                # copies of the accumulators between VGPRs and AGPRs in the GEMM main loop.
+                "-mllvm", "-amdgpu-mfma-vgpr-form"]
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{INCLUDE}", "-Wall",
             "-Wno-unused-function", "-Wno-unused-variable"] + DEVICE_FLAGS
 LDFLAGS = ["-shared", f"--offload-arch={ARCH}", "-lpthread"]

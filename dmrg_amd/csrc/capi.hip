@@ -311,6 +311,20 @@ dmrg_status dmrg_matvec_get_diagonal(dmrg_matvec *op, void *ddiag, void *stream)
     });
 }
 
+dmrg_status dmrg_autotune_enable(int enable) {
+    return guard([&] { set_autotune(enable); });
+}
+dmrg_status dmrg_autotune_reset(void) {
+    return guard([&] { clear_tuning(); });
+}
+int         dmrg_autotune_entries(void) { return tuned_entries(); }
+dmrg_status dmrg_gemm_plan_override(int cfg, int ksplit) {
+    return guard([&] {
+        DMRG_REQUIRE(cfg < 6 && ksplit >= 0 && ksplit <= 8, DMRG_ERR_ARG, "bad plan override");
+        set_plan_override(cfg, ksplit);
+    });
+}
+
 dmrg_status dmrg_matvec_time_legs(dmrg_matvec *op, const void *dx, void *dy, int nrep, int fp32, float ms_per_leg[3],
                                   void *stream) {
     return guard([&] {

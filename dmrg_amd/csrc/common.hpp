@@ -84,16 +84,27 @@ struct GemmDesc {
     Idx2        cm, cn;
     int64_t     c_batch    = 0;
     int32_t     accumulate = 0;
-    // Kernel-side loading hints, filled by the launcher.
-    int32_t a_fast_m = 1; // 1: consecutive threads walk M when loading A, 0: walk K
-    int32_t b_fast_k = 1; // 1: consecutive threads walk K when loading B, 0: walk N
+    // Split-K scratch supplied by the caller: room for part_capacity elements (0: no split-K).
+    void   *part          = nullptr;
+    int64_t part_capacity = 0;
+    // Filled by the launcher.
+    int32_t a_fast_m  = 1; // 1: consecutive threads walk M when loading A, 0: walk K
+    int32_t b_fast_k  = 1; // 1: consecutive threads walk K when loading B, 0: walk N
     int32_t xcd_remap = 1;
+    int32_t ksplit    = 1; // number of K slices (grid y)
+    int32_t kchunk    = 0; // K extent of one slice (multiple of 32)
 };
 
 // Tile configuration id (see gemm.hip for the table).
 enum class TileCfg : int { T64x64 = 0, T32x64 = 1, T64x32 = 2, T16x128 = 3, T128x64 = 4, T32x32 = 5, AUTO = -1 };
 
-void gemm(const GemmDesc &g, hipStream_t stream, TileCfg cfg = TileCfg::AUTO);
+// Launch one GEMM. cfg/ksplit = AUTO/0 pick from the tuning table or the size heuristic.
+void gemm(const GemmDesc &g, hipStream_t stream, TileCfg cfg = TileCfg::AUTO, int ksplit = 0);
 int  device_cu_count();
+// Autotuner controls: on/off, a forced plan for every GEMM (cfg < 0 clears it), cache reset and size.
+void set_autotune(int on);
+void set_plan_override(int cfg, int ksplit);
+void clear_tuning();
+int  tuned_entries();
 
 } // namespace dmrg

@@ -7,6 +7,7 @@
 // tensor index each one fuses.
 #include "contract.hpp"
 
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -32,7 +33,10 @@ Dims check_matvec_dims(const int64_t mps[3], const int64_t mpo[4], const int64_t
     return D;
 }
 
-int64_t matvec_ws_elems(const Dims &D) { return D.d * D.chiL * D.chiR * (D.DL + D.DR); }
+// Split-K slices the final leg may use (its partial slabs live in the workspace).
+constexpr int64_t MAX_KSPLIT = 8;
+
+int64_t matvec_ws_elems(const Dims &D) { return D.d * D.chiL * D.chiR * (D.DL + D.DR) + MAX_KSPLIT * D.n(); }
 
 static GemmDesc make(dmrg_dtype t, int64_t M, int64_t N, int64_t K, int nvec) {
     DMRG_REQUIRE(M < (int64_t(1) << 31) && N < (int64_t(1) << 31) && K < (int64_t(1) << 31), DMRG_ERR_DIMS,
@@ -59,6 +63,8 @@ void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *o
     const int64_t nT2 = (cL >= cR) ? cR * DR * d * cL : cL * DL * cR * d; // second intermediate per vector
     void         *T1  = w0;
     void         *T2  = w0 + static_cast<size_t>(nvec) * nT1 * es;
+    void         *PT  = w0 + static_cast<size_t>(nvec) * (nT1 + nT2) * es; // split-K partials of leg 3
+    const int64_t PC  = MAX_KSPLIT * D.n() * nvec;
 
     if(cL >= cR) {
         // Order of contraction_tblis.cpp:279-283 (theta.L first).
@@ -81,6 +87,7 @@ void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *o
         g3.A = T2, g3.am = plain(cR * DR), g3.ak = plain(1), g3.a_batch = nT2;
         g3.B = envR, g3.bk = fused(1, cR * cR, cR), g3.bn = plain(cR);
         g3.C = out, g3.cm = plain(1), g3.cn = plain(d * cL), g3.c_batch = ldy;
+        g3.part = PT, g3.part_capacity = PC;
         gemm(g3, s);
         mark(3);
     } else {
@@ -104,12 +111,15 @@ void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *o
         g3.A = envL, g3.am = plain(cL), g3.ak = fused(1, cL * cL, cL);
         g3.B = T2, g3.bk = plain(1), g3.bn = plain(cL * DL), g3.b_batch = nT2;
         g3.C = out, g3.cm = plain(d), g3.cn = fused(d * cL, 1, cR), g3.c_batch = ldy;
+        g3.part = PT, g3.part_capacity = PC;
         gemm(g3, s);
         mark(3);
     }
 }
 
-int64_t env_ws_elems(const Dims &D) { return D.d * D.chiL * D.chiR * (D.DL + D.DR); }
+int64_t env_ws_elems(const Dims &D) {
+    return D.d * D.chiL * D.chiR * (D.DL + D.DR) + MAX_KSPLIT * std::max(D.chiR * D.chiR * D.DR, D.chiL * D.chiL * D.DL);
+}
 
 void env_build_block(dmrg_dtype t, char side, const Dims &D, void *out, const void *other, const void *theta,
                      const void *mpo, void *ws, hipStream_t s) {
@@ -137,6 +147,7 @@ void env_build_block(dmrg_dtype t, char side, const Dims &D, void *out, const vo
         e3.A = theta, e3.am = plain(d * cL), e3.ak = plain(1);
         e3.B = Y, e3.bk = plain(1), e3.bn = plain(d * cL);
         e3.C = out, e3.cm = plain(1), e3.cn = plain(cR);
+        e3.part = w0 + static_cast<size_t>(d * cL * cR * (DL + DR)) * es, e3.part_capacity = MAX_KSPLIT * cR * cR * DR;
         gemm(e3, s);
     } else if(side == 'R') {
         // R'[a,a',w] = sum theta[s,a,b] R[b,b',w'] W[w,w',s,s'] conj theta[s',a',b']   (EnvBase.cpp:146-149)
@@ -159,6 +170,7 @@ void env_build_block(dmrg_dtype t, char side, const Dims &D, void *out, const vo
         e3.A = theta, e3.am = plain(d), e3.ak = fused(1, d * cL, d);
         e3.B = Y, e3.bk = plain(1), e3.bn = plain(d * cR);
         e3.C = out, e3.cm = plain(1), e3.cn = plain(cL);
+        e3.part = w0 + static_cast<size_t>(d * cL * cR * (DL + DR)) * es, e3.part_capacity = MAX_KSPLIT * cL * cL * DL;
         gemm(e3, s);
     } else {
         throw Error(DMRG_ERR_ARG, "env side must be 'L' or 'R'");

@@ -21,8 +21,12 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <cstdlib>
+#include <map>
 #include <mutex>
+#include <string>
 #include <type_traits>
+#include <vector>
 
 namespace dmrg {
 
@@ -67,20 +71,134 @@ constexpr int lds_pad(int row_elems, int elem_bytes) {
 }
 
 template <typename E>
-__device__ inline E load_elem(const E *base, int64_t off, bool valid, bool conj) {
+__device__ inline E zero_elem() {
+    if constexpr(std::is_same_v<E, cpx<double>> || std::is_same_v<E, cpx<float>>) return E{0, 0};
+    else return E(0);
+}
+
+// Branch-free masked load: an invalid element reads src[0] (present in any non-empty operand)
+// and is replaced by zero.
+template <typename E>
+__device__ inline E masked_load(const E *src, int64_t off, bool valid, bool conj) {
+    E v = src[valid ? off : 0];
     if constexpr(std::is_same_v<E, cpx<double>> || std::is_same_v<E, cpx<float>>) {
-        E v{0, 0};
-        if(valid) {
-            v = base[off];
-            if(conj) v.im = -v.im;
-        }
-        return v;
+        v.re = valid ? v.re : decltype(v.re)(0);
+        v.im = valid ? (conj ? -v.im : v.im) : decltype(v.im)(0);
     } else {
-        return valid ? base[off] : E(0);
+        v = valid ? v : E(0);
+    }
+    return v;
+}
+
+// Walks the K index of one loaded element tile by tile without divisions:
+// holds gk, gk % c and the element offset (gk % c) * s0 + (gk / c) * s1.
+struct KCursor {
+    int32_t  gk;
+    uint32_t r;
+    int64_t  off;
+};
+struct KStep {
+    uint32_t dr, c;
+    int64_t  delta, wrap;
+};
+__device__ inline KStep make_kstep(const Idx2 &x, int bk) {
+    KStep st;
+    st.c     = x.c;
+    st.dr    = static_cast<uint32_t>(bk) % x.c;
+    const int64_t dq = static_cast<int64_t>(static_cast<uint32_t>(bk) / x.c);
+    st.delta = st.dr * x.s0 + dq * x.s1;
+    st.wrap  = x.s1 - static_cast<int64_t>(x.c) * x.s0;
+    return st;
+}
+__device__ inline KCursor make_kcursor(const Idx2 &x, int gk) {
+    KCursor k;
+    k.gk  = gk;
+    k.r   = static_cast<uint32_t>(gk) % x.c;
+    k.off = idx_off(x, static_cast<uint32_t>(gk));
+    return k;
+}
+__device__ inline void advance(KCursor &k, const KStep &st, int bk) {
+    k.gk += bk;
+    k.r += st.dr;
+    k.off += st.delta;
+    if(k.r >= st.c) {
+        k.r -= st.c;
+        k.off += st.wrap;
     }
 }
 
-template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN, bool SWAP>
+// Loads one operand tile (RT rows of the non-K dimension x BK) per K step into registers and
+// then into LDS laid out [k][r]. Two address schemes:
+//  ALIGNED: every chunk of the operand's index maps is a multiple or a divisor of the tile
+//           extent, so offset(r0 + rr, k0 + kk) = offset(r0, k0) + offset(rr, kk): the per-element
+//           part is a static 32-bit value and the tile base is uniform (scalar registers).
+//  generic: per-element K cursors (KCursor) for arbitrary chunked index maps and ragged edges.
+template <typename E, int RT, int BK, int NT, bool ALIGNED>
+struct TileLoader {
+    static constexpr int L = (RT * BK + NT - 1) / NT;
+    int                  rr[L], kk[L];
+    bool                 rval[L];
+    uint32_t             eoff[L];  // ALIGNED: static in-tile offset (elements)
+    int64_t              roff[L];  // generic: offset of the non-K coordinate
+    KCursor              cur[L];   // generic: K cursors
+    KStep                st;
+    const E             *tile_base; // ALIGNED: operand + offset of (r0, 0)
+    Idx2                 kdim;
+    int                  kbase;
+
+    __device__ __forceinline__ void init(const E *src, const Idx2 &rdim, const Idx2 &kd, int r0, int rmax, int kbeg, bool fast_r, int tid) {
+        kdim  = kd;
+        kbase = kbeg;
+#pragma unroll
+        for(int e = 0; e < L; ++e) {
+            const int  idx  = tid + e * NT;
+            const bool used = idx < RT * BK;
+            rr[e]           = fast_r ? idx % RT : (idx / BK) % RT;
+            kk[e]           = fast_r ? (idx / RT) % BK : idx % BK;
+            rval[e]         = used && (r0 + rr[e]) < rmax;
+            if constexpr(ALIGNED) {
+                eoff[e] = static_cast<uint32_t>(idx_off(rdim, rr[e]) + idx_off(kd, kk[e]));
+            } else {
+                roff[e] = rval[e] ? idx_off(rdim, r0 + rr[e]) : 0;
+                cur[e]  = make_kcursor(kd, kbeg + kk[e]);
+            }
+        }
+        if constexpr(ALIGNED) tile_base = src + idx_off(rdim, r0);
+        else {
+            st        = make_kstep(kd, BK);
+            tile_base = src;
+        }
+    }
+
+    // Loads the tile at K offset k0 (ALIGNED) or at the cursors' position (generic), then advances.
+    __device__ __forceinline__ void load(E (&reg)[L], int k0, int kend, bool conj) {
+        if constexpr(ALIGNED) {
+            const E   *p     = tile_base + idx_off(kdim, static_cast<uint32_t>(k0));
+            const bool kfull = k0 + BK <= kend;
+#pragma unroll
+            for(int e = 0; e < L; ++e) {
+                const bool v = rval[e] && (kfull || k0 + kk[e] < kend);
+                reg[e]       = masked_load(p, static_cast<int64_t>(eoff[e]), v, conj);
+            }
+        } else {
+#pragma unroll
+            for(int e = 0; e < L; ++e) {
+                const bool v = rval[e] && cur[e].gk < kend;
+                reg[e]       = masked_load(tile_base, roff[e] + cur[e].off, v, conj);
+                advance(cur[e], st, BK);
+            }
+        }
+    }
+
+    template <int LD>
+    __device__ __forceinline__ void store(E (*lds)[LD], const E (&reg)[L], int tid) const {
+#pragma unroll
+        for(int e = 0; e < L; ++e)
+            if(L * NT == RT * BK || tid + e * NT < RT * BK) lds[kk[e]][rr[e]] = reg[e];
+    }
+};
+
+template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN, bool SWAP, bool ALIGNED>
 __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
     using E               = elem_t<R, CPLX>;
     using MF              = mfma_t<R>;
@@ -88,12 +206,10 @@ __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
     constexpr int NT      = 64 * WM * WN;
     constexpr int TM      = BM / (16 * WM);
     constexpr int TN      = BN / (16 * WN);
-    constexpr int LA      = BM * BK / NT;
-    constexpr int LB      = BK * BN / NT;
     constexpr int PA      = lds_pad(BM, sizeof(E));
     constexpr int PB      = lds_pad(BN, sizeof(E));
+    constexpr int NACC    = CPLX ? 3 : 1; // complex: sum ar*br, sum ai*bi, sum (ar*bi + ai*br)
     static_assert(TM >= 1 && TN >= 1, "wave tile too small");
-    static_assert(LA >= 1 && LB >= 1 && (BM * BK) % NT == 0 && (BK * BN) % NT == 0, "bad load split");
     static_assert(BK % 4 == 0, "BK must be a multiple of the MFMA k");
 
     __shared__ E As[2][BK][BM + PA];
@@ -106,7 +222,7 @@ __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
     const int wn   = wave / WM;
 
     // Tile index. Workgroups are dealt round-robin over the 8 XCDs; remap so that each
-    // XCD works a contiguous range of tiles (shared B columns stay in one L2).
+    // XCD works a contiguous range of tiles (shared operand panels stay in one L2).
     const int tilesM = (g.M + BM - 1) / BM;
     const int tilesN = (g.N + BN - 1) / BN;
     const int total  = tilesM * tilesN;
@@ -118,85 +234,41 @@ __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
     const int n0 = nt * BN;
     const int b  = blockIdx.z;
 
-    const E *A = static_cast<const E *>(g.A) + b * g.a_batch;
-    const E *B = static_cast<const E *>(g.B) + b * g.b_batch;
-    E       *C = static_cast<E *>(g.C) + b * g.c_batch;
+    // K range of this workgroup (split-K along blockIdx.y).
+    const int split = blockIdx.y;
+    const int kbeg  = split * g.kchunk;
+    const int kend  = min(g.K, kbeg + g.kchunk);
 
-    // Per-thread load coordinates and the fixed (non-K) part of their offsets.
-    int     a_mm[LA], a_kk[LA], b_kk[LB], b_nn[LB];
-    int64_t a_moff[LA], b_noff[LB];
-    bool    a_mval[LA], b_nval[LB];
-#pragma unroll
-    for(int e = 0; e < LA; ++e) {
-        const int idx = tid + e * NT;
-        if(g.a_fast_m) {
-            a_mm[e] = idx % BM;
-            a_kk[e] = idx / BM;
-        } else {
-            a_kk[e] = idx % BK;
-            a_mm[e] = idx / BK;
-        }
-        const int gm = m0 + a_mm[e];
-        a_mval[e]    = gm < g.M;
-        a_moff[e]    = a_mval[e] ? idx_off(g.am, gm) : 0;
-    }
-#pragma unroll
-    for(int e = 0; e < LB; ++e) {
-        const int idx = tid + e * NT;
-        if(g.b_fast_k) {
-            b_kk[e] = idx % BK;
-            b_nn[e] = idx / BK;
-        } else {
-            b_nn[e] = idx % BN;
-            b_kk[e] = idx / BN;
-        }
-        const int gn = n0 + b_nn[e];
-        b_nval[e]    = gn < g.N;
-        b_noff[e]    = b_nval[e] ? idx_off(g.bn, gn) : 0;
-    }
+    TileLoader<E, BM, BK, NT, ALIGNED> la;
+    TileLoader<E, BN, BK, NT, ALIGNED> lb;
+    la.init(static_cast<const E *>(g.A) + b * g.a_batch, g.am, g.ak, m0, g.M, kbeg, g.a_fast_m, tid);
+    lb.init(static_cast<const E *>(g.B) + b * g.b_batch, g.bn, g.bk, n0, g.N, kbeg, !g.b_fast_k, tid);
+    E ra[TileLoader<E, BM, BK, NT, ALIGNED>::L], rb[TileLoader<E, BN, BK, NT, ALIGNED>::L];
 
-    E ra[LA], rb[LB];
-    auto load_tiles = [&](int k0) {
+    acc_t acc[NACC][TM][TN];
 #pragma unroll
-        for(int e = 0; e < LA; ++e) {
-            const int  gk = k0 + a_kk[e];
-            const bool v  = a_mval[e] && gk < g.K;
-            ra[e]         = load_elem(A, a_moff[e] + (v ? idx_off(g.ak, gk) : 0), v, g.conjA);
-        }
+    for(int c = 0; c < NACC; ++c)
 #pragma unroll
-        for(int e = 0; e < LB; ++e) {
-            const int  gk = k0 + b_kk[e];
-            const bool v  = b_nval[e] && gk < g.K;
-            rb[e]         = load_elem(B, b_noff[e] + (v ? idx_off(g.bk, gk) : 0), v, g.conjB);
-        }
-    };
-    auto store_tiles = [&](int buf) {
+        for(int i = 0; i < TM; ++i)
 #pragma unroll
-        for(int e = 0; e < LA; ++e) As[buf][a_kk[e]][a_mm[e]] = ra[e];
-#pragma unroll
-        for(int e = 0; e < LB; ++e) Bs[buf][b_kk[e]][b_nn[e]] = rb[e];
-    };
+            for(int j = 0; j < TN; ++j) acc[c][i][j] = acc_t{0, 0, 0, 0};
 
-    acc_t acc_re[TM][TN];
-    acc_t acc_im[CPLX ? TM : 1][CPLX ? TN : 1];
-#pragma unroll
-    for(int i = 0; i < TM; ++i)
-#pragma unroll
-        for(int j = 0; j < TN; ++j) {
-            acc_re[i][j] = acc_t{0, 0, 0, 0};
-            if constexpr(CPLX) acc_im[i][j] = acc_t{0, 0, 0, 0};
-        }
-
-    const int numK = (g.K + BK - 1) / BK;
-    load_tiles(0);
-    store_tiles(0);
-    __syncthreads();
-
+    const int numK = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
     const int lrow = lane >> 4; // k row within the 4-deep MFMA step
     const int lcol = lane & 15; // m (or n) within a 16-wide block
+    if(numK > 0) {
+        la.load(ra, kbeg, kend, g.conjA);
+        lb.load(rb, kbeg, kend, g.conjB);
+        la.store(As[0], ra, tid);
+        lb.store(Bs[0], rb, tid);
+        __syncthreads();
+    }
     for(int kt = 0; kt < numK; ++kt) {
         const int buf = kt & 1;
-        if(kt + 1 < numK) load_tiles((kt + 1) * BK);
+        if(kt + 1 < numK) {
+            la.load(ra, kbeg + (kt + 1) * BK, kend, g.conjA);
+            lb.load(rb, kbeg + (kt + 1) * BK, kend, g.conjB);
+        }
 #pragma unroll
         for(int ks = 0; ks < BK / 4; ++ks) {
             const int kr = ks * 4 + lrow;
@@ -210,32 +282,37 @@ __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
 #pragma unroll
                 for(int j = 0; j < TN; ++j) {
                     if constexpr(CPLX) {
-                        // (ar + i ai)(br + i bi) = (ar br - ai bi) + i (ar bi + ai br)
-                        const R ar = fa[i].re, ai = fa[i].im, br = fb[j].re, bi = fb[j].im;
+                        // (ar + i ai)(br + i bi): rr += ar br, ii += ai bi, ri += ar bi + ai br.
                         if constexpr(!SWAP) {
-                            acc_re[i][j] = MF::op(ar, br, acc_re[i][j]);
-                            acc_re[i][j] = MF::op(-ai, bi, acc_re[i][j]);
-                            acc_im[i][j] = MF::op(ar, bi, acc_im[i][j]);
-                            acc_im[i][j] = MF::op(ai, br, acc_im[i][j]);
+                            acc[0][i][j] = MF::op(fa[i].re, fb[j].re, acc[0][i][j]);
+                            acc[1][i][j] = MF::op(fa[i].im, fb[j].im, acc[1][i][j]);
+                            acc[2][i][j] = MF::op(fa[i].re, fb[j].im, acc[2][i][j]);
+                            acc[2][i][j] = MF::op(fa[i].im, fb[j].re, acc[2][i][j]);
                         } else {
-                            acc_re[i][j] = MF::op(br, ar, acc_re[i][j]);
-                            acc_re[i][j] = MF::op(bi, -ai, acc_re[i][j]);
-                            acc_im[i][j] = MF::op(bi, ar, acc_im[i][j]);
-                            acc_im[i][j] = MF::op(br, ai, acc_im[i][j]);
+                            acc[0][i][j] = MF::op(fb[j].re, fa[i].re, acc[0][i][j]);
+                            acc[1][i][j] = MF::op(fb[j].im, fa[i].im, acc[1][i][j]);
+                            acc[2][i][j] = MF::op(fb[j].im, fa[i].re, acc[2][i][j]);
+                            acc[2][i][j] = MF::op(fb[j].re, fa[i].im, acc[2][i][j]);
                         }
                     } else {
-                        if constexpr(!SWAP) acc_re[i][j] = MF::op(fa[i], fb[j], acc_re[i][j]);
-                        else acc_re[i][j] = MF::op(fb[j], fa[i], acc_re[i][j]);
+                        if constexpr(!SWAP) acc[0][i][j] = MF::op(fa[i], fb[j], acc[0][i][j]);
+                        else acc[0][i][j] = MF::op(fb[j], fa[i], acc[0][i][j]);
                     }
                 }
         }
-        if(kt + 1 < numK) store_tiles(buf ^ 1);
+        if(kt + 1 < numK) {
+            la.store(As[buf ^ 1], ra, tid);
+            lb.store(Bs[buf ^ 1], rb, tid);
+        }
         __syncthreads();
     }
 
     // Epilogue. 16x16 accumulator layout (measured on gfx950): lane l, register v holds column l%16 of
     // row 4*(l/16)+v for v_mfma_f32_16x16x4_f32 and of row (l/16)+4*v for v_mfma_f64_16x16x4_f64.
     // Without swap rows run along M; with swap they run along N.
+    // Split-K partials go to g.part as plain column-major M x N slabs, one per (batch, split).
+    E *C = static_cast<E *>(g.C) + b * g.c_batch;
+    E *P = g.ksplit > 1 ? static_cast<E *>(g.part) + (static_cast<int64_t>(b) * g.ksplit + split) * g.M * g.N : nullptr;
 #pragma unroll
     for(int i = 0; i < TM; ++i)
 #pragma unroll
@@ -252,22 +329,64 @@ __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
                     gn = n0 + (wn * TN + j) * 16 + row;
                 }
                 if(gm < g.M && gn < g.N) {
-                    const int64_t off = idx_off(g.cm, gm) + idx_off(g.cn, gn);
-                    if constexpr(CPLX) {
-                        E val{acc_re[i][j][v], acc_im[i][j][v]};
+                    E val;
+                    if constexpr(CPLX) val = E{acc[0][i][j][v] - acc[1][i][j][v], acc[2][i][j][v]};
+                    else val = acc[0][i][j][v];
+                    if(P) {
+                        P[gm + static_cast<int64_t>(g.M) * gn] = val;
+                    } else {
+                        const int64_t off = idx_off(g.cm, gm) + idx_off(g.cn, gn);
                         if(g.accumulate) {
                             const E old = C[off];
-                            val.re += old.re;
-                            val.im += old.im;
+                            if constexpr(CPLX) {
+                                val.re += old.re;
+                                val.im += old.im;
+                            } else {
+                                val += old;
+                            }
                         }
-                        C[off] = val;
-                    } else {
-                        R val = acc_re[i][j][v];
-                        if(g.accumulate) val += C[off];
                         C[off] = val;
                     }
                 }
             }
+}
+
+// Sums the split-K partial slabs in a fixed order and writes C through its index map.
+template <typename E>
+__global__ void splitk_reduce(const GemmDesc g) {
+    const int64_t MN    = static_cast<int64_t>(g.M) * g.N;
+    const int64_t total = MN * g.batch;
+    const E      *P     = static_cast<const E *>(g.part);
+    for(int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < total;
+        t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t b   = t / MN;
+        const int64_t mn  = t - b * MN;
+        const int     m   = static_cast<int>(mn % g.M);
+        const int     n   = static_cast<int>(mn / g.M);
+        const E      *src = P + b * g.ksplit * MN + mn;
+        E             s   = src[0];
+        for(int k = 1; k < g.ksplit; ++k) {
+            const E v = src[k * MN];
+            if constexpr(std::is_same_v<E, cpx<double>> || std::is_same_v<E, cpx<float>>) {
+                s.re += v.re;
+                s.im += v.im;
+            } else {
+                s += v;
+            }
+        }
+        E            *C   = static_cast<E *>(g.C) + b * g.c_batch;
+        const int64_t off = idx_off(g.cm, m) + idx_off(g.cn, n);
+        if(g.accumulate) {
+            const E old = C[off];
+            if constexpr(std::is_same_v<E, cpx<double>> || std::is_same_v<E, cpx<float>>) {
+                s.re += old.re;
+                s.im += old.im;
+            } else {
+                s += old;
+            }
+        }
+        C[off] = s;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -298,64 +417,234 @@ int device_cu_count() {
     return cus;
 }
 
-static TileCfg choose_cfg(const GemmDesc &g) {
-    if(g.M <= 16) return TileCfg::T16x128;
+struct LaunchPlan {
+    TileCfg cfg;
+    int     ksplit;
+};
+
+static LaunchPlan heuristic_plan(const GemmDesc &g) {
     const int64_t cus  = device_cu_count();
     auto          nblk = [&](int bm, int bn) { return static_cast<int64_t>((g.M + bm - 1) / bm) * ((g.N + bn - 1) / bn) * g.batch; };
-    if(nblk(64, 64) >= 2 * cus) return TileCfg::T64x64;
-    if(nblk(32, 64) >= cus) return TileCfg::T32x64;
-    if(nblk(64, 32) >= cus) return TileCfg::T64x32;
-    return TileCfg::T32x32;
+    LaunchPlan    p{TileCfg::T64x64, 1};
+    if(g.M <= 16) p.cfg = TileCfg::T16x128;
+    else if(g.M >= 48 && g.N >= 48) p.cfg = TileCfg::T64x64;
+    else p.cfg = g.M < g.N ? TileCfg::T32x64 : TileCfg::T64x32;
+    const CfgShape sh    = cfg_shape(p.cfg);
+    const int64_t  tiles = nblk(sh.BM, sh.BN);
+    // Small grids with a long K: slice K so that about two workgroups per CU are in flight.
+    if(tiles < 2 * cus && g.K >= 512) {
+        int64_t ks = std::min<int64_t>((2 * cus + tiles - 1) / tiles, g.K / 256);
+        ks         = std::min<int64_t>(ks, 8);
+        if(g.part && ks > 1) ks = std::min<int64_t>(ks, g.part_capacity / (static_cast<int64_t>(g.M) * g.N * g.batch));
+        else ks = 1;
+        p.ksplit = static_cast<int>(std::max<int64_t>(1, ks));
+    }
+    if(p.ksplit == 1 && tiles < cus && g.M > 16 && p.cfg == TileCfg::T64x64) p.cfg = TileCfg::T32x32;
+    return p;
+}
+
+// Tile-separable index map: offset(t0 + t) = offset(t0) + offset(t) for every tile origin t0 (a multiple
+// of ext) and 0 <= t < ext, which holds when each chunk is a multiple or a divisor of ext.
+static bool separable(const Idx2 &x, int ext) {
+    return x.c >= 0x7fffffffu || (x.c % static_cast<uint32_t>(ext)) == 0 || (static_cast<uint32_t>(ext) % x.c) == 0;
+}
+// Largest in-tile offset must fit the 32-bit static offsets of the aligned loader.
+static bool small_tile_offsets(const Idx2 &r, int rt, const Idx2 &k, int bk) {
+    auto mx = [](const Idx2 &x, int ext) {
+        int64_t m = 0;
+        for(int t = 0; t < ext; ++t) m = std::max<int64_t>(m, std::llabs(idx_off(x, t)));
+        return m;
+    };
+    return mx(r, rt) + mx(k, bk) < (int64_t(1) << 31) && r.s0 >= 0 && r.s1 >= 0 && k.s0 >= 0 && k.s1 >= 0;
 }
 
 template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN>
 static void launch_cfg(const GemmDesc &g, bool swap, hipStream_t s) {
     const int  tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
-    const dim3 grid(tiles, 1, g.batch);
+    const dim3 grid(tiles, g.ksplit, g.batch);
     const dim3 block(64 * WM * WN);
-    if(swap) hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, true>), grid, block, 0, s, g);
-    else hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, false>), grid, block, 0, s, g);
+    const bool aligned = separable(g.am, BM) && separable(g.ak, BK) && separable(g.bn, BN) && separable(g.bk, BK) &&
+                         (g.kchunk % BK) == 0 && small_tile_offsets(g.am, BM, g.ak, BK) && small_tile_offsets(g.bn, BN, g.bk, BK);
+    if(aligned) {
+        if(swap) hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, true, true>), grid, block, 0, s, g);
+        else hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, false, true>), grid, block, 0, s, g);
+    } else {
+        if(swap) hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, true, false>), grid, block, 0, s, g);
+        else hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, false, false>), grid, block, 0, s, g);
+    }
+    if(g.ksplit > 1) {
+        const int64_t total = static_cast<int64_t>(g.M) * g.N * g.batch;
+        const dim3    rg(static_cast<unsigned>(std::min<int64_t>((total + 255) / 256, 8192)));
+        hipLaunchKernelGGL((splitk_reduce<elem_t<R, CPLX>>), rg, dim3(256), 0, s, g);
+    }
 }
 
 template <typename R, bool CPLX>
 static void launch_typed(const GemmDesc &g, TileCfg cfg, bool swap, hipStream_t s) {
     // K depth per LDS stage, sized so both double-buffered tiles fit in 64 KiB of LDS:
     // 16 elements for 8- and 16-byte elements, 32 for 4-byte ones; halved for the
-    // 16-byte element in the tall/wide configurations.
+    // 16-byte element in the tall/wide configurations. All configurations run 4 waves.
     constexpr int E  = static_cast<int>(sizeof(elem_t<R, CPLX>));
     constexpr int BK = E == 4 ? 32 : 16;
     constexpr int BH = E == 16 ? 8 : BK;
     switch(cfg) {
         case TileCfg::T64x64: launch_cfg<R, CPLX, 64, 64, BK, 2, 2>(g, swap, s); break;
-        case TileCfg::T32x64: launch_cfg<R, CPLX, 32, 64, BK, 1, 2>(g, swap, s); break;
-        case TileCfg::T64x32: launch_cfg<R, CPLX, 64, 32, BK, 2, 1>(g, swap, s); break;
-        case TileCfg::T16x128: launch_cfg<R, CPLX, 16, 128, BH, 1, 2>(g, swap, s); break;
+        case TileCfg::T32x64: launch_cfg<R, CPLX, 32, 64, BK, 2, 2>(g, swap, s); break;
+        case TileCfg::T64x32: launch_cfg<R, CPLX, 64, 32, BK, 2, 2>(g, swap, s); break;
+        case TileCfg::T16x128: launch_cfg<R, CPLX, 16, 128, BH, 1, 4>(g, swap, s); break;
         case TileCfg::T128x64: launch_cfg<R, CPLX, 128, 64, BH, 2, 2>(g, swap, s); break;
-        case TileCfg::T32x32: launch_cfg<R, CPLX, 32, 32, BK, 1, 1>(g, swap, s); break;
+        case TileCfg::T32x32: launch_cfg<R, CPLX, 32, 32, BK, 2, 2>(g, swap, s); break;
         default: throw Error(DMRG_ERR_ARG, "bad tile config");
     }
 }
 
 static int fast_score(const Idx2 &x) { return x.s0 == 1 ? static_cast<int>(std::min<uint32_t>(x.c, 1u << 20)) : 0; }
 
-void gemm(const GemmDesc &gin, hipStream_t stream, TileCfg cfg) {
+// ---------------------------------------------------------------------------------------
+// Autotuner: times candidate (tile configuration, K slices) plans of a GEMM with HIP events on
+// its own stream and caches the fastest per problem signature (dtype, sizes, index maps).
+
+static std::mutex                        g_tune_mu;
+static std::map<std::string, LaunchPlan> g_tuned;
+static int                               g_autotune     = 1;
+static int                               g_override_cfg = -1, g_override_ks = 0;
+
+static std::string tune_key(const GemmDesc &g) {
+    char buf[512];
+    auto I = [](const Idx2 &x) { return std::to_string(x.s0) + "," + std::to_string(x.s1) + "," + std::to_string(x.c); };
+    std::snprintf(buf, sizeof(buf), "%d|%d|%d|%d|%d|%d|%d|", static_cast<int>(g.dtype), g.M, g.N, g.K, g.batch,
+                  g.part ? 1 : 0, static_cast<int>(std::min<int64_t>(g.part_capacity / std::max<int64_t>(1, int64_t(g.M) * g.N * g.batch), 8)));
+    return std::string(buf) + I(g.am) + "|" + I(g.ak) + "|" + I(g.bk) + "|" + I(g.bn) + "|" + I(g.cm) + "|" + I(g.cn);
+}
+
+static void launch_plan(GemmDesc g, const LaunchPlan &p0, hipStream_t stream) {
+    LaunchPlan p = p0;
+    if(p.ksplit > 1) {
+        const int64_t need = static_cast<int64_t>(g.M) * g.N * g.batch * p.ksplit;
+        if(!g.part || g.part_capacity < need) p.ksplit = 1;
+    }
+    g.ksplit = 1;
+    g.kchunk = (g.K + 31) / 32 * 32; // K extent per slice, a multiple of every BK
+    if(p.ksplit > 1) {
+        const int kc = (((g.K + p.ksplit - 1) / p.ksplit) + 31) / 32 * 32;
+        g.kchunk     = kc;
+        g.ksplit     = (g.K + kc - 1) / kc;
+    }
+    if(g.kchunk <= 0) g.kchunk = 32;
+    const bool swap = (g.cm.s0 == 1 && g.cm.c >= 16) && !(g.cn.s0 == 1 && g.cn.c >= 16);
+    switch(g.dtype) {
+        case DMRG_FP64: launch_typed<double, false>(g, p.cfg, swap, stream); break;
+        case DMRG_CX64: launch_typed<double, true>(g, p.cfg, swap, stream); break;
+        case DMRG_FP32: launch_typed<float, false>(g, p.cfg, swap, stream); break;
+        case DMRG_CX32: launch_typed<float, true>(g, p.cfg, swap, stream); break;
+    }
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+static std::vector<LaunchPlan> candidates(const GemmDesc &g) {
+    std::vector<LaunchPlan> c;
+    const TileCfg cfgs[] = {TileCfg::T64x64, TileCfg::T32x64, TileCfg::T64x32, TileCfg::T32x32, TileCfg::T128x64, TileCfg::T16x128};
+    for(TileCfg t : cfgs) {
+        if(t == TileCfg::T16x128 && g.M > 32) continue;
+        if(t == TileCfg::T128x64 && g.M < 128) continue;
+        for(int ks : {1, 2, 4, 8}) {
+            if(ks > 1 && (g.K < 256 * ks || !g.part || g.part_capacity < static_cast<int64_t>(g.M) * g.N * g.batch * ks)) continue;
+            c.push_back({t, ks});
+        }
+    }
+    return c;
+}
+
+static LaunchPlan tune(const GemmDesc &g, hipStream_t stream) {
+    const auto cands = candidates(g);
+    hipEvent_t e0, e1;
+    DMRG_HIP_CHECK(hipEventCreate(&e0));
+    DMRG_HIP_CHECK(hipEventCreate(&e1));
+    LaunchPlan best = heuristic_plan(g);
+    float      best_ms = 1e30f;
+    try {
+        for(const auto &p : cands) {
+            launch_plan(g, p, stream); // warm-up (first-use code load, caches)
+            float tot = 0;
+            int   reps = 3;
+            for(int r = 0; r < reps; ++r) {
+                DMRG_HIP_CHECK(hipEventRecord(e0, stream));
+                launch_plan(g, p, stream);
+                DMRG_HIP_CHECK(hipEventRecord(e1, stream));
+                DMRG_HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0;
+                DMRG_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                tot += ms;
+            }
+            if(tot / reps < best_ms) {
+                best_ms = tot / reps;
+                best    = p;
+            }
+        }
+    } catch(...) {
+        (void) hipEventDestroy(e0);
+        (void) hipEventDestroy(e1);
+        throw;
+    }
+    (void) hipEventDestroy(e0);
+    (void) hipEventDestroy(e1);
+    return best;
+}
+
+void set_autotune(int on) { g_autotune = on; }
+void set_plan_override(int cfg, int ks) {
+    g_override_cfg = cfg;
+    g_override_ks  = ks;
+}
+void clear_tuning() {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tuned.clear();
+}
+int tuned_entries() {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    return static_cast<int>(g_tuned.size());
+}
+
+void gemm(const GemmDesc &gin, hipStream_t stream, TileCfg cfg, int ksplit) {
     if(gin.M <= 0 || gin.N <= 0 || gin.batch <= 0) return;
     GemmDesc g = gin;
-    // K == 0 is legal: every load is masked, the K loop is empty and C is written with zeros.
+    // K == 0 is legal: every load is skipped, the K loop is empty and C is written with zeros.
     if(g.K < 0) g.K = 0;
     if(g.K == 0 && g.accumulate) return;
     DMRG_REQUIRE(static_cast<int64_t>(g.M) * g.N < (int64_t(1) << 31), DMRG_ERR_DIMS, "gemm: output too large for 32-bit tile indexing");
     g.a_fast_m = fast_score(g.am) >= fast_score(g.ak) ? 1 : 0;
     g.b_fast_k = fast_score(g.bk) >= fast_score(g.bn) ? 1 : 0;
-    const bool swap = (g.cm.s0 == 1 && g.cm.c >= 16) && !(g.cn.s0 == 1 && g.cn.c >= 16);
-    if(cfg == TileCfg::AUTO) cfg = choose_cfg(g);
-    switch(g.dtype) {
-        case DMRG_FP64: launch_typed<double, false>(g, cfg, swap, stream); break;
-        case DMRG_CX64: launch_typed<double, true>(g, cfg, swap, stream); break;
-        case DMRG_FP32: launch_typed<float, false>(g, cfg, swap, stream); break;
-        case DMRG_CX32: launch_typed<float, true>(g, cfg, swap, stream); break;
+    LaunchPlan p;
+    if(g_override_cfg >= 0) {
+        p = {static_cast<TileCfg>(g_override_cfg), std::max(1, g_override_ks)};
+    } else if(cfg != TileCfg::AUTO) {
+        p = {cfg, std::max(1, ksplit)};
+    } else {
+        // Tuning needs synchronisation, so it is skipped for accumulating GEMMs (repeated launches would
+        // add up) and while the stream is being captured into a graph.
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        DMRG_HIP_CHECK(hipStreamIsCapturing(stream, &cs));
+        const bool can_tune = g_autotune && !g.accumulate && cs == hipStreamCaptureStatusNone;
+        const std::string key = tune_key(g);
+        bool              found = false;
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mu);
+            auto                        it = g_tuned.find(key);
+            if(it != g_tuned.end()) {
+                p     = it->second;
+                found = true;
+            }
+        }
+        if(!found) {
+            p = can_tune ? tune(g, stream) : heuristic_plan(g);
+            if(can_tune) {
+                std::lock_guard<std::mutex> lock(g_tune_mu);
+                g_tuned[key] = p;
+            }
+        }
+        if(ksplit > 0) p.ksplit = ksplit;
     }
-    DMRG_HIP_CHECK(hipGetLastError());
+    launch_plan(g, p, stream);
 }
 
 } // namespace dmrg

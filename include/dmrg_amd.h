@@ -135,6 +135,16 @@ dmrg_status dmrg_matvec_set_shift(dmrg_matvec *op, double re, double im);
  * (matvec_mpos.cpp:399-407); feeds the Jacobi preconditioner (matvec_mpos.cpp:1333-1416, jcb = 1). */
 dmrg_status dmrg_matvec_get_diagonal(dmrg_matvec *op, void *ddiag, void *stream);
 
+/* GEMM autotuner (no counterpart in the reference). Every contraction leg is a GEMM whose launch plan (tile
+ * configuration, K slices) is chosen by timing the candidates with HIP events the first time a problem
+ * signature is seen on a stream that is not being captured, and cached for the process. */
+dmrg_status dmrg_autotune_enable(int enable);
+dmrg_status dmrg_autotune_reset(void);
+int         dmrg_autotune_entries(void);
+/* Testing hook: force every GEMM to tile configuration cfg (0..5) with ksplit K slices; cfg < 0 restores the
+ * tuned/heuristic choice. */
+dmrg_status dmrg_gemm_plan_override(int cfg, int ksplit);
+
 /* Profiling: runs nrep device matvecs (one vector, x -> y) on stream and returns the average duration of each
  * of the three contraction legs in ms_per_leg[0..2], measured with HIP events on that stream. No counterpart in
  * the reference (its per-operator timers t_multAx, matvec_mpos.h:174-179, time the whole call). */

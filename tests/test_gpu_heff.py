@@ -203,3 +203,40 @@ def test_matvec_op_device_block_and_fp32():
         ref = O.matrix_vector_product(X[:, j].reshape(th.shape, order="F"), W, L, R)
         assert rel(Y32[:, j], ref.reshape(-1, order="F")) <= TOL32
     op.close()
+
+
+# ------------------------------------------------------------------------------ every kernel variant
+
+@pytest.fixture
+def plan_override():
+    yield lambda cfg, ks: _lib.call("dmrg_gemm_plan_override", cfg, ks)
+    _lib.call("dmrg_gemm_plan_override", -1, 0)
+
+
+@pytest.mark.parametrize("cfg", range(6), ids=["64x64", "32x64", "64x32", "16x128", "128x64", "32x32"])
+@pytest.mark.parametrize("ks", [1, 3, 8])
+def test_every_tile_config_matvec_and_env(cfg, ks, plan_override):
+    """Each tile configuration, with and without K slicing, on aligned (chi=64) and ragged shapes."""
+    plan_override(cfg, ks)
+    rng = np.random.default_rng(cfg * 10 + ks)
+    for shape in [(2, 64, 64, 5, 5), (2, 37, 29, 3, 4), (4, 48, 80, 8, 8)]:
+        for cplx in (False, True):
+            th, W, L, R = general(rng, *shape, cplx)
+            assert rel(G.matrix_vector_product(th, W, L, R), O.matrix_vector_product(th, W, L, R)) <= TOL64
+            assert rel(G.env_build_block("L", L, th, W), O.env_build_block("L", L, th, W)) <= TOL64
+            assert rel(G.env_build_block("R", R, th, W), O.env_build_block("R", R, th, W)) <= TOL64
+        th, W, L, R = general(rng, *shape, True)
+        out = G.matrix_vector_product(*(x.astype(np.complex64) for x in (th, W, L, R)))
+        assert rel(out, O.matrix_vector_product(th, W, L, R)) <= TOL32
+
+
+def test_autotuner_caches_plans():
+    _lib.call("dmrg_autotune_reset")
+    th, W, L, R = random_heff_problem(2, 96, 96, 5, 5, seed=9)
+    a = G.matrix_vector_product(th, W, L, R)
+    n = _lib.load().dmrg_autotune_entries()
+    assert n >= 3
+    b = G.matrix_vector_product(th, W, L, R)
+    assert _lib.load().dmrg_autotune_entries() == n
+    np.testing.assert_array_equal(a, b)  # same plan -> bitwise identical
+    assert rel(a, O.matrix_vector_product(th, W, L, R)) <= TOL64
