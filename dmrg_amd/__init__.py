@@ -209,6 +209,30 @@ class MatVecMPOS:
         out.close()
         return res
 
+    def eigs(self, nev: int = 1, target: str = "SA", tol: float = 1e-12, x0=None, mixed: bool = False,
+             precond: bool = False, ncv: int = 0, min_restart: int = 0, shift: float = 0.0,
+             max_matvecs: int = 100000, seed: int = 12345, stream=None):
+        """Eigenpairs on the GPU (eig::solver::eigs with PRIMME, solver.cpp:491-509). Returns
+        (eigenvalues, eigenvectors (n, nev) numpy, result dict)."""
+        cfg = _lib.EigsConfig()
+        load().dmrg_eigs_default_config(C.byref(cfg))
+        cfg.nev, cfg.ncv, cfg.min_restart = nev, ncv, min_restart
+        cfg.target = {"SA": 0, "LA": 1, "CLOSEST": 2}[target]
+        cfg.shift, cfg.tol, cfg.max_matvecs = shift, tol, max_matvecs
+        cfg.precond, cfg.mixed, cfg.seed = int(precond), int(mixed), seed
+        n = self.rows()
+        dx0 = None
+        if x0 is not None:
+            dx0 = DeviceArray.from_numpy(np.asarray(x0, dtype=self.dtype).reshape(n, order="F"))
+        dev = DeviceArray((n, nev), self.dtype)
+        res = _lib.EigsResult()
+        call("dmrg_eigs", self.h, C.byref(cfg), dx0.ptr if dx0 is not None else None, dev.ptr, C.byref(res), stream)
+        vecs = dev.to_numpy()
+        dev.close()
+        info = {k: getattr(res, k) for k, _ in res._fields_ if k not in ("eigval", "rnorm")}
+        info["rnorm"] = list(res.rnorm[:nev])
+        return np.array(res.eigval[:nev]), vecs, info
+
     def time_legs(self, dx: DeviceArray, dy: DeviceArray, nrep: int, fp32: bool = False, stream=None):
         """Average duration (ms) of each of the three contraction legs over nrep device matvecs."""
         ms = (C.c_float * 3)()

@@ -66,12 +66,28 @@ SIGNATURES: dict[str, tuple] = {
     "dmrg_matvec_multax_device_fp32": (st, [vp, vp, i64, vp, i64, C.c_int, vp]),
     "dmrg_matvec_set_shift": (st, [vp, C.c_double, C.c_double]),
     "dmrg_matvec_get_diagonal": (st, [vp, vp, vp]),
+    "dmrg_host_heev": (st, [C.c_int, dp, dp, dp]),
     "dmrg_autotune_enable": (st, [C.c_int]),
     "dmrg_autotune_reset": (st, []),
     "dmrg_autotune_entries": (C.c_int, []),
     "dmrg_gemm_plan_override": (st, [C.c_int, C.c_int]),
     "dmrg_matvec_time_legs": (st, [vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_float), vp]),
 }
+
+class EigsConfig(C.Structure):
+    _fields_ = [("nev", C.c_int), ("ncv", C.c_int), ("min_restart", C.c_int), ("target", C.c_int),
+                ("shift", C.c_double), ("tol", C.c_double), ("max_matvecs", C.c_int64), ("precond", C.c_int),
+                ("mixed", C.c_int), ("max_time_s", C.c_double), ("seed", C.c_uint64)]
+
+
+class EigsResult(C.Structure):
+    _fields_ = [("eigval", C.c_double * 16), ("rnorm", C.c_double * 16), ("nconv", C.c_int), ("iters", C.c_int),
+                ("restarts", C.c_int), ("stagnated", C.c_int), ("matvecs_fp64", C.c_int64),
+                ("matvecs_fp32", C.c_int64), ("seconds", C.c_double), ("anorm", C.c_double)]
+
+
+SIGNATURES["dmrg_eigs_default_config"] = (None, [C.POINTER(EigsConfig)])
+SIGNATURES["dmrg_eigs"] = (st, [vp, C.POINTER(EigsConfig), vp, vp, C.POINTER(EigsResult), vp])
 
 _lib = None
 

@@ -3,9 +3,12 @@
 // and records the message for dmrg_last_error().
 #include "contract.hpp"
 #include "matvec_op.hpp"
+#include "dense_eig.hpp"
+#include "eigs.hpp"
 
 #include <complex>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <vector>
 
@@ -308,6 +311,75 @@ dmrg_status dmrg_matvec_get_diagonal(dmrg_matvec *op, void *ddiag, void *stream)
     return guard([&] {
         DMRG_REQUIRE(op && ddiag, DMRG_ERR_ARG, "null argument");
         reinterpret_cast<MatVecOp *>(op)->diagonal(ddiag, static_cast<hipStream_t>(stream));
+    });
+}
+
+void dmrg_eigs_default_config(dmrg_eigs_config *cfg) {
+    if(!cfg) return;
+    cfg->nev         = 1;
+    cfg->ncv         = 0;
+    cfg->min_restart = 0;
+    cfg->target      = DMRG_TARGET_SA;
+    cfg->shift       = 0.0;
+    cfg->tol         = 1e-12;
+    cfg->max_matvecs = 100000;
+    cfg->precond     = 0;
+    cfg->mixed       = 0;
+    cfg->max_time_s  = 0.0;
+    cfg->seed        = 12345;
+}
+
+dmrg_status dmrg_eigs(dmrg_matvec *opp, const dmrg_eigs_config *cfg, const void *x0, void *evecs, dmrg_eigs_result *res,
+                      void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(opp && cfg && evecs && res, DMRG_ERR_ARG, "null argument");
+        DMRG_REQUIRE(cfg->nev >= 1 && cfg->nev <= DMRG_EIGS_MAX_NEV, DMRG_ERR_ARG, "nev out of range");
+        DMRG_REQUIRE(cfg->target >= 0 && cfg->target <= 2, DMRG_ERR_ARG, "bad target");
+        DMRG_REQUIRE(cfg->tol > 0, DMRG_ERR_ARG, "tol must be positive");
+        auto      &op = *reinterpret_cast<MatVecOp *>(opp);
+        EigsParams p;
+        p.nev         = cfg->nev;
+        p.ncv         = cfg->ncv;
+        p.min_restart = cfg->min_restart;
+        p.target      = cfg->target;
+        p.shift       = cfg->shift;
+        p.tol         = std::max(cfg->tol, std::numeric_limits<double>::epsilon());
+        p.max_matvecs = cfg->max_matvecs > 0 ? cfg->max_matvecs : 100000;
+        p.precond     = cfg->precond;
+        p.max_time_s  = cfg->max_time_s;
+        p.seed        = cfg->seed;
+        auto      s   = static_cast<hipStream_t>(stream);
+        EigsStats st, s1;
+        *res          = dmrg_eigs_result{};
+        const bool cx = op.dtype == DMRG_CX64;
+        if(cfg->mixed) {
+            if(cx) st = eigs_mixed<cpx<double>, cpx<float>>(op, s, p, static_cast<const cpx<double> *>(x0), static_cast<cpx<double> *>(evecs), res->eigval, res->rnorm, &s1);
+            else st = eigs_mixed<double, float>(op, s, p, static_cast<const double *>(x0), static_cast<double *>(evecs), res->eigval, res->rnorm, &s1);
+            res->matvecs_fp32 = s1.matvecs;
+            res->iters        = s1.iters;
+            res->restarts     = s1.restarts;
+        } else if(cx) {
+            Davidson<cpx<double>> d(op, s);
+            st = d.solve(p, static_cast<const cpx<double> *>(x0), x0 ? 1 : 0, static_cast<cpx<double> *>(evecs), res->eigval, res->rnorm);
+        } else {
+            Davidson<double> d(op, s);
+            st = d.solve(p, static_cast<const double *>(x0), x0 ? 1 : 0, static_cast<double *>(evecs), res->eigval, res->rnorm);
+        }
+        res->nconv        = st.nconv;
+        res->iters += st.iters;
+        res->restarts += st.restarts;
+        res->stagnated    = st.stagnated ? 1 : 0;
+        res->matvecs_fp64 = st.matvecs;
+        res->seconds      = st.seconds;
+        res->anorm        = st.anorm;
+    });
+}
+
+dmrg_status dmrg_host_heev(int m, const double *A, double *w, double *Z) {
+    return guard([&] {
+        DMRG_REQUIRE(m >= 0 && (m == 0 || (A && w && Z)), DMRG_ERR_ARG, "bad argument");
+        if(!hermitian_eig(m, reinterpret_cast<const std::complex<double> *>(A), m, w, reinterpret_cast<std::complex<double> *>(Z)))
+            throw Error(DMRG_ERR_NOT_CONVERGED, "tridiagonal QL did not converge");
     });
 }
 

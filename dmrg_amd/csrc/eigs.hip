@@ -1,0 +1,352 @@
+// Generalized Davidson with thick restart on the GPU (see eigs.hpp).
+#include "eigs.hpp"
+
+#include "dense_eig.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
+#include <numeric>
+
+namespace dmrg {
+
+using cx = std::complex<double>;
+
+// Layout of the small device/host exchange buffer (doubles).
+constexpr int SB_COL  = 0;                 // projected column, 2 * VEC_MAX_M
+constexpr int SB_NRM  = 2 * VEC_MAX_M;     // |t|^2 before / after orthogonalisation (2 x 2)
+constexpr int SB_RITZ = 2 * VEC_MAX_M + 4; // |r|^2, |x|^2
+constexpr int SB_Y    = 2 * VEC_MAX_M + 8; // Ritz coefficients, 2 * VEC_MAX_M
+constexpr int SB_SIZE = 4 * VEC_MAX_M + 8;
+
+template <typename T>
+Davidson<T>::Davidson(MatVecOp &o, hipStream_t st) : op(o), s(st), n(o.D.n()) {
+    scratch.reserve(static_cast<size_t>(vec_scratch_doubles()) * sizeof(double));
+    small.reserve(SB_SIZE * sizeof(double));
+    DMRG_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host), SB_SIZE * sizeof(double), hipHostMallocDefault));
+}
+
+template <typename T>
+Davidson<T>::~Davidson() {
+    if(host) (void) hipHostFree(host);
+}
+
+template <typename T>
+void Davidson<T>::matvec(const T *x, T *y, int nvec) {
+    op.multax_device(x, n, y, n, nvec, s, std::is_same_v<typename vtraits<T>::R, float>);
+}
+
+template <typename T>
+void Davidson<T>::orthonormalize_into(T *t, int m, double *dnrm_pre, double *dnrm_post) {
+    double *sm = small.as<double>();
+    multidot<T>(n, 1, t, n, t, dnrm_pre, scratch.as<double>(), s);
+    for(int pass = 0; pass < 2; ++pass) { // classical Gram-Schmidt, twice
+        if(m == 0) break;
+        multidot<T>(n, m, V.as<T>(), n, t, sm + SB_COL, scratch.as<double>(), s);
+        gemv_update<T>(n, m, V.as<T>(), n, sm + SB_COL, -1.0, 1.0, t, s);
+    }
+    multidot<T>(n, 1, t, n, t, dnrm_post, scratch.as<double>(), s);
+    scale_by_inv_norm<T>(n, t, dnrm_post, s);
+}
+
+template <typename T>
+void Davidson<T>::transform_basis(T *B, int m, const std::vector<cx> &Q, int k) {
+    // B[:, 0:k] <- B[:, 0:m] Q  (Q: m x k, column-major)
+    std::vector<T> q(static_cast<size_t>(m) * k);
+    for(size_t i = 0; i < q.size(); ++i) {
+        if constexpr(vtraits<T>::cx) q[i] = T{static_cast<typename vtraits<T>::R>(Q[i].real()), static_cast<typename vtraits<T>::R>(Q[i].imag())};
+        else q[i] = static_cast<T>(Q[i].real());
+    }
+    Ydev.reserve(q.size() * sizeof(T));
+    DMRG_HIP_CHECK(hipMemcpyAsync(Ydev.p, q.data(), q.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    GemmDesc g;
+    g.dtype = vtraits<T>::dt;
+    g.M = static_cast<int32_t>(n), g.N = k, g.K = m, g.batch = 1;
+    g.A = B, g.am = plain(1), g.ak = plain(n);
+    g.B = Ydev.p, g.bk = plain(1), g.bn = plain(m);
+    g.C = tmp.p, g.cm = plain(1), g.cn = plain(n);
+    gemm(g, s);
+    DMRG_HIP_CHECK(hipMemcpyAsync(B, tmp.p, static_cast<size_t>(n) * k * sizeof(T), hipMemcpyDeviceToDevice, s));
+    // The host copy of q must outlive the async upload.
+    DMRG_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+static void sort_by_target(const EigsParams &p, const std::vector<double> &w, std::vector<int> &ord) {
+    ord.resize(w.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    if(p.target == TARGET_LA) std::reverse(ord.begin(), ord.end());
+    else if(p.target == TARGET_CLOSEST)
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return std::fabs(w[a] - p.shift) < std::fabs(w[b] - p.shift); });
+}
+
+template <typename T>
+EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out, double *evals, double *rnorms, int nkeep) {
+    using clock = std::chrono::steady_clock;
+    const auto t0  = clock::now();
+    EigsStats  st;
+    const int  nev = std::max(1, p.nev);
+    DMRG_REQUIRE(nev <= n, DMRG_ERR_ARG, "nev larger than the problem size");
+    int mmax = p.ncv > 0 ? p.ncv : nev * static_cast<int>(std::ceil(std::log2(static_cast<double>(std::max<int64_t>(n, 2)))));
+    mmax     = static_cast<int>(std::clamp<int64_t>(mmax, 2 * nev, n));
+    mmax     = std::min(mmax, VEC_MAX_M);
+    int mmin = p.min_restart > 0 ? p.min_restart : std::max(1, mmax / 2 - 1);
+    mmin     = std::max(mmin, nev);
+    // Keep room for the retained previous Ritz vector and at least one expansion.
+    mmin = std::min(mmin, std::max(1, mmax - 1 - p.plus_k));
+    nkeep = std::max(nkeep, nev);
+    nkeep = std::min<int64_t>(nkeep, n);
+    k0    = std::min<int>(k0, mmax);
+    const double eps_t  = std::is_same_v<typename vtraits<T>::R, float> ? 1.2e-7 : 2.3e-16;
+    const size_t vbytes = static_cast<size_t>(n) * sizeof(T);
+    V.reserve(vbytes * mmax);
+    W.reserve(vbytes * mmax);
+    tmp.reserve(vbytes * mmax);
+    rbuf.reserve(vbytes);
+    T      *Vp = V.as<T>(), *Wp = W.as<T>(), *r = rbuf.as<T>();
+    double *sm = small.as<double>();
+    if(p.precond == 1) {
+        // Diagonal of H_eff in the operator's FP64 dtype; the kernel converts on the fly.
+        diag.reserve(static_cast<size_t>(n) * dtype_size(op.dtype));
+        op.diagonal(diag.p, s);
+    }
+
+    // ---- initial basis
+    int m = 0;
+    if(init && k0 > 0) {
+        DMRG_HIP_CHECK(hipMemcpyAsync(Vp, init, vbytes * k0, hipMemcpyDeviceToDevice, s));
+        for(int j = 0; j < k0; ++j) {
+            if(j != m) DMRG_HIP_CHECK(hipMemcpyAsync(Vp + m * n, Vp + j * n, vbytes, hipMemcpyDeviceToDevice, s));
+            orthonormalize_into(Vp + m * n, m, sm + SB_NRM, sm + SB_NRM + 2);
+            DMRG_HIP_CHECK(hipMemcpyAsync(host + SB_NRM, sm + SB_NRM, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+            DMRG_HIP_CHECK(hipStreamSynchronize(s));
+            if(host[SB_NRM + 2] > 1e4 * eps_t * eps_t * host[SB_NRM] && host[SB_NRM] > 0) ++m;
+        }
+    }
+    if(m == 0) {
+        random_fill<T>(n, Vp, p.seed, s);
+        orthonormalize_into(Vp, 0, sm + SB_NRM, sm + SB_NRM + 2);
+        m = 1;
+    }
+    matvec(Vp, Wp, m);
+    st.matvecs += m;
+    std::vector<cx> H(static_cast<size_t>(mmax) * mmax, cx(0.0, 0.0));
+    auto            Hat = [&](int i, int j) -> cx & { return H[i + static_cast<size_t>(mmax) * j]; };
+    for(int j = 0; j < m; ++j) {
+        multidot<T>(n, m, Vp, n, Wp + j * n, sm + SB_COL, scratch.as<double>(), s);
+        DMRG_HIP_CHECK(hipMemcpyAsync(host + SB_COL, sm + SB_COL, 2 * m * sizeof(double), hipMemcpyDeviceToHost, s));
+        DMRG_HIP_CHECK(hipStreamSynchronize(s));
+        for(int i = 0; i < m; ++i) Hat(i, j) = cx(host[SB_COL + 2 * i], host[SB_COL + 2 * i + 1]);
+    }
+
+    std::vector<double> w(mmax);
+    std::vector<cx>     Hm, Z;
+    std::vector<int>    ord;
+    std::vector<cx>     yprev;
+    double              best_r = std::numeric_limits<double>::infinity();
+    int                 since_best = 0;
+    bool                inject_random = false;
+
+    auto ritz = [&](int col_in_H, int m_, T *xout) -> double {
+        // Upload y (column col of Z), form x and r, return |r| / |x|.
+        for(int i = 0; i < m_; ++i) {
+            host[SB_Y + 2 * i]     = Z[i + static_cast<size_t>(m_) * col_in_H].real();
+            host[SB_Y + 2 * i + 1] = Z[i + static_cast<size_t>(m_) * col_in_H].imag();
+        }
+        DMRG_HIP_CHECK(hipMemcpyAsync(sm + SB_Y, host + SB_Y, 2 * m_ * sizeof(double), hipMemcpyHostToDevice, s));
+        ritz_residual<T>(n, m_, Vp, Wp, n, sm + SB_Y, w[col_in_H], xout, r, sm + SB_RITZ, scratch.as<double>(), s);
+        DMRG_HIP_CHECK(hipMemcpyAsync(host + SB_RITZ, sm + SB_RITZ, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        DMRG_HIP_CHECK(hipStreamSynchronize(s));
+        return std::sqrt(std::max(0.0, host[SB_RITZ])) / std::sqrt(std::max(1e-300, host[SB_RITZ + 1]));
+    };
+    auto rayleigh_ritz = [&]() {
+        Hm.assign(static_cast<size_t>(m) * m, cx(0.0, 0.0));
+        for(int j = 0; j < m; ++j)
+            for(int i = 0; i < m; ++i) Hm[i + static_cast<size_t>(m) * j] = Hat(i, j);
+        Z.assign(static_cast<size_t>(m) * m, cx(0.0, 0.0));
+        w.assign(m, 0.0);
+        if(!hermitian_eig(m, Hm.data(), m, w.data(), Z.data())) throw Error(DMRG_ERR_NOT_CONVERGED, "Rayleigh-Ritz eigensolver failed");
+        sort_by_target(p, w, ord);
+        for(int i = 0; i < m; ++i) st.anorm = std::max(st.anorm, std::fabs(w[i]));
+    };
+
+    std::vector<double> rn(nev, 0.0);
+    while(true) {
+        rayleigh_ritz();
+        // Residuals of the wanted pairs in order; stop at the first unconverged one (its r drives expansion).
+        int pstar = -1;
+        for(int q = 0; q < nev && q < m; ++q) {
+            rn[q] = ritz(ord[q], m, out + q * n);
+            if(rn[q] > p.tol * std::max(st.anorm, 1e-300)) {
+                pstar = q;
+                break;
+            }
+        }
+        const int nconv = pstar < 0 ? std::min(nev, m) : pstar;
+        st.nconv        = nconv;
+        if(pstar < 0 && m >= nev) break;
+        if(pstar < 0) pstar = m - 1; // fewer basis vectors than nev: keep expanding
+        const double rcur = rn[pstar];
+        if(rcur < 0.9 * best_r) {
+            best_r     = rcur;
+            since_best = 0;
+        } else if(++since_best >= p.stagnation_window && p.stagnation_window > 0) {
+            st.stagnated = true;
+            break;
+        }
+        if(st.matvecs >= p.max_matvecs) break;
+        if(p.max_time_s > 0 && std::chrono::duration<double>(clock::now() - t0).count() > p.max_time_s) break;
+
+        // ---- thick restart (GD+k)
+        if(m + 1 > mmax) {
+            std::vector<std::vector<cx>> cols;
+            for(int q = 0; q < mmin && q < m; ++q) {
+                std::vector<cx> c(m);
+                for(int i = 0; i < m; ++i) c[i] = Z[i + static_cast<size_t>(m) * ord[q]];
+                cols.push_back(std::move(c));
+            }
+            if(p.plus_k > 0 && static_cast<int>(yprev.size()) == m - 1) {
+                std::vector<cx> c(m, cx(0.0, 0.0));
+                for(int i = 0; i < m - 1; ++i) c[i] = yprev[i];
+                cols.push_back(std::move(c));
+            }
+            // Modified Gram-Schmidt in C^m (twice).
+            std::vector<std::vector<cx>> Qc;
+            for(auto &c : cols) {
+                for(int pass = 0; pass < 2; ++pass)
+                    for(auto &q : Qc) {
+                        cx d(0.0, 0.0);
+                        for(int i = 0; i < m; ++i) d += std::conj(q[i]) * c[i];
+                        for(int i = 0; i < m; ++i) c[i] -= d * q[i];
+                    }
+                double nr = 0;
+                for(auto &v : c) nr += std::norm(v);
+                nr = std::sqrt(nr);
+                if(nr > 1e-10) {
+                    for(auto &v : c) v /= nr;
+                    Qc.push_back(c);
+                }
+            }
+            const int       k = static_cast<int>(Qc.size());
+            std::vector<cx> Q(static_cast<size_t>(m) * k);
+            for(int j = 0; j < k; ++j)
+                for(int i = 0; i < m; ++i) Q[i + static_cast<size_t>(m) * j] = Qc[j][i];
+            transform_basis(Vp, m, Q, k);
+            transform_basis(Wp, m, Q, k);
+            std::vector<cx> HQ(static_cast<size_t>(m) * k, cx(0.0, 0.0)), Hn(static_cast<size_t>(k) * k, cx(0.0, 0.0));
+            for(int j = 0; j < k; ++j)
+                for(int l = 0; l < m; ++l)
+                    for(int i = 0; i < m; ++i) HQ[i + static_cast<size_t>(m) * j] += Hat(i, l) * Q[l + static_cast<size_t>(m) * j];
+            for(int j = 0; j < k; ++j)
+                for(int i = 0; i < k; ++i) {
+                    cx acc(0.0, 0.0);
+                    for(int l = 0; l < m; ++l) acc += std::conj(Q[l + static_cast<size_t>(m) * i]) * HQ[l + static_cast<size_t>(m) * j];
+                    Hn[i + static_cast<size_t>(k) * j] = acc;
+                }
+            std::fill(H.begin(), H.end(), cx(0.0, 0.0));
+            for(int j = 0; j < k; ++j)
+                for(int i = 0; i < k; ++i) Hat(i, j) = Hn[i + static_cast<size_t>(k) * j];
+            m = k;
+            ++st.restarts;
+            yprev.clear();
+        } else {
+            yprev.assign(m, cx(0.0, 0.0));
+            for(int i = 0; i < m; ++i) yprev[i] = Z[i + static_cast<size_t>(m) * ord[0]];
+        }
+
+        // ---- expansion vector t = M^-1 r (or r), orthonormalised against V
+        T *t = Vp + static_cast<int64_t>(m) * n;
+        if(inject_random) {
+            random_fill<T>(n, t, p.seed + 7919ull * (st.iters + 1), s);
+            inject_random = false;
+        } else if(p.precond == 1) {
+            const double theta = w[ord[pstar]];
+            const double floor = 1e-8 * std::max(st.anorm, 1e-300);
+            if constexpr(vtraits<T>::cx) jacobi_precond<T, cpx<double>>(n, r, diag.as<cpx<double>>(), theta, floor, t, s);
+            else jacobi_precond<T, double>(n, r, diag.as<double>(), theta, floor, t, s);
+        } else {
+            DMRG_HIP_CHECK(hipMemcpyAsync(t, r, vbytes, hipMemcpyDeviceToDevice, s));
+        }
+        orthonormalize_into(t, m, sm + SB_NRM, sm + SB_NRM + 2);
+        matvec(t, Wp + static_cast<int64_t>(m) * n, 1);
+        st.matvecs += 1;
+        multidot<T>(n, m + 1, Vp, n, Wp + static_cast<int64_t>(m) * n, sm + SB_COL, scratch.as<double>(), s);
+        DMRG_HIP_CHECK(hipMemcpyAsync(host, sm, (SB_NRM + 4) * sizeof(double), hipMemcpyDeviceToHost, s));
+        DMRG_HIP_CHECK(hipStreamSynchronize(s));
+        const double pre = host[SB_NRM], post = host[SB_NRM + 2];
+        if(!(post > 1e4 * eps_t * eps_t * pre) || !std::isfinite(post)) {
+            if(!std::isfinite(post) || !std::isfinite(pre)) throw Error(DMRG_ERR_NAN, "eigensolver: non-finite expansion vector");
+            inject_random = true; // breakdown: the correction lies in the basis
+        } else {
+            for(int i = 0; i <= m; ++i) Hat(i, m) = cx(host[SB_COL + 2 * i], host[SB_COL + 2 * i + 1]);
+            for(int i = 0; i < m; ++i) Hat(m, i) = std::conj(Hat(i, m));
+            Hat(m, m) = cx(Hat(m, m).real(), 0.0);
+            ++m;
+        }
+        ++st.iters;
+    }
+
+    // ---- outputs: nkeep Ritz vectors (the first nconv were formed in the last pass already)
+    for(int q = 0; q < std::min(nkeep, m); ++q) {
+        const double rq = ritz(ord[q], m, out + q * n);
+        if(q < nev) {
+            evals[q]  = w[ord[q]];
+            rnorms[q] = rq;
+        }
+    }
+    for(int q = m; q < nev; ++q) {
+        evals[q]  = std::numeric_limits<double>::quiet_NaN();
+        rnorms[q] = std::numeric_limits<double>::infinity();
+    }
+    st.seconds = std::chrono::duration<double>(clock::now() - t0).count();
+    return st;
+}
+
+template <typename T64, typename T32>
+EigsStats eigs_mixed(MatVecOp &op, hipStream_t s, const EigsParams &p, const T64 *init, T64 *out, double *evals,
+                     double *rnorms, EigsStats *phase1) {
+    const int64_t n    = op.D.n();
+    const int     nev  = std::max(1, p.nev);
+    const int     keep = std::min<int64_t>(nev + 2, n);
+    DevBuf        b32, b32i, b64;
+    b32.reserve(static_cast<size_t>(n) * keep * sizeof(T32));
+    b64.reserve(static_cast<size_t>(n) * keep * sizeof(T64));
+    DMRG_HIP_CHECK(hipMemsetAsync(b32.p, 0, static_cast<size_t>(n) * keep * sizeof(T32), s));
+    const T32 *init32 = nullptr;
+    if(init) {
+        b32i.reserve(static_cast<size_t>(n) * sizeof(T32));
+        convert<T32, T64>(n, init, b32i.as<T32>(), s);
+        init32 = b32i.as<T32>();
+    }
+    // Phase 1: FP32 vectors and FP32 matrix-core matvecs. The residual floor of FP32 arithmetic is
+    // ~1e-6 |A|; stop there (or at stagnation).
+    EigsParams p1        = p;
+    p1.tol               = std::max(p.tol, 5e-6);
+    p1.stagnation_window = 25;
+    std::vector<double> e1(nev), r1(nev);
+    EigsStats           s1;
+    {
+        Davidson<T32> d32(op, s);
+        s1 = d32.solve(p1, init32, init32 ? 1 : 0, b32.as<T32>(), e1.data(), r1.data(), keep);
+    }
+    if(phase1) *phase1 = s1;
+    convert<T64, T32>(n * keep, b32.as<T32>(), b64.as<T64>(), s);
+    // Phase 2: FP64 Davidson from the FP32 Ritz vectors.
+    EigsParams p2 = p;
+    p2.seed += 1;
+    Davidson<T64> d64(op, s);
+    EigsStats     s2 = d64.solve(p2, b64.as<T64>(), keep, out, evals, rnorms, nev);
+    s2.seconds += s1.seconds;
+    return s2;
+}
+
+template class Davidson<double>;
+template class Davidson<float>;
+template class Davidson<cpx<double>>;
+template class Davidson<cpx<float>>;
+template EigsStats eigs_mixed<double, float>(MatVecOp &, hipStream_t, const EigsParams &, const double *, double *, double *,
+                                             double *, EigsStats *);
+template EigsStats eigs_mixed<cpx<double>, cpx<float>>(MatVecOp &, hipStream_t, const EigsParams &, const cpx<double> *,
+                                                       cpx<double> *, double *, double *, EigsStats *);
+
+} // namespace dmrg

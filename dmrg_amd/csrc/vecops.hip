@@ -1,0 +1,296 @@
+// Block-vector kernels of the eigensolver. Every kernel streams its vectors once from HBM with
+// coalesced accesses; reductions go through per-block partials folded in a fixed order.
+#include "vecops.hpp"
+
+#include <algorithm>
+
+namespace dmrg {
+
+constexpr int VT_THREADS = 256;
+
+template <typename T>
+__device__ inline void ld(const T &v, double &re, double &im) {
+    if constexpr(vtraits<T>::cx) {
+        re = static_cast<double>(v.re);
+        im = static_cast<double>(v.im);
+    } else {
+        re = static_cast<double>(v);
+        im = 0.0;
+    }
+}
+template <typename T>
+__device__ inline T mk(double re, double im) {
+    using R = typename vtraits<T>::R;
+    if constexpr(vtraits<T>::cx) return T{static_cast<R>(re), static_cast<R>(im)};
+    else return static_cast<R>(re);
+}
+
+__device__ inline double wsum(double v) {
+#pragma unroll
+    for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block reduction of NV values; results valid in thread 0.
+template <int NV>
+__device__ inline void bsum(double (&v)[NV], double (*red)[VT_THREADS / 64]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for(int i = 0; i < NV; ++i) v[i] = wsum(v[i]);
+    __syncthreads();
+    if(lane == 0)
+#pragma unroll
+        for(int i = 0; i < NV; ++i) red[i][wave] = v[i];
+    __syncthreads();
+    if(threadIdx.x == 0)
+#pragma unroll
+        for(int i = 0; i < NV; ++i) {
+            double s = 0;
+            for(int w = 0; w < VT_THREADS / 64; ++w) s += red[i][w];
+            v[i] = s;
+        }
+}
+
+static int grid_for(int64_t n) {
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(VEC_MAX_BLOCKS, (n + VT_THREADS - 1) / VT_THREADS)));
+}
+
+// ------------------------------------------------------------------------------------------- multidot
+constexpr int MD_CH = 8; // basis columns per pass
+
+template <typename T>
+__global__ __launch_bounds__(VT_THREADS) void multidot_partial(int64_t n, int m, const T *V, int64_t ldv, const T *x,
+                                                               double *partial) {
+    __shared__ double red[2 * MD_CH][VT_THREADS / 64];
+    for(int j0 = 0; j0 < m; j0 += MD_CH) {
+        double acc[2 * MD_CH];
+#pragma unroll
+        for(int c = 0; c < 2 * MD_CH; ++c) acc[c] = 0.0;
+        for(int64_t i = blockIdx.x * static_cast<int64_t>(VT_THREADS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * VT_THREADS) {
+            double xr, xi;
+            ld(x[i], xr, xi);
+#pragma unroll
+            for(int c = 0; c < MD_CH; ++c) {
+                if(j0 + c < m) {
+                    double vr, vi;
+                    ld(V[i + (j0 + c) * ldv], vr, vi);
+                    acc[2 * c] += vr * xr + vi * xi;     // conj(v) x
+                    acc[2 * c + 1] += vr * xi - vi * xr;
+                }
+            }
+        }
+        bsum<2 * MD_CH>(acc, red);
+        if(threadIdx.x == 0)
+#pragma unroll
+            for(int c = 0; c < MD_CH; ++c)
+                if(j0 + c < m) {
+                    partial[2 * (static_cast<int64_t>(blockIdx.x) * m + j0 + c)]     = acc[2 * c];
+                    partial[2 * (static_cast<int64_t>(blockIdx.x) * m + j0 + c) + 1] = acc[2 * c + 1];
+                }
+    }
+}
+
+__global__ void fold_columns(int nblk, int m, const double *partial, double *out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if(j >= m) return;
+    double re = 0, im = 0;
+    for(int b = 0; b < nblk; ++b) {
+        re += partial[2 * (static_cast<int64_t>(b) * m + j)];
+        im += partial[2 * (static_cast<int64_t>(b) * m + j) + 1];
+    }
+    out[2 * j]     = re;
+    out[2 * j + 1] = im;
+}
+
+template <typename T>
+void multidot(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_out, double *scratch, hipStream_t s) {
+    if(m <= 0) return;
+    DMRG_REQUIRE(m <= VEC_MAX_M, DMRG_ERR_ARG, "multidot: too many columns");
+    const int g = grid_for(n);
+    hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, V, ldv, x, scratch);
+    hipLaunchKernelGGL(fold_columns, dim3((m + 63) / 64), dim3(64), 0, s, g, m, scratch, d_out);
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------------- gemv
+template <typename T>
+__global__ __launch_bounds__(VT_THREADS) void gemv_update_kernel(int64_t n, int m, const T *V, int64_t ldv, const double *h,
+                                                                 double alpha, double beta, T *x) {
+    __shared__ double hs[2 * VEC_MAX_M];
+    for(int j = threadIdx.x; j < 2 * m; j += VT_THREADS) hs[j] = h[j];
+    __syncthreads();
+    for(int64_t i = blockIdx.x * static_cast<int64_t>(VT_THREADS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * VT_THREADS) {
+        double ar = 0, ai = 0;
+        for(int j = 0; j < m; ++j) {
+            double vr, vi;
+            ld(V[i + j * ldv], vr, vi);
+            ar += vr * hs[2 * j] - vi * hs[2 * j + 1];
+            ai += vr * hs[2 * j + 1] + vi * hs[2 * j];
+        }
+        double xr = 0, xi = 0;
+        if(beta != 0.0) ld(x[i], xr, xi);
+        x[i] = mk<T>(beta * xr + alpha * ar, beta * xi + alpha * ai);
+    }
+}
+
+static int ew_grid(int64_t n) { return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(8192, (n + VT_THREADS - 1) / VT_THREADS))); }
+
+template <typename T>
+void gemv_update(int64_t n, int m, const T *V, int64_t ldv, const double *d_h, double alpha, double beta, T *x, hipStream_t s) {
+    DMRG_REQUIRE(m <= VEC_MAX_M, DMRG_ERR_ARG, "gemv: too many columns");
+    hipLaunchKernelGGL(gemv_update_kernel<T>, dim3(ew_grid(n)), dim3(VT_THREADS), 0, s, n, m, V, ldv, d_h, alpha, beta, x);
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------------- Ritz vector + residual
+template <typename T>
+__global__ __launch_bounds__(VT_THREADS) void ritz_kernel(int64_t n, int m, const T *V, const T *W, int64_t ldv, const double *y,
+                                                          double theta, T *x, T *r, double *partial) {
+    __shared__ double ys[2 * VEC_MAX_M];
+    __shared__ double red[2][VT_THREADS / 64];
+    for(int j = threadIdx.x; j < 2 * m; j += VT_THREADS) ys[j] = y[j];
+    __syncthreads();
+    double acc[2] = {0.0, 0.0};
+    for(int64_t i = blockIdx.x * static_cast<int64_t>(VT_THREADS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * VT_THREADS) {
+        double xr = 0, xi = 0, wr = 0, wi = 0;
+        for(int j = 0; j < m; ++j) {
+            double vr, vi, ur, ui;
+            ld(V[i + j * ldv], vr, vi);
+            ld(W[i + j * ldv], ur, ui);
+            const double yr = ys[2 * j], yi = ys[2 * j + 1];
+            xr += vr * yr - vi * yi;
+            xi += vr * yi + vi * yr;
+            wr += ur * yr - ui * yi;
+            wi += ur * yi + ui * yr;
+        }
+        const double rr = wr - theta * xr, ri = wi - theta * xi;
+        x[i] = mk<T>(xr, xi);
+        r[i] = mk<T>(rr, ri);
+        acc[0] += rr * rr + ri * ri;
+        acc[1] += xr * xr + xi * xi;
+    }
+    bsum<2>(acc, red);
+    if(threadIdx.x == 0) {
+        partial[2 * blockIdx.x]     = acc[0];
+        partial[2 * blockIdx.x + 1] = acc[1];
+    }
+}
+
+template <typename T>
+void ritz_residual(int64_t n, int m, const T *V, const T *W, int64_t ld, const double *d_y, double theta, T *x, T *r,
+                   double *d_out, double *scratch, hipStream_t s) {
+    DMRG_REQUIRE(m <= VEC_MAX_M, DMRG_ERR_ARG, "ritz: too many columns");
+    const int g = grid_for(n);
+    hipLaunchKernelGGL(ritz_kernel<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, V, W, ld, d_y, theta, x, r, scratch);
+    hipLaunchKernelGGL(fold_columns, dim3(1), dim3(64), 0, s, g, 1, scratch, d_out);
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------------- elementwise
+template <typename T>
+__global__ void scale_inv_norm_kernel(int64_t n, T *x, const double *nrm2) {
+    const double a = 1.0 / sqrt(nrm2[0]);
+    for(int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        double re, im;
+        ld(x[i], re, im);
+        x[i] = mk<T>(a * re, a * im);
+    }
+}
+template <typename T>
+void scale_by_inv_norm(int64_t n, T *x, const double *d_nrm2, hipStream_t s) {
+    hipLaunchKernelGGL(scale_inv_norm_kernel<T>, dim3(ew_grid(n)), dim3(VT_THREADS), 0, s, n, x, d_nrm2);
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+__global__ void scale_kernel(int64_t n, T *x, double a) {
+    for(int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        double re, im;
+        ld(x[i], re, im);
+        x[i] = mk<T>(a * re, a * im);
+    }
+}
+template <typename T>
+void scale(int64_t n, T *x, double alpha, hipStream_t s) {
+    hipLaunchKernelGGL(scale_kernel<T>, dim3(ew_grid(n)), dim3(VT_THREADS), 0, s, n, x, alpha);
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T, typename D>
+__global__ void jacobi_kernel(int64_t n, const T *r, const D *diag, double theta, double floor, T *t) {
+    for(int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        double rr, ri, dr, di;
+        ld(r[i], rr, ri);
+        ld(diag[i], dr, di);
+        double den = dr - theta;
+        if(fabs(den) < floor) den = den < 0 ? -floor : floor;
+        t[i] = mk<T>(rr / den, ri / den);
+    }
+}
+template <typename T, typename D>
+void jacobi_precond(int64_t n, const T *r, const D *diag, double theta, double floor, T *t, hipStream_t s) {
+    hipLaunchKernelGGL((jacobi_kernel<T, D>), dim3(ew_grid(n)), dim3(VT_THREADS), 0, s, n, r, diag, theta, floor, t);
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+__device__ inline uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ inline double u11(uint64_t z) { return static_cast<double>(z >> 11) * (2.0 / 9007199254740992.0) - 1.0; }
+
+template <typename T>
+__global__ void random_kernel(int64_t n, T *x, uint64_t seed) {
+    for(int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t h = splitmix64(seed * 0x100000001B3ull + static_cast<uint64_t>(i));
+        x[i]             = mk<T>(u11(h), vtraits<T>::cx ? u11(splitmix64(h)) : 0.0);
+    }
+}
+template <typename T>
+void random_fill(int64_t n, T *x, uint64_t seed, hipStream_t s) {
+    hipLaunchKernelGGL(random_kernel<T>, dim3(ew_grid(n)), dim3(VT_THREADS), 0, s, n, x, seed);
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename TO, typename TI>
+__global__ void convert_kernel(int64_t n, const TI *in, TO *out) {
+    for(int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        double re, im;
+        ld(in[i], re, im);
+        out[i] = mk<TO>(re, im);
+    }
+}
+template <typename TO, typename TI>
+void convert(int64_t n, const TI *in, TO *out, hipStream_t s) {
+    hipLaunchKernelGGL((convert_kernel<TO, TI>), dim3(ew_grid(n)), dim3(VT_THREADS), 0, s, n, in, out);
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+#define DMRG_VEC_INST(T)                                                                                                 \
+    template void multidot<T>(int64_t, int, const T *, int64_t, const T *, double *, double *, hipStream_t);          \
+    template void gemv_update<T>(int64_t, int, const T *, int64_t, const double *, double, double, T *, hipStream_t); \
+    template void ritz_residual<T>(int64_t, int, const T *, const T *, int64_t, const double *, double, T *, T *,      \
+                                   double *, double *, hipStream_t);                                                   \
+    template void scale_by_inv_norm<T>(int64_t, T *, const double *, hipStream_t);                                    \
+    template void scale<T>(int64_t, T *, double, hipStream_t);                                                         \
+    template void random_fill<T>(int64_t, T *, uint64_t, hipStream_t);
+DMRG_VEC_INST(double)
+DMRG_VEC_INST(float)
+DMRG_VEC_INST(cpx<double>)
+DMRG_VEC_INST(cpx<float>)
+template void jacobi_precond<double, double>(int64_t, const double *, const double *, double, double, double *, hipStream_t);
+template void jacobi_precond<float, double>(int64_t, const float *, const double *, double, double, float *, hipStream_t);
+template void jacobi_precond<cpx<double>, cpx<double>>(int64_t, const cpx<double> *, const cpx<double> *, double, double,
+                                                       cpx<double> *, hipStream_t);
+template void jacobi_precond<cpx<float>, cpx<double>>(int64_t, const cpx<float> *, const cpx<double> *, double, double,
+                                                      cpx<float> *, hipStream_t);
+template void convert<float, double>(int64_t, const double *, float *, hipStream_t);
+template void convert<double, float>(int64_t, const float *, double *, hipStream_t);
+template void convert<cpx<float>, cpx<double>>(int64_t, const cpx<double> *, cpx<float> *, hipStream_t);
+template void convert<cpx<double>, cpx<float>>(int64_t, const cpx<float> *, cpx<double> *, hipStream_t);
+template void convert<double, double>(int64_t, const double *, double *, hipStream_t);
+template void convert<cpx<double>, cpx<double>>(int64_t, const cpx<double> *, cpx<double> *, hipStream_t);
+
+} // namespace dmrg

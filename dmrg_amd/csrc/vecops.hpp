@@ -1,0 +1,67 @@
+// Block-vector kernels of the eigensolver (HBM-bound): projections against a basis, basis updates,
+// Ritz vectors and residuals, preconditioning. All reductions accumulate in double and are deterministic.
+#pragma once
+#include "common.hpp"
+
+#include <complex>
+#include <type_traits>
+
+namespace dmrg {
+
+template <typename T>
+struct vtraits;
+template <>
+struct vtraits<double> {
+    using R                        = double;
+    static constexpr bool       cx = false;
+    static constexpr dmrg_dtype dt = DMRG_FP64;
+};
+template <>
+struct vtraits<float> {
+    using R                        = float;
+    static constexpr bool       cx = false;
+    static constexpr dmrg_dtype dt = DMRG_FP32;
+};
+template <>
+struct vtraits<cpx<double>> {
+    using R                        = double;
+    static constexpr bool       cx = true;
+    static constexpr dmrg_dtype dt = DMRG_CX64;
+};
+template <>
+struct vtraits<cpx<float>> {
+    using R                        = float;
+    static constexpr bool       cx = true;
+    static constexpr dmrg_dtype dt = DMRG_CX32;
+};
+
+// Scratch for reductions: partial sums of up to VEC_MAX_BLOCKS blocks x VEC_MAX_M columns (complex).
+constexpr int VEC_MAX_BLOCKS = 240;
+constexpr int VEC_MAX_M      = 256;
+constexpr int64_t vec_scratch_doubles() { return 2LL * VEC_MAX_BLOCKS * VEC_MAX_M + 4 * VEC_MAX_M; }
+
+// out[j] = sum_i conj(V[i, j]) x[i], j < m (complex double, out on device, 2*m doubles).
+template <typename T>
+void multidot(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_out, double *scratch, hipStream_t s);
+// x = beta x + alpha V h   (h: complex double, m entries, device).
+template <typename T>
+void gemv_update(int64_t n, int m, const T *V, int64_t ldv, const double *d_h, double alpha, double beta, T *x, hipStream_t s);
+// x = V y, r = W y - theta x, d_out[0] = |r|^2, d_out[1] = |x|^2 (y: complex double, device).
+template <typename T>
+void ritz_residual(int64_t n, int m, const T *V, const T *W, int64_t ld, const double *d_y, double theta, T *x, T *r,
+                   double *d_out, double *scratch, hipStream_t s);
+// x *= 1 / sqrt(d_nrm2[0]) (the squared norm read on the device).
+template <typename T>
+void scale_by_inv_norm(int64_t n, T *x, const double *d_nrm2, hipStream_t s);
+template <typename T>
+void scale(int64_t n, T *x, double alpha, hipStream_t s);
+// t = r / (diag - theta), with |diag - theta| floored at floor (Jacobi preconditioner, MultPc jcb = 1).
+template <typename T, typename D>
+void jacobi_precond(int64_t n, const T *r, const D *diag, double theta, double floor, T *t, hipStream_t s);
+// Seeded pseudo-random vector with entries uniform in [-1, 1] (both parts for complex).
+template <typename T>
+void random_fill(int64_t n, T *x, uint64_t seed, hipStream_t s);
+template <typename TO, typename TI>
+void convert(int64_t n, const TI *in, TO *out, hipStream_t s);
+
+} // namespace dmrg

@@ -135,6 +135,55 @@ dmrg_status dmrg_matvec_set_shift(dmrg_matvec *op, double re, double im);
  * (matvec_mpos.cpp:399-407); feeds the Jacobi preconditioner (matvec_mpos.cpp:1333-1416, jcb = 1). */
 dmrg_status dmrg_matvec_get_diagonal(dmrg_matvec *op, void *ddiag, void *stream);
 
+/* ---------------------------------------------------------------- eigensolver */
+
+#define DMRG_EIGS_MAX_NEV 16
+typedef enum dmrg_eigs_target {
+    DMRG_TARGET_SA      = 0, /* smallest algebraic (primme_smallest; fDMRG ground state) */
+    DMRG_TARGET_LA      = 1, /* largest algebraic  (primme_largest) */
+    DMRG_TARGET_CLOSEST = 2  /* closest to shift   (primme_closest_abs; xDMRG on H^2) */
+} dmrg_eigs_target;
+
+/* Subset of eig::settings (source/math/eig/settings.h:12-70) that the GPU Davidson honours. */
+typedef struct dmrg_eigs_config {
+    int      nev;         /* maxNev, default 1 */
+    int      ncv;         /* maxNcv (basis size); <= 0: nev*ceil(log2 n) clamped to [2 nev, n] (solver.cpp:12-15) */
+    int      min_restart; /* primme_minRestartSize; <= 0: max(1, ncv/2 - 1) (solver_primme.cpp:374) */
+    int      target;      /* dmrg_eigs_target */
+    double   shift;       /* target shift for DMRG_TARGET_CLOSEST (primme_targetShifts) */
+    double   tol;         /* eps: converged when |r| <= tol * |A|est (|A|est = largest |Ritz value| seen) */
+    int64_t  max_matvecs; /* default 100000 (eigs_iter_max, settings.h:251) */
+    int      precond;     /* 0: none, 1: Jacobi (diagonal of H_eff, MultPc with jcb = 1) */
+    int      mixed;       /* 0: FP64 throughout; 1: FP32 matrix-core phase then FP64 refinement */
+    double   max_time_s;  /* 0: unlimited (reference: config.maxTime) */
+    uint64_t seed;        /* seed of the random start vector when no initial guess is given */
+} dmrg_eigs_config;
+
+typedef struct dmrg_eigs_result {
+    double  eigval[DMRG_EIGS_MAX_NEV];
+    double  rnorm[DMRG_EIGS_MAX_NEV];
+    int     nconv;        /* converged pairs; < nev when the matvec/time limit was reached */
+    int     iters;
+    int     restarts;
+    int     stagnated;
+    int64_t matvecs_fp64; /* FP64 H_eff applications (block matvecs count each vector) */
+    int64_t matvecs_fp32; /* FP32 applications (mixed mode) */
+    double  seconds;
+    double  anorm;
+} dmrg_eigs_result;
+
+void        dmrg_eigs_default_config(dmrg_eigs_config *cfg);
+/* Eigenpairs of the operator. x0: optional initial guess (DEVICE, n elements of the operator dtype, or NULL);
+ * evecs: DEVICE output, n x nev column-major. Replaces eig::solver::eigs(MatVecMPOS<T>&) with the PRIMME backend
+ * (solver.cpp:491-509, solver_primme.cpp:297-517): same problem (H_eff x = lambda x, nev pairs nearest the target),
+ * a GPU Davidson instead of PRIMME's GD_Olsen_plusK. Returns DMRG_OK also when not converged (res->nconv < nev). */
+dmrg_status dmrg_eigs(dmrg_matvec *op, const dmrg_eigs_config *cfg, const void *x0, void *evecs, dmrg_eigs_result *res,
+                      void *stream);
+
+/* Host dense Hermitian eigensolver used for the eigensolver's Rayleigh-Ritz step (the role LAPACK ?heevd
+ * plays inside PRIMME). A, Z: m x m column-major complex interleaved; w ascending. */
+dmrg_status dmrg_host_heev(int m, const double *A, double *w, double *Z);
+
 /* GEMM autotuner (no counterpart in the reference). Every contraction leg is a GEMM whose launch plan (tile
  * configuration, K slices) is chosen by timing the candidates with HIP events the first time a problem
  * signature is seen on a stream that is not being captured, and cached for the process. */
