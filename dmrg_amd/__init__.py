@@ -154,6 +154,28 @@ def nrm2(x, stream=None) -> float:
     return res.value
 
 
+def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_max: int = 64, max_sweeps: int = 20,
+                      two_site_sweeps: int = 10, var_tol: float = 1e-12, trunc_lim: float = 1e-14, eigs_tol: float = 1e-12,
+                      mixed: bool = False, complex_: bool = False, seed: int = 0, stream=None) -> dict:
+    """Finite DMRG ground state of the random transverse-field Ising chain on the GPU
+    (fdmrg::run_algorithm with ising_tf_rf, fdmrg.cpp:170-376). Returns energies, variance, entropies, timings."""
+    h = np.ascontiguousarray(np.asarray(h_rand, dtype=np.float64))
+    assert h.shape == (L,)
+    cfg = _lib.FdmrgConfig()
+    load().dmrg_fdmrg_default_config(C.byref(cfg))
+    cfg.L, cfg.J, cfg.h_tran = L, J, h_tran
+    cfg.h_rand = h.ctypes.data_as(C.POINTER(C.c_double))
+    cfg.chi_max, cfg.max_sweeps, cfg.two_site_sweeps = chi_max, max_sweeps, two_site_sweeps
+    cfg.var_tol, cfg.trunc_lim, cfg.eigs_tol = var_tol, trunc_lim, eigs_tol
+    cfg.mixed, cfg.complex_, cfg.seed = int(mixed), int(complex_), seed
+    res = _lib.FdmrgResult()
+    call("dmrg_fdmrg_ising_tf_rf", C.byref(cfg), C.byref(res), stream)
+    out = {k: getattr(res, k) for k, _ in res._fields_ if k not in ("entropies", "chi")}
+    out["entropies"] = np.array(res.entropies[: L + 1])
+    out["chi"] = np.array(res.chi[: L + 1])
+    return out
+
+
 class MatVecMPOS:
     """Device-resident effective Hamiltonian (MatVecMPOS<T>, matvec_mpos.h:23-180).
 

@@ -86,6 +86,25 @@ class EigsResult(C.Structure):
                 ("matvecs_fp32", C.c_int64), ("seconds", C.c_double), ("anorm", C.c_double)]
 
 
+MAX_L = 256
+
+
+class FdmrgConfig(C.Structure):
+    _fields_ = [("L", C.c_int), ("J", C.c_double), ("h_tran", C.c_double), ("h_rand", C.POINTER(C.c_double)),
+                ("chi_max", C.c_int), ("max_sweeps", C.c_int), ("two_site_sweeps", C.c_int), ("var_tol", C.c_double),
+                ("trunc_lim", C.c_double), ("eigs_tol", C.c_double), ("eigs_ncv", C.c_int), ("mixed", C.c_int),
+                ("complex_", C.c_int), ("seed", C.c_uint64)]
+
+
+class FdmrgResult(C.Structure):
+    _fields_ = [("energy", C.c_double), ("variance", C.c_double), ("entropies", C.c_double * (MAX_L + 1)),
+                ("chi", C.c_int64 * (MAX_L + 1)), ("sweeps", C.c_int), ("eigensolves", C.c_int), ("converged", C.c_int),
+                ("matvecs", C.c_int64), ("matvecs_fp32", C.c_int64), ("seconds", C.c_double), ("t_eigs", C.c_double),
+                ("t_env", C.c_double), ("t_svd", C.c_double), ("trunc_error_max", C.c_double)]
+
+
+SIGNATURES["dmrg_fdmrg_default_config"] = (None, [C.POINTER(FdmrgConfig)])
+SIGNATURES["dmrg_fdmrg_ising_tf_rf"] = (st, [C.POINTER(FdmrgConfig), C.POINTER(FdmrgResult), vp])
 SIGNATURES["dmrg_eigs_default_config"] = (None, [C.POINTER(EigsConfig)])
 SIGNATURES["dmrg_eigs"] = (st, [vp, C.POINTER(EigsConfig), vp, vp, C.POINTER(EigsResult), vp])
 

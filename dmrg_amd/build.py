@@ -27,7 +27,8 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                 "-mllvm", "-amdgpu-mfma-vgpr-form"]
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{INCLUDE}", "-Wall",
             "-Wno-unused-function", "-Wno-unused-variable"] + DEVICE_FLAGS
-LDFLAGS = ["-shared", f"--offload-arch={ARCH}", "-lpthread"]
+LDFLAGS = ["-shared", f"--offload-arch={ARCH}", "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver",
+           "-lrocblas", "-lpthread"]
 
 
 def sources() -> list[Path]:

@@ -5,6 +5,7 @@
 #include "matvec_op.hpp"
 #include "dense_eig.hpp"
 #include "eigs.hpp"
+#include "sweep.hpp"
 
 #include <complex>
 #include <cstring>
@@ -372,6 +373,29 @@ dmrg_status dmrg_eigs(dmrg_matvec *opp, const dmrg_eigs_config *cfg, const void 
         res->matvecs_fp64 = st.matvecs;
         res->seconds      = st.seconds;
         res->anorm        = st.anorm;
+    });
+}
+
+void dmrg_fdmrg_default_config(dmrg_fdmrg_config *cfg) {
+    if(!cfg) return;
+    *cfg                 = dmrg_fdmrg_config{};
+    cfg->L               = 16;
+    cfg->J               = 1.0;
+    cfg->h_tran          = 1.0;
+    cfg->chi_max         = 64;
+    cfg->max_sweeps      = 20;
+    cfg->two_site_sweeps = 10;
+    cfg->var_tol         = 1e-12;
+    cfg->trunc_lim       = 1e-14;
+    cfg->eigs_tol        = 1e-12;
+    cfg->seed            = 0;
+}
+
+dmrg_status dmrg_fdmrg_ising_tf_rf(const dmrg_fdmrg_config *cfg, dmrg_fdmrg_result *res, void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(cfg && res && cfg->h_rand, DMRG_ERR_ARG, "null argument");
+        DMRG_REQUIRE(cfg->L >= 2 && cfg->L <= DMRG_FDMRG_MAX_L, DMRG_ERR_ARG, "L out of range");
+        fdmrg_ising(*cfg, *res, static_cast<hipStream_t>(stream));
     });
 }
 

@@ -21,7 +21,7 @@ constexpr int SB_Y    = 2 * VEC_MAX_M + 8; // Ritz coefficients, 2 * VEC_MAX_M
 constexpr int SB_SIZE = 4 * VEC_MAX_M + 8;
 
 template <typename T>
-Davidson<T>::Davidson(MatVecOp &o, hipStream_t st) : op(o), s(st), n(o.D.n()) {
+Davidson<T>::Davidson(MatVecOp &o, hipStream_t st) : op(&o), s(st), n(o.D.n()) {
     scratch.reserve(static_cast<size_t>(vec_scratch_doubles()) * sizeof(double));
     small.reserve(SB_SIZE * sizeof(double));
     DMRG_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host), SB_SIZE * sizeof(double), hipHostMallocDefault));
@@ -34,7 +34,7 @@ Davidson<T>::~Davidson() {
 
 template <typename T>
 void Davidson<T>::matvec(const T *x, T *y, int nvec) {
-    op.multax_device(x, n, y, n, nvec, s, std::is_same_v<typename vtraits<T>::R, float>);
+    op->multax_device(x, n, y, n, nvec, s, std::is_same_v<typename vtraits<T>::R, float>);
 }
 
 template <typename T>
@@ -87,13 +87,15 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
     EigsStats  st;
     const int  nev = std::max(1, p.nev);
     DMRG_REQUIRE(nev <= n, DMRG_ERR_ARG, "nev larger than the problem size");
+    // Basis size rule of solver.cpp:12-15 and the restart size of solver_primme.cpp:373-376: the basis
+    // must hold the restart vectors, the retained previous Ritz vector and one expansion.
     int mmax = p.ncv > 0 ? p.ncv : nev * static_cast<int>(std::ceil(std::log2(static_cast<double>(std::max<int64_t>(n, 2)))));
     mmax     = static_cast<int>(std::clamp<int64_t>(mmax, 2 * nev, n));
-    mmax     = std::min(mmax, VEC_MAX_M);
     int mmin = p.min_restart > 0 ? p.min_restart : std::max(1, mmax / 2 - 1);
     mmin     = std::max(mmin, nev);
-    // Keep room for the retained previous Ritz vector and at least one expansion.
-    mmin = std::min(mmin, std::max(1, mmax - 1 - p.plus_k));
+    mmax     = static_cast<int>(std::clamp<int64_t>(mmax, mmin + 1 + std::max(2, p.plus_k), n));
+    mmax     = std::min(mmax, VEC_MAX_M);
+    mmin     = std::max(1, std::min(mmin, mmax - 1 - p.plus_k));
     nkeep = std::max(nkeep, nev);
     nkeep = std::min<int64_t>(nkeep, n);
     k0    = std::min<int>(k0, mmax);
@@ -107,8 +109,8 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
     double *sm = small.as<double>();
     if(p.precond == 1) {
         // Diagonal of H_eff in the operator's FP64 dtype; the kernel converts on the fly.
-        diag.reserve(static_cast<size_t>(n) * dtype_size(op.dtype));
-        op.diagonal(diag.p, s);
+        diag.reserve(static_cast<size_t>(n) * dtype_size(op->dtype));
+        op->diagonal(diag.p, s);
     }
 
     // ---- initial basis
@@ -185,6 +187,10 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
         const int nconv = pstar < 0 ? std::min(nev, m) : pstar;
         st.nconv        = nconv;
         if(pstar < 0 && m >= nev) break;
+        if(m >= n) { // the basis spans the whole space: the Ritz pairs are exact
+            st.nconv = std::min<int64_t>(nev, n);
+            break;
+        }
         if(pstar < 0) pstar = m - 1; // fewer basis vectors than nev: keep expanding
         const double rcur = rn[pstar];
         if(rcur < 0.9 * best_r) {
@@ -227,6 +233,8 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
                     Qc.push_back(c);
                 }
             }
+            // At most mmax - 1 vectors survive, so the next expansion has a free column.
+            if(static_cast<int>(Qc.size()) > mmax - 1) Qc.resize(mmax - 1);
             const int       k = static_cast<int>(Qc.size());
             std::vector<cx> Q(static_cast<size_t>(m) * k);
             for(int j = 0; j < k; ++j)
@@ -255,6 +263,7 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
         }
 
         // ---- expansion vector t = M^-1 r (or r), orthonormalised against V
+        DMRG_REQUIRE(m < mmax, DMRG_ERR_ARG, "eigensolver: basis full after restart");
         T *t = Vp + static_cast<int64_t>(m) * n;
         if(inject_random) {
             random_fill<T>(n, t, p.seed + 7919ull * (st.iters + 1), s);

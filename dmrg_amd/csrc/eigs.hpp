@@ -51,9 +51,14 @@ class Davidson {
     // out:  nev Ritz vectors (device, n x nev); evals/rnorms: host arrays of length nev.
     // nkeep > nev: also export the nkeep best Ritz vectors into out (n x nkeep).
     EigsStats solve(const EigsParams &p, const T *init, int k0, T *out, double *evals, double *rnorms, int nkeep = 0);
+    // Use another operator (possibly of another size); device buffers are kept and grown as needed.
+    void rebind(MatVecOp &o) {
+        op = &o;
+        n  = o.D.n();
+    }
 
   private:
-    MatVecOp   &op;
+    MatVecOp   *op;
     hipStream_t s;
     int64_t     n;
     DevBuf      V, W, tmp, rbuf, scratch, small, Ydev, diag;

@@ -70,6 +70,31 @@ MatVecOp::MatVecOp(dmrg_dtype t, int nmpo, const void *const *mpos, const int64_
     upload_mpo();
     upload_as<double>(envL, static_cast<const cx *>(envL_h), D.chiL * D.chiL * D.DL, t);
     upload_as<double>(envR, static_cast<const cx *>(envR_h), D.chiR * D.chiR * D.DR, t);
+    envL_p = envL.p;
+    envR_p = envR.p;
+}
+
+MatVecOp::MatVecOp(dmrg_dtype t, int nmpo, const void *const *mpos, const int64_t *mpo_dims, const void *envL_d,
+                   const int64_t envL_dims[3], const void *envR_d, const int64_t envR_dims[3], bool)
+    : dtype(t), nmpo_in(nmpo) {
+    rebind(nmpo, mpos, mpo_dims, envL_d, envL_dims, envR_d, envR_dims);
+}
+
+void MatVecOp::rebind(int nmpo, const void *const *mpos, const int64_t *mpo_dims, const void *envL_d,
+                      const int64_t envL_dims[3], const void *envR_d, const int64_t envR_dims[3]) {
+    DMRG_REQUIRE(envL_d && envR_d && envL_dims && envR_dims, DMRG_ERR_ARG, "null env");
+    int64_t md[4];
+    mpo_host = merge_mpos(nmpo, reinterpret_cast<const cx *const *>(mpos), mpo_dims, md);
+    const int64_t mps_dims[3] = {md[2], envL_dims[0], envR_dims[0]};
+    D                         = check_matvec_dims(mps_dims, md, envL_dims, envR_dims);
+    if(dtype == DMRG_FP64)
+        for(const auto &v : mpo_host) DMRG_REQUIRE(v.imag() == 0.0, DMRG_ERR_ARG, "mpo is not real");
+    nmpo_in     = nmpo;
+    sigma       = cx(0.0, 0.0);
+    ready_shift = false;
+    upload_mpo();
+    envL_p = envL_d;
+    envR_p = envR_d;
 }
 
 void MatVecOp::upload_mpo() {
@@ -85,8 +110,8 @@ void MatVecOp::ensure_fp32(hipStream_t s) {
     envL32.reserve(nl * sizeof(float));
     envR32.reserve(nr * sizeof(float));
     convert_f64_to_f32(nm, mpo.as<double>(), mpo32.as<float>(), s);
-    convert_f64_to_f32(nl, envL.as<double>(), envL32.as<float>(), s);
-    convert_f64_to_f32(nr, envR.as<double>(), envR32.as<float>(), s);
+    convert_f64_to_f32(nl, static_cast<const double *>(envL_p), envL32.as<float>(), s);
+    convert_f64_to_f32(nr, static_cast<const double *>(envR_p), envR32.as<float>(), s);
     have32 = true;
 }
 
@@ -101,7 +126,7 @@ void MatVecOp::multax_device(const void *dx, int64_t ldx, void *dy, int64_t ldy,
         matvec(t32, D, dx, ldx, dy, ldy, nvec, mpo32.p, envL32.p, envR32.p, w, s);
     } else {
         void *w = ws.reserve(static_cast<size_t>(nvec) * matvec_ws_elems(D) * dtype_size(dtype));
-        matvec(dtype, D, dx, ldx, dy, ldy, nvec, mpo.p, envL.p, envR.p, w, s);
+        matvec(dtype, D, dx, ldx, dy, ldy, nvec, mpo.p, envL_p, envR_p, w, s);
     }
     num_mv += nvec;
 }
@@ -134,7 +159,7 @@ void MatVecOp::set_shift(cx shift) {
 
 void MatVecOp::diagonal(void *out, hipStream_t s) {
     void *w = diagws.reserve(static_cast<size_t>(D.chiL * D.DR * D.d) * dtype_size(dtype));
-    heff_diagonal(dtype, D, mpo.p, envL.p, envR.p, w, out, s);
+    heff_diagonal(dtype, D, mpo.p, envL_p, envR_p, w, out, s);
 }
 
 void MatVecOp::time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms[3], hipStream_t s) {
@@ -147,8 +172,8 @@ void MatVecOp::time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms
         DevBuf          &wb = fp32 ? ws32 : ws;
         void            *w  = wb.reserve(static_cast<size_t>(matvec_ws_elems(D)) * dtype_size(t));
         for(int r = 0; r < nrep; ++r) {
-            matvec(t, D, dx, D.n(), dy, D.n(), 1, fp32 ? mpo32.p : mpo.p, fp32 ? envL32.p : envL.p,
-                   fp32 ? envR32.p : envR.p, w, s, ev);
+            matvec(t, D, dx, D.n(), dy, D.n(), 1, fp32 ? mpo32.p : mpo.p, fp32 ? envL32.p : envL_p,
+                   fp32 ? envR32.p : envR_p, w, s, ev);
             DMRG_HIP_CHECK(hipEventSynchronize(ev[3]));
             for(int i = 0; i < 3; ++i) {
                 float t_ms = 0;

@@ -27,6 +27,8 @@ struct DevBuf {
     }
     void *reserve(size_t b) {
         if(b > bytes) {
+            // Growing: queued kernels may still read the old buffer.
+            if(p) (void) hipDeviceSynchronize();
             release();
             if(hipMalloc(&p, b) != hipSuccess) {
                 (void) hipGetLastError();
@@ -53,7 +55,8 @@ struct MatVecOp {
     Dims                               D;
     int                                nmpo_in = 1;
     std::vector<std::complex<double>>  mpo_host; // merged MPO in cx64 (shift applied), source of the device copies
-    DevBuf                             mpo, envL, envR;       // operator dtype
+    DevBuf                             mpo, envL, envR;       // operator dtype (owned copies)
+    const void                        *envL_p = nullptr, *envR_p = nullptr; // env blocks used: owned or borrowed views
     DevBuf                             mpo32, envL32, envR32; // FP32 shadows for the mixed path
     bool                               have32 = false;
     DevBuf                             ws, ws32, xbuf, ybuf, diagws;
@@ -63,6 +66,13 @@ struct MatVecOp {
 
     MatVecOp(dmrg_dtype t, int nmpo, const void *const *mpos, const int64_t *mpo_dims, const void *envL_h,
              const int64_t envL_dims[3], const void *envR_h, const int64_t envR_dims[3]);
+    // Device-view variant: env blocks are device arrays of dtype t that stay owned by the caller and must outlive
+    // the operator; the MPOs (host, cx64) are merged and copied.
+    MatVecOp(dmrg_dtype t, int nmpo, const void *const *mpos, const int64_t *mpo_dims, const void *envL_d,
+             const int64_t envL_dims[3], const void *envR_d, const int64_t envR_dims[3], bool device_envs);
+    // Re-point a device-view operator at new MPOs and env blocks; workspaces are kept (grow-only).
+    void rebind(int nmpo, const void *const *mpos, const int64_t *mpo_dims, const void *envL_d, const int64_t envL_dims[3],
+                const void *envR_d, const int64_t envR_dims[3]);
 
     void upload_mpo();
     void ensure_fp32(hipStream_t s);

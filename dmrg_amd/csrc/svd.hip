@@ -1,0 +1,116 @@
+// Device SVD (rocSOLVER) and remapping copies used by the sweep.
+#include "svd.hpp"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cmath>
+
+namespace dmrg {
+
+#define DMRG_ROCBLAS_CHECK(expr)                                                                             \
+    do {                                                                                                     \
+        rocblas_status st_ = (expr);                                                                         \
+        if(st_ != rocblas_status_success)                                                                    \
+            throw ::dmrg::Error(DMRG_ERR_HIP, std::string(#expr) + " failed: " + rocblas_status_to_string(st_)); \
+    } while(0)
+
+DeviceSvd::DeviceSvd(hipStream_t st) : s(st) {
+    rocblas_handle h = nullptr;
+    DMRG_ROCBLAS_CHECK(rocblas_create_handle(&h));
+    DMRG_ROCBLAS_CHECK(rocblas_set_stream(h, s));
+    handle = h;
+}
+DeviceSvd::~DeviceSvd() {
+    if(handle) (void) rocblas_destroy_handle(static_cast<rocblas_handle>(handle));
+}
+
+template <typename T>
+void DeviceSvd::compute(int m, int n, const T *A, std::vector<double> &S) {
+    DMRG_REQUIRE(m > 0 && n > 0, DMRG_ERR_DIMS, "svd of an empty matrix");
+    k_ = std::min(m, n);
+    a.reserve(static_cast<size_t>(m) * n * sizeof(T));
+    sv.reserve(static_cast<size_t>(k_) * sizeof(double));
+    u.reserve(static_cast<size_t>(m) * k_ * sizeof(T));
+    vt.reserve(static_cast<size_t>(k_) * n * sizeof(T));
+    e.reserve(static_cast<size_t>(std::max(1, k_ - 1)) * sizeof(double));
+    info.reserve(sizeof(rocblas_int));
+    DMRG_HIP_CHECK(hipMemcpyAsync(a.p, A, static_cast<size_t>(m) * n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    auto h = static_cast<rocblas_handle>(handle);
+    if constexpr(std::is_same_v<T, double>) {
+        DMRG_ROCBLAS_CHECK(rocsolver_dgesvd(h, rocblas_svect_singular, rocblas_svect_singular, m, n, a.as<double>(), m,
+                                            sv.as<double>(), u.as<double>(), m, vt.as<double>(), k_, e.as<double>(),
+                                            rocblas_outofplace, info.as<rocblas_int>()));
+    } else {
+        DMRG_ROCBLAS_CHECK(rocsolver_zgesvd(h, rocblas_svect_singular, rocblas_svect_singular, m, n,
+                                            a.as<rocblas_double_complex>(), m, sv.as<double>(), u.as<rocblas_double_complex>(), m,
+                                            vt.as<rocblas_double_complex>(), k_, e.as<double>(), rocblas_outofplace,
+                                            info.as<rocblas_int>()));
+    }
+    S.resize(k_);
+    rocblas_int inf = 0;
+    DMRG_HIP_CHECK(hipMemcpyAsync(S.data(), sv.p, k_ * sizeof(double), hipMemcpyDeviceToHost, s));
+    DMRG_HIP_CHECK(hipMemcpyAsync(&inf, info.p, sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
+    DMRG_HIP_CHECK(hipStreamSynchronize(s));
+    DMRG_REQUIRE(inf == 0, DMRG_ERR_NOT_CONVERGED, "gesvd: bidiagonal QR did not converge");
+}
+template void DeviceSvd::compute<double>(int, int, const double *, std::vector<double> &);
+template void DeviceSvd::compute<cpx<double>>(int, int, const cpx<double> *, std::vector<double> &);
+
+std::pair<int, double> rank_from_truncation_error(const std::vector<double> &S, double lim, int rank_max, int rank_min) {
+    const int           k = static_cast<int>(S.size());
+    std::vector<double> terr(k + 1, 0.0);
+    double              acc = 0.0;
+    for(int i = k - 1; i >= 0; --i) {
+        acc += S[i] * S[i];
+        terr[i] = std::sqrt(acc);
+    }
+    terr[k]  = 0.0;
+    int rank = 0;
+    for(int i = 0; i <= k; ++i)
+        if(terr[i] >= lim) ++rank;
+    int rank_lim = k;
+    if(rank_max > 0) rank_lim = std::min(k, rank_max);
+    rank = std::min(rank, rank_lim);
+    if(rank_min > 0) rank = std::max(rank, std::min(k, rank_min));
+    DMRG_REQUIRE(rank > 0, DMRG_ERR_ARG, "rank <= 0");
+    return {rank, terr[rank]};
+}
+
+template <typename E>
+__global__ void remap2d_kernel(int64_t M, int64_t N, const E *in, Idx2 im, Idx2 in_, E *out, Idx2 om, Idx2 on,
+                               const double *scale, int rs) {
+    const int64_t total = M * N;
+    for(int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < total; t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const uint32_t i = static_cast<uint32_t>(t % M), j = static_cast<uint32_t>(t / M);
+        E              v = in[idx_off(im, i) + idx_off(in_, j)];
+        if(rs) {
+            const double f = scale[rs == 1 ? i : j];
+            if constexpr(std::is_same_v<E, cpx<double>>) {
+                v.re *= f;
+                v.im *= f;
+            } else {
+                v *= f;
+            }
+        }
+        out[idx_off(om, i) + idx_off(on, j)] = v;
+    }
+}
+
+void remap2d(dmrg_dtype t, int64_t M, int64_t N, const void *in, Idx2 im, Idx2 in_, void *out, Idx2 om, Idx2 on,
+             const double *d_scale, int rs, hipStream_t s) {
+    if(M <= 0 || N <= 0) return;
+    const int64_t total = M * N;
+    const dim3    g(static_cast<unsigned>(std::min<int64_t>((total + 255) / 256, 8192))), b(256);
+    if(t == DMRG_FP64)
+        hipLaunchKernelGGL(remap2d_kernel<double>, g, b, 0, s, M, N, static_cast<const double *>(in), im, in_,
+                           static_cast<double *>(out), om, on, d_scale, rs);
+    else if(t == DMRG_CX64)
+        hipLaunchKernelGGL(remap2d_kernel<cpx<double>>, g, b, 0, s, M, N, static_cast<const cpx<double> *>(in), im, in_,
+                           static_cast<cpx<double> *>(out), om, on, d_scale, rs);
+    else
+        throw Error(DMRG_ERR_UNSUPPORTED, "remap2d: FP64/CX64 only");
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+} // namespace dmrg

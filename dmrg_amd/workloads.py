@@ -51,3 +51,9 @@ def heff_macs(d: int, chiL: int, chiR: int, DL: int, DR: int) -> float:
 
 def heff_flops(d: int, chiL: int, chiR: int, DL: int, DR: int, complex_: bool) -> float:
     return heff_macs(d, chiL, chiR, DL, DR) * (8.0 if complex_ else 2.0)
+
+
+def ising_fields(L: int, w: float, seed: int) -> np.ndarray:
+    """Random fields h_i ~ Uniform[-w, w] of the ising_tf_rf model (h_mean = 0, h_wdth = 2w,
+    IsingRandomField.cpp:143-144), drawn from numpy's PCG64 with the given seed and passed explicitly."""
+    return np.random.default_rng(seed).uniform(-w, w, L)

@@ -180,6 +180,48 @@ void        dmrg_eigs_default_config(dmrg_eigs_config *cfg);
 dmrg_status dmrg_eigs(dmrg_matvec *op, const dmrg_eigs_config *cfg, const void *x0, void *evecs, dmrg_eigs_result *res,
                       void *stream);
 
+/* ---------------------------------------------------------------- finite DMRG sweep (ising_tf_rf) */
+
+#define DMRG_FDMRG_MAX_L 256
+/* Ground state of H = -sum J sz_i sz_{i+1} - sum (h_tran + h_rand[i]) sx_i on an open chain (ising_tf_rf,
+ * IsingRandomField.cpp:93-136; the random field sits on the transverse sx term, J = 0 on the last site).
+ * Fields are passed explicitly (h_rand[L]). */
+typedef struct dmrg_fdmrg_config {
+    int           L;
+    double        J;               /* J1 */
+    double        h_tran;          /* g */
+    const double *h_rand;          /* host, L entries */
+    int           chi_max;         /* bond_max */
+    int           max_sweeps;      /* single-site sweeps after the two-site phase (default 20) */
+    int           two_site_sweeps; /* two-site sweeps that grow the bonds (default 10, stop early on |dE| < 1e-12) */
+    double        var_tol;         /* variance_convergence_threshold (settings.h:275), default 1e-12 */
+    double        trunc_lim;       /* svd_truncation_min (settings.h:265), default 1e-14 */
+    double        eigs_tol;        /* eigensolver tolerance, default 1e-12 */
+    int           eigs_ncv;        /* 0: basis size rule */
+    int           mixed;           /* 1: FP32 matrix-core eigensolves with FP64 refinement */
+    int           complex_;        /* 1: complex128 tensors (the model is real; for testing the complex path) */
+    uint64_t      seed;            /* initial random product state */
+} dmrg_fdmrg_config;
+
+typedef struct dmrg_fdmrg_result {
+    double  energy;
+    double  variance;
+    double  entropies[DMRG_FDMRG_MAX_L + 1]; /* von Neumann entropy per bond 0..L (measure.cpp:134-172) */
+    int64_t chi[DMRG_FDMRG_MAX_L + 1];
+    int     sweeps;
+    int     eigensolves;
+    int     converged;                       /* variance < var_tol */
+    int64_t matvecs;
+    int64_t matvecs_fp32;
+    double  seconds;
+    double  t_eigs, t_env, t_svd;
+    double  trunc_error_max;                 /* max over bonds of |S[discarded]|_2 (svd.cpp:257-280) */
+} dmrg_fdmrg_result;
+
+void        dmrg_fdmrg_default_config(dmrg_fdmrg_config *cfg);
+/* Replaces fdmrg::run_algorithm (fdmrg.cpp:170-195) for ising_tf_rf: sweeps with all tensors on the GPU. */
+dmrg_status dmrg_fdmrg_ising_tf_rf(const dmrg_fdmrg_config *cfg, dmrg_fdmrg_result *res, void *stream);
+
 /* Host dense Hermitian eigensolver used for the eigensolver's Rayleigh-Ritz step (the role LAPACK ?heevd
  * plays inside PRIMME). A, Z: m x m column-major complex interleaved; w ascending. */
 dmrg_status dmrg_host_heev(int m, const double *A, double *w, double *Z);

@@ -102,3 +102,18 @@ def test_large_residual_with_oracle(mixed):
     assert np.linalg.norm(hx - w[0] * x) <= 1e-11 * info["anorm"], info
     assert info["nconv"] == 1
     op.close()
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7])
+@pytest.mark.parametrize("ncv", [0, 2, 3])
+def test_tiny_problems_basis_limits(n, ncv):
+    """Regression: tiny problems where the restart would fill the basis (n=4 gives a 2-vector basis by the
+    size rule) must stay within the basis and still converge."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    op = dense_operator(A)
+    w, v, info = op.eigs(nev=1, tol=1e-13, ncv=ncv)
+    wd = np.linalg.eigvalsh(A)
+    assert abs(w[0] - wd[0]) <= 1e-10 * np.abs(wd).max(), (w, wd, info)
+    op.close()
