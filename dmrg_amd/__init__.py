@@ -176,6 +176,28 @@ def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_m
     return out
 
 
+def serve(requests, workers: int = 4, hbm_budget_bytes: int = 64 << 30, mixed: bool = False, complex_: bool = False,
+          var_tol: float = 1e-12, eigs_tol: float = 1e-12, max_sweeps: int = 20, two_site_sweeps: int = 10,
+          J: float = 1.0, h_tran: float = 1.0):
+    """Run independent ising_tf_rf ground-state requests as they arrive. requests: list of dicts with keys
+    L, chi_max, h_rand, arrival_s, seed. Returns (per-request results, stats)."""
+    n = len(requests)
+    fields = [np.ascontiguousarray(np.asarray(r["h_rand"], dtype=np.float64)) for r in requests]
+    arr = (_lib.Request * max(n, 1))()
+    for i, r in enumerate(requests):
+        arr[i].L, arr[i].chi_max = int(r["L"]), int(r["chi_max"])
+        arr[i].h_rand = fields[i].ctypes.data_as(C.POINTER(C.c_double))
+        arr[i].arrival_s, arr[i].seed = float(r.get("arrival_s", 0.0)), int(r.get("seed", 0))
+    cfg = _lib.ServeConfig(workers, hbm_budget_bytes, int(mixed), int(complex_), var_tol, eigs_tol, max_sweeps,
+                           two_site_sweeps, J, h_tran)
+    out = (_lib.RequestResult * max(n, 1))()
+    st = _lib.ServeStats()
+    call("dmrg_serve", C.byref(cfg), arr, n, out, C.byref(st))
+    res = [{k: getattr(out[i], k) for k, _ in out[i]._fields_} for i in range(n)]
+    stats = {k: getattr(st, k) for k, _ in st._fields_}
+    return res, stats
+
+
 class MatVecMPOS:
     """Device-resident effective Hamiltonian (MatVecMPOS<T>, matvec_mpos.h:23-180).
 

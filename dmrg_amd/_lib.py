@@ -103,6 +103,32 @@ class FdmrgResult(C.Structure):
                 ("t_env", C.c_double), ("t_svd", C.c_double), ("trunc_error_max", C.c_double)]
 
 
+class Request(C.Structure):
+    _fields_ = [("L", C.c_int), ("chi_max", C.c_int), ("h_rand", C.POINTER(C.c_double)), ("arrival_s", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("energy", C.c_double), ("variance", C.c_double), ("entropy_mid", C.c_double), ("arrival_s", C.c_double),
+                ("start_s", C.c_double), ("finish_s", C.c_double), ("latency_s", C.c_double), ("worker", C.c_int),
+                ("status", C.c_int), ("sweeps", C.c_int), ("matvecs", C.c_int64), ("footprint_bytes", C.c_int64)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("workers", C.c_int), ("hbm_budget_bytes", C.c_int64), ("mixed", C.c_int), ("complex_", C.c_int),
+                ("var_tol", C.c_double), ("eigs_tol", C.c_double), ("max_sweeps", C.c_int), ("two_site_sweeps", C.c_int),
+                ("J", C.c_double), ("h_tran", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("makespan_s", C.c_double), ("solves_per_s", C.c_double), ("p50_latency_s", C.c_double),
+                ("p99_latency_s", C.c_double), ("mean_latency_s", C.c_double), ("completed", C.c_int), ("failed", C.c_int),
+                ("peak_admitted_bytes", C.c_int64), ("peak_concurrency", C.c_int), ("matvecs", C.c_int64)]
+
+
+SIGNATURES["dmrg_serve"] = (st, [C.POINTER(ServeConfig), C.POINTER(Request), C.c_int, C.POINTER(RequestResult),
+                                 C.POINTER(ServeStats)])
+SIGNATURES["dmrg_request_footprint"] = (i64, [C.c_int, C.c_int, C.c_int])
 SIGNATURES["dmrg_fdmrg_default_config"] = (None, [C.POINTER(FdmrgConfig)])
 SIGNATURES["dmrg_fdmrg_ising_tf_rf"] = (st, [C.POINTER(FdmrgConfig), C.POINTER(FdmrgResult), vp])
 SIGNATURES["dmrg_eigs_default_config"] = (None, [C.POINTER(EigsConfig)])

@@ -44,6 +44,21 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+CPP_DRIVER_SRC = PKG.parent / "tests" / "cpp" / "heff_cpp_driver.cpp"
+CPP_DRIVER_BIN = PKG.parent / "tests" / "cpp" / "bin" / "heff_cpp_driver"
+
+
+def build_cpp_driver() -> Path:
+    """Host-only C++ program exercising include/dmrg_amd/matvec_mpos.hpp against the built library."""
+    CPP_DRIVER_BIN.parent.mkdir(parents=True, exist_ok=True)
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{INCLUDE}", str(CPP_DRIVER_SRC), f"-L{PKG}", "-ldmrg_amd",
+           "-Wl,-rpath,$ORIGIN/../../../dmrg_amd", "-o", str(CPP_DRIVER_BIN)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ driver build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return CPP_DRIVER_BIN
+
+
 def build(verbose: bool = True, jobs: int | None = None) -> Path:
     """Compile every HIP source for gfx950 and link libdmrg_amd.so. Always rebuilds."""
     if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
@@ -59,6 +74,8 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    if CPP_DRIVER_SRC.exists():
+        build_cpp_driver()
     if verbose:
         print(f"[dmrg_amd] built {LIB} from {len(srcs)} sources", file=sys.stderr)
     return LIB

@@ -399,6 +399,15 @@ dmrg_status dmrg_fdmrg_ising_tf_rf(const dmrg_fdmrg_config *cfg, dmrg_fdmrg_resu
     });
 }
 
+dmrg_status dmrg_serve(const dmrg_serve_config *cfg, const dmrg_request *reqs, int nreq, dmrg_request_result *out,
+                       dmrg_serve_stats *stats) {
+    return guard([&] {
+        DMRG_REQUIRE(cfg && (nreq == 0 || (reqs && out)) && stats && nreq >= 0, DMRG_ERR_ARG, "null argument");
+        serve(*cfg, reqs, nreq, out, *stats);
+    });
+}
+int64_t dmrg_request_footprint(int L, int chi_max, int complex_) { return request_footprint(L, chi_max, complex_ != 0); }
+
 dmrg_status dmrg_host_heev(int m, const double *A, double *w, double *Z) {
     return guard([&] {
         DMRG_REQUIRE(m >= 0 && (m == 0 || (A && w && Z)), DMRG_ERR_ARG, "bad argument");

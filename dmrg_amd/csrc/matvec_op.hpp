@@ -2,6 +2,7 @@
 #pragma once
 #include "contract.hpp"
 
+#include <algorithm>
 #include <complex>
 #include <vector>
 
@@ -25,17 +26,20 @@ struct DevBuf {
         if(p) (void) hipFree(p);
         p = nullptr, bytes = 0;
     }
+    // Grow-only; growth is geometric (x1.5) so that buffers that follow a growing bond dimension are
+    // reallocated O(log chi) times (hipFree synchronises the device).
     void *reserve(size_t b) {
         if(b > bytes) {
-            // Growing: queued kernels may still read the old buffer.
+            const size_t nb = std::max(b, bytes + bytes / 2);
+            // Queued kernels may still read the old buffer.
             if(p) (void) hipDeviceSynchronize();
             release();
-            if(hipMalloc(&p, b) != hipSuccess) {
+            if(hipMalloc(&p, nb) != hipSuccess) {
                 (void) hipGetLastError();
                 p = nullptr;
-                throw Error(DMRG_ERR_OOM, "device allocation of " + std::to_string(b) + " bytes failed");
+                throw Error(DMRG_ERR_OOM, "device allocation of " + std::to_string(nb) + " bytes failed");
             }
-            bytes = b;
+            bytes = nb;
         }
         return p;
     }

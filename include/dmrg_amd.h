@@ -222,6 +222,56 @@ void        dmrg_fdmrg_default_config(dmrg_fdmrg_config *cfg);
 /* Replaces fdmrg::run_algorithm (fdmrg.cpp:170-195) for ising_tf_rf: sweeps with all tensors on the GPU. */
 dmrg_status dmrg_fdmrg_ising_tf_rf(const dmrg_fdmrg_config *cfg, dmrg_fdmrg_result *res, void *stream);
 
+/* ---------------------------------------------------------------- serving: many independent solves on one GPU */
+
+typedef struct dmrg_request {
+    int           L;
+    int           chi_max;
+    const double *h_rand;    /* host, L fields of this disorder realisation */
+    double        arrival_s; /* arrival time relative to the start of dmrg_serve */
+    uint64_t      seed;      /* initial state seed */
+} dmrg_request;
+
+typedef struct dmrg_request_result {
+    double  energy;
+    double  variance;
+    double  entropy_mid;
+    double  arrival_s, start_s, finish_s, latency_s; /* latency = finish - arrival */
+    int     worker;
+    int     status; /* DMRG_OK, DMRG_ERR_NOT_CONVERGED (variance above var_tol) or an error */
+    int     sweeps;
+    int64_t matvecs;
+    int64_t footprint_bytes;
+} dmrg_request_result;
+
+typedef struct dmrg_serve_config {
+    int     workers;          /* concurrent requests (one HIP stream each) */
+    int64_t hbm_budget_bytes; /* admission limit on the summed HBM footprint of running requests */
+    int     mixed;
+    int     complex_;
+    double  var_tol, eigs_tol;
+    int     max_sweeps, two_site_sweeps;
+    double  J, h_tran;
+} dmrg_serve_config;
+
+typedef struct dmrg_serve_stats {
+    double  makespan_s;
+    double  solves_per_s;
+    double  p50_latency_s, p99_latency_s, mean_latency_s;
+    int     completed, failed;
+    int64_t peak_admitted_bytes;
+    int     peak_concurrency;
+    int64_t matvecs;
+} dmrg_serve_stats;
+
+/* Runs the requests as they arrive (real time), admitting each when a worker and HBM budget are free, and
+ * fills out[nreq] and stats. The reference's counterpart is one process per disorder seed
+ * (slurm/batches_fdmrg.py:8-27); here the solves share one GPU. */
+dmrg_status dmrg_serve(const dmrg_serve_config *cfg, const dmrg_request *reqs, int nreq, dmrg_request_result *out,
+                       dmrg_serve_stats *stats);
+/* Estimated HBM footprint (bytes) that admission charges for a request. */
+int64_t     dmrg_request_footprint(int L, int chi_max, int complex_);
+
 /* Host dense Hermitian eigensolver used for the eigensolver's Rayleigh-Ritz step (the role LAPACK ?heevd
  * plays inside PRIMME). A, Z: m x m column-major complex interleaved; w ascending. */
 dmrg_status dmrg_host_heev(int m, const double *A, double *w, double *Z);

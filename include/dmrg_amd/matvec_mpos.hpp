@@ -1,0 +1,140 @@
+// C++ drop-in for the reference's H_eff operator and contraction API, forwarding to the C ABI of
+// libdmrg_amd (include/dmrg_amd.h). Header-only; link with -ldmrg_amd.
+//
+//   dmrg_amd::MatVecMPOS<T>     replaces MatVecMPOS<T> (source/math/eig/matvec/matvec_mpos.h:23-180):
+//                               same constructor inputs (MPO list + env blocks in cx64 storage), rows(),
+//                               get_shape_mps(), MultAx(T*, T*), the PRIMME callback
+//                               MultAx(void*, int*, void*, int*, int*, primme_params*, int*), set_shift(),
+//                               num_mv. Env blocks and MPOs live in HBM for the operator's lifetime.
+//   dmrg_amd::contraction::matrix_vector_product<Scalar>
+//                               replaces tools::common::contraction::matrix_vector_product
+//                               (source/tools/common/contraction.h:31-36) with host pointers.
+//   dmrg_amd::contraction::expectation_value<Scalar> (contraction_tblis.cpp:30-97).
+#pragma once
+#include "../dmrg_amd.h"
+
+#include <array>
+#include <complex>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+struct primme_params; // PRIMME's parameter block: only passed through the callback signature
+
+namespace dmrg_amd {
+
+class error : public std::runtime_error {
+  public:
+    dmrg_status status;
+    error(dmrg_status s, const std::string &what) : std::runtime_error(what), status(s) {}
+};
+
+inline void check(dmrg_status s, const char *where) {
+    if(s != DMRG_OK) throw error(s, std::string(where) + ": " + dmrg_last_error());
+}
+
+template <typename T>
+constexpr dmrg_dtype dtype_of() {
+    static_assert(std::is_same_v<T, double> || std::is_same_v<T, std::complex<double>>, "fp64 or cx64");
+    return std::is_same_v<T, double> ? DMRG_FP64 : DMRG_CX64;
+}
+
+template <typename T>
+class MatVecMPOS {
+  public:
+    using Scalar = T;
+
+    // mpos: host cx64 MPO tensors (DL, DR, d, d); envs: host cx64 blocks (chi, chi, D), as EnvBase stores them.
+    MatVecMPOS(const std::vector<const std::complex<double> *> &mpos, const std::vector<std::array<long, 4>> &mpo_dims,
+               const std::complex<double> *envL, std::array<long, 3> envL_dims, const std::complex<double> *envR,
+               std::array<long, 3> envR_dims) {
+        std::vector<const void *> p(mpos.begin(), mpos.end());
+        std::vector<int64_t>      d;
+        for(const auto &a : mpo_dims) d.insert(d.end(), a.begin(), a.end());
+        const int64_t dl[3] = {envL_dims[0], envL_dims[1], envL_dims[2]};
+        const int64_t dr[3] = {envR_dims[0], envR_dims[1], envR_dims[2]};
+        check(dmrg_matvec_create(&op_, dtype_of<T>(), static_cast<int>(p.size()), p.data(), d.data(), envL, dl, envR, dr),
+              "MatVecMPOS");
+    }
+    ~MatVecMPOS() {
+        if(op_) dmrg_matvec_destroy(op_);
+    }
+    MatVecMPOS(const MatVecMPOS &)            = delete;
+    MatVecMPOS &operator=(const MatVecMPOS &) = delete;
+
+    long rows() const { return static_cast<long>(dmrg_matvec_rows(op_)); }
+    long cols() const { return rows(); }
+    std::array<long, 3> get_shape_mps() const {
+        int64_t s[3];
+        check(dmrg_matvec_shape_mps(op_, s), "get_shape_mps");
+        return {static_cast<long>(s[0]), static_cast<long>(s[1]), static_cast<long>(s[2])};
+    }
+    long num_mv() const { return static_cast<long>(dmrg_matvec_num_mv(op_)); }
+
+    // MultAx(T*, T*) (matvec_mpos.cpp:963-974): one vector, host buffers of rows() elements.
+    void MultAx(T *mps_in, T *mps_out) { check(dmrg_matvec_multax(op_, mps_in, rows(), mps_out, rows(), 1), "MultAx"); }
+
+    // PRIMME matrixMatvec callback (matvec_mpos.cpp:976-984): blockSize columns at strides ldx / ldy.
+    // Errors are reported through *err (non-zero makes PRIMME stop), never by an exception.
+    void MultAx(void *x, int *ldx, void *y, int *ldy, int *blockSize, primme_params *, int *err) {
+        const dmrg_status s = dmrg_matvec_multax(op_, x, *ldx, y, *ldy, *blockSize);
+        *err                = (s == DMRG_OK) ? 0 : -1;
+    }
+
+    // set_shift(sigma) (matvec_mpos.cpp:1433-1472).
+    void set_shift(std::complex<double> shift) { check(dmrg_matvec_set_shift(op_, shift.real(), shift.imag()), "set_shift"); }
+
+    dmrg_matvec *handle() { return op_; }
+
+  private:
+    dmrg_matvec *op_ = nullptr;
+};
+
+namespace contraction {
+
+template <typename Scalar>
+struct DeviceCopy {
+    void  *p     = nullptr;
+    size_t bytes = 0;
+    DeviceCopy(const Scalar *host, size_t n) : bytes(n * sizeof(Scalar)) {
+        check(dmrg_malloc(&p, bytes), "dmrg_malloc");
+        if(host) check(dmrg_memcpy_htod(p, host, bytes), "dmrg_memcpy_htod");
+    }
+    ~DeviceCopy() { dmrg_free(p); }
+};
+
+// Host-pointer contraction with the reference signature: copies in, contracts on the GPU, copies out.
+template <typename Scalar>
+void matrix_vector_product(Scalar *res, const Scalar *mps, std::array<long, 3> mps_dims, const Scalar *mpo,
+                           std::array<long, 4> mpo_dims, const Scalar *envL, std::array<long, 3> envL_dims,
+                           const Scalar *envR, std::array<long, 3> envR_dims) {
+    const int64_t ms[3] = {mps_dims[0], mps_dims[1], mps_dims[2]};
+    const int64_t wo[4] = {mpo_dims[0], mpo_dims[1], mpo_dims[2], mpo_dims[3]};
+    const int64_t el[3] = {envL_dims[0], envL_dims[1], envL_dims[2]};
+    const int64_t er[3] = {envR_dims[0], envR_dims[1], envR_dims[2]};
+    const size_t  n     = static_cast<size_t>(ms[0] * ms[1] * ms[2]);
+    DeviceCopy<Scalar> dm(mps, n), dw(mpo, static_cast<size_t>(wo[0] * wo[1] * wo[2] * wo[3])),
+        dl(envL, static_cast<size_t>(el[0] * el[1] * el[2])), dr(envR, static_cast<size_t>(er[0] * er[1] * er[2])), dres(nullptr, n);
+    check(dmrg_matrix_vector_product(dtype_of<Scalar>(), dres.p, dm.p, ms, dw.p, wo, dl.p, el, dr.p, er, nullptr),
+          "matrix_vector_product");
+    check(dmrg_memcpy_dtoh(res, dres.p, n * sizeof(Scalar)), "dmrg_memcpy_dtoh");
+}
+
+template <typename Scalar>
+Scalar expectation_value(const Scalar *mps, std::array<long, 3> mps_dims, const Scalar *mpo, std::array<long, 4> mpo_dims,
+                         const Scalar *envL, std::array<long, 3> envL_dims, const Scalar *envR, std::array<long, 3> envR_dims) {
+    const int64_t ms[3] = {mps_dims[0], mps_dims[1], mps_dims[2]};
+    const int64_t wo[4] = {mpo_dims[0], mpo_dims[1], mpo_dims[2], mpo_dims[3]};
+    const int64_t el[3] = {envL_dims[0], envL_dims[1], envL_dims[2]};
+    const int64_t er[3] = {envR_dims[0], envR_dims[1], envR_dims[2]};
+    DeviceCopy<Scalar> dm(mps, static_cast<size_t>(ms[0] * ms[1] * ms[2])), dw(mpo, static_cast<size_t>(wo[0] * wo[1] * wo[2] * wo[3])),
+        dl(envL, static_cast<size_t>(el[0] * el[1] * el[2])), dr(envR, static_cast<size_t>(er[0] * er[1] * er[2]));
+    double r[2];
+    check(dmrg_expectation_value(dtype_of<Scalar>(), dm.p, ms, dw.p, wo, dl.p, el, dr.p, er, r, nullptr), "expectation_value");
+    if constexpr(std::is_same_v<Scalar, double>) return r[0];
+    else return Scalar(r[0], r[1]);
+}
+
+} // namespace contraction
+} // namespace dmrg_amd

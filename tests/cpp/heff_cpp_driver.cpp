@@ -1,0 +1,82 @@
+// Drives the C++ drop-in (include/dmrg_amd/matvec_mpos.hpp) the way the reference's callers do:
+// PRIMME's matrixMatvec function pointer with the operator as context, MultAx(T*, T*), and the
+// free matrix_vector_product / expectation_value. Reads inputs written by tests/test_gpu_cpp_adaptor.py,
+// writes the results for comparison with the oracle.
+//
+// input : int64 header {cplx, d, chiL, chiR, DL, DR, nvec, ld}, then cx64 mpo, envL, envR, then x (ld x nvec, T)
+// output: y_primme (ld x nvec, T), y_single (n, T), y_free (n, T), expval (2 doubles), num_mv (int64)
+#include "dmrg_amd/matvec_mpos.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+using cx = std::complex<double>;
+
+static void *g_matrix = nullptr; // stands in for primme->matrix
+
+template <typename T>
+static void primme_matvec_trampoline(void *x, int *ldx, void *y, int *ldy, int *bs, primme_params *pp, int *err) {
+    static_cast<dmrg_amd::MatVecMPOS<T> *>(g_matrix)->MultAx(x, ldx, y, ldy, bs, pp, err);
+}
+
+template <typename T>
+static int run(FILE *in, FILE *out, const int64_t h[8]) {
+    const int64_t d = h[1], cL = h[2], cR = h[3], DL = h[4], DR = h[5], nvec = h[6], ld = h[7];
+    const int64_t n = d * cL * cR;
+    std::vector<cx> mpo(DL * DR * d * d), eL(cL * cL * DL), eR(cR * cR * DR);
+    std::vector<T>  x(ld * nvec), y(ld * nvec, T(0)), y1(n), y2(n);
+    if(fread(mpo.data(), sizeof(cx), mpo.size(), in) != mpo.size() || fread(eL.data(), sizeof(cx), eL.size(), in) != eL.size() ||
+       fread(eR.data(), sizeof(cx), eR.size(), in) != eR.size() || fread(x.data(), sizeof(T), x.size(), in) != x.size())
+        return 2;
+    dmrg_amd::MatVecMPOS<T> op({mpo.data()}, {{DL, DR, d, d}}, eL.data(), {cL, cL, DL}, eR.data(), {cR, cR, DR});
+    if(op.rows() != n) return 3;
+    g_matrix = &op;
+    void (*matvec)(void *, int *, void *, int *, int *, primme_params *, int *) = primme_matvec_trampoline<T>;
+    int ldx = static_cast<int>(ld), ldy = static_cast<int>(ld), bs = static_cast<int>(nvec), err = 1;
+    matvec(x.data(), &ldx, y.data(), &ldy, &bs, nullptr, &err);
+    if(err != 0) return 4;
+    op.MultAx(x.data(), y1.data());
+    // Free function on the same first vector, with the operator's own (possibly real-cast) tensors.
+    std::vector<T> mpo_t(mpo.size()), eL_t(eL.size()), eR_t(eR.size());
+    if constexpr(std::is_same_v<T, double>) {
+        for(size_t i = 0; i < mpo.size(); ++i) mpo_t[i] = mpo[i].real();
+        for(size_t i = 0; i < eL.size(); ++i) eL_t[i] = eL[i].real();
+        for(size_t i = 0; i < eR.size(); ++i) eR_t[i] = eR[i].real();
+    } else {
+        mpo_t = mpo, eL_t = eL, eR_t = eR;
+    }
+    dmrg_amd::contraction::matrix_vector_product<T>(y2.data(), x.data(), {d, cL, cR}, mpo_t.data(), {DL, DR, d, d}, eL_t.data(),
+                                                    {cL, cL, DL}, eR_t.data(), {cR, cR, DR});
+    const T ev = dmrg_amd::contraction::expectation_value<T>(x.data(), {d, cL, cR}, mpo_t.data(), {DL, DR, d, d}, eL_t.data(),
+                                                             {cL, cL, DL}, eR_t.data(), {cR, cR, DR});
+    const double  evr[2] = {std::real(cx(ev)), std::imag(cx(ev))};
+    const int64_t nmv    = op.num_mv();
+    fwrite(y.data(), sizeof(T), y.size(), out);
+    fwrite(y1.data(), sizeof(T), y1.size(), out);
+    fwrite(y2.data(), sizeof(T), y2.size(), out);
+    fwrite(evr, sizeof(double), 2, out);
+    fwrite(&nmv, sizeof(int64_t), 1, out);
+    // Error path: a dimension mismatch surfaces as dmrg_amd::error with DMRG_ERR_DIMS.
+    try {
+        dmrg_amd::contraction::matrix_vector_product<T>(y2.data(), x.data(), {d, cL + 1, cR}, mpo_t.data(), {DL, DR, d, d},
+                                                        eL_t.data(), {cL, cL, DL}, eR_t.data(), {cR, cR, DR});
+        return 5;
+    } catch(const dmrg_amd::error &e) {
+        if(e.status != DMRG_ERR_DIMS) return 6;
+    }
+    return 0;
+}
+
+int main(int argc, char **argv) {
+    if(argc != 3) return 1;
+    FILE *in = fopen(argv[1], "rb"), *out = fopen(argv[2], "wb");
+    if(!in || !out) return 1;
+    int64_t h[8];
+    if(fread(h, sizeof(int64_t), 8, in) != 8) return 2;
+    const int rc = h[0] ? run<cx>(in, out, h) : run<double>(in, out, h);
+    fclose(in);
+    fclose(out);
+    if(rc) fprintf(stderr, "heff_cpp_driver failed with code %d\n", rc);
+    return rc;
+}
