@@ -103,6 +103,161 @@ def load_traffic(name: str):
     return None
 
 
+def squared_problem(th, W, L, R):
+    """Exact H^2 of the isolated workload: MPO^2 of MpoSite.cpp:57-69 and squared env blocks
+    L2[:, :, w1 + D w2] = L[:, :, w1] L[:, :, w2] (R likewise), so that H2 theta = H (H theta)."""
+    import numpy as np
+
+    D = W.shape[0]
+    t = np.tensordot(W, W.conj(), axes=([3], [2])).transpose(0, 3, 1, 4, 2, 5)
+    W2 = np.asfortranarray(t.reshape(D * D, D * D, W.shape[2], W.shape[3], order="F"))
+    L2 = np.empty((L.shape[0], L.shape[0], D * D), dtype=L.dtype, order="F")
+    R2 = np.empty((R.shape[0], R.shape[0], D * D), dtype=R.dtype, order="F")
+    for a in range(D):
+        for b in range(D):
+            L2[:, :, a + D * b] = L[:, :, a] @ L[:, :, b]
+            R2[:, :, a + D * b] = R[:, :, a] @ R[:, :, b]
+    return W2, L2, R2
+
+
+def section_h2_matvec(chi=512, steps=30):
+    """Isolated H^2 matvec (D^2 = 25 squared MPO and squared env blocks), FP64 and FP32 matrix cores."""
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import heff_flops, random_heff_problem
+
+    th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=2)
+    W2, L2, R2 = squared_problem(th, W, L, R)
+    op = G.MatVecMPOS([W2], L2, R2, dtype=np.complex128)
+    n = op.rows()
+    out = {"chi": chi, "D2": 25, "gflop_per_matvec": round(heff_flops(2, chi, chi, 25, 25, True) / 1e9, 2)}
+    for fp32 in (False, True):
+        dt = np.complex64 if fp32 else np.complex128
+        dx = G.DeviceArray.from_numpy(th.reshape(n, order="F").astype(dt))
+        dy = G.DeviceArray((n,), dt)
+        for _ in range(3):
+            op.MultAx_device(dx, dy, fp32=fp32)
+        G._lib.call("dmrg_synchronize")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            op.MultAx_device(dx, dy, fp32=fp32)
+        G._lib.call("dmrg_synchronize")
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        tf = heff_flops(2, chi, chi, 25, 25, True) / (ms * 1e-3) / 1e12
+        key = "fp32" if fp32 else "fp64"
+        out[key] = {"ms": round(ms, 4), "tflops": round(tf, 3),
+                    "frac_of_peak": round(tf / (PEAK_FP32_MFMA if fp32 else PEAK_FP64_MFMA), 4)}
+    op.close()
+    return out
+
+
+def section_eigs(chi=512, reps=6, tol=1e-10):
+    """Ground-state eigensolves of the isolated workload's H_eff (cold random starts), FP64 vs mixed
+    (FP32 matrix-core phase + FP64 refinement): solves/s, p50/p99 latency, |dE| between the modes."""
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import random_heff_problem
+
+    th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=1)
+    op = G.MatVecMPOS([W], L, R, dtype=np.complex128)
+    op.eigs(nev=1, tol=tol, seed=999)  # warm-up: first-use code loads and GEMM tuning
+    op.eigs(nev=1, tol=tol, seed=999, mixed=True)
+    res = {}
+    energies = {}
+    for mode in ("fp64", "mixed"):
+        lat, mv64, mv32, es, rel_r = [], [], [], [], []
+        t_all = time.perf_counter()
+        for k in range(reps):
+            t0 = time.perf_counter()
+            w, v, info = op.eigs(nev=1, tol=tol, mixed=(mode == "mixed"), seed=1000 + k)
+            lat.append(time.perf_counter() - t0)
+            mv64.append(info["matvecs_fp64"])
+            mv32.append(info["matvecs_fp32"])
+            es.append(w[0])
+            rel_r.append(info["rnorm"][0] / info["anorm"])
+        total = time.perf_counter() - t_all
+        energies[mode] = np.array(es)
+        res[mode] = {"solves_per_s": round(reps / total, 3), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                     "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2), "mean_matvecs_fp64": float(np.mean(mv64)),
+                     "mean_matvecs_fp32": float(np.mean(mv32)), "max_rel_residual": float(np.max(rel_r))}
+    op.close()
+    dE = float(np.max(np.abs(energies["mixed"] - energies["fp64"])))
+    spread = float(np.max(energies["fp64"]) - np.min(energies["fp64"]))
+    return {"workload": f"isolated H_eff chi={chi} cx64 (seed 1), {reps} cold starts per mode, tol {tol:g}*|H|",
+            "fp64": res["fp64"], "mixed": res["mixed"], "max_abs_dE_mixed_vs_fp64": dE,
+            "max_abs_dE_between_fp64_starts": spread, "E0": float(energies["fp64"][0])}
+
+
+def section_sweep(cpu=True):
+    """Full fDMRG sweep of ising_tf_rf: L=16, J=1, g=1, h_i ~ U[-1,1] (seed 0), chi_max=64, variance < 1e-12,
+    real FP64, on the GPU; the oracle's CPU sweep of the same algorithm timed on the host cores."""
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import ising_fields
+    from oracle import heff as O  # checker: closed-form energies and entropies
+
+    L = 16
+    h = ising_fields(L, 1.0, seed=0)
+    G.fdmrg_ising_tf_rf(L, h, chi_max=64)  # warm-up (code loads, tuning)
+    t0 = time.perf_counter()
+    r = G.fdmrg_ising_tf_rf(L, h, chi_max=64)
+    gpu_s = time.perf_counter() - t0
+    e0 = O.ising_free_fermion_ground_energy(1.0, 1.0, h)
+    S0 = O.ising_free_fermion_entropies(1.0, 1.0, h)
+    out = {"workload": "ising_tf_rf L=16 J=1 g=1 h~U[-1,1] seed 0, chi_max=64, single-site to var<1e-12, FP64 real",
+           "gpu_s": round(gpu_s, 3), "energy": r["energy"], "exact_energy": e0, "abs_dE_vs_exact": abs(r["energy"] - e0),
+           "variance": r["variance"], "max_entropy_err_vs_exact": float(np.max(np.abs(r["entropies"] - S0))),
+           "sweeps": r["sweeps"], "eigensolves": r["eigensolves"], "t_eigs_s": round(r["t_eigs"], 3),
+           "t_env_s": round(r["t_env"], 3), "t_svd_s": round(r["t_svd"], 3)}
+    if cpu:
+        from oracle import sweep as S  # cpu_baseline leg: the oracle sweep is the thing timed here
+
+        threads = None
+        try:
+            from threadpoolctl import threadpool_info
+
+            info = [i for i in threadpool_info() if i.get("user_api") == "blas"]
+            threads = int(info[0]["num_threads"]) if info else None
+        except Exception:
+            pass
+        t0 = time.perf_counter()
+        rc = S.fdmrg_ising(L, h, chi_max=64)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_reference_sweep"] = {"seconds": round(cpu_s, 2), "cores": threads or os.cpu_count(), "kind": "port",
+                                      "abs_dE_vs_exact": abs(rc["energy"] - e0), "gpu_speedup": round(cpu_s / gpu_s, 1)}
+    return out
+
+
+def section_serve(nreq=32, workers=8, rate=3.0, seed=4):
+    """Serving stream of independent ising_tf_rf realisations: L in {16,20,24,32}, chi_max in {64,128,256},
+    h_i ~ U[-w,w] with w in {0.5,2,6}, seeds 100.., Poisson arrivals at `rate` per second."""
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import ising_fields
+
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for i in range(nreq):
+        L = int(rng.choice([16, 20, 24, 32]))
+        chi = int(rng.choice([64, 128, 256]))
+        w = float(rng.choice([0.5, 2.0, 6.0]))
+        t += float(rng.exponential(1.0 / rate))
+        reqs.append({"L": L, "chi_max": chi, "h_rand": ising_fields(L, w, seed=100 + i), "arrival_s": t, "seed": 100 + i})
+    res, st = G.serve(reqs, workers=workers, hbm_budget_bytes=128 << 30)
+    return {"workload": f"{nreq} requests, L in {{16,20,24,32}}, chi_max in {{64,128,256}}, w in {{0.5,2,6}}, "
+                        f"Poisson arrivals {rate}/s, {workers} workers, FP64",
+            "solves_per_s": round(st["solves_per_s"], 3), "p50_latency_s": round(st["p50_latency_s"], 3),
+            "p99_latency_s": round(st["p99_latency_s"], 3), "mean_latency_s": round(st["mean_latency_s"], 3),
+            "completed": st["completed"], "failed": st["failed"], "makespan_s": round(st["makespan_s"], 2),
+            "peak_concurrency": st["peak_concurrency"], "peak_admitted_gb": round(st["peak_admitted_bytes"] / 2**30, 2),
+            "max_variance": max(r["variance"] for r in res)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +265,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--chi", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="matvec only (no eigensolve, sweep or serving sections)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the per-request "
@@ -154,6 +310,11 @@ def main():
                       "frac_of_fp32_peak": round(r32["tflops"] / PEAK_FP32_MFMA, 4)},
         "chi_sweep_tflops": sweep,
     }
+    if not args.quick:
+        out["h2_matvec"] = section_h2_matvec(args.chi)
+        out["eigs"] = section_eigs(args.chi)
+        out["sweep"] = section_sweep(cpu=not args.no_cpu_baseline)
+        out["serve"] = section_serve()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.chi)
     print(json.dumps(out), flush=True)

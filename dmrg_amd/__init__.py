@@ -156,7 +156,8 @@ def nrm2(x, stream=None) -> float:
 
 def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_max: int = 64, max_sweeps: int = 20,
                       two_site_sweeps: int = 10, var_tol: float = 1e-12, trunc_lim: float = 1e-14, eigs_tol: float = 1e-12,
-                      mixed: bool = False, complex_: bool = False, seed: int = 0, stream=None) -> dict:
+                      mixed: bool = False, complex_: bool = False, seed: int = 0, mixed_min_n: int = 0,
+                      stream=None) -> dict:
     """Finite DMRG ground state of the random transverse-field Ising chain on the GPU
     (fdmrg::run_algorithm with ising_tf_rf, fdmrg.cpp:170-376). Returns energies, variance, entropies, timings."""
     h = np.ascontiguousarray(np.asarray(h_rand, dtype=np.float64))
@@ -167,7 +168,7 @@ def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_m
     cfg.h_rand = h.ctypes.data_as(C.POINTER(C.c_double))
     cfg.chi_max, cfg.max_sweeps, cfg.two_site_sweeps = chi_max, max_sweeps, two_site_sweeps
     cfg.var_tol, cfg.trunc_lim, cfg.eigs_tol = var_tol, trunc_lim, eigs_tol
-    cfg.mixed, cfg.complex_, cfg.seed = int(mixed), int(complex_), seed
+    cfg.mixed, cfg.complex_, cfg.seed, cfg.mixed_min_n = int(mixed), int(complex_), seed, mixed_min_n
     res = _lib.FdmrgResult()
     call("dmrg_fdmrg_ising_tf_rf", C.byref(cfg), C.byref(res), stream)
     out = {k: getattr(res, k) for k, _ in res._fields_ if k not in ("entropies", "chi")}

@@ -93,7 +93,7 @@ class FdmrgConfig(C.Structure):
     _fields_ = [("L", C.c_int), ("J", C.c_double), ("h_tran", C.c_double), ("h_rand", C.POINTER(C.c_double)),
                 ("chi_max", C.c_int), ("max_sweeps", C.c_int), ("two_site_sweeps", C.c_int), ("var_tol", C.c_double),
                 ("trunc_lim", C.c_double), ("eigs_tol", C.c_double), ("eigs_ncv", C.c_int), ("mixed", C.c_int),
-                ("complex_", C.c_int), ("seed", C.c_uint64)]
+                ("complex_", C.c_int), ("seed", C.c_uint64), ("mixed_min_n", C.c_int64)]
 
 
 class FdmrgResult(C.Structure):

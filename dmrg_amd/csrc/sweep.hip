@@ -229,7 +229,8 @@ struct Fdmrg {
         double    ev = 0, rn = 0;
         EigsStats st;
         xbuf.reserve(static_cast<size_t>(n) * es());
-        if(cfg.mixed) {
+        const int64_t mixed_min = cfg.mixed_min_n > 0 ? cfg.mixed_min_n : 131072;
+        if(cfg.mixed && n >= mixed_min) {
             EigsStats s1;
             st = eigs_mixed<T, std::conditional_t<std::is_same_v<T, double>, float, cpx<float>>>(op, s, p, th, xbuf.as<T>(), &ev, &rn, &s1);
             res->matvecs_fp32 += s1.matvecs;

@@ -90,16 +90,21 @@ __global__ __launch_bounds__(VT_THREADS) void multidot_partial(int64_t n, int m,
     }
 }
 
-__global__ void fold_columns(int nblk, int m, const double *partial, double *out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if(j >= m) return;
-    double re = 0, im = 0;
-    for(int b = 0; b < nblk; ++b) {
-        re += partial[2 * (static_cast<int64_t>(b) * m + j)];
-        im += partial[2 * (static_cast<int64_t>(b) * m + j) + 1];
+// out[j] = sum_b partial[b][j] for j < m: one block per column, each thread a fixed strided subset of the
+// partials, then a fixed-order tree (deterministic).
+__global__ __launch_bounds__(VT_THREADS) void fold_columns(int nblk, int m, const double *partial, double *out) {
+    __shared__ double red[2][VT_THREADS / 64];
+    const int         j   = blockIdx.x;
+    double            acc[2] = {0.0, 0.0};
+    for(int b = threadIdx.x; b < nblk; b += VT_THREADS) {
+        acc[0] += partial[2 * (static_cast<int64_t>(b) * m + j)];
+        acc[1] += partial[2 * (static_cast<int64_t>(b) * m + j) + 1];
     }
-    out[2 * j]     = re;
-    out[2 * j + 1] = im;
+    bsum<2>(acc, red);
+    if(threadIdx.x == 0) {
+        out[2 * j]     = acc[0];
+        out[2 * j + 1] = acc[1];
+    }
 }
 
 template <typename T>
@@ -108,7 +113,7 @@ void multidot(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_o
     DMRG_REQUIRE(m <= VEC_MAX_M, DMRG_ERR_ARG, "multidot: too many columns");
     const int g = grid_for(n);
     hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, V, ldv, x, scratch);
-    hipLaunchKernelGGL(fold_columns, dim3((m + 63) / 64), dim3(64), 0, s, g, m, scratch, d_out);
+    hipLaunchKernelGGL(fold_columns, dim3(m), dim3(VT_THREADS), 0, s, g, m, scratch, d_out);
     DMRG_HIP_CHECK(hipGetLastError());
 }
 
@@ -182,7 +187,7 @@ void ritz_residual(int64_t n, int m, const T *V, const T *W, int64_t ld, const d
     DMRG_REQUIRE(m <= VEC_MAX_M, DMRG_ERR_ARG, "ritz: too many columns");
     const int g = grid_for(n);
     hipLaunchKernelGGL(ritz_kernel<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, V, W, ld, d_y, theta, x, r, scratch);
-    hipLaunchKernelGGL(fold_columns, dim3(1), dim3(64), 0, s, g, 1, scratch, d_out);
+    hipLaunchKernelGGL(fold_columns, dim3(1), dim3(VT_THREADS), 0, s, g, 1, scratch, d_out);
     DMRG_HIP_CHECK(hipGetLastError());
 }
 

@@ -36,7 +36,7 @@ struct vtraits<cpx<float>> {
 };
 
 // Scratch for reductions: partial sums of up to VEC_MAX_BLOCKS blocks x VEC_MAX_M columns (complex).
-constexpr int VEC_MAX_BLOCKS = 240;
+constexpr int VEC_MAX_BLOCKS = 2048; // enough waves in flight to stream the basis at HBM rate
 constexpr int VEC_MAX_M      = 256;
 constexpr int64_t vec_scratch_doubles() { return 2LL * VEC_MAX_BLOCKS * VEC_MAX_M + 4 * VEC_MAX_M; }
 

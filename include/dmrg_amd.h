@@ -198,9 +198,10 @@ typedef struct dmrg_fdmrg_config {
     double        trunc_lim;       /* svd_truncation_min (settings.h:265), default 1e-14 */
     double        eigs_tol;        /* eigensolver tolerance, default 1e-12 */
     int           eigs_ncv;        /* 0: basis size rule */
-    int           mixed;           /* 1: FP32 matrix-core eigensolves with FP64 refinement */
+    int           mixed;           /* 1: FP32 matrix-core eigensolves with FP64 refinement for problems of size >= mixed_min_n */
     int           complex_;        /* 1: complex128 tensors (the model is real; for testing the complex path) */
     uint64_t      seed;            /* initial random product state */
+    int64_t       mixed_min_n;     /* <= 0: 131072 (smaller warm-started solves converge faster in FP64 alone) */
 } dmrg_fdmrg_config;
 
 typedef struct dmrg_fdmrg_result {

@@ -41,7 +41,7 @@ def test_sweep_mixed_matches_fp64():
     L = 12
     h = ising_fields(L, 1.0, seed=3)
     r64 = G.fdmrg_ising_tf_rf(L, h, chi_max=32)
-    rmx = G.fdmrg_ising_tf_rf(L, h, chi_max=32, mixed=True)
+    rmx = G.fdmrg_ising_tf_rf(L, h, chi_max=32, mixed=True, mixed_min_n=1)
     assert abs(rmx["energy"] - r64["energy"]) <= 1e-10 * abs(r64["energy"])
     assert rmx["variance"] < 1e-12 and rmx["matvecs_fp32"] > 0
     np.testing.assert_allclose(rmx["entropies"], r64["entropies"], atol=1e-8)
