@@ -55,11 +55,12 @@ def gpu_matvec_bench(chi, steps, warmup, fp32=False, legs_nrep=50):
     G._lib.call("dmrg_synchronize")
     dt = time.perf_counter() - t0
     flops = heff_flops(d, chi, chi, D, D, True)
-    legs_ms = op.time_legs(dx, dy, legs_nrep, fp32=fp32)
+    legs_ms = op.time_legs(dx, dy, legs_nrep, fp32=fp32)  # [leg1, leg2, leg3 main, leg3 reduce]
     macs = leg_macs(d, chi, chi, D, D)
     op.close()
+    kernel_ms = legs_ms[:3]
     return {"ms_per_step": dt / steps * 1e3, "tflops": flops * steps / dt / 1e12, "legs_ms": legs_ms,
-            "legs_tflops": [8.0 * m / (t * 1e-3) / 1e12 for m, t in zip(macs, legs_ms)], "flops": flops}
+            "legs_tflops": [8.0 * m / (t * 1e-3) / 1e12 for m, t in zip(macs, kernel_ms)], "flops": flops}
 
 
 def cpu_baseline(chi=512, budget_s=12.0, max_reps=200):
@@ -266,7 +267,26 @@ def main():
     ap.add_argument("--chi", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="matvec only (no eigensolve, sweep or serving sections)")
+    ap.add_argument("--profile-matvec", action="store_true", help="only the chi=512 FP64 matvec steps, then exit")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="one 256 MiB device-to-device copy (scales the HBM byte counters of a profiler run), then exit")
     args = ap.parse_args()
+    if args.profile_matvec:
+        # Only the chi=512 FP64 matvec steps (for profiler passes); pair with DMRG_TUNE_CACHE to reuse plans.
+        r = gpu_matvec_bench(args.chi, args.steps, args.warmup, legs_nrep=1)
+        print(json.dumps({"profile_matvec_ms": round(r["ms_per_step"], 5), "legs_ms": r["legs_ms"]}))
+        return
+    if args.copy_calibration:
+        import dmrg_amd as G
+
+        nbytes = 256 << 20
+        a, b2 = G.DeviceArray((nbytes // 8,), "float64"), G.DeviceArray((nbytes // 8,), "float64")
+        a.zero()
+        G._lib.call("dmrg_synchronize")
+        G._lib.call("dmrg_memcpy_dtod", b2.ptr, a.ptr, nbytes)
+        G._lib.call("dmrg_synchronize")
+        print(json.dumps({"copy_bytes": nbytes}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the per-request "
               f"sharding across GPUs is described in DESIGN.md).", file=sys.stderr)
@@ -282,13 +302,16 @@ def main():
         if chi != args.chi:
             sweep[str(chi)] = round(gpu_matvec_bench(chi, 200, 10)["tflops"], 3)
 
+    # Dominant kernel: the longest single kernel launch (leg 1, leg 2, or leg 3's main kernel).
     i_dom = max(range(3), key=lambda i: r64["legs_ms"][i])
     macs = leg_macs(2, args.chi, args.chi, 5, 5)
     achieved = r64["legs_tflops"][i_dom]
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_MFMA, 4), "traffic": load_traffic(f"leg{i_dom + 1}_chi{args.chi}"),
                 "kernel": f"gemm leg {i_dom + 1} of 3 (complex FP64, {macs[i_dom]:.3e} MACs per launch)",
-                "legs_ms": [round(x, 4) for x in r64["legs_ms"]]}
+                "kernel_ms": round(r64["legs_ms"][i_dom], 4),
+                "legs_ms": {"leg1": round(r64["legs_ms"][0], 4), "leg2_mpo": round(r64["legs_ms"][1], 4),
+                            "leg3_main": round(r64["legs_ms"][2], 4), "leg3_splitk_reduce": round(r64["legs_ms"][3], 4)}}
     out = {
         "metric": METRIC,
         "value": round(r64["tflops"], 4),

@@ -279,8 +279,8 @@ class MatVecMPOS:
         return np.array(res.eigval[:nev]), vecs, info
 
     def time_legs(self, dx: DeviceArray, dy: DeviceArray, nrep: int, fp32: bool = False, stream=None):
-        """Average duration (ms) of each of the three contraction legs over nrep device matvecs."""
-        ms = (C.c_float * 3)()
+        """Average durations (ms) over nrep device matvecs: [leg 1, leg 2, leg 3 main kernel, leg 3 split-K reduce]."""
+        ms = (C.c_float * 4)()
         call("dmrg_matvec_time_legs", self.h, dx.ptr, dy.ptr, nrep, int(fp32), ms, stream)
         return [float(v) for v in ms]
 

@@ -430,7 +430,7 @@ dmrg_status dmrg_gemm_plan_override(int cfg, int ksplit) {
     });
 }
 
-dmrg_status dmrg_matvec_time_legs(dmrg_matvec *op, const void *dx, void *dy, int nrep, int fp32, float ms_per_leg[3],
+dmrg_status dmrg_matvec_time_legs(dmrg_matvec *op, const void *dx, void *dy, int nrep, int fp32, float ms_per_leg[4],
                                   void *stream) {
     return guard([&] {
         DMRG_REQUIRE(op && dx && dy && ms_per_leg && nrep > 0, DMRG_ERR_ARG, "bad argument");

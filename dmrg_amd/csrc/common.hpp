@@ -93,6 +93,8 @@ struct GemmDesc {
     int32_t xcd_remap = 1;
     int32_t ksplit    = 1; // number of K slices (grid y)
     int32_t kchunk    = 0; // K extent of one slice (multiple of 32)
+    // Profiling: if set, this hipEvent_t is recorded right after the main kernel (before a split-K reduce).
+    void *ev_after_main = nullptr;
 };
 
 // Tile configuration id (see gemm.hip for the table).

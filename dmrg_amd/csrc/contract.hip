@@ -88,8 +88,9 @@ void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *o
         g3.B = envR, g3.bk = fused(1, cR * cR, cR), g3.bn = plain(cR);
         g3.C = out, g3.cm = plain(1), g3.cn = plain(d * cL), g3.c_batch = ldy;
         g3.part = PT, g3.part_capacity = PC;
+        if(ev) g3.ev_after_main = ev[3];
         gemm(g3, s);
-        mark(3);
+        mark(4);
     } else {
         // Order of contraction_tblis.cpp:285-289 (theta.R first).
         // Leg 1: U[s,a,b',w'] = sum_b theta[s,a,b] R[b,b',w']            m=(s,a) k=b n=(b',w')
@@ -112,8 +113,9 @@ void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *o
         g3.B = T2, g3.bk = plain(1), g3.bn = plain(cL * DL), g3.b_batch = nT2;
         g3.C = out, g3.cm = plain(d), g3.cn = fused(d * cL, 1, cR), g3.c_batch = ldy;
         g3.part = PT, g3.part_capacity = PC;
+        if(ev) g3.ev_after_main = ev[3];
         gemm(g3, s);
-        mark(3);
+        mark(4);
     }
 }
 

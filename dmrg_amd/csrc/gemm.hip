@@ -472,6 +472,7 @@ static void launch_cfg(const GemmDesc &g, bool swap, hipStream_t s) {
         if(swap) hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, true, false>), grid, block, 0, s, g);
         else hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, false, false>), grid, block, 0, s, g);
     }
+    if(g.ev_after_main) DMRG_HIP_CHECK(hipEventRecord(static_cast<hipEvent_t>(g.ev_after_main), s));
     if(g.ksplit > 1) {
         const int64_t total = static_cast<int64_t>(g.M) * g.N * g.batch;
         const dim3    rg(static_cast<unsigned>(std::min<int64_t>((total + 255) / 256, 8192)));
@@ -591,6 +592,40 @@ static LaunchPlan tune(const GemmDesc &g, hipStream_t stream) {
     return best;
 }
 
+// Optional persistent cache: with DMRG_TUNE_CACHE=<file>, tuned plans are loaded at first use and each new
+// plan is appended, so that separate processes (e.g. profiler passes) launch the same kernels.
+static const char *tune_cache_path() {
+    static const char *p = std::getenv("DMRG_TUNE_CACHE");
+    return (p && *p) ? p : nullptr;
+}
+static void load_tune_cache_locked() {
+    static bool loaded = false;
+    if(loaded) return;
+    loaded = true;
+    const char *path = tune_cache_path();
+    if(!path) return;
+    FILE *f = std::fopen(path, "r");
+    if(!f) return;
+    char line[1024];
+    while(std::fgets(line, sizeof(line), f)) {
+        std::string l(line);
+        const size_t sp = l.rfind(' ');
+        if(sp == std::string::npos) continue;
+        int cfg = 0, ks = 1;
+        if(std::sscanf(l.c_str() + sp + 1, "%d:%d", &cfg, &ks) != 2) continue;
+        g_tuned[l.substr(0, sp)] = LaunchPlan{static_cast<TileCfg>(cfg), ks};
+    }
+    std::fclose(f);
+}
+static void append_tune_cache_locked(const std::string &key, const LaunchPlan &p) {
+    const char *path = tune_cache_path();
+    if(!path) return;
+    if(FILE *f = std::fopen(path, "a")) {
+        std::fprintf(f, "%s %d:%d\n", key.c_str(), static_cast<int>(p.cfg), p.ksplit);
+        std::fclose(f);
+    }
+}
+
 void set_autotune(int on) { g_autotune = on; }
 void set_plan_override(int cfg, int ks) {
     g_override_cfg = cfg;
@@ -629,6 +664,7 @@ void gemm(const GemmDesc &gin, hipStream_t stream, TileCfg cfg, int ksplit) {
         bool              found = false;
         {
             std::lock_guard<std::mutex> lock(g_tune_mu);
+            load_tune_cache_locked();
             auto                        it = g_tuned.find(key);
             if(it != g_tuned.end()) {
                 p     = it->second;
@@ -640,6 +676,7 @@ void gemm(const GemmDesc &gin, hipStream_t stream, TileCfg cfg, int ksplit) {
             if(can_tune) {
                 std::lock_guard<std::mutex> lock(g_tune_mu);
                 g_tuned[key] = p;
+                append_tune_cache_locked(key, p);
             }
         }
         if(ksplit > 0) p.ksplit = ksplit;

@@ -162,10 +162,10 @@ void MatVecOp::diagonal(void *out, hipStream_t s) {
     heff_diagonal(dtype, D, mpo.p, envL_p, envR_p, w, out, s);
 }
 
-void MatVecOp::time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms[3], hipStream_t s) {
-    hipEvent_t ev[4];
+void MatVecOp::time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms[4], hipStream_t s) {
+    hipEvent_t ev[5];
     for(auto &e : ev) DMRG_HIP_CHECK(hipEventCreate(&e));
-    double acc[3] = {0, 0, 0};
+    double acc[4] = {0, 0, 0, 0};
     try {
         if(fp32) ensure_fp32(s);
         const dmrg_dtype t  = fp32 ? dtype_single_of(dtype) : dtype;
@@ -174,8 +174,8 @@ void MatVecOp::time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms
         for(int r = 0; r < nrep; ++r) {
             matvec(t, D, dx, D.n(), dy, D.n(), 1, fp32 ? mpo32.p : mpo.p, fp32 ? envL32.p : envL_p,
                    fp32 ? envR32.p : envR_p, w, s, ev);
-            DMRG_HIP_CHECK(hipEventSynchronize(ev[3]));
-            for(int i = 0; i < 3; ++i) {
+            DMRG_HIP_CHECK(hipEventSynchronize(ev[4]));
+            for(int i = 0; i < 4; ++i) {
                 float t_ms = 0;
                 DMRG_HIP_CHECK(hipEventElapsedTime(&t_ms, ev[i], ev[i + 1]));
                 acc[i] += t_ms;
@@ -187,7 +187,7 @@ void MatVecOp::time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms
         throw;
     }
     for(auto &e : ev) (void) hipEventDestroy(e);
-    for(int i = 0; i < 3; ++i) ms[i] = static_cast<float>(acc[i] / nrep);
+    for(int i = 0; i < 4; ++i) ms[i] = static_cast<float>(acc[i] / nrep);
 }
 
 } // namespace dmrg

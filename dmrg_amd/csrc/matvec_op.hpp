@@ -84,7 +84,7 @@ struct MatVecOp {
     void multax_host(const void *x, int64_t ldx, void *y, int64_t ldy, int nvec);
     void set_shift(std::complex<double> shift);
     void diagonal(void *out, hipStream_t s);
-    void time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms[3], hipStream_t s);
+    void time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms[4], hipStream_t s);
 };
 
 } // namespace dmrg

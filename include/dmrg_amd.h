@@ -287,10 +287,11 @@ int         dmrg_autotune_entries(void);
  * tuned/heuristic choice. */
 dmrg_status dmrg_gemm_plan_override(int cfg, int ksplit);
 
-/* Profiling: runs nrep device matvecs (one vector, x -> y) on stream and returns the average duration of each
- * of the three contraction legs in ms_per_leg[0..2], measured with HIP events on that stream. No counterpart in
- * the reference (its per-operator timers t_multAx, matvec_mpos.h:174-179, time the whole call). */
-dmrg_status dmrg_matvec_time_legs(dmrg_matvec *op, const void *dx, void *dy, int nrep, int fp32, float ms_per_leg[3],
+/* Profiling: runs nrep device matvecs (one vector, x -> y) on stream and returns average durations measured with
+ * HIP events on that stream: ms[0] leg 1 kernel, ms[1] leg 2 (MPO) kernel, ms[2] leg 3 main kernel, ms[3] leg 3
+ * split-K reduce (0 when leg 3 is not split). No counterpart in the reference (its per-operator timers t_multAx,
+ * matvec_mpos.h:174-179, time the whole call). */
+dmrg_status dmrg_matvec_time_legs(dmrg_matvec *op, const void *dx, void *dy, int nrep, int fp32, float ms[4],
                                   void *stream);
 
 #ifdef __cplusplus
