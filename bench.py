@@ -259,6 +259,95 @@ def section_serve(nreq=32, workers=8, rate=3.0, seed=4):
             "max_variance": max(r["variance"] for r in res)}
 
 
+def section_two_site(chi=1024, D=8, steps=10, tol=1e-10):
+    """Two-site workload: theta (4, chi, chi) cx64, env blocks (chi, chi, 8) Hermitian per w, two random
+    range-6 site MPOs (8, 8, 2, 2) merged into (8, 8, 4, 4), seed 3. Matvec FP64 vs FP32 matrix cores, one
+    FP64 and one mixed ground-state solve from theta, then the truncated SVD of the solution as a
+    (2 chi) x (2 chi) matrix down to rank chi: |S[chi:]|_2 and sum S[chi:]^2, for each rocSOLVER routine."""
+    import ctypes as C
+
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import heff_flops, theta_as_matrix, two_site_problem
+
+    th, mpos, L, R = two_site_problem(chi, D, seed=3)
+    op = G.MatVecMPOS(mpos, L, R, dtype=np.complex128)
+    del L, R
+    n = op.rows()
+    flops = heff_flops(4, chi, chi, D, D, True)
+    out = {"workload": f"two-site theta (4,{chi},{chi}) cx64, env ({chi},{chi},{D}), merged MPO ({D},{D},4,4), seed 3",
+           "gflop_per_matvec": round(flops / 1e9, 1)}
+    for fp32 in (False, True):
+        dt = np.complex64 if fp32 else np.complex128
+        dx = G.DeviceArray.from_numpy(th.reshape(n, order="F").astype(dt))
+        dy = G.DeviceArray((n,), dt)
+        for _ in range(2):
+            op.MultAx_device(dx, dy, fp32=fp32)
+        G._lib.call("dmrg_synchronize")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            op.MultAx_device(dx, dy, fp32=fp32)
+        G._lib.call("dmrg_synchronize")
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        tf = flops / (ms * 1e-3) / 1e12
+        out["fp32" if fp32 else "fp64"] = {"ms": round(ms, 3), "tflops": round(tf, 3),
+                                           "frac_of_peak": round(tf / (PEAK_FP32_MFMA if fp32 else PEAK_FP64_MFMA), 4)}
+        dx.close()
+        dy.close()
+    sol = {}
+    vec = None
+    for mode in ("fp64", "mixed"):
+        t0 = time.perf_counter()
+        w, v, info = op.eigs(nev=1, tol=tol, x0=th, mixed=(mode == "mixed"))
+        sol[mode] = {"seconds": round(time.perf_counter() - t0, 3), "E0": float(w[0]),
+                     "matvecs_fp64": info["matvecs_fp64"], "matvecs_fp32": info["matvecs_fp32"],
+                     "rel_residual": info["rnorm"][0] / info["anorm"]}
+        if mode == "fp64":
+            vec = v[:, 0]
+    sol["abs_dE_mixed_vs_fp64"] = abs(sol["mixed"]["E0"] - sol["fp64"]["E0"])
+    out["eigs"] = sol
+    op.close()
+    M = theta_as_matrix(vec.reshape((4, chi, chi), order="F"))
+    m, k = M.shape[0], min(M.shape)
+    dM = G.DeviceArray.from_numpy(M)
+    dU, dV = G.DeviceArray((m, k), np.complex128), G.DeviceArray((k, M.shape[1]), np.complex128)
+    S = np.empty(k)
+    svd = {"matrix": f"{m}x{M.shape[1]} complex, keep rank {chi}"}
+    for method, name in ((0, "gesvd"), (1, "gesdd"), (2, "gesvdj")):
+        rank, terr, disc = C.c_int64(), C.c_double(), C.c_double()
+        t0 = time.perf_counter()
+        G._lib.call("dmrg_svd_truncate", G.DMRG_CX64, m, M.shape[1], dM.ptr, 1e-14, chi, method, dU.ptr,
+                    S.ctypes.data_as(C.POINTER(C.c_double)), dV.ptr, C.byref(rank), C.byref(terr), C.byref(disc), None)
+        svd[name] = {"seconds": round(time.perf_counter() - t0, 3), "rank": rank.value,
+                     "truncation_error": terr.value, "discarded_weight": disc.value, "S0": float(S[0])}
+    for a in (dM, dU, dV):
+        a.close()
+    out["svd"] = svd
+    return out
+
+
+def host_buffer_matvec(chi=512, reps=50):
+    """PCIe-inclusive rate: the PRIMME-callback form MultAx(x, y) with host buffers (x copied in, y out)."""
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import heff_flops, random_heff_problem
+
+    th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=1)
+    op = G.MatVecMPOS([W], L, R, dtype=np.complex128)
+    x = np.ascontiguousarray(th.reshape(-1, order="F"))
+    for _ in range(3):
+        op.MultAx(x)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        op.MultAx(x)
+    dt = (time.perf_counter() - t0) / reps
+    op.close()
+    return {"ms_per_matvec": round(dt * 1e3, 4), "tflops": round(heff_flops(2, chi, chi, 5, 5, True) / dt / 1e12, 3),
+            "note": "host x and y (8 MiB each) cross PCIe every call; not the headline value"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -267,10 +356,14 @@ def main():
     ap.add_argument("--chi", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="matvec only (no eigensolve, sweep or serving sections)")
+    ap.add_argument("--two-site", action="store_true", help="only the two-site chi=1024 section, then exit")
     ap.add_argument("--profile-matvec", action="store_true", help="only the chi=512 FP64 matvec steps, then exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="one 256 MiB device-to-device copy (scales the HBM byte counters of a profiler run), then exit")
     args = ap.parse_args()
+    if args.two_site:
+        print(json.dumps(section_two_site()))
+        return
     if args.profile_matvec:
         # Only the chi=512 FP64 matvec steps (for profiler passes); pair with DMRG_TUNE_CACHE to reuse plans.
         r = gpu_matvec_bench(args.chi, args.steps, args.warmup, legs_nrep=1)
@@ -332,12 +425,14 @@ def main():
         "fp32_mfma": {"tflops": round(r32["tflops"], 4), "ms_per_step": round(r32["ms_per_step"], 5),
                       "frac_of_fp32_peak": round(r32["tflops"] / PEAK_FP32_MFMA, 4)},
         "chi_sweep_tflops": sweep,
+        "host_buffer_matvec": host_buffer_matvec(args.chi),
     }
     if not args.quick:
         out["h2_matvec"] = section_h2_matvec(args.chi)
         out["eigs"] = section_eigs(args.chi)
         out["sweep"] = section_sweep(cpu=not args.no_cpu_baseline)
         out["serve"] = section_serve()
+        out["two_site_chi1024"] = section_two_site()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.chi)
     print(json.dumps(out), flush=True)

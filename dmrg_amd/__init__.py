@@ -18,7 +18,8 @@ from ._lib import DMRG_CX32, DMRG_CX64, DMRG_FP32, DMRG_FP64, DmrgError, call, l
 
 __all__ = [
     "DeviceArray", "MatVecMPOS", "matrix_vector_product", "env_build_block", "expectation_value",
-    "dot", "nrm2", "device_info", "DmrgError", "dtype_code",
+    "dot", "nrm2", "device_info", "DmrgError", "dtype_code", "heff_dense", "dense_eigh", "env_set_edge",
+    "svd_truncate",
 ]
 
 _CODE = {np.dtype(np.float64): DMRG_FP64, np.dtype(np.complex128): DMRG_CX64,
@@ -154,6 +155,70 @@ def nrm2(x, stream=None) -> float:
     return res.value
 
 
+def heff_dense(mpo, envL, envR, transpose: bool = False, stream=None) -> np.ndarray:
+    """Explicit H_eff (n x n): contract_mpo_env (TensorsFinite.cpp:119-142) so that H @ theta.ravel('F') is the
+    matvec; transpose=True gives MatVecMPO::get_matrix's orientation (matvec_mpo.cpp:313-332)."""
+    w, l, r = _dev(mpo), _dev(envL), _dev(envR)
+    n = w.shape[2] * l.shape[0] * r.shape[0]
+    out = DeviceArray((n, n), w.dtype)
+    try:
+        call("dmrg_heff_dense", w.code, out.ptr, n, w.ptr, _dims(w.shape), l.ptr, _dims(l.shape), r.ptr,
+             _dims(r.shape), int(transpose), stream)
+        return out.to_numpy()
+    finally:
+        out.close()
+
+
+def dense_eigh(A, vectors: bool = True, stream=None):
+    """All eigenpairs of a Hermitian matrix on the GPU (rocSOLVER syevd/heevd; the reference's dense
+    eig::solver::eig branch, solver.cpp:205-245). Returns (w ascending, V or None)."""
+    a = DeviceArray.from_numpy(np.asarray(A))
+    n = a.shape[0]
+    w = np.empty(n, dtype=np.float64)
+    try:
+        call("dmrg_dense_eigh", a.code, n, a.ptr, n, w.ctypes.data_as(C.POINTER(C.c_double)), int(vectors), stream)
+        return w, (a.to_numpy() if vectors else None)
+    finally:
+        a.close()
+
+
+def env_set_edge(chi: int, edge, stream=None) -> np.ndarray:
+    """EnvBase::set_edge_dims (EnvBase.cpp:254-285): block (chi, chi, D) with block[i, i, :] = edge."""
+    e = np.ascontiguousarray(edge)
+    out = DeviceArray((chi, chi, e.shape[0]), e.dtype)
+    try:
+        call("dmrg_env_set_edge", out.code, out.ptr, chi, e.shape[0], e.ctypes.data_as(C.c_void_p), stream)
+        return out.to_numpy()
+    finally:
+        out.close()
+
+
+def svd_truncate(A, trunc_lim: float = 1e-14, rank_max: int = 0, method: int = 0, stream=None):
+    """Truncated SVD on the GPU (svd::solver, svd_lapacke.cpp:82-330; rank rule svd.cpp:257-280).
+
+    Returns dict(U (m x r), S (all singular values), Vt (r x n), rank, truncation_error = |S[r:]|_2,
+    discarded_weight = sum S[r:]^2). A may be a numpy matrix or a DeviceArray."""
+    a = _dev(A)
+    m, n = a.shape
+    k = min(m, n)
+    U, Vt = DeviceArray((m, k), a.dtype), DeviceArray((k, n), a.dtype)
+    S = np.empty(k, dtype=np.float64)
+    rank, terr, disc = C.c_int64(), C.c_double(), C.c_double()
+    try:
+        call("dmrg_svd_truncate", a.code, m, n, a.ptr, trunc_lim, rank_max, method, U.ptr,
+             S.ctypes.data_as(C.POINTER(C.c_double)), Vt.ptr, C.byref(rank), C.byref(terr), C.byref(disc), stream)
+        r = rank.value
+        u = U.to_numpy()[:, :r] if r else U.to_numpy()[:, :0]
+        vt_flat = Vt.to_numpy().ravel(order="F")[: r * n]
+        vt = vt_flat.reshape((r, n), order="F")
+        return {"U": u, "S": S, "Vt": vt, "rank": r, "truncation_error": terr.value, "discarded_weight": disc.value}
+    finally:
+        U.close()
+        Vt.close()
+        if a is not A:
+            a.close()
+
+
 def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_max: int = 64, max_sweeps: int = 20,
                       two_site_sweeps: int = 10, var_tol: float = 1e-12, trunc_lim: float = 1e-14, eigs_tol: float = 1e-12,
                       mixed: bool = False, complex_: bool = False, seed: int = 0, mixed_min_n: int = 0,
@@ -243,6 +308,16 @@ class MatVecMPOS:
         ld = ld or self.rows()
         fn = "dmrg_matvec_multax_device_fp32" if fp32 else "dmrg_matvec_multax_device"
         call(fn, self.h, dx.ptr, ld, dy.ptr, ld, nvec, stream)
+
+    def get_matrix(self, transpose: bool = True, stream=None) -> np.ndarray:
+        """Dense operator matrix (MatVecMPO::get_matrix orientation by default, matvec_mpo.cpp:313-332)."""
+        n = self.rows()
+        out = DeviceArray((n, n), self.dtype)
+        try:
+            call("dmrg_matvec_get_matrix", self.h, out.ptr, n, int(transpose), stream)
+            return out.to_numpy()
+        finally:
+            out.close()
 
     def set_shift(self, shift: complex) -> None:
         call("dmrg_matvec_set_shift", self.h, float(np.real(shift)), float(np.imag(shift)))

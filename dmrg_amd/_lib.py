@@ -72,6 +72,12 @@ SIGNATURES: dict[str, tuple] = {
     "dmrg_autotune_entries": (C.c_int, []),
     "dmrg_gemm_plan_override": (st, [C.c_int, C.c_int]),
     "dmrg_matvec_time_legs": (st, [vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_float), vp]),
+    "dmrg_heff_dense": (st, [C.c_int, vp, i64, vp, i64p, vp, i64p, vp, i64p, C.c_int, vp]),
+    "dmrg_matvec_get_matrix": (st, [vp, vp, i64, C.c_int, vp]),
+    "dmrg_dense_eigh": (st, [C.c_int, i64, vp, i64, dp, C.c_int, vp]),
+    "dmrg_env_set_edge": (st, [C.c_int, vp, i64, i64, vp, vp]),
+    "dmrg_svd_truncate": (st, [C.c_int, i64, i64, vp, C.c_double, i64, C.c_int, vp, dp, vp, C.POINTER(C.c_int64), dp, dp,
+                               vp]),
 }
 
 class EigsConfig(C.Structure):

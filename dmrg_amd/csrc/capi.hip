@@ -6,10 +6,14 @@
 #include "dense_eig.hpp"
 #include "eigs.hpp"
 #include "sweep.hpp"
+#include "dense.hpp"
+#include "svd.hpp"
 
 #include <complex>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -41,6 +45,31 @@ static dmrg_status guard(F &&f) {
 // Serialises the stateless entry points that share the per-device workspace.
 static std::mutex g_stateless_mu;
 
+// Held for the duration of one stateless call: takes the lock, makes the call's stream wait for the
+// previous stateless call's kernels on this device (they may still be using the shared workspace),
+// and marks this call's end on its stream. Stateless calls on different streams therefore run one
+// after the other on the device; operators (dmrg_matvec) own their workspaces and are not affected.
+class StatelessScope {
+  public:
+    explicit StatelessScope(hipStream_t s) : lock_(g_stateless_mu), s_(s) {
+        int dev = 0;
+        DMRG_HIP_CHECK(hipGetDevice(&dev));
+        static std::vector<hipEvent_t> evs;
+        if(static_cast<int>(evs.size()) <= dev) evs.resize(dev + 1, nullptr);
+        if(!evs[dev]) DMRG_HIP_CHECK(hipEventCreateWithFlags(&evs[dev], hipEventDisableTiming));
+        ev_ = evs[dev];
+        DMRG_HIP_CHECK(hipStreamWaitEvent(s_, ev_, 0));
+    }
+    ~StatelessScope() { (void) hipEventRecord(ev_, s_); }
+    StatelessScope(const StatelessScope &)            = delete;
+    StatelessScope &operator=(const StatelessScope &) = delete;
+
+  private:
+    std::lock_guard<std::mutex> lock_;
+    hipStream_t                 s_;
+    hipEvent_t                  ev_ = nullptr;
+};
+
 // Small per-device buffer for reduction results.
 static double *reduction_buffer() {
     static std::mutex            mu;
@@ -55,6 +84,18 @@ static double *reduction_buffer() {
         bufs[dev] = static_cast<double *>(p);
     }
     return bufs[dev];
+}
+
+// rocSOLVER-backed SVD / eigensolver objects, one per (host thread, device, stream); each owns a rocBLAS
+// handle and grow-only work buffers, released when the thread ends.
+static DeviceSvd &solver_for(hipStream_t s) {
+    thread_local std::map<std::pair<int, hipStream_t>, std::unique_ptr<DeviceSvd>> cache;
+    int dev = 0;
+    DMRG_HIP_CHECK(hipGetDevice(&dev));
+    auto &slot = cache[{dev, s}];
+    if(!slot) slot = std::make_unique<DeviceSvd>(s);
+    slot->set_stream(s);
+    return *slot;
 }
 
 } // namespace dmrg
@@ -176,7 +217,7 @@ dmrg_status dmrg_matrix_vector_product(dmrg_dtype dtype, void *res, const void *
         DMRG_REQUIRE(res && mps && mpo && envL && envR && mps_dims && mpo_dims && envL_dims && envR_dims, DMRG_ERR_ARG,
                      "null argument");
         const Dims                  D = check_matvec_dims(mps_dims, mpo_dims, envL_dims, envR_dims);
-        std::lock_guard<std::mutex> lock(g_stateless_mu);
+        StatelessScope scope(static_cast<hipStream_t>(stream));
         void *ws = scratch_workspace(static_cast<size_t>(matvec_ws_elems(D)) * dtype_size(dtype));
         matvec(dtype, D, mps, D.n(), res, D.n(), 1, mpo, envL, envR, ws, static_cast<hipStream_t>(stream));
     });
@@ -205,7 +246,7 @@ dmrg_status dmrg_env_build_block(dmrg_dtype dtype, char side, void *block_out, c
             DMRG_REQUIRE(mps_dims[2] == other_dims[0], DMRG_ERR_DIMS, "envR: mps dim 2 != right-block dim 0");
             DMRG_REQUIRE(mpo_dims[1] == other_dims[2], DMRG_ERR_DIMS, "envR: mpo dim 1 != right-block dim 2");
         }
-        std::lock_guard<std::mutex> lock(g_stateless_mu);
+        StatelessScope scope(static_cast<hipStream_t>(stream));
         void *ws = scratch_workspace(static_cast<size_t>(env_ws_elems(D)) * dtype_size(dtype));
         env_build_block(dtype, side, D, block_out, otherblock, mps, mpo, ws, static_cast<hipStream_t>(stream));
     });
@@ -214,7 +255,7 @@ dmrg_status dmrg_env_build_block(dmrg_dtype dtype, char side, void *block_out, c
 dmrg_status dmrg_dot(dmrg_dtype dtype, int64_t n, const void *x, const void *y, double result[2], void *stream) {
     return guard([&] {
         DMRG_REQUIRE(result && (n == 0 || (x && y)) && n >= 0, DMRG_ERR_ARG, "bad argument");
-        std::lock_guard<std::mutex> lock(g_stateless_mu);
+        StatelessScope scope(static_cast<hipStream_t>(stream));
         double *buf = reduction_buffer();
         auto    s   = static_cast<hipStream_t>(stream);
         dot_device(dtype, n, x, y, buf, s);
@@ -227,7 +268,7 @@ dmrg_status dmrg_dot(dmrg_dtype dtype, int64_t n, const void *x, const void *y, 
 dmrg_status dmrg_nrm2(dmrg_dtype dtype, int64_t n, const void *x, double *result, void *stream) {
     return guard([&] {
         double r[2];
-        std::lock_guard<std::mutex> lock(g_stateless_mu);
+        StatelessScope scope(static_cast<hipStream_t>(stream));
         double *buf = reduction_buffer();
         auto    s   = static_cast<hipStream_t>(stream);
         nrm2sq_device(dtype, n, x, buf, s);
@@ -243,7 +284,7 @@ dmrg_status dmrg_expectation_value(dmrg_dtype dtype, const void *mps, const int6
     return guard([&] {
         DMRG_REQUIRE(result && mps && mpo && envL && envR, DMRG_ERR_ARG, "null argument");
         const Dims                  D = check_matvec_dims(mps_dims, mpo_dims, envL_dims, envR_dims);
-        std::lock_guard<std::mutex> lock(g_stateless_mu);
+        StatelessScope scope(static_cast<hipStream_t>(stream));
         const size_t es = dtype_size(dtype);
         char *ws = static_cast<char *>(scratch_workspace(static_cast<size_t>(matvec_ws_elems(D) + D.n()) * es));
         void *hv = ws + static_cast<size_t>(matvec_ws_elems(D)) * es;
@@ -435,6 +476,79 @@ dmrg_status dmrg_matvec_time_legs(dmrg_matvec *op, const void *dx, void *dy, int
     return guard([&] {
         DMRG_REQUIRE(op && dx && dy && ms_per_leg && nrep > 0, DMRG_ERR_ARG, "bad argument");
         reinterpret_cast<MatVecOp *>(op)->time_legs(dx, dy, nrep, fp32 != 0, ms_per_leg, static_cast<hipStream_t>(stream));
+    });
+}
+
+// ------------------------------------------------------------------------- dense path, edges, SVD
+
+dmrg_status dmrg_heff_dense(dmrg_dtype dtype, void *H, int64_t ldh, const void *mpo, const int64_t mpo_dims[4],
+                            const void *envL, const int64_t envL_dims[3], const void *envR, const int64_t envR_dims[3],
+                            int transpose, void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(H && mpo && envL && envR && mpo_dims && envL_dims && envR_dims, DMRG_ERR_ARG, "null argument");
+        const int64_t mps_dims[3] = {mpo_dims[2], envL_dims[0], envR_dims[0]};
+        const Dims    D           = check_matvec_dims(mps_dims, mpo_dims, envL_dims, envR_dims);
+        heff_dense(dtype, D, mpo, envL, envR, H, ldh, transpose != 0, static_cast<hipStream_t>(stream));
+    });
+}
+
+dmrg_status dmrg_matvec_get_matrix(dmrg_matvec *opp, void *H, int64_t ldh, int transpose, void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(opp && H, DMRG_ERR_ARG, "null argument");
+        auto *op = reinterpret_cast<MatVecOp *>(opp);
+        heff_dense(op->dtype, op->D, op->mpo.p, op->envL_p, op->envR_p, H, ldh, transpose != 0, static_cast<hipStream_t>(stream));
+    });
+}
+
+dmrg_status dmrg_dense_eigh(dmrg_dtype dtype, int64_t n, void *A, int64_t lda, double *w, int vectors, void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(A && w, DMRG_ERR_ARG, "null argument");
+        DMRG_REQUIRE(n > 0 && n < INT32_MAX && lda >= n && lda < INT32_MAX, DMRG_ERR_DIMS, "eigh: bad n / lda");
+        auto               &solver = solver_for(static_cast<hipStream_t>(stream));
+        std::vector<double> ev;
+        if(dtype == DMRG_FP64) solver.eigh(static_cast<int>(n), static_cast<double *>(A), static_cast<int>(lda), vectors != 0, ev);
+        else if(dtype == DMRG_CX64)
+            solver.eigh(static_cast<int>(n), static_cast<cpx<double> *>(A), static_cast<int>(lda), vectors != 0, ev);
+        else throw Error(DMRG_ERR_UNSUPPORTED, "eigh: FP64 or CX64");
+        std::memcpy(w, ev.data(), static_cast<size_t>(n) * sizeof(double));
+    });
+}
+
+dmrg_status dmrg_env_set_edge(dmrg_dtype dtype, void *block_out, int64_t chi, int64_t mpo_dim, const void *edge,
+                              void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(block_out && edge, DMRG_ERR_ARG, "null argument");
+        env_set_edge(dtype, block_out, chi, mpo_dim, edge, static_cast<hipStream_t>(stream));
+    });
+}
+
+dmrg_status dmrg_svd_truncate(dmrg_dtype dtype, int64_t m, int64_t n, const void *A, double trunc_lim, int64_t rank_max,
+                              int method, void *U, double *S, void *Vt, int64_t *rank, double *truncation_error,
+                              double *discarded_weight, void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(A && U && S && Vt && rank, DMRG_ERR_ARG, "null argument");
+        DMRG_REQUIRE(m > 0 && n > 0 && m < INT32_MAX && n < INT32_MAX && m * n < (int64_t(1) << 40), DMRG_ERR_DIMS,
+                     "svd: bad matrix size");
+        DMRG_REQUIRE(dtype == DMRG_FP64 || dtype == DMRG_CX64, DMRG_ERR_UNSUPPORTED, "svd: FP64 or CX64");
+        auto                s      = static_cast<hipStream_t>(stream);
+        auto               &solver = solver_for(s);
+        std::vector<double> sv;
+        if(dtype == DMRG_FP64) solver.compute(static_cast<int>(m), static_cast<int>(n), static_cast<const double *>(A), sv, method);
+        else solver.compute(static_cast<int>(m), static_cast<int>(n), static_cast<const cpx<double> *>(A), sv, method);
+        const int64_t k     = solver.k();
+        const auto    rk    = rank_from_truncation_error(sv, trunc_lim, rank_max > 0 ? static_cast<int>(std::min<int64_t>(rank_max, k)) : 0, 1);
+        const int64_t r     = rk.first;
+        const size_t  es    = dtype_size(dtype);
+        DMRG_HIP_CHECK(hipMemcpyAsync(U, solver.U(), static_cast<size_t>(m * r) * es, hipMemcpyDeviceToDevice, s));
+        DMRG_HIP_CHECK(hipMemcpy2DAsync(Vt, static_cast<size_t>(r) * es, solver.Vt(), static_cast<size_t>(k) * es,
+                                        static_cast<size_t>(r) * es, static_cast<size_t>(n), hipMemcpyDeviceToDevice, s));
+        DMRG_HIP_CHECK(hipStreamSynchronize(s));
+        double disc = 0.0;
+        for(int64_t i = k - 1; i >= r; --i) disc += sv[i] * sv[i];
+        std::memcpy(S, sv.data(), static_cast<size_t>(k) * sizeof(double));
+        *rank = r;
+        if(truncation_error) *truncation_error = rk.second;
+        if(discarded_weight) *discarded_weight = disc;
     });
 }
 

@@ -21,41 +21,87 @@ DeviceSvd::DeviceSvd(hipStream_t st) : s(st) {
     DMRG_ROCBLAS_CHECK(rocblas_set_stream(h, s));
     handle = h;
 }
+void DeviceSvd::set_stream(hipStream_t st) {
+    s = st;
+    DMRG_ROCBLAS_CHECK(rocblas_set_stream(static_cast<rocblas_handle>(handle), s));
+}
 DeviceSvd::~DeviceSvd() {
     if(handle) (void) rocblas_destroy_handle(static_cast<rocblas_handle>(handle));
 }
 
 template <typename T>
-void DeviceSvd::compute(int m, int n, const T *A, std::vector<double> &S) {
+void DeviceSvd::compute(int m, int n, const T *A, std::vector<double> &S, int method) {
     DMRG_REQUIRE(m > 0 && n > 0, DMRG_ERR_DIMS, "svd of an empty matrix");
+    DMRG_REQUIRE(method >= 0 && method <= 2, DMRG_ERR_ARG, "svd method must be 0 (gesvd), 1 (gesdd) or 2 (gesvdj)");
     k_ = std::min(m, n);
     a.reserve(static_cast<size_t>(m) * n * sizeof(T));
     sv.reserve(static_cast<size_t>(k_) * sizeof(double));
     u.reserve(static_cast<size_t>(m) * k_ * sizeof(T));
     vt.reserve(static_cast<size_t>(k_) * n * sizeof(T));
     e.reserve(static_cast<size_t>(std::max(1, k_ - 1)) * sizeof(double));
-    info.reserve(sizeof(rocblas_int));
+    info.reserve(sizeof(rocblas_int) * 2);
+    res.reserve(sizeof(double));
     DMRG_HIP_CHECK(hipMemcpyAsync(a.p, A, static_cast<size_t>(m) * n * sizeof(T), hipMemcpyDeviceToDevice, s));
-    auto h = static_cast<rocblas_handle>(handle);
+    auto        h  = static_cast<rocblas_handle>(handle);
+    const auto  sv_ = rocblas_svect_singular;
+    rocblas_int *inf = info.as<rocblas_int>();
     if constexpr(std::is_same_v<T, double>) {
-        DMRG_ROCBLAS_CHECK(rocsolver_dgesvd(h, rocblas_svect_singular, rocblas_svect_singular, m, n, a.as<double>(), m,
-                                            sv.as<double>(), u.as<double>(), m, vt.as<double>(), k_, e.as<double>(),
-                                            rocblas_outofplace, info.as<rocblas_int>()));
+        double *pa = a.as<double>(), *pu = u.as<double>(), *pv = vt.as<double>();
+        if(method == 0)
+            DMRG_ROCBLAS_CHECK(rocsolver_dgesvd(h, sv_, sv_, m, n, pa, m, sv.as<double>(), pu, m, pv, k_, e.as<double>(),
+                                                rocblas_outofplace, inf));
+        else if(method == 1)
+            DMRG_ROCBLAS_CHECK(rocsolver_dgesdd(h, sv_, sv_, m, n, pa, m, sv.as<double>(), pu, m, pv, k_, inf));
+        else
+            DMRG_ROCBLAS_CHECK(rocsolver_dgesvdj(h, sv_, sv_, m, n, pa, m, 0.0, res.as<double>(), 100, inf + 1,
+                                                 sv.as<double>(), pu, m, pv, k_, inf));
     } else {
-        DMRG_ROCBLAS_CHECK(rocsolver_zgesvd(h, rocblas_svect_singular, rocblas_svect_singular, m, n,
-                                            a.as<rocblas_double_complex>(), m, sv.as<double>(), u.as<rocblas_double_complex>(), m,
-                                            vt.as<rocblas_double_complex>(), k_, e.as<double>(), rocblas_outofplace,
-                                            info.as<rocblas_int>()));
+        using Z = rocblas_double_complex;
+        Z *pa = a.as<Z>(), *pu = u.as<Z>(), *pv = vt.as<Z>();
+        if(method == 0)
+            DMRG_ROCBLAS_CHECK(rocsolver_zgesvd(h, sv_, sv_, m, n, pa, m, sv.as<double>(), pu, m, pv, k_, e.as<double>(),
+                                                rocblas_outofplace, inf));
+        else if(method == 1)
+            DMRG_ROCBLAS_CHECK(rocsolver_zgesdd(h, sv_, sv_, m, n, pa, m, sv.as<double>(), pu, m, pv, k_, inf));
+        else
+            DMRG_ROCBLAS_CHECK(rocsolver_zgesvdj(h, sv_, sv_, m, n, pa, m, 0.0, res.as<double>(), 100, inf + 1,
+                                                 sv.as<double>(), pu, m, pv, k_, inf));
     }
     S.resize(k_);
-    rocblas_int inf = 0;
+    rocblas_int hinf = 0;
     DMRG_HIP_CHECK(hipMemcpyAsync(S.data(), sv.p, k_ * sizeof(double), hipMemcpyDeviceToHost, s));
-    DMRG_HIP_CHECK(hipMemcpyAsync(&inf, info.p, sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
+    DMRG_HIP_CHECK(hipMemcpyAsync(&hinf, info.p, sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
     DMRG_HIP_CHECK(hipStreamSynchronize(s));
-    DMRG_REQUIRE(inf == 0, DMRG_ERR_NOT_CONVERGED, "gesvd: bidiagonal QR did not converge");
+    DMRG_REQUIRE(hinf == 0, DMRG_ERR_NOT_CONVERGED, "svd: rocSOLVER reported no convergence (info " + std::to_string(hinf) + ")");
+    for(int i = 1; i < k_; ++i)
+        DMRG_REQUIRE(S[i] <= S[i - 1], DMRG_ERR_NOT_CONVERGED, "svd: singular values are not in descending order");
 }
-template void DeviceSvd::compute<double>(int, int, const double *, std::vector<double> &);
-template void DeviceSvd::compute<cpx<double>>(int, int, const cpx<double> *, std::vector<double> &);
+template void DeviceSvd::compute<double>(int, int, const double *, std::vector<double> &, int);
+template void DeviceSvd::compute<cpx<double>>(int, int, const cpx<double> *, std::vector<double> &, int);
+
+template <typename T>
+void DeviceSvd::eigh(int n, T *A, int lda, bool vectors, std::vector<double> &w) {
+    DMRG_REQUIRE(n > 0 && lda >= n, DMRG_ERR_DIMS, "eigh: need n > 0 and lda >= n");
+    sv.reserve(static_cast<size_t>(n) * sizeof(double));
+    e.reserve(static_cast<size_t>(n) * sizeof(double));
+    info.reserve(sizeof(rocblas_int) * 2);
+    auto       h  = static_cast<rocblas_handle>(handle);
+    const auto ev = vectors ? rocblas_evect_original : rocblas_evect_none;
+    if constexpr(std::is_same_v<T, double>)
+        DMRG_ROCBLAS_CHECK(rocsolver_dsyevd(h, ev, rocblas_fill_lower, n, A, lda, sv.as<double>(), e.as<double>(),
+                                            info.as<rocblas_int>()));
+    else
+        DMRG_ROCBLAS_CHECK(rocsolver_zheevd(h, ev, rocblas_fill_lower, n, reinterpret_cast<rocblas_double_complex *>(A), lda,
+                                            sv.as<double>(), e.as<double>(), info.as<rocblas_int>()));
+    w.resize(n);
+    rocblas_int hinf = 0;
+    DMRG_HIP_CHECK(hipMemcpyAsync(w.data(), sv.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    DMRG_HIP_CHECK(hipMemcpyAsync(&hinf, info.p, sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
+    DMRG_HIP_CHECK(hipStreamSynchronize(s));
+    DMRG_REQUIRE(hinf == 0, DMRG_ERR_NOT_CONVERGED, "eigh: rocSOLVER reported no convergence (info " + std::to_string(hinf) + ")");
+}
+template void DeviceSvd::eigh<double>(int, double *, int, bool, std::vector<double> &);
+template void DeviceSvd::eigh<cpx<double>>(int, cpx<double> *, int, bool, std::vector<double> &);
 
 std::pair<int, double> rank_from_truncation_error(const std::vector<double> &S, double lim, int rank_max, int rank_min) {
     const int           k = static_cast<int>(S.size());

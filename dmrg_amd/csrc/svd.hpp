@@ -9,14 +9,21 @@ namespace dmrg {
 
 // Thin SVD A = U diag(S) Vt of a device m x n column-major matrix (double or complex double) on rocSOLVER.
 // U: m x k, Vt: k x n (k = min(m, n)), S: descending, copied to the host. A is left untouched.
+// method: 0 bidiagonal QR (gesvd), 1 divide and conquer (gesdd), 2 one-sided Jacobi (gesvdj), the routines
+// the reference selects between in svd_lapacke.cpp:162-290.
 class DeviceSvd {
   public:
     explicit DeviceSvd(hipStream_t s);
     ~DeviceSvd();
     DeviceSvd(const DeviceSvd &) = delete;
     DeviceSvd &operator=(const DeviceSvd &) = delete;
+    void set_stream(hipStream_t s);
     template <typename T>
-    void compute(int m, int n, const T *A, std::vector<double> &S);
+    void compute(int m, int n, const T *A, std::vector<double> &S, int method = 0);
+    // Hermitian eigendecomposition in place (rocSOLVER syevd / heevd): A (n x n, lda) is overwritten by the
+    // eigenvectors when vectors is set; w receives the n eigenvalues in ascending order.
+    template <typename T>
+    void eigh(int n, T *A, int lda, bool vectors, std::vector<double> &w);
     void *U() const { return u.p; }
     void *Vt() const { return vt.p; }
     int   k() const { return k_; }
@@ -24,7 +31,7 @@ class DeviceSvd {
   private:
     hipStream_t s;
     void       *handle = nullptr;
-    DevBuf      a, sv, u, vt, e, info;
+    DevBuf      a, sv, u, vt, e, info, res;
     int         k_ = 0;
 };
 

@@ -57,3 +57,48 @@ def ising_fields(L: int, w: float, seed: int) -> np.ndarray:
     """Random fields h_i ~ Uniform[-w, w] of the ising_tf_rf model (h_mean = 0, h_wdth = 2w,
     IsingRandomField.cpp:143-144), drawn from numpy's PCG64 with the given seed and passed explicitly."""
     return np.random.default_rng(seed).uniform(-w, w, L)
+
+
+_PZ = np.array([[1.0, 0.0], [0.0, -1.0]])
+_PX = np.array([[0.0, 1.0], [1.0, 0.0]])
+
+
+def random_range_mpo_site(rng, D: int = 8, xi: float = 1.0) -> np.ndarray:
+    """One real site MPO (D, D, 2, 2) of a random l-bit-style chain (cf. LBit.cpp:147-260):
+    H = sum_i (h_i Z_i + g_i X_i) + sum_i sum_{r=1}^{D-2} J_{i,r} Z_i Z_{i+r},  J_{i,r} ~ N(0,1) e^{-r/xi},
+    h_i, g_i ~ N(0,1). Finite-state layout with rows = left bond, columns = right bond, row D-1 starting
+    strings and column 0 ending them (the convention of the ising MPO, IsingRandomField.cpp:107-136):
+    W[0,0] = W[D-1,D-1] = I, W[D-1,0] = h Z + g X, W[D-1,r] = J_r Z, W[r,r-1] = I (r >= 2), W[1,0] = Z.
+    Every physical block is real symmetric."""
+    W = np.zeros((D, D, 2, 2), dtype=np.float64, order="F")
+    I2 = np.eye(2)
+    W[0, 0] = I2
+    W[D - 1, D - 1] = I2
+    W[D - 1, 0] = rng.standard_normal() * _PZ + rng.standard_normal() * _PX
+    for r in range(1, D - 1):
+        W[D - 1, r] = rng.standard_normal() * np.exp(-r / xi) * _PZ
+        if r >= 2:
+            W[r, r - 1] = I2
+    W[1, 0] = _PZ
+    return W
+
+
+def two_site_problem(chi: int, D: int, seed: int):
+    """Two-site workload: theta (4, chi, chi) cx64 N(0,1) normalised, env blocks (chi, chi, D) Hermitian per w,
+    and the two site MPOs (D, D, 2, 2) of random_range_mpo_site (merged by the operator into (D, D, 4, 4))."""
+    rng = np.random.default_rng(seed)
+    theta = rng.standard_normal((4, chi, chi)) + 1j * rng.standard_normal((4, chi, chi))
+    theta = np.asfortranarray(theta / np.linalg.norm(theta))
+    envL = _herm_blocks(rng, chi, D, True)
+    envR = _herm_blocks(rng, chi, D, True)
+    mpos = [random_range_mpo_site(rng, D), random_range_mpo_site(rng, D)]
+    return theta, mpos, envL, envR
+
+
+def theta_as_matrix(theta: np.ndarray) -> np.ndarray:
+    """Two-site theta (d1*d2, chiL, chiR) with d1 fastest (contraction_tblis.cpp:478-505) as the split matrix
+    M[(s1, a), (s2, b)] of shape (d1*chiL, d2*chiR) (split.cpp:21-346), for d1 = d2."""
+    d = int(round(np.sqrt(theta.shape[0])))
+    chiL, chiR = theta.shape[1], theta.shape[2]
+    t = theta.reshape((d, d, chiL, chiR), order="F").transpose(0, 2, 1, 3)  # (s1, a, s2, b)
+    return np.asfortranarray(t.reshape((d * chiL, d * chiR), order="F"))

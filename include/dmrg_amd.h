@@ -69,7 +69,9 @@ dmrg_status dmrg_event_elapsed_ms(float *ms, void *start, void *stop);
  * Replaces tools::common::contraction::matrix_vector_product<Scalar>(res, mps, mps_dims, mpo, mpo_dims,
  * envL, envL_dims, envR, envR_dims)  (contraction.h:31-36; contraction_tblis.cpp:213-292), Scalar in
  * {fp64, cx64}; FP32/CX32 run the same contraction on FP32 matrix cores. Same dimension checks
- * (contraction_tblis.cpp:230-234) -> DMRG_ERR_DIMS. Workspace is held by the library. */
+ * (contraction_tblis.cpp:230-234) -> DMRG_ERR_DIMS. Workspace is held by the library; the stateless
+ * entry points (this one, dmrg_env_build_block, dmrg_expectation_value, dmrg_dot, dmrg_nrm2) share it, so
+ * calls of them on different streams are ordered one after the other on the device. */
 dmrg_status dmrg_matrix_vector_product(dmrg_dtype dtype, void *res, const void *mps, const int64_t mps_dims[3],
                                        const void *mpo, const int64_t mpo_dims[4], const void *envL,
                                        const int64_t envL_dims[3], const void *envR, const int64_t envR_dims[3],
@@ -293,6 +295,45 @@ dmrg_status dmrg_gemm_plan_override(int cfg, int ksplit);
  * matvec_mpos.h:174-179, time the whole call). */
 dmrg_status dmrg_matvec_time_legs(dmrg_matvec *op, const void *dx, void *dy, int nrep, int fp32, float ms[4],
                                   void *stream);
+
+/* ------------------------------------------------------------ dense path, edges, truncated SVD */
+
+/* Explicit H_eff matrix H (n x n, leading dimension ldh >= n, device, dtype FP64 or CX64) of one MPO
+ * (DL,DR,d,d) and env blocks (chiL,chiL,DL), (chiR,chiR,DR); n = d*chiL*chiR <= 65535.
+ * transpose = 0: H[(s',a',b'),(s,a,b)] = sum L[a,a',w] W[w,w',s,s'] R[b,b',w'] so that H x is the matvec; the
+ * orientation of contract_mpo_env / TensorsFinite::get_effective_hamiltonian (TensorsFinite.cpp:119-142).
+ * transpose = 1: its transpose, the orientation of MatVecMPO::get_tensor/get_matrix (matvec_mpo.cpp:313-332). */
+dmrg_status dmrg_heff_dense(dmrg_dtype dtype, void *H, int64_t ldh, const void *mpo, const int64_t mpo_dims[4],
+                            const void *envL, const int64_t envL_dims[3], const void *envR, const int64_t envR_dims[3],
+                            int transpose, void *stream);
+/* The same matrix for an operator (merged MPO with its shift, its env blocks): MatVecMPO::get_matrix
+ * (matvec_mpo.cpp:329-332) with transpose = 1. */
+dmrg_status dmrg_matvec_get_matrix(dmrg_matvec *op, void *H, int64_t ldh, int transpose, void *stream);
+
+/* All eigenvalues (ascending, host w[n]) and optionally eigenvectors (overwriting A, device, column-major)
+ * of a Hermitian n x n device matrix, lower triangle referenced (rocSOLVER syevd / heevd). Replaces the dense
+ * eig::solver::eig path (solver.cpp:205-245, dsyevr/zheevr) used for n <= 4096 (optimize_energy_eig.cpp:18-58).
+ * Synchronises the stream. */
+dmrg_status dmrg_dense_eigh(dmrg_dtype dtype, int64_t n, void *A, int64_t lda, double *w, int vectors, void *stream);
+
+/* Chain-end environment block (chi, chi, mpo_dim), device: zero except block[i,i,:] = edge[:] for all i < chi,
+ * edge a host vector of mpo_dim <= 128 elements of dtype (FP64 or CX64). Replaces EnvBase::set_edge_dims
+ * (EnvBase.cpp:254-285); the edge itself is MpoSite::get_MPO_edge_left/right (MpoSite.cpp:450-547): the unit
+ * vector on the last MPO row for the left edge, on the first column for the right edge. */
+dmrg_status dmrg_env_set_edge(dmrg_dtype dtype, void *block_out, int64_t chi, int64_t mpo_dim, const void *edge,
+                              void *stream);
+
+/* Truncated SVD A = U S Vt of a device m x n column-major matrix (FP64 or CX64), the step of split::split_mps
+ * (split.cpp:21-346) done by svd::solver (svd_lapacke.cpp:82-330). rank = #{k : ||S[k:]||_2 >= trunc_lim}
+ * capped at rank_max (> 0; 0 = no cap), at least 1 (svd.cpp:257-280). Outputs: U (device, m x rank, ld m),
+ * Vt (device, rank x n, ld rank), S (host, all min(m,n) singular values, descending), rank,
+ * truncation_error = ||S[rank:]||_2 (the reference's definition, a norm) and discarded_weight = sum S[rank:]^2
+ * (either may be NULL). U and Vt must hold m*min(m,n) and min(m,n)*n elements. method: 0 gesvd (bidiagonal QR),
+ * 1 gesdd (divide and conquer), 2 gesvdj (Jacobi), the routines of svd_lapacke.cpp:162-290.
+ * Synchronises the stream. */
+dmrg_status dmrg_svd_truncate(dmrg_dtype dtype, int64_t m, int64_t n, const void *A, double trunc_lim, int64_t rank_max,
+                              int method, void *U, double *S, void *Vt, int64_t *rank, double *truncation_error,
+                              double *discarded_weight, void *stream);
 
 #ifdef __cplusplus
 }
