@@ -66,6 +66,7 @@ SIGNATURES: dict[str, tuple] = {
     "dmrg_matvec_multax_device_fp32": (st, [vp, vp, i64, vp, i64, C.c_int, vp]),
     "dmrg_matvec_set_shift": (st, [vp, C.c_double, C.c_double]),
     "dmrg_matvec_get_diagonal": (st, [vp, vp, vp]),
+    "dmrg_matvec_precond": (st, [vp, vp, i64, vp, i64, C.c_int, dp]),
     "dmrg_host_heev": (st, [C.c_int, dp, dp, dp]),
     "dmrg_autotune_enable": (st, [C.c_int]),
     "dmrg_autotune_reset": (st, []),

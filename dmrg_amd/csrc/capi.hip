@@ -356,6 +356,14 @@ dmrg_status dmrg_matvec_get_diagonal(dmrg_matvec *op, void *ddiag, void *stream)
     });
 }
 
+dmrg_status dmrg_matvec_precond(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec,
+                                const double *shifts) {
+    return guard([&] {
+        DMRG_REQUIRE(op && (nvec == 0 || (x && y)) && nvec >= 0, DMRG_ERR_ARG, "bad argument");
+        reinterpret_cast<MatVecOp *>(op)->precond_host(x, ldx, y, ldy, nvec, shifts);
+    });
+}
+
 void dmrg_eigs_default_config(dmrg_eigs_config *cfg) {
     if(!cfg) return;
     cfg->nev         = 1;

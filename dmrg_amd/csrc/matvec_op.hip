@@ -1,5 +1,6 @@
 // MatVecOp: device-resident H_eff operator (MatVecMPOS<T>, matvec_mpos.cpp:40-130, 963-984, 1433-1472).
 #include "matvec_op.hpp"
+#include "vecops.hpp"
 
 #include <cstring>
 
@@ -99,7 +100,8 @@ void MatVecOp::rebind(int nmpo, const void *const *mpos, const int64_t *mpo_dims
 
 void MatVecOp::upload_mpo() {
     upload_as<double>(mpo, mpo_host.data(), static_cast<int64_t>(mpo_host.size()), dtype);
-    have32 = false;
+    have32     = false;
+    diag_valid = false;
 }
 
 void MatVecOp::ensure_fp32(hipStream_t s) {
@@ -160,6 +162,34 @@ void MatVecOp::set_shift(cx shift) {
 void MatVecOp::diagonal(void *out, hipStream_t s) {
     void *w = diagws.reserve(static_cast<size_t>(D.chiL * D.DR * D.d) * dtype_size(dtype));
     heff_diagonal(dtype, D, mpo.p, envL_p, envR_p, w, out, s);
+}
+
+void MatVecOp::precond_host(const void *x, int64_t ldx, void *y, int64_t ldy, int nvec, const double *shifts) {
+    // MultPc with jcbMaxBlockSize = 1 (matvec_mpos.cpp:1333-1361): out = in / (diag(A) - shift), no floor.
+    if(nvec == 0) return;
+    const int64_t n  = D.n();
+    const size_t  es = dtype_size(dtype);
+    DMRG_REQUIRE(ldx >= n && ldy >= n, DMRG_ERR_ARG, "leading dimension smaller than rows()");
+    if(!diag_valid) {
+        diagc.reserve(static_cast<size_t>(n) * es);
+        diagonal(diagc.p, nullptr);
+        diag_valid = true;
+    }
+    xbuf.reserve(static_cast<size_t>(nvec) * n * es);
+    ybuf.reserve(static_cast<size_t>(nvec) * n * es);
+    DMRG_HIP_CHECK(hipMemcpy2D(xbuf.p, n * es, x, ldx * es, n * es, nvec, hipMemcpyHostToDevice));
+    for(int i = 0; i < nvec; ++i) {
+        const size_t off = static_cast<size_t>(i) * n * es;
+        if(dtype == DMRG_FP64)
+            jacobi_precond<double, double>(n, reinterpret_cast<const double *>(xbuf.as<char>() + off), diagc.as<double>(),
+                                           shifts ? shifts[i] : 0.0, 0.0, reinterpret_cast<double *>(ybuf.as<char>() + off),
+                                           nullptr);
+        else
+            jacobi_precond<cpx<double>, cpx<double>>(n, reinterpret_cast<const cpx<double> *>(xbuf.as<char>() + off),
+                                                     diagc.as<cpx<double>>(), shifts ? shifts[i] : 0.0, 0.0,
+                                                     reinterpret_cast<cpx<double> *>(ybuf.as<char>() + off), nullptr);
+    }
+    DMRG_HIP_CHECK(hipMemcpy2D(y, ldy * es, ybuf.p, n * es, n * es, nvec, hipMemcpyDeviceToHost));
 }
 
 void MatVecOp::time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms[4], hipStream_t s) {

@@ -63,7 +63,8 @@ struct MatVecOp {
     const void                        *envL_p = nullptr, *envR_p = nullptr; // env blocks used: owned or borrowed views
     DevBuf                             mpo32, envL32, envR32; // FP32 shadows for the mixed path
     bool                               have32 = false;
-    DevBuf                             ws, ws32, xbuf, ybuf, diagws;
+    DevBuf                             ws, ws32, xbuf, ybuf, diagws, diagc;
+    bool                               diag_valid = false; // diagc holds the current operator's diagonal
     std::complex<double>               sigma{0.0, 0.0};
     bool                               ready_shift = false;
     int64_t                            num_mv      = 0;
@@ -84,6 +85,8 @@ struct MatVecOp {
     void multax_host(const void *x, int64_t ldx, void *y, int64_t ldy, int nvec);
     void set_shift(std::complex<double> shift);
     void diagonal(void *out, hipStream_t s);
+    // Jacobi preconditioner on host buffers: y[:, i] = x[:, i] / (diag(H_eff) - shifts[i]).
+    void precond_host(const void *x, int64_t ldx, void *y, int64_t ldy, int nvec, const double *shifts);
     void time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms[4], hipStream_t s);
 };
 

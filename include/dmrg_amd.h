@@ -136,6 +136,12 @@ dmrg_status dmrg_matvec_set_shift(dmrg_matvec *op, double re, double im);
 /* Diagonal of H_eff (length rows), DEVICE output, operator dtype. Replaces MatVecMPOS::get_diagonal_new
  * (matvec_mpos.cpp:399-407); feeds the Jacobi preconditioner (matvec_mpos.cpp:1333-1416, jcb = 1). */
 dmrg_status dmrg_matvec_get_diagonal(dmrg_matvec *op, void *ddiag, void *stream);
+/* Jacobi preconditioner on host buffers, the PRIMME applyPreconditioner callback MatVecMPOS::MultPc with
+ * jcbMaxBlockSize = 1 (matvec_mpos.cpp:1333-1361): y[:, i] = x[:, i] / (diag(H_eff) - shifts[i]) for i < nvec
+ * (shifts = PRIMME's ShiftsForPreconditioner; NULL means 0). The diagonal is computed once per operator state
+ * (again after set_shift). */
+dmrg_status dmrg_matvec_precond(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec,
+                                const double *shifts);
 
 /* ---------------------------------------------------------------- eigensolver */
 

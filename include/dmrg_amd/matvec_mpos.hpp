@@ -10,11 +10,20 @@
 //                               replaces tools::common::contraction::matrix_vector_product
 //                               (source/tools/common/contraction.h:31-36) with host pointers.
 //   dmrg_amd::contraction::expectation_value<Scalar> (contraction_tblis.cpp:30-97).
+//
+// MatVecMPOS<T> also carries the members eig::solver::eigs_primme<MatrixProductType> reads
+// (solver_primme.cpp:297-517): Scalar, can_shift / can_shift_invert / can_precondition, set_shift,
+// isReadyShift, isReadyFactorOp, preconditioner, set_jcbMaxBlockSize, rows, num_mv / num_op / num_pc,
+// t_multAx / t_multOPv / t_multPc, and the Jacobi MultPc callback (optimize_energy.cpp:34-42), so that it can
+// stand in for the reference's operator type in that template. Shift-invert (FactorOP / MultOPv) is not
+// provided: isReadyFactorOp() is false, which keeps PRIMME on MultAx.
 #pragma once
 #include "../dmrg_amd.h"
 
 #include <array>
+#include <chrono>
 #include <complex>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -40,10 +49,35 @@ constexpr dmrg_dtype dtype_of() {
     return std::is_same_v<T, double> ? DMRG_FP64 : DMRG_CX64;
 }
 
+// Accepts assignments from any enumeration (eig::Preconditioner, eig::Factorization ...) and keeps the value.
+struct EnumValue {
+    int value = 0;
+    template <typename E>
+    EnumValue &operator=(E e) {
+        value = static_cast<int>(e);
+        return *this;
+    }
+};
+
+// Accumulating wall-clock timer with the get_time() the reference's tid::ur timers expose.
+struct Timer {
+    double seconds = 0.0;
+    double get_time() const { return seconds; }
+};
+
 template <typename T>
 class MatVecMPOS {
   public:
-    using Scalar = T;
+    using Scalar                            = T;
+    static constexpr bool can_shift         = true;
+    static constexpr bool can_shift_invert  = false;
+    static constexpr bool can_precondition  = true;
+    EnumValue             preconditioner;   // eig::Preconditioner: NONE (0) or JACOBI
+    EnumValue             factorization;    // accepted and ignored (no shift-invert)
+    long                  num_op = 0, num_pc = 0;
+    std::unique_ptr<Timer> t_multAx  = std::make_unique<Timer>();
+    std::unique_ptr<Timer> t_multOPv = std::make_unique<Timer>();
+    std::unique_ptr<Timer> t_multPc  = std::make_unique<Timer>();
 
     // mpos: host cx64 MPO tensors (DL, DR, d, d); envs: host cx64 blocks (chi, chi, D), as EnvBase stores them.
     MatVecMPOS(const std::vector<const std::complex<double> *> &mpos, const std::vector<std::array<long, 4>> &mpo_dims,
@@ -73,22 +107,60 @@ class MatVecMPOS {
     long num_mv() const { return static_cast<long>(dmrg_matvec_num_mv(op_)); }
 
     // MultAx(T*, T*) (matvec_mpos.cpp:963-974): one vector, host buffers of rows() elements.
-    void MultAx(T *mps_in, T *mps_out) { check(dmrg_matvec_multax(op_, mps_in, rows(), mps_out, rows(), 1), "MultAx"); }
+    void MultAx(T *mps_in, T *mps_out) {
+        const auto t0 = std::chrono::steady_clock::now();
+        check(dmrg_matvec_multax(op_, mps_in, rows(), mps_out, rows(), 1), "MultAx");
+        t_multAx->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
 
     // PRIMME matrixMatvec callback (matvec_mpos.cpp:976-984): blockSize columns at strides ldx / ldy.
     // Errors are reported through *err (non-zero makes PRIMME stop), never by an exception.
     void MultAx(void *x, int *ldx, void *y, int *ldy, int *blockSize, primme_params *, int *err) {
-        const dmrg_status s = dmrg_matvec_multax(op_, x, *ldx, y, *ldy, *blockSize);
-        *err                = (s == DMRG_OK) ? 0 : -1;
+        const auto        t0 = std::chrono::steady_clock::now();
+        const dmrg_status s  = dmrg_matvec_multax(op_, x, *ldx, y, *ldy, *blockSize);
+        t_multAx->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *err = (s == DMRG_OK) ? 0 : -1;
     }
 
+    // Jacobi preconditioner, PRIMME applyPreconditioner callback (matvec_mpos.cpp:1333-1361 with
+    // jcbMaxBlockSize = 1): y_i = x_i / (diag(H_eff) - primme->ShiftsForPreconditioner[i]). A template on the
+    // parameter block so that its fields are only looked up where PRIMME's header is included.
+    template <typename P>
+    void MultPc(void *x, int *ldx, void *y, int *ldy, int *blockSize, P *primme, int *err) {
+        MultPc(x, *ldx, y, *ldy, *blockSize, primme ? primme->ShiftsForPreconditioner : nullptr, err);
+    }
+    void MultPc(void *x, long ldx, void *y, long ldy, int blockSize, const double *shifts, int *err) {
+        if(preconditioner.value == 0) { // eig::Preconditioner::NONE: y is left as PRIMME passed it
+            *err = 0;
+            return;
+        }
+        const auto        t0 = std::chrono::steady_clock::now();
+        const dmrg_status s  = dmrg_matvec_precond(op_, x, ldx, y, ldy, blockSize, shifts);
+        t_multPc->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        num_pc += blockSize;
+        *err = (s == DMRG_OK) ? 0 : -1;
+    }
+    // Block sizes above 1 select the reference's block-Jacobi preconditioner; this operator applies the
+    // diagonal (size 1) form for every setting.
+    void set_jcbMaxBlockSize(long size) { jcb_max_block_size_ = size; }
+    long get_jcbMaxBlockSize() const { return jcb_max_block_size_; }
+
     // set_shift(sigma) (matvec_mpos.cpp:1433-1472).
-    void set_shift(std::complex<double> shift) { check(dmrg_matvec_set_shift(op_, shift.real(), shift.imag()), "set_shift"); }
+    void set_shift(std::complex<double> shift) {
+        check(dmrg_matvec_set_shift(op_, shift.real(), shift.imag()), "set_shift");
+        ready_shift_ = true;
+    }
+    bool isReadyShift() const { return ready_shift_; }
+    bool isReadyFactorOp() const { return false; }
+    void FactorOP() { throw error(DMRG_ERR_UNSUPPORTED, "MatVecMPOS: shift-invert is not provided"); }
+    void MultOPv(void *, int *, void *, int *, int *, primme_params *, int *err) { *err = -1; }
 
     dmrg_matvec *handle() { return op_; }
 
   private:
-    dmrg_matvec *op_ = nullptr;
+    dmrg_matvec *op_                 = nullptr;
+    long         jcb_max_block_size_ = 1;
+    bool         ready_shift_        = false;
 };
 
 namespace contraction {

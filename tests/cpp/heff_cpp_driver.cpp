@@ -4,8 +4,42 @@
 // writes the results for comparison with the oracle.
 //
 // input : int64 header {cplx, d, chiL, chiR, DL, DR, nvec, ld}, then cx64 mpo, envL, envR, then x (ld x nvec, T)
-// output: y_primme (ld x nvec, T), y_single (n, T), y_free (n, T), expval (2 doubles), num_mv (int64)
+// output: y_primme (ld x nvec, T), y_single (n, T), y_free (n, T), expval (2 doubles), num_mv (int64),
+//         y_precond (ld x nvec, T; Jacobi with shifts 0.3, -0.7, 1.1, ...), y_shifted (n, T; H - 0.25 after
+//         set_shift(0.25)), num_pc (int64)
 #include "dmrg_amd/matvec_mpos.hpp"
+
+static void *g_op = nullptr; // the preconditioner callback's operator
+
+// Stand-in for PRIMME's parameter block: only the field the preconditioner callback reads.
+struct primme_params {
+    double *ShiftsForPreconditioner;
+};
+
+// The operator requirements of eig::solver::eigs_primme (solver_primme.cpp:297-517) and of the Jacobi callback
+// (optimize_energy.cpp:34-42), exercised the way they use them.
+enum class Preconditioner { NONE, JACOBI };
+template <typename M>
+static int solver_requirements(M &matrix, void *x, int ld, void *y, int nvec, double *shifts) {
+    using Scalar = typename M::Scalar;
+    static_assert(M::can_shift && M::can_precondition && !M::can_shift_invert, "operator capabilities");
+    static_assert(sizeof(Scalar) == 8 || sizeof(Scalar) == 16, "fp64 or cx64");
+    if(matrix.isReadyFactorOp()) return 7;
+    if(matrix.isReadyShift()) return 8;
+    matrix.preconditioner = Preconditioner::JACOBI;
+    matrix.set_jcbMaxBlockSize(1);
+    primme_params pp{shifts};
+    int           ldx = ld, ldy = ld, bs = nvec, err = 1;
+    void (*pc)(void *, int *, void *, int *, int *, primme_params *, int *) = [](void *a, int *la, void *b, int *lb, int *n,
+                                                                               primme_params *p, int *e) {
+        static_cast<M *>(g_op)->MultPc(a, la, b, lb, n, p, e);
+    };
+    g_op = &matrix;
+    pc(x, &ldx, y, &ldy, &bs, &pp, &err);
+    if(err != 0) return 9;
+    if(matrix.t_multOPv->get_time() != 0.0 || matrix.num_op != 0) return 10;
+    return 0;
+}
 
 #include <cstdio>
 #include <cstdlib>
@@ -57,6 +91,19 @@ static int run(FILE *in, FILE *out, const int64_t h[8]) {
     fwrite(y2.data(), sizeof(T), y2.size(), out);
     fwrite(evr, sizeof(double), 2, out);
     fwrite(&nmv, sizeof(int64_t), 1, out);
+    std::vector<T>      ypc(ld * nvec, T(0)), ysh(n);
+    std::vector<double> shifts(nvec);
+    for(int64_t i = 0; i < nvec; ++i) shifts[i] = (i % 3 == 0) ? 0.3 : (i % 3 == 1 ? -0.7 : 1.1);
+    if(const int rc = solver_requirements(op, x.data(), static_cast<int>(ld), ypc.data(), static_cast<int>(nvec), shifts.data()))
+        return rc;
+    op.set_shift(cx(0.25, 0.0));
+    if(!op.isReadyShift()) return 11;
+    op.MultAx(x.data(), ysh.data());
+    if(op.t_multAx->get_time() <= 0.0) return 12;
+    const int64_t npc = op.num_pc;
+    fwrite(ypc.data(), sizeof(T), ypc.size(), out);
+    fwrite(ysh.data(), sizeof(T), ysh.size(), out);
+    fwrite(&npc, sizeof(int64_t), 1, out);
     // Error path: a dimension mismatch surfaces as dmrg_amd::error with DMRG_ERR_DIMS.
     try {
         dmrg_amd::contraction::matrix_vector_product<T>(y2.data(), x.data(), {d, cL + 1, cR}, mpo_t.data(), {DL, DR, d, d},

@@ -56,3 +56,30 @@ def test_cpp_adaptor(tmp_path, cplx, shape):
     evr = O.expectation_value(th, W, L, R)
     assert abs(complex(ev[0], ev[1]) - evr) <= 1e-12 * abs(evr) * 10
     assert nmv == nvec + 1
+    # Jacobi preconditioner callback: y_i = x_i / (diag(H_eff) - shift_i)
+    Ypc = np.frombuffer(raw[off + 8:off + 8 + ld * nvec * isz].tobytes(), dtype=dt).reshape(ld, nvec, order="F")
+    off += 8 + ld * nvec * isz
+    ysh = np.frombuffer(raw[off:off + n * isz].tobytes(), dtype=dt)
+    off += n * isz
+    npc = np.frombuffer(raw[off:off + 8].tobytes(), dtype=np.int64)[0]
+    diag = heff_diagonal(W, L, R)
+    for j, sh in enumerate([0.3, -0.7, 1.1][:nvec]):
+        ref = X[:n, j] / (diag - sh)
+        assert np.linalg.norm(Ypc[:n, j] - ref) <= 1e-12 * np.linalg.norm(ref)
+    assert npc == nvec
+    # set_shift(0.25) subtracts 0.25 x (one MPO: the whole shift lands on it)
+    ref_sh = ref0 - 0.25 * th.reshape(-1, order="F")
+    assert np.linalg.norm(ysh - ref_sh) <= 1e-12 * np.linalg.norm(ref_sh)
+
+
+def heff_diagonal(W, L, R):
+    """diag H_eff[(s,a,b),(s,a,b)] = sum_{w,w'} L[a,a,w] W[w,w',s,s] R[b,b,w'] (from the brute-force definition)."""
+    Ld = np.einsum("aaw->aw", L)
+    Rd = np.einsum("bbv->bv", R)
+    Wd = np.einsum("wvss->wvs", W)
+    return np.einsum("aw,wvs,bv->sab", Ld, Wd, Rd).reshape(-1, order="F")
+
+
+def test_diagonal_helper_matches_brute_force():
+    th, W, L, R = random_heff_problem(2, 3, 4, 3, 2, seed=9, complex_=True)
+    assert np.allclose(heff_diagonal(W, L, R), np.diag(O.dense_heff(W, L, R)), rtol=0, atol=1e-13)
