@@ -319,6 +319,21 @@ class MatVecMPOS:
         finally:
             out.close()
 
+    def set_jcb_block_size(self, size: int) -> None:
+        """set_jcbMaxBlockSize: 1 = Jacobi, > 1 = block-Jacobi preconditioner in MultPc."""
+        call("dmrg_matvec_set_jcb_block_size", self.h, size)
+
+    def MultPc(self, x: np.ndarray, shifts) -> np.ndarray:
+        """Preconditioner callback on host vectors (MultPc, matvec_mpos.cpp:1333-1361): column i is
+        preconditioned with shifts[i]."""
+        x = np.asfortranarray(np.asarray(x, dtype=self.dtype))
+        x2 = x.reshape(self.rows(), -1, order="F")
+        y = np.empty_like(x2, order="F")
+        sh = np.ascontiguousarray(np.broadcast_to(np.asarray(shifts, dtype=np.float64), (x2.shape[1],)))
+        call("dmrg_matvec_precond", self.h, x2.ctypes.data_as(C.c_void_p), x2.shape[0], y.ctypes.data_as(C.c_void_p),
+             y.shape[0], x2.shape[1], sh.ctypes.data_as(C.POINTER(C.c_double)))
+        return y.reshape(x.shape, order="F")
+
     def set_shift(self, shift: complex) -> None:
         call("dmrg_matvec_set_shift", self.h, float(np.real(shift)), float(np.imag(shift)))
 
@@ -330,8 +345,8 @@ class MatVecMPOS:
         return res
 
     def eigs(self, nev: int = 1, target: str = "SA", tol: float = 1e-12, x0=None, mixed: bool = False,
-             precond: bool = False, ncv: int = 0, min_restart: int = 0, shift: float = 0.0,
-             max_matvecs: int = 100000, seed: int = 12345, stream=None):
+             precond: int = 0, ncv: int = 0, min_restart: int = 0, shift: float = 0.0,
+             max_matvecs: int = 100000, seed: int = 12345, jcb: int = 128, stream=None):
         """Eigenpairs on the GPU (eig::solver::eigs with PRIMME, solver.cpp:491-509). Returns
         (eigenvalues, eigenvectors (n, nev) numpy, result dict)."""
         cfg = _lib.EigsConfig()
@@ -339,7 +354,7 @@ class MatVecMPOS:
         cfg.nev, cfg.ncv, cfg.min_restart = nev, ncv, min_restart
         cfg.target = {"SA": 0, "LA": 1, "CLOSEST": 2}[target]
         cfg.shift, cfg.tol, cfg.max_matvecs = shift, tol, max_matvecs
-        cfg.precond, cfg.mixed, cfg.seed = int(precond), int(mixed), seed
+        cfg.precond, cfg.mixed, cfg.seed, cfg.jcb_block = int(precond), int(mixed), seed, jcb
         n = self.rows()
         dx0 = None
         if x0 is not None:

@@ -67,6 +67,7 @@ SIGNATURES: dict[str, tuple] = {
     "dmrg_matvec_set_shift": (st, [vp, C.c_double, C.c_double]),
     "dmrg_matvec_get_diagonal": (st, [vp, vp, vp]),
     "dmrg_matvec_precond": (st, [vp, vp, i64, vp, i64, C.c_int, dp]),
+    "dmrg_matvec_set_jcb_block_size": (st, [vp, i64]),
     "dmrg_host_heev": (st, [C.c_int, dp, dp, dp]),
     "dmrg_autotune_enable": (st, [C.c_int]),
     "dmrg_autotune_reset": (st, []),
@@ -84,7 +85,7 @@ SIGNATURES: dict[str, tuple] = {
 class EigsConfig(C.Structure):
     _fields_ = [("nev", C.c_int), ("ncv", C.c_int), ("min_restart", C.c_int), ("target", C.c_int),
                 ("shift", C.c_double), ("tol", C.c_double), ("max_matvecs", C.c_int64), ("precond", C.c_int),
-                ("mixed", C.c_int), ("max_time_s", C.c_double), ("seed", C.c_uint64)]
+                ("mixed", C.c_int), ("max_time_s", C.c_double), ("seed", C.c_uint64), ("jcb_block", C.c_int64)]
 
 
 class EigsResult(C.Structure):

@@ -356,6 +356,13 @@ dmrg_status dmrg_matvec_get_diagonal(dmrg_matvec *op, void *ddiag, void *stream)
     });
 }
 
+dmrg_status dmrg_matvec_set_jcb_block_size(dmrg_matvec *op, int64_t size) {
+    return guard([&] {
+        DMRG_REQUIRE(op, DMRG_ERR_ARG, "null operator");
+        reinterpret_cast<MatVecOp *>(op)->set_jcb(size);
+    });
+}
+
 dmrg_status dmrg_matvec_precond(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec,
                                 const double *shifts) {
     return guard([&] {
@@ -377,6 +384,7 @@ void dmrg_eigs_default_config(dmrg_eigs_config *cfg) {
     cfg->mixed       = 0;
     cfg->max_time_s  = 0.0;
     cfg->seed        = 12345;
+    cfg->jcb_block   = 128;
 }
 
 dmrg_status dmrg_eigs(dmrg_matvec *opp, const dmrg_eigs_config *cfg, const void *x0, void *evecs, dmrg_eigs_result *res,
@@ -398,6 +406,7 @@ dmrg_status dmrg_eigs(dmrg_matvec *opp, const dmrg_eigs_config *cfg, const void 
         p.precond     = cfg->precond;
         p.max_time_s  = cfg->max_time_s;
         p.seed        = cfg->seed;
+        p.jcb         = cfg->jcb_block;
         auto      s   = static_cast<hipStream_t>(stream);
         EigsStats st, s1;
         *res          = dmrg_eigs_result{};

@@ -13,6 +13,12 @@ namespace dmrg {
 void heff_dense(dmrg_dtype t, const Dims &D, const void *mpo, const void *envL, const void *envR, void *H, int64_t ldh,
                 bool transpose, hipStream_t s);
 
+// Diagonal blocks of (H_eff - shift I) for the block-Jacobi preconditioner (MatVecMPOS::get_diagonal_block,
+// matvec_mpos.cpp:500-771): nb = ceil(n / jcb) blocks of jcb x jcb at offsets k*jcb, stored one after the other
+// (column-major, leading dimension jcb). Rows/columns past n in the last block hold the identity.
+void heff_diag_blocks(dmrg_dtype t, const Dims &D, const void *mpo, const void *envL, const void *envR, int64_t jcb,
+                      double shift, void *out, hipStream_t s);
+
 // Edge block (chi, chi, Dm): zero except block[i, i, :] = edge[:] for every i < chi. Dm <= EDGE_MAX.
 constexpr int EDGE_MAX = 128;
 void env_set_edge(dmrg_dtype t, void *out, int64_t chi, int64_t Dm, const void *edge_host, hipStream_t s);

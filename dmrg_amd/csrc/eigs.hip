@@ -107,6 +107,7 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
     rbuf.reserve(vbytes);
     T      *Vp = V.as<T>(), *Wp = W.as<T>(), *r = rbuf.as<T>();
     double *sm = small.as<double>();
+    bool pc_built = false;
     if(p.precond == 1) {
         // Diagonal of H_eff in the operator's FP64 dtype; the kernel converts on the fly.
         diag.reserve(static_cast<size_t>(n) * dtype_size(op->dtype));
@@ -268,6 +269,24 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
         if(inject_random) {
             random_fill<T>(n, t, p.seed + 7919ull * (st.iters + 1), s);
             inject_random = false;
+        } else if(p.precond == 2) {
+            // Block-Jacobi: built once per operator state, at the first Ritz value that needs it (readyCalcPc,
+            // matvec_mpos.cpp:1086-1088); the FP64 phase of a mixed solve reuses the FP32 phase's blocks.
+            if(!pc_built) {
+                op->set_jcb(p.jcb);
+                if(!op->pc_ready) op->block_pc_build(w[ord[pstar]], s);
+                pc_built = true;
+            }
+            if constexpr(std::is_same_v<typename vtraits<T>::R, double>) {
+                op->block_pc_apply(r, t, s);
+            } else { // FP32 phase: the block inverses are FP64
+                using TD = std::conditional_t<vtraits<T>::cx, cpx<double>, double>;
+                pcw.reserve(2 * static_cast<size_t>(n) * sizeof(TD));
+                TD *r64 = pcw.as<TD>(), *t64 = r64 + n;
+                convert<TD, T>(n, r, r64, s);
+                op->block_pc_apply(r64, t64, s);
+                convert<T, TD>(n, t64, t, s);
+            }
         } else if(p.precond == 1) {
             const double theta = w[ord[pstar]];
             const double floor = 1e-8 * std::max(st.anorm, 1e-300);

@@ -21,7 +21,8 @@ struct EigsParams {
     double  shift       = 0.0; // sigma for TARGET_CLOSEST
     double  tol         = 1e-12; // converged when |r| <= tol * |A|_est
     int64_t max_matvecs = 100000;
-    int     precond     = 0; // 1: Jacobi (diagonal of H_eff)
+    int     precond     = 0; // 1: Jacobi (diagonal of H_eff), 2: block-Jacobi with blocks of jcb
+    int64_t jcb         = 128;
     int     plus_k      = 1; // previous Ritz vectors retained at restart
     double  max_time_s  = 0; // 0: no limit
     uint64_t seed       = 12345;
@@ -61,7 +62,7 @@ class Davidson {
     MatVecOp   *op;
     hipStream_t s;
     int64_t     n;
-    DevBuf      V, W, tmp, rbuf, scratch, small, Ydev, diag;
+    DevBuf      V, W, tmp, rbuf, scratch, small, Ydev, diag, pcw;
     double     *host = nullptr; // pinned
     void        matvec(const T *x, T *y, int nvec);
     void        orthonormalize_into(T *t, int m, double *dnrm_pre, double *dnrm_post);

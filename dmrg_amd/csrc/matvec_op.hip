@@ -102,6 +102,7 @@ void MatVecOp::upload_mpo() {
     upload_as<double>(mpo, mpo_host.data(), static_cast<int64_t>(mpo_host.size()), dtype);
     have32     = false;
     diag_valid = false;
+    pc_ready   = false;
 }
 
 void MatVecOp::ensure_fp32(hipStream_t s) {
@@ -170,6 +171,16 @@ void MatVecOp::precond_host(const void *x, int64_t ldx, void *y, int64_t ldy, in
     const int64_t n  = D.n();
     const size_t  es = dtype_size(dtype);
     DMRG_REQUIRE(ldx >= n && ldy >= n, DMRG_ERR_ARG, "leading dimension smaller than rows()");
+    if(jcb > 1) {
+        xbuf.reserve(static_cast<size_t>(nvec) * n * es);
+        ybuf.reserve(static_cast<size_t>(nvec) * n * es);
+        DMRG_HIP_CHECK(hipMemcpy2D(xbuf.p, n * es, x, ldx * es, n * es, nvec, hipMemcpyHostToDevice));
+        if(!pc_ready) block_pc_build(shifts ? shifts[0] : 0.0, nullptr);
+        for(int i = 0; i < nvec; ++i)
+            block_pc_apply(xbuf.as<char>() + static_cast<size_t>(i) * n * es, ybuf.as<char>() + static_cast<size_t>(i) * n * es, nullptr);
+        DMRG_HIP_CHECK(hipMemcpy2D(y, ldy * es, ybuf.p, n * es, n * es, nvec, hipMemcpyDeviceToHost));
+        return;
+    }
     if(!diag_valid) {
         diagc.reserve(static_cast<size_t>(n) * es);
         diagonal(diagc.p, nullptr);

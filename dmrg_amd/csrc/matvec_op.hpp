@@ -85,8 +85,23 @@ struct MatVecOp {
     void multax_host(const void *x, int64_t ldx, void *y, int64_t ldy, int nvec);
     void set_shift(std::complex<double> shift);
     void diagonal(void *out, hipStream_t s);
-    // Jacobi preconditioner on host buffers: y[:, i] = x[:, i] / (diag(H_eff) - shifts[i]).
+    // Preconditioner on host buffers: Jacobi y[:, i] = x[:, i] / (diag(H_eff) - shifts[i]) when jcb == 1,
+    // block-Jacobi (below) when jcb > 1.
     void precond_host(const void *x, int64_t ldx, void *y, int64_t ldy, int nvec, const double *shifts);
+
+    // Block-Jacobi preconditioner (jcbMaxBlockSize > 1, matvec_mpos.cpp:1086-1331 with LU factorisation): the
+    // inverses of the diagonal blocks of (H_eff - shift) at offsets k*jcb, built once at the first use after the
+    // operator, its shift or the block size changed (readyCalcPc), with the shift of that first use.
+    int64_t jcb      = 1;
+    bool    pc_ready = false;
+    double  pc_shift = 0.0;
+    DevBuf  pcinv, pcx, pcy, pcpiv, pcinfo;
+    void   *blas = nullptr; // rocblas_handle, created on first use
+    void    set_jcb(int64_t size);
+    void    block_pc_build(double shift, hipStream_t s);
+    // y = P x for one vector in the operator's dtype (device buffers of n elements).
+    void    block_pc_apply(const void *x, void *y, hipStream_t s);
+    ~MatVecOp();
     void time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms[4], hipStream_t s);
 };
 

@@ -1,0 +1,111 @@
+// Block-Jacobi preconditioner of the H_eff operator: explicit inverses of the diagonal blocks of
+// (H_eff - shift), factorised with batched LU (rocSOLVER getrf/getri, pivoted) and applied as one batched
+// GEMV (rocBLAS). Mirrors MatVecMPOS::CalcPc / MultPc for jcbMaxBlockSize > 1 with Factorization::LU
+// (matvec_mpos.cpp:1086-1331,1362-1378): blocks at offsets k*jcb, the last one possibly smaller.
+#include "dense.hpp"
+#include "matvec_op.hpp"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <vector>
+
+namespace dmrg {
+
+#define DMRG_ROCBLAS_OK(expr)                                                                                \
+    do {                                                                                                     \
+        rocblas_status st_ = (expr);                                                                         \
+        if(st_ != rocblas_status_success)                                                                    \
+            throw ::dmrg::Error(DMRG_ERR_HIP, std::string(#expr) + " failed: " + rocblas_status_to_string(st_)); \
+    } while(0)
+
+// Inverses of all blocks may not exceed this many bytes (n * jcb elements).
+constexpr size_t BLOCK_PC_MAX_BYTES = size_t(16) << 30;
+
+MatVecOp::~MatVecOp() {
+    if(blas) (void) rocblas_destroy_handle(static_cast<rocblas_handle>(blas));
+}
+
+void MatVecOp::set_jcb(int64_t size) {
+    DMRG_REQUIRE(size >= 1, DMRG_ERR_ARG, "block size must be >= 1");
+    if(size != jcb) pc_ready = false;
+    jcb = size;
+}
+
+void MatVecOp::block_pc_build(double shift, hipStream_t s) {
+    DMRG_REQUIRE(dtype == DMRG_FP64 || dtype == DMRG_CX64, DMRG_ERR_UNSUPPORTED, "block-Jacobi: FP64 or CX64 operator");
+    const int64_t n  = D.n();
+    const int64_t b  = std::min<int64_t>(jcb, n);
+    const int64_t nb = (n + b - 1) / b;
+    const size_t  es = dtype_size(dtype);
+    DMRG_REQUIRE(static_cast<size_t>(nb * b * b) * es <= BLOCK_PC_MAX_BYTES, DMRG_ERR_OOM,
+                 "block-Jacobi inverses would exceed " + std::to_string(BLOCK_PC_MAX_BYTES >> 30) + " GiB; use a smaller block size");
+    DMRG_REQUIRE(b < INT32_MAX && nb < INT32_MAX, DMRG_ERR_DIMS, "block-Jacobi: too many blocks");
+    if(!blas) {
+        rocblas_handle h = nullptr;
+        DMRG_ROCBLAS_OK(rocblas_create_handle(&h));
+        blas = h;
+    }
+    auto h = static_cast<rocblas_handle>(blas);
+    DMRG_ROCBLAS_OK(rocblas_set_stream(h, s));
+    pcinv.reserve(static_cast<size_t>(nb * b * b) * es);
+    pcpiv.reserve(static_cast<size_t>(nb * b) * sizeof(rocblas_int));
+    pcinfo.reserve(static_cast<size_t>(nb) * sizeof(rocblas_int));
+    heff_diag_blocks(dtype, D, mpo.p, envL_p, envR_p, b, shift, pcinv.p, s);
+    const rocblas_int  bi = static_cast<rocblas_int>(b), nbi = static_cast<rocblas_int>(nb);
+    const rocblas_stride sa = b * b, sp = b;
+    if(dtype == DMRG_FP64) {
+        DMRG_ROCBLAS_OK(rocsolver_dgetrf_strided_batched(h, bi, bi, pcinv.as<double>(), bi, sa, pcpiv.as<rocblas_int>(), sp,
+                                                         pcinfo.as<rocblas_int>(), nbi));
+        DMRG_ROCBLAS_OK(rocsolver_dgetri_strided_batched(h, bi, pcinv.as<double>(), bi, sa, pcpiv.as<rocblas_int>(), sp,
+                                                         pcinfo.as<rocblas_int>(), nbi));
+    } else {
+        auto *a = pcinv.as<rocblas_double_complex>();
+        DMRG_ROCBLAS_OK(rocsolver_zgetrf_strided_batched(h, bi, bi, a, bi, sa, pcpiv.as<rocblas_int>(), sp, pcinfo.as<rocblas_int>(), nbi));
+        DMRG_ROCBLAS_OK(rocsolver_zgetri_strided_batched(h, bi, a, bi, sa, pcpiv.as<rocblas_int>(), sp, pcinfo.as<rocblas_int>(), nbi));
+    }
+    std::vector<rocblas_int> info(nb);
+    DMRG_HIP_CHECK(hipMemcpyAsync(info.data(), pcinfo.p, nb * sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
+    DMRG_HIP_CHECK(hipStreamSynchronize(s));
+    for(int64_t k = 0; k < nb; ++k)
+        DMRG_REQUIRE(info[k] == 0, DMRG_ERR_NOT_CONVERGED,
+                     "block-Jacobi: diagonal block " + std::to_string(k) + " of (H_eff - shift) is singular");
+    pc_shift = shift;
+    pc_ready = true;
+}
+
+void MatVecOp::block_pc_apply(const void *x, void *y, hipStream_t s) {
+    DMRG_REQUIRE(pc_ready, DMRG_ERR_ARG, "block-Jacobi preconditioner not built");
+    const int64_t n  = D.n();
+    const int64_t b  = std::min<int64_t>(jcb, n);
+    const int64_t nb = (n + b - 1) / b;
+    const size_t  es = dtype_size(dtype);
+    auto          h  = static_cast<rocblas_handle>(blas);
+    DMRG_ROCBLAS_OK(rocblas_set_stream(h, s));
+    const void *xi = x;
+    void       *yo = y;
+    if(nb * b != n) { // ragged last block: pad to nb*b with zeros
+        pcx.reserve(static_cast<size_t>(nb * b) * es);
+        pcy.reserve(static_cast<size_t>(nb * b) * es);
+        DMRG_HIP_CHECK(hipMemcpyAsync(pcx.p, x, static_cast<size_t>(n) * es, hipMemcpyDeviceToDevice, s));
+        DMRG_HIP_CHECK(hipMemsetAsync(pcx.as<char>() + static_cast<size_t>(n) * es, 0, static_cast<size_t>(nb * b - n) * es, s));
+        xi = pcx.p;
+        yo = pcy.p;
+    }
+    const rocblas_int    bi = static_cast<rocblas_int>(b), nbi = static_cast<rocblas_int>(nb);
+    const rocblas_stride sa = b * b, sv = b;
+    if(dtype == DMRG_FP64) {
+        const double one = 1.0, zero = 0.0;
+        DMRG_ROCBLAS_OK(rocblas_dgemv_strided_batched(h, rocblas_operation_none, bi, bi, &one, pcinv.as<double>(), bi, sa,
+                                                      static_cast<const double *>(xi), 1, sv, &zero, static_cast<double *>(yo), 1, sv,
+                                                      nbi));
+    } else {
+        const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+        DMRG_ROCBLAS_OK(rocblas_zgemv_strided_batched(h, rocblas_operation_none, bi, bi, &one, pcinv.as<rocblas_double_complex>(),
+                                                      bi, sa, static_cast<const rocblas_double_complex *>(xi), 1, sv, &zero,
+                                                      static_cast<rocblas_double_complex *>(yo), 1, sv, nbi));
+    }
+    if(yo != y) DMRG_HIP_CHECK(hipMemcpyAsync(y, yo, static_cast<size_t>(n) * es, hipMemcpyDeviceToDevice, s));
+}
+
+} // namespace dmrg

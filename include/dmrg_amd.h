@@ -140,6 +140,10 @@ dmrg_status dmrg_matvec_get_diagonal(dmrg_matvec *op, void *ddiag, void *stream)
  * jcbMaxBlockSize = 1 (matvec_mpos.cpp:1333-1361): y[:, i] = x[:, i] / (diag(H_eff) - shifts[i]) for i < nvec
  * (shifts = PRIMME's ShiftsForPreconditioner; NULL means 0). The diagonal is computed once per operator state
  * (again after set_shift). */
+/* MatVecMPOS::set_jcbMaxBlockSize (matvec_mpos.h): 1 selects the Jacobi (diagonal) preconditioner of
+ * dmrg_matvec_precond, > 1 the block-Jacobi one (inverses of the diagonal blocks of (H_eff - shift) of this size,
+ * built at the next dmrg_matvec_precond call with its first shift, and again after set_shift or a size change). */
+dmrg_status dmrg_matvec_set_jcb_block_size(dmrg_matvec *op, int64_t size);
 dmrg_status dmrg_matvec_precond(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec,
                                 const double *shifts);
 
@@ -161,10 +165,13 @@ typedef struct dmrg_eigs_config {
     double   shift;       /* target shift for DMRG_TARGET_CLOSEST (primme_targetShifts) */
     double   tol;         /* eps: converged when |r| <= tol * |A|est (|A|est = largest |Ritz value| seen) */
     int64_t  max_matvecs; /* default 100000 (eigs_iter_max, settings.h:251) */
-    int      precond;     /* 0: none, 1: Jacobi (diagonal of H_eff, MultPc with jcb = 1) */
+    int      precond;     /* 0: none, 1: Jacobi (diagonal of H_eff, MultPc with jcb = 1), 2: block-Jacobi with
+                             blocks of jcb_block (MultPc with jcbMaxBlockSize > 1, LU; built once per solve at the
+                             first Ritz value) */
     int      mixed;       /* 0: FP64 throughout; 1: FP32 matrix-core phase then FP64 refinement */
     double   max_time_s;  /* 0: unlimited (reference: config.maxTime) */
     uint64_t seed;        /* seed of the random start vector when no initial guess is given */
+    int64_t  jcb_block;   /* block size of precond = 2 (jcbMaxBlockSize, default 128, settings.h:261) */
 } dmrg_eigs_config;
 
 typedef struct dmrg_eigs_result {
