@@ -270,11 +270,13 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
             random_fill<T>(n, t, p.seed + 7919ull * (st.iters + 1), s);
             inject_random = false;
         } else if(p.precond == 2) {
-            // Block-Jacobi: built once per operator state, at the first Ritz value that needs it (readyCalcPc,
-            // matvec_mpos.cpp:1086-1088); the FP64 phase of a mixed solve reuses the FP32 phase's blocks.
+            // Block-Jacobi: built once per operator state (readyCalcPc, matvec_mpos.cpp:1086-1088) at the target
+            // shift when there is one, else at the first Ritz value that needs it; the FP64 phase of a mixed solve
+            // reuses the FP32 phase's blocks.
             if(!pc_built) {
                 op->set_jcb(p.jcb);
-                if(!op->pc_ready) op->block_pc_build(w[ord[pstar]], s);
+                const double pc_shift = p.target == TARGET_CLOSEST ? p.shift : w[ord[pstar]];
+                if(!op->pc_ready || op->pc_shift != pc_shift) op->block_pc_build(pc_shift, s);
                 pc_built = true;
             }
             if constexpr(std::is_same_v<typename vtraits<T>::R, double>) {

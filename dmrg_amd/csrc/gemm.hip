@@ -13,8 +13,9 @@
 // waves run MFMA on the current one. Each wave owns a (BM/WM) x (BN/WN) sub-tile
 // built from 16x16 blocks:
 //   FP64: v_mfma_f64_16x16x4_f64,  FP32: v_mfma_f32_16x16x4_f32.
-// Complex products use the 4-real-product form (4 MFMAs per block per k-step),
-// which keeps FP64 rounding at the level of a real GEMM.
+// Complex products use the 3-real-product (Gauss) form, 3 MFMAs per block per k-step:
+//   P1 = sum ar br, P2 = sum ai bi, P3 = sum (ar + ai)(br + bi);  re = P1 - P2, im = P3 - P1 - P2.
+// Its rounding error is bounded by a small multiple of eps * sum |a||b|, as for the 4-product form.
 // The "swap" variant feeds B as the MFMA's first operand so that the lane index
 // of the accumulator runs along M; it is picked when C is unit-stride in M, so
 // the epilogue stores are coalesced either way.
@@ -70,23 +71,41 @@ constexpr int lds_pad(int row_elems, int elem_bytes) {
     return 0;
 }
 
+// LDS row pads (elements) per load direction. A tile stored [k][r] is written by lanes that walk r
+// (operand contiguous along r: rows stay contiguous, the read-friendly pad applies) or k (operand contiguous
+// along K: consecutive lanes write consecutive rows, so the row pitch must spread them over the banks).
+// 16-byte elements: one pad of 1 serves both. The K-direction pads were chosen with a bank model of
+// 64 banks x 4 bytes (and 32 x 4) over the fragment reads and the tile writes.
+constexpr int pad_rfast(int row_elems, int elem_bytes) { return elem_bytes >= 16 ? 1 : lds_pad(row_elems, elem_bytes); }
+constexpr int pad_kfast(int row_elems, int elem_bytes) { return (elem_bytes == 4 && row_elems >= 64) ? 9 : 1; }
+constexpr int imax(int a, int b) { return a > b ? a : b; }
+
 template <typename E>
 __device__ inline E zero_elem() {
     if constexpr(std::is_same_v<E, cpx<double>> || std::is_same_v<E, cpx<float>>) return E{0, 0};
     else return E(0);
 }
 
-// Branch-free masked load: an invalid element reads src[0] (present in any non-empty operand)
-// and is replaced by zero.
+// Element loads that never wait: the aligned loader reads through a raw buffer resource whose base is
+// the tile origin; an element outside the matrix gets an offset past the buffer's range, which the
+// hardware answers with zero and without touching memory. No branch or use of the value follows the
+// load, so the wait for it lands at the LDS store, after the MFMA work of the current K step.
+constexpr uint32_t BUF_RANGE = 0x7fffffffu; // bytes addressable through one tile resource
+constexpr uint32_t BUF_OOB   = 0x80000000u; // offset that reads as zero
+constexpr int      BUF_FLAGS = 0x00020000;  // dword 3 of a gfx9 buffer resource (32-bit data format)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const void *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), static_cast<short>(0), static_cast<int>(BUF_RANGE), BUF_FLAGS);
+}
 template <typename E>
-__device__ inline E masked_load(const E *src, int64_t off, bool valid, bool conj) {
-    E v = src[valid ? off : 0];
-    if constexpr(std::is_same_v<E, cpx<double>> || std::is_same_v<E, cpx<float>>) {
-        v.re = valid ? v.re : decltype(v.re)(0);
-        v.im = valid ? (conj ? -v.im : v.im) : decltype(v.im)(0);
-    } else {
-        v = valid ? v : E(0);
-    }
+__device__ __forceinline__ E buffer_load_elem(__amdgpu_buffer_rsrc_t r, uint32_t off_bytes) {
+    if constexpr(sizeof(E) == 16) return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(off_bytes), 0, 0));
+    else if constexpr(sizeof(E) == 8) return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(off_bytes), 0, 0));
+    else return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b32(r, static_cast<int>(off_bytes), 0, 0));
+}
+template <typename E>
+__device__ __forceinline__ E conj_elem(E v, bool conj) {
+    if constexpr(std::is_same_v<E, cpx<double>> || std::is_same_v<E, cpx<float>>) v.im = conj ? -v.im : v.im;
     return v;
 }
 
@@ -138,7 +157,8 @@ struct TileLoader {
     static constexpr int L = (RT * BK + NT - 1) / NT;
     int                  rr[L], kk[L];
     bool                 rval[L];
-    uint32_t             eoff[L];  // ALIGNED: static in-tile offset (elements)
+    bool                 lval[L];  // generic: validity of the element loaded last
+    uint32_t             eoff[L];  // ALIGNED: static in-tile offset (bytes)
     int64_t              roff[L];  // generic: offset of the non-K coordinate
     KCursor              cur[L];   // generic: K cursors
     KStep                st;
@@ -157,7 +177,7 @@ struct TileLoader {
             kk[e]           = fast_r ? (idx / RT) % BK : idx % BK;
             rval[e]         = used && (r0 + rr[e]) < rmax;
             if constexpr(ALIGNED) {
-                eoff[e] = static_cast<uint32_t>(idx_off(rdim, rr[e]) + idx_off(kd, kk[e]));
+                eoff[e] = static_cast<uint32_t>((idx_off(rdim, rr[e]) + idx_off(kd, kk[e])) * static_cast<int64_t>(sizeof(E)));
             } else {
                 roff[e] = rval[e] ? idx_off(rdim, r0 + rr[e]) : 0;
                 cur[e]  = make_kcursor(kd, kbeg + kk[e]);
@@ -170,31 +190,37 @@ struct TileLoader {
         }
     }
 
-    // Loads the tile at K offset k0 (ALIGNED) or at the cursors' position (generic), then advances.
-    __device__ __forceinline__ void load(E (&reg)[L], int k0, int kend, bool conj) {
+    // Issues the loads of the tile at K offset k0 (ALIGNED) or at the cursors' position (generic), then
+    // advances. Elements outside the matrix read as zero (ALIGNED) or are zeroed at store (generic).
+    __device__ __forceinline__ void load(E (&reg)[L], int k0, int kend) {
         if constexpr(ALIGNED) {
-            const E   *p     = tile_base + idx_off(kdim, static_cast<uint32_t>(k0));
+            const auto rs    = tile_rsrc(tile_base + idx_off(kdim, static_cast<uint32_t>(k0)));
             const bool kfull = k0 + BK <= kend;
 #pragma unroll
             for(int e = 0; e < L; ++e) {
                 const bool v = rval[e] && (kfull || k0 + kk[e] < kend);
-                reg[e]       = masked_load(p, static_cast<int64_t>(eoff[e]), v, conj);
+                reg[e]       = buffer_load_elem<E>(rs, v ? eoff[e] : BUF_OOB);
             }
         } else {
 #pragma unroll
             for(int e = 0; e < L; ++e) {
                 const bool v = rval[e] && cur[e].gk < kend;
-                reg[e]       = masked_load(tile_base, roff[e] + cur[e].off, v, conj);
+                lval[e]      = v;
+                reg[e]       = tile_base[v ? roff[e] + cur[e].off : 0];
                 advance(cur[e], st, BK);
             }
         }
     }
 
-    template <int LD>
-    __device__ __forceinline__ void store(E (*lds)[LD], const E (&reg)[L], int tid) const {
+    // Writes the tile into LDS laid out [k][r] with a row pitch of `pitch` elements.
+    __device__ __forceinline__ void store(E *lds, int pitch, const E (&reg)[L], int tid, bool conj) const {
 #pragma unroll
         for(int e = 0; e < L; ++e)
-            if(L * NT == RT * BK || tid + e * NT < RT * BK) lds[kk[e]][rr[e]] = reg[e];
+            if(L * NT == RT * BK || tid + e * NT < RT * BK) {
+                E v = reg[e];
+                if constexpr(!ALIGNED) v = lval[e] ? v : zero_elem<E>();
+                lds[kk[e] * pitch + rr[e]] = conj_elem(v, conj);
+            }
     }
 };
 
@@ -206,14 +232,15 @@ __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
     constexpr int NT      = 64 * WM * WN;
     constexpr int TM      = BM / (16 * WM);
     constexpr int TN      = BN / (16 * WN);
-    constexpr int PA      = lds_pad(BM, sizeof(E));
-    constexpr int PB      = lds_pad(BN, sizeof(E));
-    constexpr int NACC    = CPLX ? 3 : 1; // complex: sum ar*br, sum ai*bi, sum (ar*bi + ai*br)
+    constexpr int EB      = static_cast<int>(sizeof(E));
+    constexpr int LDA     = BM + imax(pad_rfast(BM, EB), pad_kfast(BM, EB)); // allocated row length of A's tile
+    constexpr int LDB     = BN + imax(pad_rfast(BN, EB), pad_kfast(BN, EB));
+    constexpr int NACC    = CPLX ? 3 : 1; // complex: sum ar*br, sum ai*bi, sum (ar+ai)*(br+bi)
     static_assert(TM >= 1 && TN >= 1, "wave tile too small");
     static_assert(BK % 4 == 0, "BK must be a multiple of the MFMA k");
 
-    __shared__ E As[2][BK][BM + PA];
-    __shared__ E Bs[2][BK][BN + PB];
+    __shared__ E As[2][BK * LDA];
+    __shared__ E Bs[2][BK * LDB];
 
     const int tid  = threadIdx.x;
     const int lane = tid & 63;
@@ -239,6 +266,9 @@ __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
     const int kbeg  = split * g.kchunk;
     const int kend  = min(g.K, kbeg + g.kchunk);
 
+    // Row pitches: A is loaded along M when g.a_fast_m, B along N when !g.b_fast_k.
+    const int pa = g.a_fast_m ? BM + pad_rfast(BM, EB) : BM + pad_kfast(BM, EB);
+    const int pb = !g.b_fast_k ? BN + pad_rfast(BN, EB) : BN + pad_kfast(BN, EB);
     TileLoader<E, BM, BK, NT, ALIGNED> la;
     TileLoader<E, BN, BK, NT, ALIGNED> lb;
     la.init(static_cast<const E *>(g.A) + b * g.a_batch, g.am, g.ak, m0, g.M, kbeg, g.a_fast_m, tid);
@@ -257,42 +287,47 @@ __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
     const int lrow = lane >> 4; // k row within the 4-deep MFMA step
     const int lcol = lane & 15; // m (or n) within a 16-wide block
     if(numK > 0) {
-        la.load(ra, kbeg, kend, g.conjA);
-        lb.load(rb, kbeg, kend, g.conjB);
-        la.store(As[0], ra, tid);
-        lb.store(Bs[0], rb, tid);
+        la.load(ra, kbeg, kend);
+        lb.load(rb, kbeg, kend);
+        la.store(As[0], pa, ra, tid, g.conjA);
+        lb.store(Bs[0], pb, rb, tid, g.conjB);
         __syncthreads();
     }
     for(int kt = 0; kt < numK; ++kt) {
         const int buf = kt & 1;
         if(kt + 1 < numK) {
-            la.load(ra, kbeg + (kt + 1) * BK, kend, g.conjA);
-            lb.load(rb, kbeg + (kt + 1) * BK, kend, g.conjB);
+            la.load(ra, kbeg + (kt + 1) * BK, kend);
+            lb.load(rb, kbeg + (kt + 1) * BK, kend);
         }
 #pragma unroll
         for(int ks = 0; ks < BK / 4; ++ks) {
             const int kr = ks * 4 + lrow;
             E         fa[TM], fb[TN];
 #pragma unroll
-            for(int i = 0; i < TM; ++i) fa[i] = As[buf][kr][(wm * TM + i) * 16 + lcol];
+            for(int i = 0; i < TM; ++i) fa[i] = As[buf][kr * pa + (wm * TM + i) * 16 + lcol];
 #pragma unroll
-            for(int j = 0; j < TN; ++j) fb[j] = Bs[buf][kr][(wn * TN + j) * 16 + lcol];
+            for(int j = 0; j < TN; ++j) fb[j] = Bs[buf][kr * pb + (wn * TN + j) * 16 + lcol];
+            R sa[CPLX ? TM : 1], sb[CPLX ? TN : 1];
+            if constexpr(CPLX) {
+#pragma unroll
+                for(int i = 0; i < TM; ++i) sa[i] = fa[i].re + fa[i].im;
+#pragma unroll
+                for(int j = 0; j < TN; ++j) sb[j] = fb[j].re + fb[j].im;
+            }
 #pragma unroll
             for(int i = 0; i < TM; ++i)
 #pragma unroll
                 for(int j = 0; j < TN; ++j) {
                     if constexpr(CPLX) {
-                        // (ar + i ai)(br + i bi): rr += ar br, ii += ai bi, ri += ar bi + ai br.
+                        // (ar + i ai)(br + i bi): P1 += ar br, P2 += ai bi, P3 += (ar + ai)(br + bi).
                         if constexpr(!SWAP) {
                             acc[0][i][j] = MF::op(fa[i].re, fb[j].re, acc[0][i][j]);
                             acc[1][i][j] = MF::op(fa[i].im, fb[j].im, acc[1][i][j]);
-                            acc[2][i][j] = MF::op(fa[i].re, fb[j].im, acc[2][i][j]);
-                            acc[2][i][j] = MF::op(fa[i].im, fb[j].re, acc[2][i][j]);
+                            acc[2][i][j] = MF::op(sa[i], sb[j], acc[2][i][j]);
                         } else {
                             acc[0][i][j] = MF::op(fb[j].re, fa[i].re, acc[0][i][j]);
                             acc[1][i][j] = MF::op(fb[j].im, fa[i].im, acc[1][i][j]);
-                            acc[2][i][j] = MF::op(fb[j].im, fa[i].re, acc[2][i][j]);
-                            acc[2][i][j] = MF::op(fb[j].re, fa[i].im, acc[2][i][j]);
+                            acc[2][i][j] = MF::op(sb[j], sa[i], acc[2][i][j]);
                         }
                     } else {
                         if constexpr(!SWAP) acc[0][i][j] = MF::op(fa[i], fb[j], acc[0][i][j]);
@@ -301,8 +336,8 @@ __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
                 }
         }
         if(kt + 1 < numK) {
-            la.store(As[buf ^ 1], ra, tid);
-            lb.store(Bs[buf ^ 1], rb, tid);
+            la.store(As[buf ^ 1], pa, ra, tid, g.conjA);
+            lb.store(Bs[buf ^ 1], pb, rb, tid, g.conjB);
         }
         __syncthreads();
     }
@@ -330,7 +365,10 @@ __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
                 }
                 if(gm < g.M && gn < g.N) {
                     E val;
-                    if constexpr(CPLX) val = E{acc[0][i][j][v] - acc[1][i][j][v], acc[2][i][j][v]};
+                    if constexpr(CPLX) {
+                        const R p1 = acc[0][i][j][v], p2 = acc[1][i][j][v];
+                        val        = E{p1 - p2, acc[2][i][j][v] - p1 - p2};
+                    }
                     else val = acc[0][i][j][v];
                     if(P) {
                         P[gm + static_cast<int64_t>(g.M) * gn] = val;
@@ -448,14 +486,15 @@ static LaunchPlan heuristic_plan(const GemmDesc &g) {
 static bool separable(const Idx2 &x, int ext) {
     return x.c >= 0x7fffffffu || (x.c % static_cast<uint32_t>(ext)) == 0 || (static_cast<uint32_t>(ext) % x.c) == 0;
 }
-// Largest in-tile offset must fit the 32-bit static offsets of the aligned loader.
-static bool small_tile_offsets(const Idx2 &r, int rt, const Idx2 &k, int bk) {
+// Largest in-tile offset (bytes) must stay inside the aligned loader's buffer range.
+static bool small_tile_offsets(const Idx2 &r, int rt, const Idx2 &k, int bk, int elem_bytes) {
     auto mx = [](const Idx2 &x, int ext) {
         int64_t m = 0;
         for(int t = 0; t < ext; ++t) m = std::max<int64_t>(m, std::llabs(idx_off(x, t)));
         return m;
     };
-    return mx(r, rt) + mx(k, bk) < (int64_t(1) << 31) && r.s0 >= 0 && r.s1 >= 0 && k.s0 >= 0 && k.s1 >= 0;
+    return (mx(r, rt) + mx(k, bk)) * elem_bytes < static_cast<int64_t>(BUF_RANGE) && r.s0 >= 0 && r.s1 >= 0 && k.s0 >= 0 &&
+           k.s1 >= 0;
 }
 
 template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN>
@@ -464,7 +503,8 @@ static void launch_cfg(const GemmDesc &g, bool swap, hipStream_t s) {
     const dim3 grid(tiles, g.ksplit, g.batch);
     const dim3 block(64 * WM * WN);
     const bool aligned = separable(g.am, BM) && separable(g.ak, BK) && separable(g.bn, BN) && separable(g.bk, BK) &&
-                         (g.kchunk % BK) == 0 && small_tile_offsets(g.am, BM, g.ak, BK) && small_tile_offsets(g.bn, BN, g.bk, BK);
+                         (g.kchunk % BK) == 0 && small_tile_offsets(g.am, BM, g.ak, BK, static_cast<int>(sizeof(elem_t<R, CPLX>))) &&
+                         small_tile_offsets(g.bn, BN, g.bk, BK, static_cast<int>(sizeof(elem_t<R, CPLX>)));
     if(aligned) {
         if(swap) hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, true, true>), grid, block, 0, s, g);
         else hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, false, true>), grid, block, 0, s, g);

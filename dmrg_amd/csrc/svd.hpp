@@ -9,8 +9,8 @@ namespace dmrg {
 
 // Thin SVD A = U diag(S) Vt of a device m x n column-major matrix (double or complex double) on rocSOLVER.
 // U: m x k, Vt: k x n (k = min(m, n)), S: descending, copied to the host. A is left untouched.
-// method: 0 bidiagonal QR (gesvd), 1 divide and conquer (gesdd), 2 one-sided Jacobi (gesvdj), the routines
-// the reference selects between in svd_lapacke.cpp:162-290.
+// method: 0 bidiagonal QR (gesvd). The reference also selects gesdd / gejsv (svd_lapacke.cpp:162-290); their
+// rocSOLVER counterparts (1 gesdd, 2 gesvdj) are rejected because they lose the small singular values.
 class DeviceSvd {
   public:
     explicit DeviceSvd(hipStream_t s);
@@ -29,6 +29,8 @@ class DeviceSvd {
     int   k() const { return k_; }
 
   private:
+    template <typename T>
+    void        sort_descending(int m, int n, std::vector<double> &S);
     hipStream_t s;
     void       *handle = nullptr;
     DevBuf      a, sv, u, vt, e, info, res;

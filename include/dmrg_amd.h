@@ -166,8 +166,8 @@ typedef struct dmrg_eigs_config {
     double   tol;         /* eps: converged when |r| <= tol * |A|est (|A|est = largest |Ritz value| seen) */
     int64_t  max_matvecs; /* default 100000 (eigs_iter_max, settings.h:251) */
     int      precond;     /* 0: none, 1: Jacobi (diagonal of H_eff, MultPc with jcb = 1), 2: block-Jacobi with
-                             blocks of jcb_block (MultPc with jcbMaxBlockSize > 1, LU; built once per solve at the
-                             first Ritz value) */
+                             blocks of jcb_block (MultPc with jcbMaxBlockSize > 1, LU), built at the target shift
+                             for DMRG_TARGET_CLOSEST, else at the first Ritz value */
     int      mixed;       /* 0: FP64 throughout; 1: FP32 matrix-core phase then FP64 refinement */
     double   max_time_s;  /* 0: unlimited (reference: config.maxTime) */
     uint64_t seed;        /* seed of the random start vector when no initial guess is given */
@@ -341,8 +341,9 @@ dmrg_status dmrg_env_set_edge(dmrg_dtype dtype, void *block_out, int64_t chi, in
  * capped at rank_max (> 0; 0 = no cap), at least 1 (svd.cpp:257-280). Outputs: U (device, m x rank, ld m),
  * Vt (device, rank x n, ld rank), S (host, all min(m,n) singular values, descending), rank,
  * truncation_error = ||S[rank:]||_2 (the reference's definition, a norm) and discarded_weight = sum S[rank:]^2
- * (either may be NULL). U and Vt must hold m*min(m,n) and min(m,n)*n elements. method: 0 gesvd (bidiagonal QR),
- * 1 gesdd (divide and conquer), 2 gesvdj (Jacobi), the routines of svd_lapacke.cpp:162-290.
+ * (either may be NULL). U and Vt must hold m*min(m,n) and min(m,n)*n elements. method: 0 = gesvd (bidiagonal
+ * QR, rocSOLVER). The reference can also pick gesdd or gejsv (svd_lapacke.cpp:162-290); methods 1 (gesdd) and
+ * 2 (gesvdj) return DMRG_ERR_UNSUPPORTED because rocSOLVER's versions lose the small singular values.
  * Synchronises the stream. */
 dmrg_status dmrg_svd_truncate(dmrg_dtype dtype, int64_t m, int64_t n, const void *A, double trunc_lim, int64_t rank_max,
                               int method, void *U, double *S, void *Vt, int64_t *rank, double *truncation_error,

@@ -64,11 +64,16 @@ def test_cpp_adaptor(tmp_path, cplx, shape):
     npc = np.frombuffer(raw[off:off + 8].tobytes(), dtype=np.int64)[0]
     diag = heff_diagonal(W, L, R)
     for j, sh in enumerate([0.3, -0.7, 1.1][:nvec]):
-        ref = X[:n, j] / (diag - sh)
-        assert np.linalg.norm(Ypc[:n, j] - ref) <= 1e-12 * np.linalg.norm(ref)
+        # backward error: (diag - shift) y = x, robust to small denominators
+        resid = (diag - sh) * Ypc[:n, j] - X[:n, j]
+        assert np.linalg.norm(resid) <= 1e-12 * np.linalg.norm(X[:n, j]) * np.max(np.abs(diag - sh))
     assert npc == nvec
-    # set_shift(0.25) subtracts 0.25 x (one MPO: the whole shift lands on it)
-    ref_sh = ref0 - 0.25 * th.reshape(-1, order="F")
+    # set_shift(0.25) adds (0 - 0.25) I to the MPO's bottom-left block W[DL-1, 0] (matvec_mpos.cpp:1433-1472)
+    Ws = np.array(W, dtype=np.complex128, order="F")
+    Ws[DL - 1, 0] += -0.25 * np.eye(d)
+    if not cplx:
+        Ws = Ws.real.copy(order="F")
+    ref_sh = O.matrix_vector_product(th, Ws, L, R).reshape(-1, order="F")
     assert np.linalg.norm(ysh - ref_sh) <= 1e-12 * np.linalg.norm(ref_sh)
 
 

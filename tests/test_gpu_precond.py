@@ -49,19 +49,37 @@ def test_multpc_matches_dense(dtype, jcb):
         op.close()
 
 
-@pytest.mark.parametrize("precond,jcb", [(1, 1), (2, 64), (2, 512)])
 @pytest.mark.parametrize("mixed", [False, True])
-def test_eigs_with_preconditioners(precond, jcb, mixed):
-    print(f"[case] eigs precond={precond} jcb={jcb} mixed={mixed}", file=sys.stderr, flush=True)
+def test_eigs_jacobi_ground_state(mixed):
+    print(f"[case] eigs Jacobi mixed={mixed}", file=sys.stderr, flush=True)
     th, W, L, R = random_heff_problem(2, 40, 40, 5, 5, seed=8)
     op = G.MatVecMPOS([W], L, R, dtype=np.complex128)
     try:
         w0, _, i0 = op.eigs(nev=1, tol=1e-12)
-        w1, v1, i1 = op.eigs(nev=1, tol=1e-12, precond=precond, jcb=jcb, mixed=mixed)
+        w1, v1, i1 = op.eigs(nev=1, tol=1e-12, precond=1, mixed=mixed)
         assert abs(w1[0] - w0[0]) <= 1e-10 * i0["anorm"]
         r = op.MultAx(v1[:, 0]) - w1[0] * v1[:, 0]
         assert np.linalg.norm(r) <= 1e-10 * i0["anorm"]
-        print(f"[info] matvecs none={i0['matvecs_fp64']} precond={i1['matvecs_fp64']}+{i1['matvecs_fp32']}",
-              file=sys.stderr, flush=True)
+    finally:
+        op.close()
+
+
+@pytest.mark.parametrize("jcb", [32, 128])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_eigs_block_jacobi_interior(jcb, mixed):
+    """Block-Jacobi built at the target shift (the setting of the reference's xDMRG solves): the eigenvalue
+    closest to an interior shift, checked against the dense spectrum."""
+    print(f"[case] eigs block-Jacobi jcb={jcb} mixed={mixed}", file=sys.stderr, flush=True)
+    th, W, L, R = random_heff_problem(2, 20, 20, 5, 5, seed=8)
+    op = G.MatVecMPOS([W], L, R, dtype=np.complex128)
+    try:
+        H = op.get_matrix(transpose=False)
+        w = np.linalg.eigvalsh(H)
+        sigma = w[6] + 0.1 * (w[5] - w[6])  # closest to w[6]
+        w1, v1, i1 = op.eigs(nev=1, target="CLOSEST", shift=sigma, tol=1e-12, precond=2, jcb=jcb, mixed=mixed)
+        assert abs(w1[0] - w[6]) <= 1e-10 * np.max(np.abs(w))
+        r = H @ v1[:, 0] - w1[0] * v1[:, 0]
+        assert np.linalg.norm(r) <= 1e-10 * np.max(np.abs(w))
+        print(f"[info] matvecs {i1['matvecs_fp64']}+{i1['matvecs_fp32']}", file=sys.stderr, flush=True)
     finally:
         op.close()
