@@ -297,6 +297,8 @@ def section_two_site(chi=1024, D=8, steps=10, tol=1e-10):
         dy.close()
     sol = {}
     vec = None
+    for mode in ("fp64", "mixed"):  # warm-up: first-use allocations of the eigensolver bases
+        op.eigs(nev=1, tol=tol, x0=th, mixed=(mode == "mixed"), max_matvecs=8)
     for mode in ("fp64", "mixed"):
         t0 = time.perf_counter()
         w, v, info = op.eigs(nev=1, tol=tol, x0=th, mixed=(mode == "mixed"))
