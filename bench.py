@@ -167,9 +167,19 @@ def section_eigs(chi=512, reps=6, tol=1e-10):
     op.eigs(nev=1, tol=tol, seed=999, mixed=True)
     res = {}
     energies = {}
+    n = op.rows()
+    dv, dy = G.DeviceArray((n,), np.complex128), G.DeviceArray((n,), np.complex128)
+
+    def variance(v):
+        """<H^2> - <H>^2 of the normalised vector v (FP64 matvec), the energy variance of the solution."""
+        v = v / np.linalg.norm(v)
+        G._lib.call("dmrg_memcpy_htod", dv.ptr, np.ascontiguousarray(v).ctypes.data_as(__import__("ctypes").c_void_p), v.nbytes)
+        op.MultAx_device(dv, dy)
+        e = G.dot(dv, dy)
+        return max(0.0, G.nrm2(dy) ** 2 - abs(e) ** 2)
+
     for mode in ("fp64", "mixed"):
-        lat, mv64, mv32, es, rel_r = [], [], [], [], []
-        t_all = time.perf_counter()
+        lat, mv64, mv32, es, rel_r, var = [], [], [], [], [], []
         for k in range(reps):
             t0 = time.perf_counter()
             w, v, info = op.eigs(nev=1, tol=tol, mixed=(mode == "mixed"), seed=1000 + k)
@@ -178,16 +188,22 @@ def section_eigs(chi=512, reps=6, tol=1e-10):
             mv32.append(info["matvecs_fp32"])
             es.append(w[0])
             rel_r.append(info["rnorm"][0] / info["anorm"])
-        total = time.perf_counter() - t_all
+            var.append(variance(v[:, 0]))
+        total = float(np.sum(lat))  # solve time only (the variance check is outside)
         energies[mode] = np.array(es)
         res[mode] = {"solves_per_s": round(reps / total, 3), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
                      "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2), "mean_matvecs_fp64": float(np.mean(mv64)),
-                     "mean_matvecs_fp32": float(np.mean(mv32)), "max_rel_residual": float(np.max(rel_r))}
+                     "mean_matvecs_fp32": float(np.mean(mv32)), "max_rel_residual": float(np.max(rel_r)),
+                     "median_variance": float(np.median(var)), "max_variance": float(np.max(var))}
+    dv.close()
+    dy.close()
     op.close()
     dE = float(np.max(np.abs(energies["mixed"] - energies["fp64"])))
     spread = float(np.max(energies["fp64"]) - np.min(energies["fp64"]))
     return {"workload": f"isolated H_eff chi={chi} cx64 (seed 1), {reps} cold starts per mode, tol {tol:g}*|H|",
             "fp64": res["fp64"], "mixed": res["mixed"], "max_abs_dE_mixed_vs_fp64": dE,
+            "variance_ratio_mixed_vs_fp64": res["mixed"]["median_variance"] / max(res["fp64"]["median_variance"], 1e-300),
+            "speedup_mixed_vs_fp64": round(res["mixed"]["solves_per_s"] / res["fp64"]["solves_per_s"], 3),
             "max_abs_dE_between_fp64_starts": spread, "E0": float(energies["fp64"][0])}
 
 

@@ -18,7 +18,8 @@ constexpr int SB_COL  = 0;                 // projected column, 2 * VEC_MAX_M
 constexpr int SB_NRM  = 2 * VEC_MAX_M;     // |t|^2 before / after orthogonalisation (2 x 2)
 constexpr int SB_RITZ = 2 * VEC_MAX_M + 4; // |r|^2, |x|^2
 constexpr int SB_Y    = 2 * VEC_MAX_M + 8; // Ritz coefficients, 2 * VEC_MAX_M
-constexpr int SB_SIZE = 4 * VEC_MAX_M + 8;
+constexpr int SB_FLAG = 4 * VEC_MAX_M + 8; // second-pass flag of the DGKS orthogonalisation (int)
+constexpr int SB_SIZE = 4 * VEC_MAX_M + 10;
 
 template <typename T>
 Davidson<T>::Davidson(MatVecOp &o, hipStream_t st) : op(&o), s(st), n(o.D.n()) {
@@ -47,6 +48,25 @@ void Davidson<T>::orthonormalize_into(T *t, int m, double *dnrm_pre, double *dnr
         gemv_update<T>(n, m, V.as<T>(), n, sm + SB_COL, -1.0, 1.0, t, s);
     }
     multidot<T>(n, 1, t, n, t, dnrm_post, scratch.as<double>(), s);
+    scale_by_inv_norm<T>(n, t, dnrm_post, s);
+}
+
+// t = V[:, m] <- src orthogonalised against V[:, 0:m] and normalised. One classical Gram-Schmidt pass whose
+// first projection also yields |src|^2 and whose update yields |t|^2; a second pass runs (decided on the
+// device, no host round trip) only when the norm dropped below 1/sqrt(2) of its value (DGKS criterion).
+// |src|^2 and the final |t|^2 land in dnrm_pre / dnrm_post for the breakdown test. src may be t itself.
+template <typename T>
+void Davidson<T>::expand_from(const T *src, T *t, int m, double *dnrm_pre, double *dnrm_post) {
+    double *sm   = small.as<double>();
+    int    *flag = reinterpret_cast<int *>(sm + SB_FLAG);
+    double *scr  = scratch.as<double>();
+    multidot_x<T>(n, m, V.as<T>(), n, src, sm + SB_COL, dnrm_pre, nullptr, scr, s);
+    gemv_update_norm<T>(n, m, V.as<T>(), n, sm + SB_COL, -1.0, 1.0, src, t, dnrm_post, nullptr, scr, s);
+    if(m > 0) {
+        dgks_flag(dnrm_pre, dnrm_post, 0.5, flag, s);
+        multidot_x<T>(n, m, V.as<T>(), n, t, sm + SB_COL, nullptr, flag, scr, s);
+        gemv_update_norm<T>(n, m, V.as<T>(), n, sm + SB_COL, -1.0, 1.0, t, t, dnrm_post, flag, scr, s);
+    }
     scale_by_inv_norm<T>(n, t, dnrm_post, s);
 }
 
@@ -150,17 +170,23 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
     int                 since_best = 0;
     bool                inject_random = false;
 
-    auto ritz = [&](int col_in_H, int m_, T *xout) -> double {
-        // Upload y (column col of Z), form x and r, return |r| / |x|.
+    // Ritz vector x = V y and residual r = W y - theta x for column col of Z; |r|^2, |x|^2 go to SB_RITZ.
+    auto ritz_launch = [&](int col_in_H, int m_, T *xout) {
         for(int i = 0; i < m_; ++i) {
             host[SB_Y + 2 * i]     = Z[i + static_cast<size_t>(m_) * col_in_H].real();
             host[SB_Y + 2 * i + 1] = Z[i + static_cast<size_t>(m_) * col_in_H].imag();
         }
         DMRG_HIP_CHECK(hipMemcpyAsync(sm + SB_Y, host + SB_Y, 2 * m_ * sizeof(double), hipMemcpyHostToDevice, s));
         ritz_residual<T>(n, m_, Vp, Wp, n, sm + SB_Y, w[col_in_H], xout, r, sm + SB_RITZ, scratch.as<double>(), s);
+    };
+    auto ritz_norm = [&]() { // |r| / |x| from the host copy of SB_RITZ
+        return std::sqrt(std::max(0.0, host[SB_RITZ])) / std::sqrt(std::max(1e-300, host[SB_RITZ + 1]));
+    };
+    auto ritz = [&](int col_in_H, int m_, T *xout) -> double {
+        ritz_launch(col_in_H, m_, xout);
         DMRG_HIP_CHECK(hipMemcpyAsync(host + SB_RITZ, sm + SB_RITZ, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
         DMRG_HIP_CHECK(hipStreamSynchronize(s));
-        return std::sqrt(std::max(0.0, host[SB_RITZ])) / std::sqrt(std::max(1e-300, host[SB_RITZ + 1]));
+        return ritz_norm();
     };
     auto rayleigh_ritz = [&]() {
         Hm.assign(static_cast<size_t>(m) * m, cx(0.0, 0.0));
@@ -172,10 +198,90 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
         sort_by_target(p, w, ord);
         for(int i = 0; i < m; ++i) st.anorm = std::max(st.anorm, std::fabs(w[i]));
     };
+    // t = M^-1 r with the configured preconditioner (theta: the Ritz value of the pair being refined).
+    auto apply_precond = [&](double theta, T *t) {
+        if(p.precond == 2) {
+            // Block-Jacobi: built once per operator state (readyCalcPc, matvec_mpos.cpp:1086-1088) at the target
+            // shift when there is one, else at the first Ritz value that needs it; the FP64 phase of a mixed solve
+            // reuses the FP32 phase's blocks.
+            if(!pc_built) {
+                op->set_jcb(p.jcb);
+                const double pc_shift = p.target == TARGET_CLOSEST ? p.shift : theta;
+                if(!op->pc_ready || op->pc_shift != pc_shift) op->block_pc_build(pc_shift, s);
+                pc_built = true;
+            }
+            if constexpr(std::is_same_v<typename vtraits<T>::R, double>) {
+                op->block_pc_apply(r, t, s);
+            } else { // FP32 phase: the block inverses are FP64
+                using TD = std::conditional_t<vtraits<T>::cx, cpx<double>, double>;
+                pcw.reserve(2 * static_cast<size_t>(n) * sizeof(TD));
+                TD *r64 = pcw.as<TD>(), *t64 = r64 + n;
+                convert<TD, T>(n, r, r64, s);
+                op->block_pc_apply(r64, t64, s);
+                convert<T, TD>(n, t64, t, s);
+            }
+        } else {
+            const double floor = 1e-8 * std::max(st.anorm, 1e-300);
+            if constexpr(vtraits<T>::cx) jacobi_precond<T, cpx<double>>(n, r, diag.as<cpx<double>>(), theta, floor, t, s);
+            else jacobi_precond<T, double>(n, r, diag.as<double>(), theta, floor, t, s);
+        }
+    };
+    // After the synchronisation that brought the new H column and the norms of the expansion vector to the host:
+    // grow the basis, or flag a breakdown (the correction lay in the basis) for a random expansion next time.
+    auto accept_expansion = [&](int &mm) {
+        const double pre = host[SB_NRM], post = host[SB_NRM + 2];
+        if(!(post > 1e4 * eps_t * eps_t * pre) || !std::isfinite(post)) {
+            if(!std::isfinite(post) || !std::isfinite(pre)) throw Error(DMRG_ERR_NAN, "eigensolver: non-finite expansion vector");
+            inject_random = true;
+            return;
+        }
+        for(int i = 0; i <= mm; ++i) Hat(i, mm) = cx(host[SB_COL + 2 * i], host[SB_COL + 2 * i + 1]);
+        for(int i = 0; i < mm; ++i) Hat(mm, i) = std::conj(Hat(i, mm));
+        Hat(mm, mm) = cx(Hat(mm, mm).real(), 0.0);
+        ++mm;
+    };
 
     std::vector<double> rn(nev, 0.0);
     while(true) {
         rayleigh_ritz();
+        // One wanted pair and no restart due: the residual check is deferred to the single synchronisation
+        // of the iteration, and the expansion is computed meanwhile (discarded if the pair has converged).
+        const bool speculate = nev == 1 && m + 1 <= mmax && m < n && st.matvecs < p.max_matvecs;
+        if(speculate) {
+            ritz_launch(ord[0], m, out);
+            yprev.assign(m, cx(0.0, 0.0));
+            for(int i = 0; i < m; ++i) yprev[i] = Z[i + static_cast<size_t>(m) * ord[0]];
+            T *t = Vp + static_cast<int64_t>(m) * n;
+            const T *src = r;
+            if(inject_random) {
+                random_fill<T>(n, t, p.seed + 7919ull * (st.iters + 1), s);
+                src           = t;
+                inject_random = false;
+            } else if(p.precond != 0) {
+                apply_precond(w[ord[0]], t);
+                src = t;
+            }
+            expand_from(src, t, m, sm + SB_NRM, sm + SB_NRM + 2);
+            matvec(t, Wp + static_cast<int64_t>(m) * n, 1);
+            st.matvecs += 1;
+            multidot<T>(n, m + 1, Vp, n, Wp + static_cast<int64_t>(m) * n, sm + SB_COL, scratch.as<double>(), s);
+            DMRG_HIP_CHECK(hipMemcpyAsync(host, sm, (SB_RITZ + 2) * sizeof(double), hipMemcpyDeviceToHost, s));
+            DMRG_HIP_CHECK(hipStreamSynchronize(s));
+            rn[0]    = ritz_norm();
+            st.nconv = rn[0] <= p.tol * std::max(st.anorm, 1e-300) ? 1 : 0;
+            if(st.nconv == 1) break; // converged: the expansion is dropped (m unchanged)
+            if(rn[0] < 0.9 * best_r) {
+                best_r     = rn[0];
+                since_best = 0;
+            } else if(++since_best >= p.stagnation_window && p.stagnation_window > 0) {
+                st.stagnated = true;
+                break;
+            }
+            if(p.max_time_s > 0 && std::chrono::duration<double>(clock::now() - t0).count() > p.max_time_s) break;
+            accept_expansion(m);
+            ++st.iters;
+            continue;
+        }
         // Residuals of the wanted pairs in order; stop at the first unconverged one (its r drives expansion).
         int pstar = -1;
         for(int q = 0; q < nev && q < m; ++q) {
@@ -265,54 +371,23 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
 
         // ---- expansion vector t = M^-1 r (or r), orthonormalised against V
         DMRG_REQUIRE(m < mmax, DMRG_ERR_ARG, "eigensolver: basis full after restart");
-        T *t = Vp + static_cast<int64_t>(m) * n;
+        T       *t   = Vp + static_cast<int64_t>(m) * n;
+        const T *src = r;
         if(inject_random) {
             random_fill<T>(n, t, p.seed + 7919ull * (st.iters + 1), s);
+            src           = t;
             inject_random = false;
-        } else if(p.precond == 2) {
-            // Block-Jacobi: built once per operator state (readyCalcPc, matvec_mpos.cpp:1086-1088) at the target
-            // shift when there is one, else at the first Ritz value that needs it; the FP64 phase of a mixed solve
-            // reuses the FP32 phase's blocks.
-            if(!pc_built) {
-                op->set_jcb(p.jcb);
-                const double pc_shift = p.target == TARGET_CLOSEST ? p.shift : w[ord[pstar]];
-                if(!op->pc_ready || op->pc_shift != pc_shift) op->block_pc_build(pc_shift, s);
-                pc_built = true;
-            }
-            if constexpr(std::is_same_v<typename vtraits<T>::R, double>) {
-                op->block_pc_apply(r, t, s);
-            } else { // FP32 phase: the block inverses are FP64
-                using TD = std::conditional_t<vtraits<T>::cx, cpx<double>, double>;
-                pcw.reserve(2 * static_cast<size_t>(n) * sizeof(TD));
-                TD *r64 = pcw.as<TD>(), *t64 = r64 + n;
-                convert<TD, T>(n, r, r64, s);
-                op->block_pc_apply(r64, t64, s);
-                convert<T, TD>(n, t64, t, s);
-            }
-        } else if(p.precond == 1) {
-            const double theta = w[ord[pstar]];
-            const double floor = 1e-8 * std::max(st.anorm, 1e-300);
-            if constexpr(vtraits<T>::cx) jacobi_precond<T, cpx<double>>(n, r, diag.as<cpx<double>>(), theta, floor, t, s);
-            else jacobi_precond<T, double>(n, r, diag.as<double>(), theta, floor, t, s);
-        } else {
-            DMRG_HIP_CHECK(hipMemcpyAsync(t, r, vbytes, hipMemcpyDeviceToDevice, s));
+        } else if(p.precond != 0) {
+            apply_precond(w[ord[pstar]], t);
+            src = t;
         }
-        orthonormalize_into(t, m, sm + SB_NRM, sm + SB_NRM + 2);
+        expand_from(src, t, m, sm + SB_NRM, sm + SB_NRM + 2);
         matvec(t, Wp + static_cast<int64_t>(m) * n, 1);
         st.matvecs += 1;
         multidot<T>(n, m + 1, Vp, n, Wp + static_cast<int64_t>(m) * n, sm + SB_COL, scratch.as<double>(), s);
         DMRG_HIP_CHECK(hipMemcpyAsync(host, sm, (SB_NRM + 4) * sizeof(double), hipMemcpyDeviceToHost, s));
         DMRG_HIP_CHECK(hipStreamSynchronize(s));
-        const double pre = host[SB_NRM], post = host[SB_NRM + 2];
-        if(!(post > 1e4 * eps_t * eps_t * pre) || !std::isfinite(post)) {
-            if(!std::isfinite(post) || !std::isfinite(pre)) throw Error(DMRG_ERR_NAN, "eigensolver: non-finite expansion vector");
-            inject_random = true; // breakdown: the correction lies in the basis
-        } else {
-            for(int i = 0; i <= m; ++i) Hat(i, m) = cx(host[SB_COL + 2 * i], host[SB_COL + 2 * i + 1]);
-            for(int i = 0; i < m; ++i) Hat(m, i) = std::conj(Hat(i, m));
-            Hat(m, m) = cx(Hat(m, m).real(), 0.0);
-            ++m;
-        }
+        accept_expansion(m);
         ++st.iters;
     }
 

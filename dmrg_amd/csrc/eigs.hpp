@@ -66,6 +66,7 @@ class Davidson {
     double     *host = nullptr; // pinned
     void        matvec(const T *x, T *y, int nvec);
     void        orthonormalize_into(T *t, int m, double *dnrm_pre, double *dnrm_post);
+    void        expand_from(const T *src, T *t, int m, double *dnrm_pre, double *dnrm_post);
     void        project_column(int m, std::vector<std::complex<double>> &H, int ldh, int col);
     void        transform_basis(T *B, int m, const std::vector<std::complex<double>> &Q, int k);
 };

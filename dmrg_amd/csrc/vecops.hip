@@ -58,11 +58,13 @@ static int grid_for(int64_t n) {
 // ------------------------------------------------------------------------------------------- multidot
 constexpr int MD_CH = 8; // basis columns per pass
 
+// Columns j < m are V[:, j]; column m (when with_self) is x itself, so x^H x comes out of the same pass.
 template <typename T>
-__global__ __launch_bounds__(VT_THREADS) void multidot_partial(int64_t n, int m, const T *V, int64_t ldv, const T *x,
-                                                               double *partial) {
+__global__ __launch_bounds__(VT_THREADS) void multidot_partial(int64_t n, int m, int mc, const T *V, int64_t ldv, const T *x,
+                                                               const int *skip, double *partial) {
+    if(skip && *skip == 0) return;
     __shared__ double red[2 * MD_CH][VT_THREADS / 64];
-    for(int j0 = 0; j0 < m; j0 += MD_CH) {
+    for(int j0 = 0; j0 < mc; j0 += MD_CH) {
         double acc[2 * MD_CH];
 #pragma unroll
         for(int c = 0; c < 2 * MD_CH; ++c) acc[c] = 0.0;
@@ -71,9 +73,10 @@ __global__ __launch_bounds__(VT_THREADS) void multidot_partial(int64_t n, int m,
             ld(x[i], xr, xi);
 #pragma unroll
             for(int c = 0; c < MD_CH; ++c) {
-                if(j0 + c < m) {
+                if(j0 + c < mc) {
                     double vr, vi;
-                    ld(V[i + (j0 + c) * ldv], vr, vi);
+                    if(j0 + c < m) ld(V[i + (j0 + c) * ldv], vr, vi);
+                    else vr = xr, vi = xi;
                     acc[2 * c] += vr * xr + vi * xi;     // conj(v) x
                     acc[2 * c + 1] += vr * xi - vi * xr;
                 }
@@ -83,16 +86,18 @@ __global__ __launch_bounds__(VT_THREADS) void multidot_partial(int64_t n, int m,
         if(threadIdx.x == 0)
 #pragma unroll
             for(int c = 0; c < MD_CH; ++c)
-                if(j0 + c < m) {
-                    partial[2 * (static_cast<int64_t>(blockIdx.x) * m + j0 + c)]     = acc[2 * c];
-                    partial[2 * (static_cast<int64_t>(blockIdx.x) * m + j0 + c) + 1] = acc[2 * c + 1];
+                if(j0 + c < mc) {
+                    partial[2 * (static_cast<int64_t>(blockIdx.x) * mc + j0 + c)]     = acc[2 * c];
+                    partial[2 * (static_cast<int64_t>(blockIdx.x) * mc + j0 + c) + 1] = acc[2 * c + 1];
                 }
     }
 }
 
-// out[j] = sum_b partial[b][j] for j < m: one block per column, each thread a fixed strided subset of the
-// partials, then a fixed-order tree (deterministic).
-__global__ __launch_bounds__(VT_THREADS) void fold_columns(int nblk, int m, const double *partial, double *out) {
+// out[j] = sum_b partial[b][j] for j < m (column j == m_out goes to self_out instead, when given): one block
+// per column, each thread a fixed strided subset of the partials, then a fixed-order tree (deterministic).
+__global__ __launch_bounds__(VT_THREADS) void fold_columns(int nblk, int m, const double *partial, double *out, int m_out,
+                                                           double *self_out, const int *skip) {
+    if(skip && *skip == 0) return;
     __shared__ double red[2][VT_THREADS / 64];
     const int         j   = blockIdx.x;
     double            acc[2] = {0.0, 0.0};
@@ -102,8 +107,9 @@ __global__ __launch_bounds__(VT_THREADS) void fold_columns(int nblk, int m, cons
     }
     bsum<2>(acc, red);
     if(threadIdx.x == 0) {
-        out[2 * j]     = acc[0];
-        out[2 * j + 1] = acc[1];
+        double *o = (j == m_out && self_out) ? self_out : out + 2 * j;
+        o[0]      = acc[0];
+        o[1]      = acc[1];
     }
 }
 
@@ -112,8 +118,20 @@ void multidot(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_o
     if(m <= 0) return;
     DMRG_REQUIRE(m <= VEC_MAX_M, DMRG_ERR_ARG, "multidot: too many columns");
     const int g = grid_for(n);
-    hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, V, ldv, x, scratch);
-    hipLaunchKernelGGL(fold_columns, dim3(m), dim3(VT_THREADS), 0, s, g, m, scratch, d_out);
+    hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, m, V, ldv, x, nullptr, scratch);
+    hipLaunchKernelGGL(fold_columns, dim3(m), dim3(VT_THREADS), 0, s, g, m, scratch, d_out, -1, nullptr, nullptr);
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void multidot_x(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_out, double *d_self, const int *skip,
+                double *scratch, hipStream_t s) {
+    const int mc = m + (d_self ? 1 : 0);
+    if(mc <= 0) return;
+    DMRG_REQUIRE(mc <= VEC_MAX_M, DMRG_ERR_ARG, "multidot: too many columns");
+    const int g = grid_for(n);
+    hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, mc, V, ldv, x, skip, scratch);
+    hipLaunchKernelGGL(fold_columns, dim3(mc), dim3(VT_THREADS), 0, s, g, mc, scratch, d_out, d_self ? m : -1, d_self, skip);
     DMRG_HIP_CHECK(hipGetLastError());
 }
 
@@ -138,12 +156,67 @@ __global__ __launch_bounds__(VT_THREADS) void gemv_update_kernel(int64_t n, int 
     }
 }
 
+// y = beta x + alpha V h (x may alias y) and per-block partials of |y|^2.
+template <typename T>
+__global__ __launch_bounds__(VT_THREADS) void gemv_update_norm_kernel(int64_t n, int m, const T *V, int64_t ldv, const double *h,
+                                                                      double alpha, double beta, const T *x, T *y,
+                                                                      const int *skip, double *partial) {
+    if(skip && *skip == 0) return;
+    __shared__ double hs[2 * VEC_MAX_M];
+    __shared__ double red[2][VT_THREADS / 64];
+    for(int j = threadIdx.x; j < 2 * m; j += VT_THREADS) hs[j] = h[j];
+    __syncthreads();
+    double acc[2] = {0.0, 0.0};
+    for(int64_t i = blockIdx.x * static_cast<int64_t>(VT_THREADS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * VT_THREADS) {
+        double ar = 0, ai = 0;
+        for(int j = 0; j < m; ++j) {
+            double vr, vi;
+            ld(V[i + j * ldv], vr, vi);
+            ar += vr * hs[2 * j] - vi * hs[2 * j + 1];
+            ai += vr * hs[2 * j + 1] + vi * hs[2 * j];
+        }
+        double xr = 0, xi = 0;
+        if(beta != 0.0) ld(x[i], xr, xi);
+        const T yv = mk<T>(beta * xr + alpha * ar, beta * xi + alpha * ai);
+        y[i]       = yv;
+        double yr, yi;
+        ld(yv, yr, yi); // the norm of the stored (possibly FP32-rounded) vector
+        acc[0] += yr * yr + yi * yi;
+    }
+    bsum<2>(acc, red);
+    if(threadIdx.x == 0) {
+        partial[2 * blockIdx.x]     = acc[0];
+        partial[2 * blockIdx.x + 1] = 0.0;
+    }
+}
+
+// flag = 1 when |t|^2 after a projection fell below eta2 times its value before (the DGKS test for a
+// second orthogonalisation pass), else 0.
+__global__ void dgks_kernel(const double *pre, const double *post, double eta2, int *flag) {
+    if(threadIdx.x == 0 && blockIdx.x == 0) flag[0] = (post[0] < eta2 * pre[0]) ? 1 : 0;
+}
+
 static int ew_grid(int64_t n) { return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(8192, (n + VT_THREADS - 1) / VT_THREADS))); }
 
 template <typename T>
 void gemv_update(int64_t n, int m, const T *V, int64_t ldv, const double *d_h, double alpha, double beta, T *x, hipStream_t s) {
     DMRG_REQUIRE(m <= VEC_MAX_M, DMRG_ERR_ARG, "gemv: too many columns");
     hipLaunchKernelGGL(gemv_update_kernel<T>, dim3(ew_grid(n)), dim3(VT_THREADS), 0, s, n, m, V, ldv, d_h, alpha, beta, x);
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void gemv_update_norm(int64_t n, int m, const T *V, int64_t ldv, const double *d_h, double alpha, double beta, const T *x, T *y,
+                      double *d_nrm2, const int *skip, double *scratch, hipStream_t s) {
+    DMRG_REQUIRE(m <= VEC_MAX_M, DMRG_ERR_ARG, "gemv: too many columns");
+    const int g = grid_for(n);
+    hipLaunchKernelGGL(gemv_update_norm_kernel<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, V, ldv, d_h, alpha, beta, x, y, skip, scratch);
+    hipLaunchKernelGGL(fold_columns, dim3(1), dim3(VT_THREADS), 0, s, g, 1, scratch, d_nrm2, -1, nullptr, skip);
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+void dgks_flag(const double *d_pre, const double *d_post, double eta2, int *d_flag, hipStream_t s) {
+    hipLaunchKernelGGL(dgks_kernel, dim3(1), dim3(64), 0, s, d_pre, d_post, eta2, d_flag);
     DMRG_HIP_CHECK(hipGetLastError());
 }
 
@@ -187,7 +260,7 @@ void ritz_residual(int64_t n, int m, const T *V, const T *W, int64_t ld, const d
     DMRG_REQUIRE(m <= VEC_MAX_M, DMRG_ERR_ARG, "ritz: too many columns");
     const int g = grid_for(n);
     hipLaunchKernelGGL(ritz_kernel<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, V, W, ld, d_y, theta, x, r, scratch);
-    hipLaunchKernelGGL(fold_columns, dim3(1), dim3(VT_THREADS), 0, s, g, 1, scratch, d_out);
+    hipLaunchKernelGGL(fold_columns, dim3(1), dim3(VT_THREADS), 0, s, g, 1, scratch, d_out, -1, nullptr, nullptr);
     DMRG_HIP_CHECK(hipGetLastError());
 }
 
@@ -280,7 +353,11 @@ void convert(int64_t n, const TI *in, TO *out, hipStream_t s) {
                                    double *, double *, hipStream_t);                                                   \
     template void scale_by_inv_norm<T>(int64_t, T *, const double *, hipStream_t);                                    \
     template void scale<T>(int64_t, T *, double, hipStream_t);                                                         \
-    template void random_fill<T>(int64_t, T *, uint64_t, hipStream_t);
+    template void random_fill<T>(int64_t, T *, uint64_t, hipStream_t);                                               \
+    template void multidot_x<T>(int64_t, int, const T *, int64_t, const T *, double *, double *, const int *, double *,  \
+                                hipStream_t);                                                                          \
+    template void gemv_update_norm<T>(int64_t, int, const T *, int64_t, const double *, double, double, const T *, T *,  \
+                                      double *, const int *, double *, hipStream_t);
 DMRG_VEC_INST(double)
 DMRG_VEC_INST(float)
 DMRG_VEC_INST(cpx<double>)

@@ -43,6 +43,17 @@ constexpr int64_t vec_scratch_doubles() { return 2LL * VEC_MAX_BLOCKS * VEC_MAX_
 // out[j] = sum_i conj(V[i, j]) x[i], j < m (complex double, out on device, 2*m doubles).
 template <typename T>
 void multidot(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_out, double *scratch, hipStream_t s);
+// out[j] = sum_i conj(V[i, j]) x[i] for j < m and, when d_self is given, d_self[0] = |x|^2 from the same pass.
+// skip (device int, optional): when *skip == 0 the kernels do nothing (outputs keep their values).
+template <typename T>
+void multidot_x(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_out, double *d_self, const int *skip,
+                double *scratch, hipStream_t s);
+// y = beta x + alpha V h (x may alias y) and d_nrm2[0] = |y|^2; skip as above.
+template <typename T>
+void gemv_update_norm(int64_t n, int m, const T *V, int64_t ldv, const double *d_h, double alpha, double beta, const T *x, T *y,
+                      double *d_nrm2, const int *skip, double *scratch, hipStream_t s);
+// d_flag[0] = (d_post[0] < eta2 * d_pre[0]): whether a second orthogonalisation pass is needed (DGKS).
+void dgks_flag(const double *d_pre, const double *d_post, double eta2, int *d_flag, hipStream_t s);
 // x = beta x + alpha V h   (h: complex double, m entries, device).
 template <typename T>
 void gemv_update(int64_t n, int m, const T *V, int64_t ldv, const double *d_h, double alpha, double beta, T *x, hipStream_t s);
