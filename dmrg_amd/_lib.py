@@ -73,6 +73,7 @@ SIGNATURES: dict[str, tuple] = {
     "dmrg_autotune_reset": (st, []),
     "dmrg_autotune_entries": (C.c_int, []),
     "dmrg_gemm_plan_override": (st, [C.c_int, C.c_int]),
+    "dmrg_gemm_hybrid_override": (st, [C.c_int, C.c_int, C.c_int]),
     "dmrg_matvec_time_legs": (st, [vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_float), vp]),
     "dmrg_heff_dense": (st, [C.c_int, vp, i64, vp, i64p, vp, i64p, vp, i64p, C.c_int, vp]),
     "dmrg_matvec_get_matrix": (st, [vp, vp, i64, C.c_int, vp]),

@@ -481,9 +481,16 @@ dmrg_status dmrg_autotune_reset(void) {
     return guard([&] { clear_tuning(); });
 }
 int         dmrg_autotune_entries(void) { return tuned_entries(); }
+dmrg_status dmrg_gemm_hybrid_override(int cfg, int nsplit, int ksplit2) {
+    return guard([&] {
+        DMRG_REQUIRE(cfg >= 0 && cfg <= static_cast<int>(TileCfg::T64x32) && nsplit > 0 && ksplit2 >= 1 && ksplit2 <= 8,
+                     DMRG_ERR_ARG, "bad hybrid plan override");
+        set_plan_override(cfg, 1, nsplit, ksplit2);
+    });
+}
 dmrg_status dmrg_gemm_plan_override(int cfg, int ksplit) {
     return guard([&] {
-        DMRG_REQUIRE(cfg < 6 && ksplit >= 0 && ksplit <= 8, DMRG_ERR_ARG, "bad plan override");
+        DMRG_REQUIRE(cfg <= static_cast<int>(TileCfg::SMALLK) && ksplit >= 0 && ksplit <= 8, DMRG_ERR_ARG, "bad plan override");
         set_plan_override(cfg, ksplit);
     });
 }

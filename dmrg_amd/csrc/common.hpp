@@ -98,14 +98,18 @@ struct GemmDesc {
 };
 
 // Tile configuration id (see gemm.hip for the table).
-enum class TileCfg : int { T64x64 = 0, T32x64 = 1, T64x32 = 2, T16x128 = 3, T128x64 = 4, T32x32 = 5, AUTO = -1 };
+// SMALLK: a column-streaming kernel for GEMMs whose K (and M) are tiny, such as the MPO leg of the matvec
+// (K = d * D); it is HBM-bound and skips the tile machinery. GEMMs outside its limits fall back to T64x64.
+enum class TileCfg : int { T64x64 = 0, T32x64 = 1, T64x32 = 2, T16x128 = 3, T128x64 = 4, T32x32 = 5, SMALLK = 6, AUTO = -1 };
+constexpr int SMALLK_MAX_K = 32;
+constexpr int SMALLK_MAX_M = 64;
 
 // Launch one GEMM. cfg/ksplit = AUTO/0 pick from the tuning table or the size heuristic.
 void gemm(const GemmDesc &g, hipStream_t stream, TileCfg cfg = TileCfg::AUTO, int ksplit = 0);
 int  device_cu_count();
 // Autotuner controls: on/off, a forced plan for every GEMM (cfg < 0 clears it), cache reset and size.
 void set_autotune(int on);
-void set_plan_override(int cfg, int ksplit);
+void set_plan_override(int cfg, int ksplit, int nsplit = 0, int ksplit2 = 1);
 void clear_tuning();
 int  tuned_entries();
 

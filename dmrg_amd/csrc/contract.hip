@@ -63,7 +63,7 @@ void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *o
     const int64_t nT2 = (cL >= cR) ? cR * DR * d * cL : cL * DL * cR * d; // second intermediate per vector
     void         *T1  = w0;
     void         *T2  = w0 + static_cast<size_t>(nvec) * nT1 * es;
-    void         *PT  = w0 + static_cast<size_t>(nvec) * (nT1 + nT2) * es; // split-K partials of leg 3
+    void         *PT  = w0 + static_cast<size_t>(nvec) * (nT1 + nT2) * es; // split-K partials (leg 1, then leg 3)
     const int64_t PC  = MAX_KSPLIT * D.n() * nvec;
 
     if(cL >= cR) {
@@ -73,6 +73,7 @@ void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *o
         g1.A = theta, g1.am = fused(1, d * cL, d), g1.ak = plain(d), g1.a_batch = ldx;
         g1.B = envL, g1.bk = plain(1), g1.bn = plain(cL);
         g1.C = T1, g1.cm = plain(1), g1.cn = plain(d * cR), g1.c_batch = nT1;
+        g1.part = PT, g1.part_capacity = PC; // leg 3's partial slabs are free until leg 3
         gemm(g1, s);
         mark(1);
         // Leg 2: T2[b,w',s',a'] = sum_{s,w} W[w,w',s,s'] T1[s,b,a',w]  m=(w',s') k=(s,w) n=(b,a')
@@ -98,6 +99,7 @@ void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *o
         g1.A = theta, g1.am = plain(1), g1.ak = plain(d * cL), g1.a_batch = ldx;
         g1.B = envR, g1.bk = plain(1), g1.bn = plain(cR);
         g1.C = T1, g1.cm = plain(1), g1.cn = plain(d * cL), g1.c_batch = nT1;
+        g1.part = PT, g1.part_capacity = PC;
         gemm(g1, s);
         mark(1);
         // Leg 2: V[a,w,b',s'] = sum_{s,w'} W[w,w',s,s'] U[s,a,b',w']    m=(w,s') k=(s,w') n=(a,b')

@@ -389,6 +389,54 @@ __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
             }
 }
 
+// GEMM with tiny K and M (K <= SMALLK_MAX_K, M <= SMALLK_MAX_M): A sits in LDS, every thread streams one column
+// n of B (K elements, kept in registers) and writes the M elements of C's column n. Consecutive threads take
+// consecutive n, so B and C are read and written along their n index maps; the kernel is HBM-bound.
+template <typename R, bool CPLX, int KMAX>
+__global__ __launch_bounds__(256) void smallk_kernel(const GemmDesc g) {
+    using E = elem_t<R, CPLX>;
+    __shared__ E As[SMALLK_MAX_M * SMALLK_MAX_K];
+    const int    b  = blockIdx.y;
+    const E     *A  = static_cast<const E *>(g.A) + b * g.a_batch;
+    const E     *B  = static_cast<const E *>(g.B) + b * g.b_batch;
+    E           *C  = static_cast<E *>(g.C) + b * g.c_batch;
+    for(int t = threadIdx.x; t < g.M * g.K; t += blockDim.x) {
+        const int m = t % g.M, k = t / g.M;
+        As[t]       = conj_elem(A[idx_off(g.am, m) + idx_off(g.ak, k)], g.conjA != 0);
+    }
+    __syncthreads();
+    for(int64_t n = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; n < g.N; n += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t boff = idx_off(g.bn, static_cast<uint32_t>(n));
+        const int64_t coff = idx_off(g.cn, static_cast<uint32_t>(n));
+        E             bv[KMAX];
+#pragma unroll
+        for(int k = 0; k < KMAX; ++k)
+            if(k < g.K) bv[k] = conj_elem(B[boff + idx_off(g.bk, k)], g.conjB != 0);
+        for(int m = 0; m < g.M; ++m) {
+            R ar = 0, ai = 0;
+#pragma unroll
+            for(int k = 0; k < KMAX; ++k)
+                if(k < g.K) {
+                    const E a = As[m + g.M * k];
+                    if constexpr(CPLX) {
+                        ar = fma(a.re, bv[k].re, fma(-a.im, bv[k].im, ar));
+                        ai = fma(a.re, bv[k].im, fma(a.im, bv[k].re, ai));
+                    } else {
+                        ar = fma(a, bv[k], ar);
+                    }
+                }
+            E *dst = C + coff + idx_off(g.cm, m);
+            if constexpr(CPLX) {
+                if(g.accumulate) ar += dst->re, ai += dst->im;
+                *dst = E{ar, ai};
+            } else {
+                if(g.accumulate) ar += *dst;
+                *dst = ar;
+            }
+        }
+    }
+}
+
 // Sums the split-K partial slabs in a fixed order and writes C through its index map.
 template <typename E>
 __global__ void splitk_reduce(const GemmDesc g) {
@@ -455,9 +503,14 @@ int device_cu_count() {
     return cus;
 }
 
+// A plan: tile configuration and K slices; with nsplit > 0 the columns [0, nsplit) run as one launch without
+// K slicing and the tail [nsplit, N) as a second launch with ksplit2 K slices, so that the tail (the part of
+// the grid that would occupy the GPU only partially) is spread over all CUs.
 struct LaunchPlan {
     TileCfg cfg;
     int     ksplit;
+    int     nsplit  = 0;
+    int     ksplit2 = 1;
 };
 
 static LaunchPlan heuristic_plan(const GemmDesc &g) {
@@ -535,6 +588,19 @@ static void launch_typed(const GemmDesc &g, TileCfg cfg, bool swap, hipStream_t 
         case TileCfg::T16x128: launch_cfg<R, CPLX, 16, 128, BH, 1, 4>(g, swap, s); break;
         case TileCfg::T128x64: launch_cfg<R, CPLX, 128, 64, BH, 2, 2>(g, swap, s); break;
         case TileCfg::T32x32: launch_cfg<R, CPLX, 32, 32, BK, 2, 2>(g, swap, s); break;
+        case TileCfg::SMALLK: {
+            if(g.K > SMALLK_MAX_K || g.M > SMALLK_MAX_M) { // outside the kernel's limits
+                launch_cfg<R, CPLX, 64, 64, BK, 2, 2>(g, swap, s);
+                break;
+            }
+            const int64_t blocks = std::min<int64_t>((static_cast<int64_t>(g.N) + 255) / 256, 8192);
+            const dim3    grid(static_cast<unsigned>(blocks), g.batch);
+            if(g.K <= 8) hipLaunchKernelGGL((smallk_kernel<R, CPLX, 8>), grid, dim3(256), 0, s, g);
+            else if(g.K <= 16) hipLaunchKernelGGL((smallk_kernel<R, CPLX, 16>), grid, dim3(256), 0, s, g);
+            else hipLaunchKernelGGL((smallk_kernel<R, CPLX, SMALLK_MAX_K>), grid, dim3(256), 0, s, g);
+            if(g.ev_after_main) DMRG_HIP_CHECK(hipEventRecord(static_cast<hipEvent_t>(g.ev_after_main), s));
+            break;
+        }
         default: throw Error(DMRG_ERR_ARG, "bad tile config");
     }
 }
@@ -549,6 +615,7 @@ static std::mutex                        g_tune_mu;
 static std::map<std::string, LaunchPlan> g_tuned;
 static int                               g_autotune     = 1;
 static int                               g_override_cfg = -1, g_override_ks = 0;
+static int                               g_override_nsplit = 0, g_override_ks2 = 1;
 
 static std::string tune_key(const GemmDesc &g) {
     char buf[512];
@@ -558,7 +625,32 @@ static std::string tune_key(const GemmDesc &g) {
     return std::string(buf) + I(g.am) + "|" + I(g.ak) + "|" + I(g.bk) + "|" + I(g.bn) + "|" + I(g.cm) + "|" + I(g.cn);
 }
 
-static void launch_plan(GemmDesc g, const LaunchPlan &p0, hipStream_t stream) {
+static void launch_plan_single(GemmDesc g, const LaunchPlan &p0, hipStream_t stream);
+
+// Columns [n0, n0 + nn) of g as a GEMM of their own (valid when the B and C column maps are separable at n0).
+static GemmDesc column_slice(const GemmDesc &g, int n0, int nn) {
+    GemmDesc h     = g;
+    const size_t es = dtype_size(g.dtype);
+    h.B            = static_cast<const char *>(g.B) + idx_off(g.bn, static_cast<uint32_t>(n0)) * static_cast<int64_t>(es);
+    h.C            = static_cast<char *>(g.C) + idx_off(g.cn, static_cast<uint32_t>(n0)) * static_cast<int64_t>(es);
+    h.N            = nn;
+    h.ev_after_main = nullptr;
+    return h;
+}
+static bool column_separable(const Idx2 &x, int n0) { return x.c >= 0x7fffffffu || (static_cast<uint32_t>(n0) % x.c) == 0; }
+
+static void launch_plan(const GemmDesc &g, const LaunchPlan &p, hipStream_t stream) {
+    if(p.nsplit > 0 && p.nsplit < g.N && g.batch == 1 && column_separable(g.bn, p.nsplit) && column_separable(g.cn, p.nsplit)) {
+        GemmDesc head = column_slice(g, 0, p.nsplit), tail = column_slice(g, p.nsplit, g.N - p.nsplit);
+        launch_plan_single(head, LaunchPlan{p.cfg, 1}, stream);
+        tail.ev_after_main = g.ev_after_main;
+        launch_plan_single(tail, LaunchPlan{p.cfg, p.ksplit2}, stream);
+        return;
+    }
+    launch_plan_single(g, LaunchPlan{p.cfg, p.ksplit}, stream);
+}
+
+static void launch_plan_single(GemmDesc g, const LaunchPlan &p0, hipStream_t stream) {
     LaunchPlan p = p0;
     if(p.ksplit > 1) {
         const int64_t need = static_cast<int64_t>(g.M) * g.N * g.batch * p.ksplit;
@@ -582,9 +674,64 @@ static void launch_plan(GemmDesc g, const LaunchPlan &p0, hipStream_t stream) {
     DMRG_HIP_CHECK(hipGetLastError());
 }
 
+// Workgroups of configuration t that one CU holds at once (aligned, swapped variant as the representative).
+template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN>
+static int occupancy_of() {
+    int nb = 0;
+    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, true, true>, 64 * WM * WN, 0) != hipSuccess)
+        return 0;
+    return nb;
+}
+template <typename R, bool CPLX>
+static int occupancy_typed(TileCfg t) {
+    constexpr int E  = static_cast<int>(sizeof(elem_t<R, CPLX>));
+    constexpr int BK = E == 4 ? 32 : 16;
+    switch(t) {
+        case TileCfg::T64x64: return occupancy_of<R, CPLX, 64, 64, BK, 2, 2>();
+        case TileCfg::T32x64: return occupancy_of<R, CPLX, 32, 64, BK, 2, 2>();
+        case TileCfg::T64x32: return occupancy_of<R, CPLX, 64, 32, BK, 2, 2>();
+        default: return 0;
+    }
+}
+static int resident_blocks(dmrg_dtype dt, TileCfg t) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_pair(static_cast<int>(dt), static_cast<int>(t));
+    auto       it  = cache.find(key);
+    if(it != cache.end()) return it->second;
+    int v = 0;
+    switch(dt) {
+        case DMRG_FP64: v = occupancy_typed<double, false>(t); break;
+        case DMRG_CX64: v = occupancy_typed<double, true>(t); break;
+        case DMRG_FP32: v = occupancy_typed<float, false>(t); break;
+        case DMRG_CX32: v = occupancy_typed<float, true>(t); break;
+    }
+    cache[key] = v;
+    return v;
+}
+
 static std::vector<LaunchPlan> candidates(const GemmDesc &g) {
     std::vector<LaunchPlan> c;
     const TileCfg cfgs[] = {TileCfg::T64x64, TileCfg::T32x64, TileCfg::T64x32, TileCfg::T32x32, TileCfg::T128x64, TileCfg::T16x128};
+    if(g.K <= SMALLK_MAX_K && g.M <= SMALLK_MAX_M) c.push_back({TileCfg::SMALLK, 1});
+    // Hybrid plans: whole waves of tiles without K slicing, the remaining tile columns sliced over K.
+    if(g.part && g.batch == 1 && g.K >= 128) {
+        const TileCfg hcfgs[] = {TileCfg::T64x64, TileCfg::T32x64, TileCfg::T64x32};
+        for(TileCfg t : hcfgs) {
+            const CfgShape sh    = cfg_shape(t);
+            const int      tm    = (g.M + sh.BM - 1) / sh.BM, tn = (g.N + sh.BN - 1) / sh.BN;
+            const int64_t  slots = static_cast<int64_t>(device_cu_count()) * resident_blocks(g.dtype, t);
+            if(slots <= 0 || static_cast<int64_t>(tm) * tn <= slots) continue;
+            const int64_t full_cols = (static_cast<int64_t>(tm) * tn / slots) * slots / tm; // tile columns in whole waves
+            if(full_cols <= 0 || full_cols >= tn) continue;
+            const int nsplit = static_cast<int>(full_cols * sh.BN);
+            for(int ks2 : {2, 4, 8}) {
+                if(g.K < 64 * ks2 || g.part_capacity < static_cast<int64_t>(g.M) * (g.N - nsplit) * ks2) continue;
+                c.push_back({t, 1, nsplit, ks2});
+            }
+        }
+    }
     for(TileCfg t : cfgs) {
         if(t == TileCfg::T16x128 && g.M > 32) continue;
         if(t == TileCfg::T128x64 && g.M < 128) continue;
@@ -651,9 +798,9 @@ static void load_tune_cache_locked() {
         std::string l(line);
         const size_t sp = l.rfind(' ');
         if(sp == std::string::npos) continue;
-        int cfg = 0, ks = 1;
-        if(std::sscanf(l.c_str() + sp + 1, "%d:%d", &cfg, &ks) != 2) continue;
-        g_tuned[l.substr(0, sp)] = LaunchPlan{static_cast<TileCfg>(cfg), ks};
+        int cfg = 0, ks = 1, nsplit = 0, ks2 = 1;
+        if(std::sscanf(l.c_str() + sp + 1, "%d:%d:%d:%d", &cfg, &ks, &nsplit, &ks2) < 2) continue;
+        g_tuned[l.substr(0, sp)] = LaunchPlan{static_cast<TileCfg>(cfg), ks, nsplit, ks2};
     }
     std::fclose(f);
 }
@@ -661,15 +808,17 @@ static void append_tune_cache_locked(const std::string &key, const LaunchPlan &p
     const char *path = tune_cache_path();
     if(!path) return;
     if(FILE *f = std::fopen(path, "a")) {
-        std::fprintf(f, "%s %d:%d\n", key.c_str(), static_cast<int>(p.cfg), p.ksplit);
+        std::fprintf(f, "%s %d:%d:%d:%d\n", key.c_str(), static_cast<int>(p.cfg), p.ksplit, p.nsplit, p.ksplit2);
         std::fclose(f);
     }
 }
 
 void set_autotune(int on) { g_autotune = on; }
-void set_plan_override(int cfg, int ks) {
-    g_override_cfg = cfg;
-    g_override_ks  = ks;
+void set_plan_override(int cfg, int ks, int nsplit, int ks2) {
+    g_override_cfg    = cfg;
+    g_override_ks     = ks;
+    g_override_nsplit = nsplit;
+    g_override_ks2    = ks2;
 }
 void clear_tuning() {
     std::lock_guard<std::mutex> lock(g_tune_mu);
@@ -691,7 +840,7 @@ void gemm(const GemmDesc &gin, hipStream_t stream, TileCfg cfg, int ksplit) {
     g.b_fast_k = fast_score(g.bk) >= fast_score(g.bn) ? 1 : 0;
     LaunchPlan p;
     if(g_override_cfg >= 0) {
-        p = {static_cast<TileCfg>(g_override_cfg), std::max(1, g_override_ks)};
+        p = {static_cast<TileCfg>(g_override_cfg), std::max(1, g_override_ks), g_override_nsplit, std::max(1, g_override_ks2)};
     } else if(cfg != TileCfg::AUTO) {
         p = {cfg, std::max(1, ksplit)};
     } else {
@@ -719,7 +868,7 @@ void gemm(const GemmDesc &gin, hipStream_t stream, TileCfg cfg, int ksplit) {
                 append_tune_cache_locked(key, p);
             }
         }
-        if(ksplit > 0) p.ksplit = ksplit;
+        if(ksplit > 0) p = LaunchPlan{p.cfg, ksplit};
     }
     launch_plan(g, p, stream);
 }

@@ -298,9 +298,14 @@ dmrg_status dmrg_host_heev(int m, const double *A, double *w, double *Z);
 dmrg_status dmrg_autotune_enable(int enable);
 dmrg_status dmrg_autotune_reset(void);
 int         dmrg_autotune_entries(void);
-/* Testing hook: force every GEMM to tile configuration cfg (0..5) with ksplit K slices; cfg < 0 restores the
- * tuned/heuristic choice. */
+/* Testing hook: force every GEMM to tile configuration cfg (0..5: 64x64, 32x64, 64x32, 16x128, 128x64, 32x32;
+ * 6: the column-streaming kernel for tiny K, which falls back to 64x64 beyond K = 32 or M = 64) with ksplit K
+ * slices; cfg < 0 restores the tuned/heuristic choice. */
 dmrg_status dmrg_gemm_plan_override(int cfg, int ksplit);
+/* Testing hook: force every GEMM to a hybrid plan of configuration cfg (0..2): output columns [0, nsplit) in one
+ * launch without K slicing, the rest in a second launch with ksplit2 K slices (GEMMs without split-K scratch,
+ * or whose column maps do not split at nsplit, run as one launch). Cleared by dmrg_gemm_plan_override(-1, 0). */
+dmrg_status dmrg_gemm_hybrid_override(int cfg, int nsplit, int ksplit2);
 
 /* Profiling: runs nrep device matvecs (one vector, x -> y) on stream and returns average durations measured with
  * HIP events on that stream: ms[0] leg 1 kernel, ms[1] leg 2 (MPO) kernel, ms[2] leg 3 main kernel, ms[3] leg 3

@@ -213,7 +213,7 @@ def plan_override():
     _lib.call("dmrg_gemm_plan_override", -1, 0)
 
 
-@pytest.mark.parametrize("cfg", range(6), ids=["64x64", "32x64", "64x32", "16x128", "128x64", "32x32"])
+@pytest.mark.parametrize("cfg", range(7), ids=["64x64", "32x64", "64x32", "16x128", "128x64", "32x32", "smallk"])
 @pytest.mark.parametrize("ks", [1, 3, 8])
 def test_every_tile_config_matvec_and_env(cfg, ks, plan_override):
     """Each tile configuration, with and without K slicing, on aligned (chi=64) and ragged shapes."""
@@ -228,6 +228,25 @@ def test_every_tile_config_matvec_and_env(cfg, ks, plan_override):
         th, W, L, R = general(rng, *shape, True)
         out = G.matrix_vector_product(*(x.astype(np.complex64) for x in (th, W, L, R)))
         assert rel(out, O.matrix_vector_product(th, W, L, R)) <= TOL32
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2], ids=["64x64", "32x64", "64x32"])
+@pytest.mark.parametrize("nsplit,ks2", [(64, 2), (128, 4), (192, 8)])
+def test_hybrid_plans(cfg, nsplit, ks2):
+    """Hybrid plans: columns [0, nsplit) in one launch, the tail sliced over K, on aligned and ragged shapes."""
+    _lib.call("dmrg_gemm_hybrid_override", cfg, nsplit, ks2)
+    try:
+        rng = np.random.default_rng(cfg * 100 + nsplit + ks2)
+        for shape in [(2, 64, 64, 5, 5), (2, 96, 40, 3, 4), (4, 48, 80, 8, 8), (2, 130, 120, 5, 5)]:
+            for cplx in (False, True):
+                th, W, L, R = general(rng, *shape, cplx)
+                assert rel(G.matrix_vector_product(th, W, L, R), O.matrix_vector_product(th, W, L, R)) <= TOL64
+                assert rel(G.env_build_block("R", R, th, W), O.env_build_block("R", R, th, W)) <= TOL64
+            th, W, L, R = general(rng, *shape, True)
+            out = G.matrix_vector_product(*(x.astype(np.complex64) for x in (th, W, L, R)))
+            assert rel(out, O.matrix_vector_product(th, W, L, R)) <= TOL32
+    finally:
+        _lib.call("dmrg_gemm_plan_override", -1, 0)
 
 
 def test_autotuner_caches_plans():
