@@ -332,7 +332,7 @@ def section_two_site(chi=1024, D=8, steps=10, tol=1e-10):
     dU, dV = G.DeviceArray((m, k), np.complex128), G.DeviceArray((k, M.shape[1]), np.complex128)
     S = np.empty(k)
     svd = {"matrix": f"{m}x{M.shape[1]} complex, keep rank {chi}"}
-    for method, name in ((0, "gesvd"),):
+    for method, name in ((3, "jacobi"), (0, "gesvd")):
         rank, terr, disc = C.c_int64(), C.c_double(), C.c_double()
         t0 = time.perf_counter()
         G._lib.call("dmrg_svd_truncate", G.DMRG_CX64, m, M.shape[1], dM.ptr, 1e-14, chi, method, dU.ptr,

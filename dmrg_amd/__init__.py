@@ -193,8 +193,9 @@ def env_set_edge(chi: int, edge, stream=None) -> np.ndarray:
         out.close()
 
 
-def svd_truncate(A, trunc_lim: float = 1e-14, rank_max: int = 0, method: int = 0, stream=None):
-    """Truncated SVD on the GPU (svd::solver, svd_lapacke.cpp:82-330; rank rule svd.cpp:257-280).
+def svd_truncate(A, trunc_lim: float = 1e-14, rank_max: int = 0, method: int = 3, stream=None):
+    """Truncated SVD on the GPU (svd::solver, svd_lapacke.cpp:82-330; rank rule svd.cpp:257-280); method 3 is the
+    one-sided Jacobi SVD (gesvj), 0 rocSOLVER's gesvd.
 
     Returns dict(U (m x r), S (all singular values), Vt (r x n), rank, truncation_error = |S[r:]|_2,
     discarded_weight = sum S[r:]^2). A may be a numpy matrix or a DeviceArray."""

@@ -9,8 +9,12 @@ namespace dmrg {
 
 // Thin SVD A = U diag(S) Vt of a device m x n column-major matrix (double or complex double) on rocSOLVER.
 // U: m x k, Vt: k x n (k = min(m, n)), S: descending, copied to the host. A is left untouched.
-// method: 0 bidiagonal QR (gesvd). The reference also selects gesdd / gejsv (svd_lapacke.cpp:162-290); their
-// rocSOLVER counterparts (1 gesdd, 2 gesvdj) are rejected because they lose the small singular values.
+// method: 0 bidiagonal QR (rocSOLVER gesvd), 3 one-sided Jacobi (this library's kernels; the gesvj routine the
+// reference picks for small problems, svd.cpp:122-124). The reference also selects gesdd (svd_lapacke.cpp:162-290);
+// rocSOLVER's 1 gesdd and 2 gesvdj are rejected because they lose the small singular values.
+constexpr int SVD_GESVD  = 0;
+constexpr int SVD_JACOBI = 3;
+
 class DeviceSvd {
   public:
     explicit DeviceSvd(hipStream_t s);
@@ -24,6 +28,8 @@ class DeviceSvd {
     // eigenvectors when vectors is set; w receives the n eigenvalues in ascending order.
     template <typename T>
     void eigh(int n, T *A, int lda, bool vectors, std::vector<double> &w);
+    template <typename T>
+    void jacobi(int m, int n, const T *A, std::vector<double> &S);
     void *U() const { return u.p; }
     void *Vt() const { return vt.p; }
     int   k() const { return k_; }

@@ -297,7 +297,7 @@ struct Fdmrg {
         remap2d(DT, d * a, d * b, theta.p, fused(1, d * d, d), fused(d, d * d * a, d), theta2.p, plain(1), plain(d * a), nullptr, 0, s);
         const auto          t0 = clock::now();
         std::vector<double> S;
-        svd.compute<T>(static_cast<int>(d * a), static_cast<int>(d * b), theta2.as<T>(), S);
+        svd.compute<T>(static_cast<int>(d * a), static_cast<int>(d * b), theta2.as<T>(), S, SVD_JACOBI);
         auto [r, te] = rank_from_truncation_error(S, cfg.trunc_lim > 0 ? cfg.trunc_lim : 1e-14, cfg.chi_max);
         double nrm = 0;
         for(int k = 0; k < r; ++k) nrm += S[k] * S[k];
@@ -343,7 +343,7 @@ struct Fdmrg {
         std::vector<double> S;
         if(right && c + 1 < L) {
             // theta as (d a) x b: A_c = U, M_{c+1} <- (S Vt) M_{c+1}
-            svd.compute<T>(static_cast<int>(d * a), static_cast<int>(b), theta.as<T>(), S);
+            svd.compute<T>(static_cast<int>(d * a), static_cast<int>(b), theta.as<T>(), S, SVD_JACOBI);
             const int k = svd.k();
             sdev.reserve(S.size() * sizeof(double));
             DMRG_HIP_CHECK(hipMemcpyAsync(sdev.p, S.data(), k * sizeof(double), hipMemcpyHostToDevice, s));
@@ -370,7 +370,7 @@ struct Fdmrg {
             // theta as a x (d b): X[a, (s, b)] = theta[s + d a + d a_dim b]; B_c = Vt, M_{c-1} <- M_{c-1} (U S)
             theta2.reserve(static_cast<size_t>(n) * es());
             remap2d(DT, a, d * b, theta.p, plain(d), fused(1, d * a, d), theta2.p, plain(1), plain(a), nullptr, 0, s);
-            svd.compute<T>(static_cast<int>(a), static_cast<int>(d * b), theta2.as<T>(), S);
+            svd.compute<T>(static_cast<int>(a), static_cast<int>(d * b), theta2.as<T>(), S, SVD_JACOBI);
             const int k = svd.k();
             sdev.reserve(S.size() * sizeof(double));
             DMRG_HIP_CHECK(hipMemcpyAsync(sdev.p, S.data(), k * sizeof(double), hipMemcpyHostToDevice, s));

@@ -347,8 +347,9 @@ dmrg_status dmrg_env_set_edge(dmrg_dtype dtype, void *block_out, int64_t chi, in
  * Vt (device, rank x n, ld rank), S (host, all min(m,n) singular values, descending), rank,
  * truncation_error = ||S[rank:]||_2 (the reference's definition, a norm) and discarded_weight = sum S[rank:]^2
  * (either may be NULL). U and Vt must hold m*min(m,n) and min(m,n)*n elements. method: 0 = gesvd (bidiagonal
- * QR, rocSOLVER). The reference can also pick gesdd or gejsv (svd_lapacke.cpp:162-290); methods 1 (gesdd) and
- * 2 (gesvdj) return DMRG_ERR_UNSUPPORTED because rocSOLVER's versions lose the small singular values.
+ * QR, rocSOLVER); 3 = one-sided Jacobi on the GPU (this library's kernels; the gesvj routine that the reference's
+ * automatic choice takes for small problems, svd.cpp:122-124, and the routine the sweep uses). Methods 1 (gesdd)
+ * and 2 (gesvdj) return DMRG_ERR_UNSUPPORTED because rocSOLVER's versions lose the small singular values.
  * Synchronises the stream. */
 dmrg_status dmrg_svd_truncate(dmrg_dtype dtype, int64_t m, int64_t n, const void *A, double trunc_lim, int64_t rank_max,
                               int method, void *U, double *S, void *Vt, int64_t *rank, double *truncation_error,

@@ -2,7 +2,9 @@
 
 Tolerances: explicit H_eff relative 2-norm error <= 1e-12 against the oracle's literal loops; dense
 eigenvalues <= 1e-12 * |A| against numpy (LAPACK) and the reference's LAPACKE fixtures; singular values
-<= 1e-12 * S[0] against numpy, orthonormality and reconstruction <= 1e-11; edge blocks exact.
+<= 1e-12 * S[0] against numpy, orthonormality and reconstruction <= 1e-11, for rocSOLVER's gesvd and the
+one-sided Jacobi kernels (both the LDS-resident path for n <= 32 and the launch-per-round path); edge blocks
+exact.
 """
 import sys
 from pathlib import Path
@@ -144,10 +146,11 @@ def check_svd(A, res, lim, rank_max):
     assert abs(err - np.linalg.norm(S_ref[r:])) <= 1e-11 * S_ref[0]
 
 
+@pytest.mark.parametrize("method", [0, 3], ids=["gesvd", "jacobi"])
 @pytest.mark.parametrize("cplx", [False, True])
-@pytest.mark.parametrize("m,n", [(1, 1), (1, 7), (9, 1), (30, 30), (64, 40), (37, 90), (300, 200)])
-def test_svd_truncate_random(m, n, cplx):
-    print(f"[case] svd m={m} n={n} cplx={cplx}", file=sys.stderr, flush=True)
+@pytest.mark.parametrize("m,n", [(1, 1), (1, 7), (9, 1), (2, 2), (30, 30), (31, 31), (50, 32), (33, 70), (64, 40), (37, 90), (300, 200)])
+def test_svd_truncate_random(m, n, cplx, method):
+    print(f"[case] svd m={m} n={n} cplx={cplx} method={method}", file=sys.stderr, flush=True)
     rng = np.random.default_rng(m * 1000 + n)
     A = rng.standard_normal((m, n)) + (1j * rng.standard_normal((m, n)) if cplx else 0)
     # graded spectrum so that truncation actually drops something
@@ -155,7 +158,7 @@ def test_svd_truncate_random(m, n, cplx):
     s = np.exp(-0.5 * np.arange(len(s)))
     A = np.asfortranarray((u * s) @ vt)
     for lim, rank_max in ((1e-14, 0), (1e-3, 0), (1e-14, 5)):
-        check_svd(A, G.svd_truncate(A, lim, rank_max), lim, rank_max)
+        check_svd(A, G.svd_truncate(A, lim, rank_max, method), lim, rank_max)
 
 
 @pytest.mark.parametrize("key", ["m6", "m96", "m0", "m181", "m174", "m5", "m101", "m7"])
@@ -164,20 +167,22 @@ def test_svd_reference_matrices(key):
     tests/golden/make_svd_fixtures.py)."""
     print(f"[case] svd fixture {key}", file=sys.stderr, flush=True)
     A = np.asfortranarray(np.load(GOLDEN / "svd_matrices.npz")[key])
-    check_svd(A, G.svd_truncate(A, 1e-10, 0), 1e-10, 0)
+    for method in (0, 3):
+        check_svd(A, G.svd_truncate(A, 1e-10, 0, method), 1e-10, 0)
 
 
 def test_svd_rank_deficient_and_rejected_routines():
-    """Exact zeros and ties at the rounding level come back in descending order; rocSOLVER's gesdd / gesvdj
-    paths (method 1, 2) are refused."""
+    """Exact zeros and ties at the rounding level come back in descending order (QR and Jacobi); rocSOLVER's
+    gesdd / gesvdj paths (method 1, 2) are refused."""
     rng = np.random.default_rng(17)
     X = rng.standard_normal((80, 6)) + 1j * rng.standard_normal((80, 6))
     A = X @ X.conj().T  # 80 x 80, rank 6
     A = np.asfortranarray(np.hstack([A, A[:, :3]]))  # 80 x 83, rank 6
-    res = G.svd_truncate(A, 1e-8, 0)
-    assert np.all(np.diff(res["S"]) <= 0)
-    assert res["rank"] == 6
-    check_svd(A, res, 1e-8, 0)
+    for method in (0, 3):
+        res = G.svd_truncate(A, 1e-8, 0, method)
+        assert np.all(np.diff(res["S"]) <= 0)
+        assert res["rank"] == 6
+        check_svd(A, res, 1e-8, 0)
     for method in (1, 2):
         with pytest.raises(DmrgError) as e:
             G.svd_truncate(A, 1e-14, 0, method)
