@@ -171,12 +171,14 @@ def section_eigs(chi=512, reps=6, tol=1e-10):
     dv, dy = G.DeviceArray((n,), np.complex128), G.DeviceArray((n,), np.complex128)
 
     def variance(v):
-        """<H^2> - <H>^2 of the normalised vector v (FP64 matvec), the energy variance of the solution."""
-        v = v / np.linalg.norm(v)
-        G._lib.call("dmrg_memcpy_htod", dv.ptr, np.ascontiguousarray(v).ctypes.data_as(__import__("ctypes").c_void_p), v.nbytes)
+        """Energy variance <H^2> - <H>^2 of the normalised vector v (FP64 matvec), evaluated as |Hv - <H> v|^2,
+        which equals it exactly and has no cancellation between two numbers of size <H>^2."""
+        v = np.ascontiguousarray(v / np.linalg.norm(v))
+        G._lib.call("dmrg_memcpy_htod", dv.ptr, v.ctypes.data_as(__import__("ctypes").c_void_p), v.nbytes)
         op.MultAx_device(dv, dy)
-        e = G.dot(dv, dy)
-        return max(0.0, G.nrm2(dy) ** 2 - abs(e) ** 2)
+        hv = dy.to_numpy()
+        e = np.vdot(v, hv).real
+        return float(np.linalg.norm(hv - e * v) ** 2)
 
     for mode in ("fp64", "mixed"):
         lat, mv64, mv32, es, rel_r, var = [], [], [], [], [], []

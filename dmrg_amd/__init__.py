@@ -347,15 +347,21 @@ class MatVecMPOS:
 
     def eigs(self, nev: int = 1, target: str = "SA", tol: float = 1e-12, x0=None, mixed: bool = False,
              precond: int = 0, ncv: int = 0, min_restart: int = 0, shift: float = 0.0,
-             max_matvecs: int = 100000, seed: int = 12345, jcb: int = 128, stream=None):
+             max_matvecs: int = 100000, seed: int = 12345, jcb: int = 128, inner_max=None, inner_tol=None,
+             stream=None):
         """Eigenpairs on the GPU (eig::solver::eigs with PRIMME, solver.cpp:491-509). Returns
-        (eigenvalues, eigenvectors (n, nev) numpy, result dict)."""
+        (eigenvalues, eigenvectors (n, nev) numpy, result dict). mixed: FP32 phase, then FP64 refinement whose
+        expansions come from at most inner_max FP32 correction-equation CG steps (None: library default)."""
         cfg = _lib.EigsConfig()
         load().dmrg_eigs_default_config(C.byref(cfg))
         cfg.nev, cfg.ncv, cfg.min_restart = nev, ncv, min_restart
         cfg.target = {"SA": 0, "LA": 1, "CLOSEST": 2}[target]
         cfg.shift, cfg.tol, cfg.max_matvecs = shift, tol, max_matvecs
         cfg.precond, cfg.mixed, cfg.seed, cfg.jcb_block = int(precond), int(mixed), seed, jcb
+        if inner_max is not None:
+            cfg.inner_max = int(inner_max)
+        if inner_tol is not None:
+            cfg.inner_tol = float(inner_tol)
         n = self.rows()
         dx0 = None
         if x0 is not None:

@@ -86,7 +86,8 @@ SIGNATURES: dict[str, tuple] = {
 class EigsConfig(C.Structure):
     _fields_ = [("nev", C.c_int), ("ncv", C.c_int), ("min_restart", C.c_int), ("target", C.c_int),
                 ("shift", C.c_double), ("tol", C.c_double), ("max_matvecs", C.c_int64), ("precond", C.c_int),
-                ("mixed", C.c_int), ("max_time_s", C.c_double), ("seed", C.c_uint64), ("jcb_block", C.c_int64)]
+                ("mixed", C.c_int), ("max_time_s", C.c_double), ("seed", C.c_uint64), ("jcb_block", C.c_int64),
+                ("inner_max", C.c_int), ("inner_tol", C.c_double)]
 
 
 class EigsResult(C.Structure):

@@ -385,6 +385,8 @@ void dmrg_eigs_default_config(dmrg_eigs_config *cfg) {
     cfg->max_time_s  = 0.0;
     cfg->seed        = 12345;
     cfg->jcb_block   = 128;
+    cfg->inner_max   = 20;
+    cfg->inner_tol   = 1e-2;
 }
 
 dmrg_status dmrg_eigs(dmrg_matvec *opp, const dmrg_eigs_config *cfg, const void *x0, void *evecs, dmrg_eigs_result *res,
@@ -407,6 +409,9 @@ dmrg_status dmrg_eigs(dmrg_matvec *opp, const dmrg_eigs_config *cfg, const void 
         p.max_time_s  = cfg->max_time_s;
         p.seed        = cfg->seed;
         p.jcb         = cfg->jcb_block;
+        DMRG_REQUIRE(cfg->inner_max >= 0 && cfg->inner_tol > 0 && cfg->inner_tol < 1, DMRG_ERR_ARG, "bad inner_max / inner_tol");
+        p.inner_max = cfg->mixed ? cfg->inner_max : 0;
+        p.inner_tol = cfg->inner_tol;
         auto      s   = static_cast<hipStream_t>(stream);
         EigsStats st, s1;
         *res          = dmrg_eigs_result{};
@@ -414,7 +419,7 @@ dmrg_status dmrg_eigs(dmrg_matvec *opp, const dmrg_eigs_config *cfg, const void 
         if(cfg->mixed) {
             if(cx) st = eigs_mixed<cpx<double>, cpx<float>>(op, s, p, static_cast<const cpx<double> *>(x0), static_cast<cpx<double> *>(evecs), res->eigval, res->rnorm, &s1);
             else st = eigs_mixed<double, float>(op, s, p, static_cast<const double *>(x0), static_cast<double *>(evecs), res->eigval, res->rnorm, &s1);
-            res->matvecs_fp32 = s1.matvecs;
+            res->matvecs_fp32 = s1.matvecs + st.inner_matvecs;
             res->iters        = s1.iters;
             res->restarts     = s1.restarts;
         } else if(cx) {

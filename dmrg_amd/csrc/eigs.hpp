@@ -7,6 +7,7 @@
 #include "vecops.hpp"
 
 #include <complex>
+#include <memory>
 #include <vector>
 
 namespace dmrg {
@@ -29,6 +30,10 @@ struct EigsParams {
     // Stagnation stop (used by the FP32 phase of the mixed solver): stop when the residual has not
     // dropped by 10% over this many iterations (0: off).
     int stagnation_window = 0;
+    // FP64 solves only: expand with t from at most inner_max FP32 CG steps on the correction equation
+    // (Jacobi-Davidson), stopped early at a residual reduction of inner_tol (0: plain Davidson expansion).
+    int    inner_max = 0;
+    double inner_tol = 1e-2;
 };
 
 struct EigsStats {
@@ -39,6 +44,49 @@ struct EigsStats {
     double  anorm    = 0.0;
     double  seconds  = 0.0;
     bool    stagnated = false;
+    int64_t inner_matvecs = 0; // FP32 applications inside the correction solves
+};
+
+template <typename T>
+struct lower_precision {
+    using type = T;
+};
+template <>
+struct lower_precision<double> {
+    using type = float;
+};
+template <>
+struct lower_precision<cpx<double>> {
+    using type = cpx<float>;
+};
+
+// Approximate solution of the Jacobi-Davidson correction equation for a Ritz pair (theta, x) with residual r,
+//   sigma (I - x x^H)(H - theta)(I - x x^H) t = -sigma (I - x x^H) r,   t orthogonal to x,
+// by conjugate gradients in FP32 with matrix-core matvecs. sigma = +1 refines the lowest eigenvalue (the
+// projected operator is positive definite once theta lies below the second eigenvalue), -1 the largest.
+// The iteration stops after max_it steps, at a residual reduction of rtol (checked every few steps) or at
+// a breakdown (p^H A p <= 0, the operator is not definite); t = r when no step was taken.
+template <typename T32>
+class CorrectionSolver {
+  public:
+    CorrectionSolver(MatVecOp &op, hipStream_t s);
+    ~CorrectionSolver();
+    CorrectionSolver(const CorrectionSolver &) = delete;
+    CorrectionSolver &operator=(const CorrectionSolver &) = delete;
+    // x, r, t: device vectors of the FP64 type (length n). Returns the number of FP32 matvecs.
+    template <typename T64>
+    int solve(const T64 *x, const T64 *r, double theta, double sigma, int max_it, double rtol, T64 *t);
+    void rebind(MatVecOp &o) {
+        op = &o;
+        n  = o.D.n();
+    }
+
+  private:
+    MatVecOp   *op;
+    hipStream_t s;
+    int64_t     n;
+    DevBuf      xp, q, rv, tv, scratch, dev;
+    double     *host = nullptr; // pinned copy of the state
 };
 
 template <typename T>
@@ -56,6 +104,7 @@ class Davidson {
     void rebind(MatVecOp &o) {
         op = &o;
         n  = o.D.n();
+        if(inner) inner->rebind(o);
     }
 
   private:
@@ -63,6 +112,7 @@ class Davidson {
     hipStream_t s;
     int64_t     n;
     DevBuf      V, W, tmp, rbuf, scratch, small, Ydev, diag, pcw;
+    std::unique_ptr<CorrectionSolver<typename lower_precision<T>::type>> inner;
     double     *host = nullptr; // pinned
     void        matvec(const T *x, T *y, int nvec);
     void        orthonormalize_into(T *t, int m, double *dnrm_pre, double *dnrm_post);
@@ -73,6 +123,8 @@ class Davidson {
 
 // Mixed precision: FP32 matrix-core Davidson to a loose tolerance, then FP64 Davidson seeded with the
 // best FP32 Ritz vectors, to the requested tolerance (FP64 Rayleigh-Ritz, orthogonalisation, residuals).
+// With inner_max > 0 (one wanted pair at the lowest or largest eigenvalue, no preconditioner) the FP64 phase
+// expands with FP32 correction-equation solves, so that most matvecs stay on the FP32 matrix cores.
 template <typename T64, typename T32>
 EigsStats eigs_mixed(MatVecOp &op, hipStream_t s, const EigsParams &p, const T64 *init, T64 *out, double *evals,
                      double *rnorms, EigsStats *phase1);

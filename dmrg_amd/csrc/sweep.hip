@@ -232,8 +232,9 @@ struct Fdmrg {
         const int64_t mixed_min = cfg.mixed_min_n > 0 ? cfg.mixed_min_n : 131072;
         if(cfg.mixed && n >= mixed_min) {
             EigsStats s1;
+            p.inner_max = 20; // FP32 correction solves in the FP64 phase
             st = eigs_mixed<T, std::conditional_t<std::is_same_v<T, double>, float, cpx<float>>>(op, s, p, th, xbuf.as<T>(), &ev, &rn, &s1);
-            res->matvecs_fp32 += s1.matvecs;
+            res->matvecs_fp32 += s1.matvecs + st.inner_matvecs;
         } else {
             if(!dav) dav = std::make_unique<Davidson<T>>(op, s);
             dav->rebind(op);

@@ -346,6 +346,146 @@ void convert(int64_t n, const TI *in, TO *out, hipStream_t s) {
     DMRG_HIP_CHECK(hipGetLastError());
 }
 
+// ------------------------------------------------------------------------------------------- projected CG
+// Kernels of the FP32 correction-equation solver (CG on A = sigma P (H - theta) P, P = I - x x^H). The scalars
+// live on the device: dots = {x^H q (2), p^H q (2)}; state as laid out by the CG_* offsets. Every block of a
+// kernel derives the same step from the same inputs; only block 0 writes the state, into the slot of the
+// next iteration (k + 1) & 1, which no block of the current kernel reads.
+
+template <typename T>
+__global__ __launch_bounds__(VT_THREADS) void cg_shift_kernel(int64_t n, const T *p, T *q, double theta, double sigma, const double *state,
+                                                              int k) {
+    if(state[CG_STOP + (k & 1)] != 0.0) return;
+    for(int64_t i = blockIdx.x * static_cast<int64_t>(VT_THREADS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * VT_THREADS) {
+        double qr, qi, pr, pi;
+        ld(q[i], qr, qi);
+        ld(p[i], pr, pi);
+        q[i] = mk<T>(sigma * (qr - theta * pr), sigma * (qi - theta * pi));
+    }
+}
+
+// r = -sigma (r - x c) with c = x^H r (dots[0..1]); p = r; t = 0; partial |r|^2 per block.
+template <typename T>
+__global__ __launch_bounds__(VT_THREADS) void cg_init_kernel(int64_t n, const T *x, const double *c, double sigma, T *r, T *p, T *t,
+                                                             double *partial) {
+    __shared__ double red[1][VT_THREADS / 64];
+    const double      cr = c[0], ci = c[1];
+    double            acc[1] = {0.0};
+    for(int64_t i = blockIdx.x * static_cast<int64_t>(VT_THREADS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * VT_THREADS) {
+        double rr, ri, xr, xi;
+        ld(r[i], rr, ri);
+        ld(x[i], xr, xi);
+        const double vr = -sigma * (rr - (xr * cr - xi * ci)), vi = -sigma * (ri - (xr * ci + xi * cr));
+        const T      v  = mk<T>(vr, vi);
+        r[i]            = v;
+        p[i]            = v;
+        t[i]            = mk<T>(0.0, 0.0);
+        double wr, wi;
+        ld(v, wr, wi);
+        acc[0] += wr * wr + wi * wi;
+    }
+    bsum<1>(acc, red);
+    if(threadIdx.x == 0) {
+        partial[2 * blockIdx.x]     = acc[0];
+        partial[2 * blockIdx.x + 1] = 0.0;
+    }
+}
+
+// state <- start of the iteration: rho_0 = |r|^2 (rr[0]); stopped when r = 0.
+__global__ void cg_start_kernel(const double *rr, double *state) {
+    state[CG_RHO]      = rr[0];
+    state[CG_RHO + 1]  = 0.0;
+    state[CG_RHO0]     = rr[0];
+    state[CG_STOP]     = rr[0] > 0.0 ? 0.0 : 1.0;
+    state[CG_STOP + 1] = 0.0;
+    state[CG_ITS]      = 0.0;
+    state[CG_RR]       = rr[0];
+}
+
+// alpha = rho / p^H q; t += alpha p; r -= alpha (q - x (x^H q)); partial |r|^2. No update on a breakdown (p^H q <= 0).
+template <typename T>
+__global__ __launch_bounds__(VT_THREADS) void cg_update_kernel(int64_t n, const T *x, const T *p, const T *q, const double *dots,
+                                                               const double *state, int k, T *t, T *r, double *partial) {
+    __shared__ double red[1][VT_THREADS / 64];
+    const double      pq   = dots[2];
+    const bool        live = state[CG_STOP + (k & 1)] == 0.0 && pq > 0.0;
+    const double      a    = live ? state[CG_RHO + (k & 1)] / pq : 0.0;
+    const double      cr = dots[0], ci = dots[1];
+    double            acc[1] = {0.0};
+    if(live) {
+        for(int64_t i = blockIdx.x * static_cast<int64_t>(VT_THREADS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * VT_THREADS) {
+            double xr, xi, pr, pi, qr, qi, tr, ti, rr, ri;
+            ld(x[i], xr, xi);
+            ld(p[i], pr, pi);
+            ld(q[i], qr, qi);
+            ld(t[i], tr, ti);
+            ld(r[i], rr, ri);
+            t[i]            = mk<T>(tr + a * pr, ti + a * pi);
+            const double yr = qr - (xr * cr - xi * ci), yi = qi - (xr * ci + xi * cr);
+            const T      v  = mk<T>(rr - a * yr, ri - a * yi);
+            r[i]            = v;
+            double wr, wi;
+            ld(v, wr, wi);
+            acc[0] += wr * wr + wi * wi;
+        }
+    }
+    bsum<1>(acc, red);
+    if(threadIdx.x == 0) {
+        partial[2 * blockIdx.x]     = acc[0];
+        partial[2 * blockIdx.x + 1] = 0.0;
+    }
+}
+
+// beta = |r|^2 / rho; p = r + beta p; block 0 advances the state (rho, stop flag, step count).
+template <typename T>
+__global__ __launch_bounds__(VT_THREADS) void cg_direction_kernel(int64_t n, const T *r, const double *dots, const double *rr,
+                                                                  double *state, int k, double rtol2, T *p) {
+    const bool   stopped = state[CG_STOP + (k & 1)] != 0.0;
+    const bool   live    = !stopped && dots[2] > 0.0;
+    const double rho     = state[CG_RHO + (k & 1)];
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+        const int nx          = (k + 1) & 1;
+        state[CG_RHO + nx]    = live ? rr[0] : rho;
+        state[CG_STOP + nx]   = (!live || rr[0] <= rtol2 * state[CG_RHO0]) ? 1.0 : 0.0;
+        state[CG_ITS]         = state[CG_ITS] + (live ? 1.0 : 0.0);
+        if(live) state[CG_RR] = rr[0];
+    }
+    if(!live) return;
+    const double b = rr[0] / rho;
+    for(int64_t i = blockIdx.x * static_cast<int64_t>(VT_THREADS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * VT_THREADS) {
+        double rr_, ri, pr, pi;
+        ld(r[i], rr_, ri);
+        ld(p[i], pr, pi);
+        p[i] = mk<T>(rr_ + b * pr, ri + b * pi);
+    }
+}
+
+template <typename T>
+void cg_init(int64_t n, const T *x, const double *d_c, double sigma, T *r, T *p, T *t, double *d_rr, double *state, double *scratch,
+             hipStream_t s) {
+    const int g = grid_for(n);
+    hipLaunchKernelGGL(cg_init_kernel<T>, dim3(g), dim3(VT_THREADS), 0, s, n, x, d_c, sigma, r, p, t, scratch);
+    hipLaunchKernelGGL(fold_columns, dim3(1), dim3(VT_THREADS), 0, s, g, 1, scratch, d_rr, -1, nullptr, nullptr);
+    hipLaunchKernelGGL(cg_start_kernel, dim3(1), dim3(1), 0, s, d_rr, state);
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void cg_shift(int64_t n, const T *p, T *q, double theta, double sigma, const double *state, int k, hipStream_t s) {
+    hipLaunchKernelGGL(cg_shift_kernel<T>, dim3(grid_for(n)), dim3(VT_THREADS), 0, s, n, p, q, theta, sigma, state, k);
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void cg_step(int64_t n, const T *x, T *p, const T *q, const double *d_dots, T *t, T *r, double *d_rr, double *state, int k,
+             double rtol2, double *scratch, hipStream_t s) {
+    const int g = grid_for(n);
+    hipLaunchKernelGGL(cg_update_kernel<T>, dim3(g), dim3(VT_THREADS), 0, s, n, x, p, q, d_dots, state, k, t, r, scratch);
+    hipLaunchKernelGGL(fold_columns, dim3(1), dim3(VT_THREADS), 0, s, g, 1, scratch, d_rr, -1, nullptr, nullptr);
+    hipLaunchKernelGGL(cg_direction_kernel<T>, dim3(g), dim3(VT_THREADS), 0, s, n, r, d_dots, d_rr, state, k, rtol2, p);
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
 #define DMRG_VEC_INST(T)                                                                                                 \
     template void multidot<T>(int64_t, int, const T *, int64_t, const T *, double *, double *, hipStream_t);          \
     template void gemv_update<T>(int64_t, int, const T *, int64_t, const double *, double, double, T *, hipStream_t); \
@@ -360,6 +500,14 @@ void convert(int64_t n, const TI *in, TO *out, hipStream_t s) {
                                       double *, const int *, double *, hipStream_t);
 DMRG_VEC_INST(double)
 DMRG_VEC_INST(float)
+#define DMRG_CG_INST(T)                                                                                                  \
+    template void cg_init<T>(int64_t, const T *, const double *, double, T *, T *, T *, double *, double *, double *,     \
+                             hipStream_t);                                                                               \
+    template void cg_shift<T>(int64_t, const T *, T *, double, double, const double *, int, hipStream_t);               \
+    template void cg_step<T>(int64_t, const T *, T *, const T *, const double *, T *, T *, double *, double *, int, double, \
+                             double *, hipStream_t);
+DMRG_CG_INST(float)
+DMRG_CG_INST(cpx<float>)
 DMRG_VEC_INST(cpx<double>)
 DMRG_VEC_INST(cpx<float>)
 template void jacobi_precond<double, double>(int64_t, const double *, const double *, double, double, double *, hipStream_t);

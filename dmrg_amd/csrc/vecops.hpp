@@ -75,4 +75,24 @@ void random_fill(int64_t n, T *x, uint64_t seed, hipStream_t s);
 template <typename TO, typename TI>
 void convert(int64_t n, const TI *in, TO *out, hipStream_t s);
 
+// Conjugate gradients on A t = b with A = sigma P (H - theta) P, P = I - x x^H (|x| = 1), the matvec H p
+// supplied by the caller between the steps. Device state (doubles) at these offsets:
+constexpr int CG_RHO   = 0; // |r|^2 of the current iteration (two slots, iteration k uses slot k & 1)
+constexpr int CG_STOP  = 2; // stop flag (two slots): converged to rtol or broke down
+constexpr int CG_RHO0  = 4; // |b|^2
+constexpr int CG_ITS   = 5; // completed steps
+constexpr int CG_RR    = 6; // |r|^2 after the last completed step
+constexpr int CG_STATE = 8;
+// r holds the FP32 residual of the Ritz pair on entry: b = -sigma P r goes to r and p, t = 0; d_c = x^H r.
+template <typename T>
+void cg_init(int64_t n, const T *x, const double *d_c, double sigma, T *r, T *p, T *t, double *d_rr, double *state, double *scratch,
+             hipStream_t s);
+// q = sigma (q - theta p), after q = H p (skipped once stopped).
+template <typename T>
+void cg_shift(int64_t n, const T *p, T *q, double theta, double sigma, const double *state, int k, hipStream_t s);
+// One CG step of iteration k given d_dots = {x^H q, p^H q} (complex): t, r, p and the state advance.
+template <typename T>
+void cg_step(int64_t n, const T *x, T *p, const T *q, const double *d_dots, T *t, T *r, double *d_rr, double *state, int k,
+             double rtol2, double *scratch, hipStream_t s);
+
 } // namespace dmrg

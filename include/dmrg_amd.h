@@ -172,6 +172,10 @@ typedef struct dmrg_eigs_config {
     double   max_time_s;  /* 0: unlimited (reference: config.maxTime) */
     uint64_t seed;        /* seed of the random start vector when no initial guess is given */
     int64_t  jcb_block;   /* block size of precond = 2 (jcbMaxBlockSize, default 128, settings.h:261) */
+    int      inner_max;   /* mixed = 1, one pair at SA or LA, precond = 0: the FP64 phase expands with at most this
+                             many FP32 conjugate-gradient steps on the Jacobi-Davidson correction equation
+                             (default 20; 0: plain FP64 Davidson refinement) */
+    double   inner_tol;   /* those steps stop at this reduction of the correction residual (default 1e-2) */
 } dmrg_eigs_config;
 
 typedef struct dmrg_eigs_result {
@@ -182,7 +186,7 @@ typedef struct dmrg_eigs_result {
     int     restarts;
     int     stagnated;
     int64_t matvecs_fp64; /* FP64 H_eff applications (block matvecs count each vector) */
-    int64_t matvecs_fp32; /* FP32 applications (mixed mode) */
+    int64_t matvecs_fp32; /* FP32 applications (mixed mode: FP32 phase plus correction solves) */
     double  seconds;
     double  anorm;
 } dmrg_eigs_result;

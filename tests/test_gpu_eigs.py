@@ -4,6 +4,7 @@ free-fermion ground energy, and residual checks with the oracle matvec at larger
 Tolerances: eigenvalues within 1e-10 (relative to |A|) of the exact value for FP64 and for the mixed
 FP32+FP64 mode; residual |Hx - lambda x| <= tol |A| with tol = 1e-12 unless stated.
 """
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -89,12 +90,35 @@ def test_ising_full_chain_free_fermion(L):
     op.close()
 
 
-@pytest.mark.parametrize("mixed", [False, True])
-def test_large_residual_with_oracle(mixed):
+@pytest.mark.parametrize("target", ["SA", "LA"])
+@pytest.mark.parametrize("inner_max", [0, 5, 20])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_mixed_correction_solves(target, inner_max, cplx):
+    """Mixed solves whose FP64 phase expands with FP32 correction-equation CG steps (inner_max > 0) or with
+    plain residuals (0): same eigenpair as the dense reference at both ends of the spectrum."""
+    print(f"[case] mixed target={target} inner_max={inner_max} cplx={cplx}", file=sys.stderr, flush=True)
+    th, W, L, R = random_heff_problem(2, 20, 24, 5, 5, seed=4, complex_=cplx)
+    H = dense_from_oracle(th, W, L, R)
+    wd = np.linalg.eigvalsh(H)
+    ref = wd[0] if target == "SA" else wd[-1]
+    anorm = np.abs(wd).max()
+    op = G.MatVecMPOS([W], L, R, dtype=th.dtype)
+    try:
+        w, v, info = op.eigs(nev=1, target=target, tol=1e-12, mixed=True, inner_max=inner_max)
+        assert info["nconv"] == 1, info
+        assert abs(w[0] - ref) <= 1e-10 * anorm, (w[0], ref, info)
+        x = v[:, 0]
+        assert np.linalg.norm(H @ x - w[0] * x) <= 1e-11 * anorm
+    finally:
+        op.close()
+
+
+@pytest.mark.parametrize("mixed,inner_max", [(False, None), (True, 0), (True, 20)])
+def test_large_residual_with_oracle(mixed, inner_max):
     """chi=128 complex (n=32768): residual and Rayleigh quotient checked with the oracle matvec."""
     th, W, L, R = random_heff_problem(2, 128, 128, 5, 5, seed=1)
     op = G.MatVecMPOS([W], L, R, dtype=np.complex128)
-    w, v, info = op.eigs(nev=1, tol=1e-12, mixed=mixed)
+    w, v, info = op.eigs(nev=1, tol=1e-12, mixed=mixed, inner_max=inner_max)
     x = v[:, 0].reshape(th.shape, order="F")
     hx = O.matrix_vector_product(x, W, L, R)
     lam = np.vdot(x, hx).real / np.vdot(x, x).real
