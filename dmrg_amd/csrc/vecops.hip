@@ -56,18 +56,52 @@ static int grid_for(int64_t n) {
 }
 
 // ------------------------------------------------------------------------------------------- multidot
-constexpr int MD_CH = 8; // basis columns per pass
+constexpr int MD_CH        = 16;  // basis columns per pass (32 double accumulators per thread)
+constexpr int MD_PER_THREAD = 4;  // rows per thread and pass at full grid: amortises the block reduction
+
+// Reduce-scatter of NV values across the 64 lanes of a wavefront: after the call lane l holds in v[0] the
+// wave total of value index rs_index<NV>(l) (valid in the lanes whose low bits below the last exchange are
+// zero). Halving the set exchanged at each step costs NV - 1 + (6 - log2 NV) shuffles instead of 6 NV.
+template <int H, int O, int NV>
+__device__ inline void rs_step(double (&v)[NV], int lane) {
+    if constexpr(H >= 1) {
+        const bool up = (lane & O) != 0;
+#pragma unroll
+        for(int k = 0; k < H; ++k) {
+            const double send = up ? v[k] : v[k + H];
+            const double keep = up ? v[k + H] : v[k];
+            v[k]              = keep + __shfl_xor(send, O, 64);
+        }
+        rs_step<H / 2, O / 2, NV>(v, lane);
+    } else {
+#pragma unroll
+        for(int o = O; o >= 1; o >>= 1) v[0] += __shfl_xor(v[0], o, 64); // remaining low lane bits
+    }
+}
+template <int NV>
+__device__ inline int rs_index(int lane) {
+    int idx = 0;
+#pragma unroll
+    for(int h = NV / 2, o = 32; h >= 1; h >>= 1, o >>= 1) idx += (lane & o) ? h : 0;
+    return idx;
+}
+template <int NV>
+__device__ inline bool rs_owner(int lane) {
+    return (lane & (64 / NV - 1)) == 0;
+}
 
 // Columns j < m are V[:, j]; column m (when with_self) is x itself, so x^H x comes out of the same pass.
 template <typename T>
 __global__ __launch_bounds__(VT_THREADS) void multidot_partial(int64_t n, int m, int mc, const T *V, int64_t ldv, const T *x,
                                                                const int *skip, double *partial) {
     if(skip && *skip == 0) return;
-    __shared__ double red[2 * MD_CH][VT_THREADS / 64];
+    constexpr int     NV = 2 * MD_CH;
+    __shared__ double red[NV][VT_THREADS / 64];
+    const int         lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for(int j0 = 0; j0 < mc; j0 += MD_CH) {
-        double acc[2 * MD_CH];
+        double acc[NV];
 #pragma unroll
-        for(int c = 0; c < 2 * MD_CH; ++c) acc[c] = 0.0;
+        for(int c = 0; c < NV; ++c) acc[c] = 0.0;
         for(int64_t i = blockIdx.x * static_cast<int64_t>(VT_THREADS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * VT_THREADS) {
             double xr, xi;
             ld(x[i], xr, xi);
@@ -77,20 +111,29 @@ __global__ __launch_bounds__(VT_THREADS) void multidot_partial(int64_t n, int m,
                     double vr, vi;
                     if(j0 + c < m) ld(V[i + (j0 + c) * ldv], vr, vi);
                     else vr = xr, vi = xi;
-                    acc[2 * c] += vr * xr + vi * xi;     // conj(v) x
+                    acc[2 * c] += vr * xr + vi * xi; // conj(v) x
                     acc[2 * c + 1] += vr * xi - vi * xr;
                 }
             }
         }
-        bsum<2 * MD_CH>(acc, red);
-        if(threadIdx.x == 0)
+        rs_step<NV / 2, 32, NV>(acc, lane);
+        if(rs_owner<NV>(lane)) red[rs_index<NV>(lane)][wave] = acc[0];
+        __syncthreads();
+        if(threadIdx.x < NV) {
+            const int c = threadIdx.x >> 1;
+            if(j0 + c < mc) {
+                double t = 0;
 #pragma unroll
-            for(int c = 0; c < MD_CH; ++c)
-                if(j0 + c < mc) {
-                    partial[2 * (static_cast<int64_t>(blockIdx.x) * mc + j0 + c)]     = acc[2 * c];
-                    partial[2 * (static_cast<int64_t>(blockIdx.x) * mc + j0 + c) + 1] = acc[2 * c + 1];
-                }
+                for(int w = 0; w < VT_THREADS / 64; ++w) t += red[threadIdx.x][w];
+                partial[2 * (static_cast<int64_t>(blockIdx.x) * mc + j0 + c) + (threadIdx.x & 1)] = t;
+            }
+        }
+        __syncthreads();
     }
+}
+
+static int md_grid(int64_t n) {
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(VEC_MAX_BLOCKS, (n + VT_THREADS * MD_PER_THREAD - 1) / (VT_THREADS * MD_PER_THREAD))));
 }
 
 // out[j] = sum_b partial[b][j] for j < m (column j == m_out goes to self_out instead, when given): one block
@@ -117,7 +160,7 @@ template <typename T>
 void multidot(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_out, double *scratch, hipStream_t s) {
     if(m <= 0) return;
     DMRG_REQUIRE(m <= VEC_MAX_M, DMRG_ERR_ARG, "multidot: too many columns");
-    const int g = grid_for(n);
+    const int g = md_grid(n);
     hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, m, V, ldv, x, nullptr, scratch);
     hipLaunchKernelGGL(fold_columns, dim3(m), dim3(VT_THREADS), 0, s, g, m, scratch, d_out, -1, nullptr, nullptr);
     DMRG_HIP_CHECK(hipGetLastError());
@@ -129,7 +172,7 @@ void multidot_x(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d
     const int mc = m + (d_self ? 1 : 0);
     if(mc <= 0) return;
     DMRG_REQUIRE(mc <= VEC_MAX_M, DMRG_ERR_ARG, "multidot: too many columns");
-    const int g = grid_for(n);
+    const int g = md_grid(n);
     hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, mc, V, ldv, x, skip, scratch);
     hipLaunchKernelGGL(fold_columns, dim3(mc), dim3(VT_THREADS), 0, s, g, mc, scratch, d_out, d_self ? m : -1, d_self, skip);
     DMRG_HIP_CHECK(hipGetLastError());
