@@ -277,6 +277,51 @@ def section_serve(nreq=32, workers=8, rate=3.0, seed=4):
             "max_variance": max(r["variance"] for r in res)}
 
 
+def section_expansion(chi=512, reps=5):
+    """Subspace expansion at chi: the optimally mixed block (8 matvecs: 4 with H over D=5 envs, 4 with the
+    variance operator over D^2=25 envs) and the L/R expansion terms (energy block with the MPO, variance block
+    with the MPO^2)."""
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import random_heff_problem
+
+    th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=1)
+    _, W2, L2, R2 = random_heff_problem(2, chi, chi, 25, 25, seed=2)
+    h1 = G.MatVecMPOS([W], L, R, dtype=np.complex128)
+    h2 = G.MatVecMPOS([W2], L2, R2, dtype=np.complex128)
+    dth = G.DeviceArray.from_numpy(th)
+    G.optimally_mixed_block(h1, h2, dth)  # warm-up (GEMM plans)
+    G._lib.call("dmrg_synchronize")
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m, alpha, val = G.optimally_mixed_block(h1, h2, dth)
+        m.close()
+    ms_mixed = (time.perf_counter() - t0) / reps * 1e3
+    out = {"workload": f"theta (2,{chi},{chi}) cx64, H envs D=5, variance envs D=25, seed 1/2",
+           "mixed_block": {"ms": round(ms_mixed, 3), "matvecs": 8, "alphas": [round(a, 6) for a in alpha]}}
+    dW, dW2 = G.DeviceArray.from_numpy(W), G.DeviceArray.from_numpy(W2)
+    dL, dR, dL2 = G.DeviceArray.from_numpy(L), G.DeviceArray.from_numpy(R), G.DeviceArray.from_numpy(L2)
+    for name, side, blk, mpo, D in (("term_L_energy", "L", dL, dW, 5), ("term_R_energy", "R", dR, dW, 5),
+                                    ("term_L_variance", "L", dL2, dW2, 25)):
+        P = G.env_expansion_term(side, blk, dth, mpo, 0.5)
+        P.close()
+        G._lib.call("dmrg_synchronize")
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            P = G.env_expansion_term(side, blk, dth, mpo, 0.5)
+            P.close()
+        G._lib.call("dmrg_synchronize")
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        flop = 8.0 * (chi * chi * D * 2 * chi + 2 * 2 * D * D * chi * chi)  # block.mps GEMM + MPO leg
+        out[name] = {"ms": round(ms, 3), "tflops": round(flop / ms / 1e9, 2)}
+    for x in (dth, dW, dW2, dL, dR, dL2):
+        x.close()
+    h1.close()
+    h2.close()
+    return out
+
+
 def section_two_site(chi=1024, D=8, steps=10, tol=1e-10):
     """Two-site workload: theta (4, chi, chi) cx64, env blocks (chi, chi, 8) Hermitian per w, two random
     range-6 site MPOs (8, 8, 2, 2) merged into (8, 8, 4, 4), seed 3. Matvec FP64 vs FP32 matrix cores, one
@@ -368,6 +413,45 @@ def host_buffer_matvec(chi=512, reps=50):
             "note": "host x and y (8 MiB each) cross PCIe every call; not the headline value"}
 
 
+def profile_eigs(mode, chi=512, tol=1e-10):
+    """Warm-up plus 3 eigensolves of the chi cx64 workload in one mode; the per-solve matvec counts."""
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import random_heff_problem
+
+    th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=1)
+    op = G.MatVecMPOS([W], L, R, dtype=np.complex128)
+    op.eigs(nev=1, tol=tol, seed=999, mixed=(mode == "mixed"))
+    out = []
+    for k in range(3):
+        t0 = time.perf_counter()
+        w, _, info = op.eigs(nev=1, tol=tol, seed=1000 + k, mixed=(mode == "mixed"))
+        out.append({"ms": round((time.perf_counter() - t0) * 1e3, 2), "E": float(w[0]), "matvecs_fp64": info["matvecs_fp64"],
+                    "matvecs_fp32": info["matvecs_fp32"]})
+    op.close()
+    return {"profile_eigs": mode, "chi": chi, "solves": out}
+
+
+def profile_svd(n, seed=5):
+    """Warm-up plus 3 Jacobi SVDs (method 3) of a random n x n complex matrix resident on the device."""
+    import numpy as np
+
+    import dmrg_amd as G
+
+    rng = np.random.default_rng(seed)
+    A = np.asfortranarray(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    dA = G.DeviceArray.from_numpy(A)
+    G.svd_truncate(dA, 1e-14, 0, 3)
+    ms = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        G.svd_truncate(dA, 1e-14, 0, 3)
+        ms.append(round((time.perf_counter() - t0) * 1e3, 2))
+    dA.close()
+    return {"profile_svd": n, "ms": ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -380,6 +464,10 @@ def main():
     ap.add_argument("--profile-matvec", action="store_true", help="only the chi=512 FP64 matvec steps, then exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="one 256 MiB device-to-device copy (scales the HBM byte counters of a profiler run), then exit")
+    ap.add_argument("--profile-eigs", choices=("fp64", "mixed"),
+                    help="one warm-up and 3 timed chi=512 eigensolves in this mode (for profiler passes), then exit")
+    ap.add_argument("--profile-svd", type=int, metavar="N",
+                    help="3 Jacobi SVDs of a random N x N complex matrix (for profiler passes), then exit")
     args = ap.parse_args()
     if args.two_site:
         print(json.dumps(section_two_site()))
@@ -388,6 +476,12 @@ def main():
         # Only the chi=512 FP64 matvec steps (for profiler passes); pair with DMRG_TUNE_CACHE to reuse plans.
         r = gpu_matvec_bench(args.chi, args.steps, args.warmup, legs_nrep=1)
         print(json.dumps({"profile_matvec_ms": round(r["ms_per_step"], 5), "legs_ms": r["legs_ms"]}))
+        return
+    if args.profile_eigs:
+        print(json.dumps(profile_eigs(args.profile_eigs, args.chi)))
+        return
+    if args.profile_svd:
+        print(json.dumps(profile_svd(args.profile_svd)))
         return
     if args.copy_calibration:
         import dmrg_amd as G
@@ -401,8 +495,8 @@ def main():
         print(json.dumps({"copy_bytes": nbytes}))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the per-request "
-              f"sharding across GPUs is described in DESIGN.md).", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (requests would be "
+              f"spread over GPUs one per device; that layer is not built).", file=sys.stderr)
         sys.exit(2)
 
     import dmrg_amd as G
@@ -452,6 +546,7 @@ def main():
         out["eigs"] = section_eigs(args.chi)
         out["sweep"] = section_sweep(cpu=not args.no_cpu_baseline)
         out["serve"] = section_serve()
+        out["expansion"] = section_expansion(args.chi)
         out["two_site_chi1024"] = section_two_site()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.chi)

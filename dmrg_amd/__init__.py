@@ -130,6 +130,29 @@ def env_build_block(side: str, otherblock, mps, mpo, stream=None):
     return res
 
 
+def env_expansion_term(side: str, block, mps, mpo, alpha: float = 1.0, stream=None):
+    """Subspace-expansion term alpha * (block . mps . mpo) (EnvBase::get_expansion_term, EnvBase.cpp:303-361).
+    side 'L': shape (d, chiL, chiR*DR); side 'R': shape (d, chiL*DL, chiR) in the reference's storage order."""
+    on_dev = isinstance(mps, DeviceArray)
+    m = _dev(mps)
+    b, w = _dev(block), _dev(mpo)
+    d, cL, cR = m.shape
+    if side == "L":
+        shape = (d, cL, cR * w.shape[1])
+    elif side == "R":
+        shape = (d, cL * w.shape[0], cR)
+    else:
+        raise ValueError("side must be 'L' or 'R'")
+    out = DeviceArray(shape, m.dtype)
+    call("dmrg_env_expansion_term", m.code, C.c_char(side.encode()), out.ptr, b.ptr, _dims(b.shape), m.ptr,
+         _dims(m.shape), w.ptr, _dims(w.shape), float(alpha), stream)
+    if on_dev:
+        return out
+    res = out.to_numpy()
+    out.close()
+    return res
+
+
 def expectation_value(mps, mpo, envL, envR, stream=None) -> complex:
     """<mps|H_eff|mps> (contraction_tblis.cpp:30-97)."""
     m, w, l, r = _dev(mps), _dev(mpo), _dev(envL), _dev(envR)
@@ -263,6 +286,27 @@ def serve(requests, workers: int = 4, hbm_budget_bytes: int = 64 << 30, mixed: b
     res = [{k: getattr(out[i], k) for k, _ in out[i]._fields_} for i in range(n)]
     stats = {k: getattr(st, k) for k, _ in st._fields_}
     return res, stats
+
+
+def optimally_mixed_block(h1: "MatVecMPOS", h2: "MatVecMPOS", theta, mode: int = 3, stream=None):
+    """get_optimally_mixed_block_1site (env.cpp:96-239) with h1 the energy operator and h2 the operator on the
+    variance envs. mode bit 1 = H1, bit 2 = H2. Returns (mixed theta, (alpha_mps, alpha_h1v, alpha_h2v),
+    opt_val); numpy theta gives a numpy result of theta's shape, a DeviceArray a DeviceArray."""
+    on_dev = isinstance(theta, DeviceArray)
+    t = theta if on_dev else DeviceArray.from_numpy(np.asarray(theta, dtype=h1.dtype))
+    out = DeviceArray(t.shape, t.dtype)
+    alphas = (C.c_double * 3)()
+    val = C.c_double()
+    try:
+        call("dmrg_optimally_mixed_block", h1.h, h2.h, t.ptr, int(mode), out.ptr, alphas, C.byref(val), stream)
+    finally:
+        if not on_dev:
+            t.close()
+    if on_dev:
+        return out, tuple(alphas), val.value
+    res = out.to_numpy()
+    out.close()
+    return res, tuple(alphas), val.value
 
 
 class MatVecMPOS:

@@ -53,6 +53,8 @@ SIGNATURES: dict[str, tuple] = {
     "dmrg_event_elapsed_ms": (st, [C.POINTER(C.c_float), vp, vp]),
     "dmrg_matrix_vector_product": (st, [C.c_int, vp, vp, i64p, vp, i64p, vp, i64p, vp, i64p, vp]),
     "dmrg_env_build_block": (st, [C.c_int, C.c_char, vp, vp, i64p, vp, i64p, vp, i64p, vp]),
+    "dmrg_env_expansion_term": (st, [C.c_int, C.c_char, vp, vp, i64p, vp, i64p, vp, i64p, C.c_double, vp]),
+    "dmrg_optimally_mixed_block": (st, [vp, vp, vp, C.c_int, vp, C.POINTER(C.c_double), C.POINTER(C.c_double), vp]),
     "dmrg_expectation_value": (st, [C.c_int, vp, i64p, vp, i64p, vp, i64p, vp, i64p, dp, vp]),
     "dmrg_dot": (st, [C.c_int, i64, vp, vp, dp, vp]),
     "dmrg_nrm2": (st, [C.c_int, i64, vp, dp, vp]),

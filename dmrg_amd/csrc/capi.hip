@@ -5,6 +5,7 @@
 #include "matvec_op.hpp"
 #include "dense_eig.hpp"
 #include "eigs.hpp"
+#include "expand.hpp"
 #include "sweep.hpp"
 #include "dense.hpp"
 #include "svd.hpp"
@@ -249,6 +250,58 @@ dmrg_status dmrg_env_build_block(dmrg_dtype dtype, char side, void *block_out, c
         StatelessScope scope(static_cast<hipStream_t>(stream));
         void *ws = scratch_workspace(static_cast<size_t>(env_ws_elems(D)) * dtype_size(dtype));
         env_build_block(dtype, side, D, block_out, otherblock, mps, mpo, ws, static_cast<hipStream_t>(stream));
+    });
+}
+
+dmrg_status dmrg_env_expansion_term(dmrg_dtype dtype, char side, void *P, const void *block, const int64_t block_dims[3],
+                                    const void *mps, const int64_t mps_dims[3], const void *mpo, const int64_t mpo_dims[4],
+                                    double alpha, void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(P && block && mps && mpo && block_dims && mps_dims && mpo_dims, DMRG_ERR_ARG, "null argument");
+        DMRG_REQUIRE(side == 'L' || side == 'R', DMRG_ERR_ARG, "side must be 'L' or 'R'");
+        DMRG_REQUIRE(!std::isnan(alpha), DMRG_ERR_ARG, "the mixing factor alpha is nan");
+        DMRG_REQUIRE(mps_dims[0] == mpo_dims[2] && mpo_dims[2] == mpo_dims[3], DMRG_ERR_DIMS, "expansion: mps dim 0 != mpo dim 2");
+        DMRG_REQUIRE(block_dims[0] == block_dims[1], DMRG_ERR_DIMS, "expansion: block must be square in its bond indices");
+        if(side == 'L') {
+            DMRG_REQUIRE(block_dims[0] == mps_dims[1], DMRG_ERR_DIMS, "expansion L: block dim 0 != mps dim 1");
+            DMRG_REQUIRE(block_dims[2] == mpo_dims[0], DMRG_ERR_DIMS, "expansion L: block dim 2 != mpo dim 0");
+        } else {
+            DMRG_REQUIRE(block_dims[0] == mps_dims[2], DMRG_ERR_DIMS, "expansion R: block dim 0 != mps dim 2");
+            DMRG_REQUIRE(block_dims[2] == mpo_dims[1], DMRG_ERR_DIMS, "expansion R: block dim 2 != mpo dim 1");
+        }
+        Dims D;
+        D.d = mps_dims[0], D.chiL = mps_dims[1], D.chiR = mps_dims[2], D.DL = mpo_dims[0], D.DR = mpo_dims[1];
+        const int64_t nP = D.d * D.chiL * D.chiR * (side == 'L' ? D.DR : D.DL);
+        auto          s  = static_cast<hipStream_t>(stream);
+        StatelessScope scope(s);
+        if(nP == 0) return;
+        if(alpha == 0.0) {
+            DMRG_HIP_CHECK(hipMemsetAsync(P, 0, static_cast<size_t>(nP) * dtype_size(dtype), s));
+            return;
+        }
+        void *ws = scratch_workspace(static_cast<size_t>(D.d * D.chiL * D.chiR * (side == 'L' ? D.DL : D.DR)) * dtype_size(dtype));
+        env_expansion_term(dtype, side, D, P, block, mps, mpo, ws, s);
+        if(alpha != 1.0) {
+            switch(dtype) {
+                case DMRG_FP64: scale<double>(nP, static_cast<double *>(P), alpha, s); break;
+                case DMRG_CX64: scale<cpx<double>>(nP, static_cast<cpx<double> *>(P), alpha, s); break;
+                case DMRG_FP32: scale<float>(nP, static_cast<float *>(P), alpha, s); break;
+                case DMRG_CX32: scale<cpx<float>>(nP, static_cast<cpx<float> *>(P), alpha, s); break;
+                default: throw Error(DMRG_ERR_ARG, "bad dtype");
+            }
+        }
+    });
+}
+
+dmrg_status dmrg_optimally_mixed_block(dmrg_matvec *h1, dmrg_matvec *h2, const void *theta, int mode, void *mixed,
+                                       double alphas[3], double *opt_val, void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(h1 && h2 && theta && mixed && alphas, DMRG_ERR_ARG, "null argument");
+        MixedBlockResult r;
+        optimally_mixed_block(*reinterpret_cast<MatVecOp *>(h1), *reinterpret_cast<MatVecOp *>(h2), theta, mode, mixed, r,
+                              static_cast<hipStream_t>(stream));
+        for(int i = 0; i < 3; ++i) alphas[i] = r.alpha[i];
+        if(opt_val) *opt_val = r.opt_val;
     });
 }
 

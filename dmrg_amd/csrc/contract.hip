@@ -181,6 +181,45 @@ void env_build_block(dmrg_dtype t, char side, const Dims &D, void *out, const vo
     }
 }
 
+// Expansion term of subspace expansion (EnvBase.cpp:303-361), alpha applied by the caller:
+//   side 'L': P[s',a',b,wR] = sum_{a,w,s} block[a,a',w] mps[s,a,b] W[w,wR,s,s'], P dims (d, chiL, chiR*DR);
+//   side 'R': the reference forms R2[b',a,wL,s'] = sum_{b,w,s} block[b,b',w] mps[s,a,b] W[wL,w,s,s'],
+//             shuffles it to [s',b',a,wL] and reshapes that buffer to (d, chiL*DL, chiR) as it lies, so P's
+//             storage offset is s' + d(b' + chiR(a + chiL wL)).
+void env_expansion_term(dmrg_dtype t, char side, const Dims &D, void *P, const void *block, const void *mps, const void *mpo,
+                        void *ws, hipStream_t s) {
+    const int64_t d = D.d, cL = D.chiL, cR = D.chiR, DL = D.DL, DR = D.DR;
+    if(side == 'L') {
+        // X[(a',w),(s,b)] = sum_a block[a,a',w] mps[s,a,b]          m=(a',w) k=a n=(s,b)
+        GemmDesc e1 = make(t, cL * DL, d * cR, cL, 1);
+        e1.A = block, e1.am = fused(cL, cL * cL, cL), e1.ak = plain(1);
+        e1.B = mps, e1.bk = plain(d), e1.bn = fused(1, d * cL, d);
+        e1.C = ws, e1.cm = plain(1), e1.cn = plain(cL * DL);
+        gemm(e1, s);
+        // P[s',a',b,wR] = sum_{w,s} W[w,wR,s,s'] X[(a',w),(s,b)]     m=(s',wR) k=(w,s) n=(a',b)
+        GemmDesc e2 = make(t, d * DR, cL * cR, DL * d, 1);
+        e2.A = mpo, e2.am = fused(DL * DR * d, DL, d), e2.ak = fused(1, DL * DR, DL);
+        e2.B = ws, e2.bk = fused(cL, cL * DL, DL), e2.bn = fused(1, cL * DL * d, cL);
+        e2.C = P, e2.cm = fused(1, d * cL * cR, d), e2.cn = fused(d, d * cL, cL);
+        gemm(e2, s);
+    } else if(side == 'R') {
+        // X[(b',w),(s,a)] = sum_b block[b,b',w] mps[s,a,b]          m=(b',w) k=b n=(s,a)
+        GemmDesc e1 = make(t, cR * DR, d * cL, cR, 1);
+        e1.A = block, e1.am = fused(cR, cR * cR, cR), e1.ak = plain(1);
+        e1.B = mps, e1.bk = plain(d * cL), e1.bn = plain(1);
+        e1.C = ws, e1.cm = plain(1), e1.cn = plain(cR * DR);
+        gemm(e1, s);
+        // P at s' + d(b' + chiR(a + chiL wL)) = sum_{w,s} W[wL,w,s,s'] X[(b',w),(s,a)]   m=(s',wL) k=(w,s) n=(b',a)
+        GemmDesc e2 = make(t, d * DL, cR * cL, DR * d, 1);
+        e2.A = mpo, e2.am = fused(DL * DR * d, 1, d), e2.ak = fused(DL, DL * DR, DR);
+        e2.B = ws, e2.bk = fused(cR, cR * DR, DR), e2.bn = fused(1, cR * DR * d, cR);
+        e2.C = P, e2.cm = fused(1, d * cR * cL, d), e2.cn = fused(d, d * cR, cR);
+        gemm(e2, s);
+    } else {
+        throw Error(DMRG_ERR_ARG, "env side must be 'L' or 'R'");
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Stateless workspace
 

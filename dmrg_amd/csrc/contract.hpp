@@ -37,6 +37,11 @@ int64_t env_ws_elems(const Dims &D);
 void    env_build_block(dmrg_dtype t, char side, const Dims &D, void *out, const void *other, const void *theta,
                         const void *mpo, void *ws, hipStream_t s);
 
+// Subspace-expansion term without the factor alpha (EnvBase.cpp:303-361); ws holds d*chiL*chiR*D elements,
+// D = DL for side 'L' and DR for side 'R'.
+void env_expansion_term(dmrg_dtype t, char side, const Dims &D, void *P, const void *block, const void *mps, const void *mpo,
+                        void *ws, hipStream_t s);
+
 // Grow-only device workspace shared by the stateless entry points (one per device).
 void *scratch_workspace(size_t bytes);
 

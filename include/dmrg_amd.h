@@ -86,6 +86,17 @@ dmrg_status dmrg_env_build_block(dmrg_dtype dtype, char side, void *block_out, c
                                  const int64_t other_dims[3], const void *mps, const int64_t mps_dims[3],
                                  const void *mpo, const int64_t mpo_dims[4], void *stream);
 
+/* Subspace-expansion term P = alpha * (block . mps . mpo). Replaces EnvBase::get_expansion_term(mps, mpo, alpha)
+ * (EnvBase.cpp:303-361); the pre-truncation of its MpsSite overload (:363-455) is an SVD of mps that callers
+ * compose with dmrg_svd_truncate. side 'L': block (chiL, chiL, DL) of the left env (energy block with the MPO,
+ * variance block with the MPO^2), P (d, chiL, chiR*DR), P[s',a',b + chiR wR] = sum block[a,a',w] mps[s,a,b]
+ * W[w,wR,s,s']. side 'R': block (chiR, chiR, DR), P (d, chiL*DL, chiR) holding, at storage offset
+ * s' + d(b' + chiR(a + chiL wL)), sum block[b,b',w] mps[s,a,b] W[wL,w,s,s'] (the reference's shuffle {3,0,1,2}
+ * followed by a plain reshape). alpha = 0 gives zeros; a NaN alpha is DMRG_ERR_ARG (EnvBase.cpp:358). */
+dmrg_status dmrg_env_expansion_term(dmrg_dtype dtype, char side, void *P, const void *block, const int64_t block_dims[3],
+                                    const void *mps, const int64_t mps_dims[3], const void *mpo, const int64_t mpo_dims[4],
+                                    double alpha, void *stream);
+
 /* <theta|H_eff|theta> = sum conj(theta) . (H_eff theta).
  * Replaces tools::common::contraction::expectation_value (contraction_tblis.cpp:30-97). result[0..1] = re, im. */
 dmrg_status dmrg_expectation_value(dmrg_dtype dtype, const void *mps, const int64_t mps_dims[3], const void *mpo,
@@ -198,6 +209,18 @@ void        dmrg_eigs_default_config(dmrg_eigs_config *cfg);
  * a GPU Davidson instead of PRIMME's GD_Olsen_plusK. Returns DMRG_OK also when not converged (res->nconv < nev). */
 dmrg_status dmrg_eigs(dmrg_matvec *op, const dmrg_eigs_config *cfg, const void *x0, void *evecs, dmrg_eigs_result *res,
                       void *stream);
+
+/* Optimally mixed block of subspace expansion. Replaces tools::finite::env::get_optimally_mixed_block_1site
+ * (env.cpp:96-239): V = MGS([theta, H1 theta, H2 theta]) (env.cpp:74-93), K1 = V^H H1 V, K2 = V^H H2 V (lower
+ * triangles), K3 = K2 - K1 K1 with the mode's decoupling, the eigenvector of K3's lowest eigenvalue (or, when a
+ * K has a row below tol, the one with the largest weight on theta), mixed = V * col. h1: the energy operator,
+ * h2: the operator on the variance envs (MPO^2), same shape and dtype (FP64 or CX64). mode: bit 1 = H1,
+ * bit 2 = H2 (EnvExpandMode values). theta, mixed: DEVICE, n elements. alphas = {alpha_mps, alpha_h1v,
+ * alpha_h2v}; opt_val = the chosen K3 eigenvalue. The eigenvector is rotated so that its largest component is
+ * real positive before its real part is taken (the reference keeps Eigen's unnormalised phase): alphas and
+ * mixed match the reference up to one overall sign for real problems. */
+dmrg_status dmrg_optimally_mixed_block(dmrg_matvec *h1, dmrg_matvec *h2, const void *theta, int mode, void *mixed,
+                                       double alphas[3], double *opt_val, void *stream);
 
 /* ---------------------------------------------------------------- finite DMRG sweep (ising_tf_rf) */
 

@@ -1,0 +1,85 @@
+"""GPU parity of subspace expansion: the expansion term (EnvBase.cpp:303-361) and the optimally mixed block
+(env.cpp:96-239), through the C ABI, against the oracle (oracle/expansion.py).
+
+Tolerances: expansion term relative 2-norm error <= 1e-12 (FP64/CX64); mixed block: alphas and the mixed
+theta within 1e-10 after fixing one overall sign (the reference's eigenvector sign is Eigen's choice), the
+chosen K3 eigenvalue within 1e-10 of the scale of K3.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import dmrg_amd as G
+from dmrg_amd import DmrgError
+from dmrg_amd.workloads import random_heff_problem
+from oracle import expansion as X
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
+
+
+def rand(rng, cplx, *shape):
+    a = rng.standard_normal(shape)
+    return np.asfortranarray(a + 1j * rng.standard_normal(shape) if cplx else a)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("shape", [(2, 1, 1, 1, 1), (2, 3, 5, 2, 3), (2, 7, 4, 5, 5), (4, 9, 6, 3, 8), (2, 64, 48, 5, 5),
+                                   (2, 33, 65, 25, 25)])
+def test_expansion_term_matches_oracle(shape, side, cplx):
+    print(f"[case] expansion term side={side} shape={shape} cplx={cplx}", file=sys.stderr, flush=True)
+    d, cL, cR, DL, DR = shape
+    rng = np.random.default_rng(sum(shape) * 3 + (1 if cplx else 0))
+    mps, mpo = rand(rng, cplx, d, cL, cR), rand(rng, cplx, DL, DR, d, d)
+    block = rand(rng, cplx, cL, cL, DL) if side == "L" else rand(rng, cplx, cR, cR, DR)
+    for alpha in (1.0, 0.25):
+        P = G.env_expansion_term(side, block, mps, mpo, alpha)
+        ref = X.expansion_term(side, block, mps, mpo, alpha)
+        assert P.shape == ref.shape
+        assert rel(P, ref) <= 1e-12
+    assert not np.any(G.env_expansion_term(side, block, mps, mpo, 0.0))
+
+
+def test_expansion_term_errors():
+    rng = np.random.default_rng(1)
+    mps, mpo = rand(rng, True, 2, 3, 4), rand(rng, True, 2, 3, 2, 2)
+    with pytest.raises(DmrgError):
+        G.env_expansion_term("L", rand(rng, True, 4, 4, 2), mps, mpo)  # block dim 0 != chiL
+    with pytest.raises(DmrgError):
+        G.env_expansion_term("R", rand(rng, True, 4, 4, 2), mps, mpo)  # block dim 2 != DR
+    with pytest.raises(DmrgError):
+        G.env_expansion_term("L", rand(rng, True, 3, 3, 2), mps, mpo, float("nan"))
+
+
+def ops(cplx, chiL, chiR, seed):
+    th, W1, L1, R1 = random_heff_problem(2, chiL, chiR, 5, 5, seed=seed, complex_=cplx)
+    _, W2, L2, R2 = random_heff_problem(2, chiL, chiR, 4, 6, seed=seed + 50, complex_=cplx)
+    if not cplx:
+        L1, R1, L2, R2 = (np.ascontiguousarray(x.real) for x in (L1, R1, L2, R2))
+    return th, (W1, L1, R1), (W2, L2, R2)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("chi", [(3, 4), (12, 10), (64, 64)])
+def test_optimally_mixed_block_matches_oracle(chi, mode, cplx):
+    print(f"[case] mixed block chi={chi} mode={mode} cplx={cplx}", file=sys.stderr, flush=True)
+    th, (W1, L1, R1), (W2, L2, R2) = ops(cplx, *chi, seed=chi[0] + mode)
+    dt = np.complex128 if cplx else np.float64
+    h1 = G.MatVecMPOS([W1], L1, R1, dtype=dt)
+    h2 = G.MatVecMPOS([W2], L2, R2, dtype=dt)
+    try:
+        mixed, alpha, val = G.optimally_mixed_block(h1, h2, th.astype(dt), mode=mode)
+        mref, aref, vref, _, _, K3 = X.optimally_mixed_block(th.astype(dt), W1, L1, R1, L2, R2, mode=mode, mpo2=W2)
+        sgn = 1.0 if np.dot(alpha, aref) >= 0 else -1.0
+        assert np.max(np.abs(sgn * np.array(alpha) - np.array(aref))) <= 1e-10, (alpha, aref)
+        assert rel(sgn * mixed, mref) <= 1e-10
+        assert abs(val - vref) <= 1e-10 * max(1.0, np.abs(K3).max())
+    finally:
+        h1.close()
+        h2.close()
