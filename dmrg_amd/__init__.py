@@ -246,9 +246,10 @@ def svd_truncate(A, trunc_lim: float = 1e-14, rank_max: int = 0, method: int = 3
 def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_max: int = 64, max_sweeps: int = 20,
                       two_site_sweeps: int = 10, var_tol: float = 1e-12, trunc_lim: float = 1e-14, eigs_tol: float = 1e-12,
                       mixed: bool = False, complex_: bool = False, seed: int = 0, mixed_min_n: int = 0,
-                      stream=None) -> dict:
+                      xdmrg: bool = False, energy_target: float = 0.0, stream=None) -> dict:
     """Finite DMRG ground state of the random transverse-field Ising chain on the GPU
-    (fdmrg::run_algorithm with ising_tf_rf, fdmrg.cpp:170-376). Returns energies, variance, entropies, timings."""
+    (fdmrg::run_algorithm with ising_tf_rf, fdmrg.cpp:170-376), or with xdmrg the eigenstate found by minimising
+    <(H - energy_target)^2> (xdmrg.cpp:297-341, folded spectrum). Returns energies, variance, entropies, timings."""
     h = np.ascontiguousarray(np.asarray(h_rand, dtype=np.float64))
     assert h.shape == (L,)
     cfg = _lib.FdmrgConfig()
@@ -258,6 +259,7 @@ def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_m
     cfg.chi_max, cfg.max_sweeps, cfg.two_site_sweeps = chi_max, max_sweeps, two_site_sweeps
     cfg.var_tol, cfg.trunc_lim, cfg.eigs_tol = var_tol, trunc_lim, eigs_tol
     cfg.mixed, cfg.complex_, cfg.seed, cfg.mixed_min_n = int(mixed), int(complex_), seed, mixed_min_n
+    cfg.xdmrg, cfg.energy_target = int(xdmrg), float(energy_target)
     res = _lib.FdmrgResult()
     call("dmrg_fdmrg_ising_tf_rf", C.byref(cfg), C.byref(res), stream)
     out = {k: getattr(res, k) for k, _ in res._fields_ if k not in ("entropies", "chi")}
@@ -418,6 +420,31 @@ class MatVecMPOS:
         info = {k: getattr(res, k) for k, _ in res._fields_ if k not in ("eigval", "rnorm")}
         info["rnorm"] = list(res.rnorm[:nev])
         return np.array(res.eigval[:nev]), vecs, info
+
+    def eigs_folded_spectrum(self, h: "MatVecMPOS", x0, energy_shift: float, target: str = "CLOSEST",
+                             shift: float = 0.0, tol: float = 1e-12, precond: int = 0, ncv: int = 0,
+                             max_matvecs: int = 100000, stream=None):
+        """Folded-spectrum executor on this operator (the MPO^2 of H - energy_shift on the variance envs), h = H
+        (optimize_folded_spectrum.cpp:235-342). Returns (energy, variance, eigenvector, result dict)."""
+        cfg = _lib.EigsConfig()
+        load().dmrg_eigs_default_config(C.byref(cfg))
+        cfg.nev, cfg.ncv, cfg.target = 1, ncv, {"SA": 0, "LA": 1, "CLOSEST": 2}[target]
+        cfg.shift, cfg.tol, cfg.max_matvecs, cfg.precond = shift, tol, max_matvecs, int(precond)
+        n = self.rows()
+        dx0 = DeviceArray.from_numpy(np.asarray(x0, dtype=self.dtype).reshape(n, order="F"))
+        dev = DeviceArray((n, 1), self.dtype)
+        res = _lib.EigsResult()
+        e, v = C.c_double(), C.c_double()
+        try:
+            call("dmrg_eigs_folded_spectrum", h.h, self.h, C.byref(cfg), dx0.ptr, float(energy_shift), dev.ptr,
+                 C.byref(e), C.byref(v), C.byref(res), stream)
+            vec = dev.to_numpy()[:, 0]
+        finally:
+            dx0.close()
+            dev.close()
+        info = {k: getattr(res, k) for k, _ in res._fields_ if k not in ("eigval", "rnorm")}
+        info["eigval"], info["rnorm"] = res.eigval[0], res.rnorm[0]
+        return e.value, v.value, vec, info
 
     def time_legs(self, dx: DeviceArray, dy: DeviceArray, nrep: int, fp32: bool = False, stream=None):
         """Average durations (ms) over nrep device matvecs: [leg 1, leg 2, leg 3 main kernel, leg 3 split-K reduce]."""

@@ -105,7 +105,8 @@ class FdmrgConfig(C.Structure):
     _fields_ = [("L", C.c_int), ("J", C.c_double), ("h_tran", C.c_double), ("h_rand", C.POINTER(C.c_double)),
                 ("chi_max", C.c_int), ("max_sweeps", C.c_int), ("two_site_sweeps", C.c_int), ("var_tol", C.c_double),
                 ("trunc_lim", C.c_double), ("eigs_tol", C.c_double), ("eigs_ncv", C.c_int), ("mixed", C.c_int),
-                ("complex_", C.c_int), ("seed", C.c_uint64), ("mixed_min_n", C.c_int64)]
+                ("complex_", C.c_int), ("seed", C.c_uint64), ("mixed_min_n", C.c_int64), ("xdmrg", C.c_int),
+                ("energy_target", C.c_double)]
 
 
 class FdmrgResult(C.Structure):
@@ -141,6 +142,8 @@ class ServeStats(C.Structure):
 SIGNATURES["dmrg_serve"] = (st, [C.POINTER(ServeConfig), C.POINTER(Request), C.c_int, C.POINTER(RequestResult),
                                  C.POINTER(ServeStats)])
 SIGNATURES["dmrg_request_footprint"] = (i64, [C.c_int, C.c_int, C.c_int])
+SIGNATURES["dmrg_eigs_folded_spectrum"] = (st, [vp, vp, C.POINTER(EigsConfig), vp, C.c_double, vp, C.POINTER(C.c_double),
+                                                C.POINTER(C.c_double), C.POINTER(EigsResult), vp])
 SIGNATURES["dmrg_fdmrg_default_config"] = (None, [C.POINTER(FdmrgConfig)])
 SIGNATURES["dmrg_fdmrg_ising_tf_rf"] = (st, [C.POINTER(FdmrgConfig), C.POINTER(FdmrgResult), vp])
 SIGNATURES["dmrg_eigs_default_config"] = (None, [C.POINTER(EigsConfig)])

@@ -492,6 +492,39 @@ dmrg_status dmrg_eigs(dmrg_matvec *opp, const dmrg_eigs_config *cfg, const void 
     });
 }
 
+dmrg_status dmrg_eigs_folded_spectrum(dmrg_matvec *h, dmrg_matvec *h2, const dmrg_eigs_config *cfg, const void *x0,
+                                      double energy_shift, void *evecs, double *energy, double *variance,
+                                      dmrg_eigs_result *res, void *stream) {
+    const dmrg_status st = dmrg_eigs(h2, cfg, x0, evecs, res, stream);
+    if(st != DMRG_OK) return st;
+    return guard([&] {
+        DMRG_REQUIRE(h && energy && variance, DMRG_ERR_ARG, "null argument");
+        auto &H  = *reinterpret_cast<MatVecOp *>(h);
+        auto &H2 = *reinterpret_cast<MatVecOp *>(h2);
+        DMRG_REQUIRE(H.dtype == H2.dtype && H.D.n() == H2.D.n(), DMRG_ERR_DIMS, "folded spectrum: H and H^2 act on different spaces");
+        auto          s  = static_cast<hipStream_t>(stream);
+        const int64_t n  = H.D.n();
+        const size_t  es = dtype_size(H.dtype);
+        DevBuf        y;
+        y.reserve(static_cast<size_t>(n) * es + 8 * sizeof(double));
+        double *red = reinterpret_cast<double *>(static_cast<char *>(y.p) + static_cast<size_t>(n) * es);
+        for(int q = 0; q < cfg->nev; ++q) {
+            const void *x = static_cast<const char *>(evecs) + static_cast<size_t>(q) * n * es;
+            H.multax_device(x, n, y.p, n, 1, s, false);
+            dot_device(H.dtype, n, x, y.p, red, s);
+            H2.multax_device(x, n, y.p, n, 1, s, false);
+            dot_device(H.dtype, n, x, y.p, red + 2, s);
+            dot_device(H.dtype, n, x, x, red + 4, s);
+            double r[6];
+            DMRG_HIP_CHECK(hipMemcpyAsync(r, red, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
+            DMRG_HIP_CHECK(hipStreamSynchronize(s));
+            energy[q]       = r[0] / r[4];
+            const double dE = energy[q] - energy_shift;
+            variance[q]     = r[2] / r[4] - dE * dE; // <(H - E_shift)^2> - (E - E_shift)^2 (xdmrg.cpp:488-496)
+        }
+    });
+}
+
 void dmrg_fdmrg_default_config(dmrg_fdmrg_config *cfg) {
     if(!cfg) return;
     *cfg                 = dmrg_fdmrg_config{};

@@ -7,6 +7,8 @@
 //     with H^2 built from energy-shifted MPOs as set_energy_shift_mpo does (MpoSite.cpp:57-69).
 #include "sweep.hpp"
 
+#include "dense.hpp"
+
 #include "eigs.hpp"
 #include "svd.hpp"
 
@@ -66,6 +68,9 @@ std::vector<cx> mpo_squared(const std::vector<cx> &W, const int64_t d[4], int64_
 
 namespace {
 
+// settings::precision::eig_max_size (source/config/settings.h:249): largest problem for the dense solver.
+constexpr int64_t EIG_MAX_SIZE = 4096;
+
 template <typename T>
 struct Fdmrg {
     using clock = std::chrono::steady_clock;
@@ -79,12 +84,13 @@ struct Fdmrg {
     std::vector<DevBuf>                       M;             // MPS site tensors (d, chi[i], chi[i+1])
     std::vector<int64_t>                      chi;           // chi[0..L]
     std::vector<DevBuf>                       EL, ER, EL2, ER2; // EL[i]: left of site i; ER[i]: right of site i
-    DevBuf                                    theta, theta2, xbuf, ws, tmp, sdev;
+    DevBuf                                    theta, theta2, xbuf, ws, tmp, sdev, dense;
     DeviceSvd                                 svd;
     std::unique_ptr<MatVecOp>                 hop, h2op;
     std::unique_ptr<Davidson<T>>              dav;
     std::vector<double>                       entropy, terr;
     double                                    eshift = 0.0;
+    bool                                      folded = false; // xDMRG: solves on (H - E_tgt)^2
     dmrg_fdmrg_result                        *res;
     double t_eigs = 0, t_env = 0, t_svd = 0;
 
@@ -205,23 +211,50 @@ struct Fdmrg {
         double rnorm;
     };
 
-    // Ground state of H_eff on the active sites [i, i+ns) with theta as the start vector; theta <- eigenvector.
-    Solve solve(int i, int ns, T *th) {
-        const auto t0 = clock::now();
+    // Point hop (H, or with sq the shifted H^2 on the variance envs) at the active sites [i, i+ns).
+    MatVecOp &bind_op(std::unique_ptr<MatVecOp> &h, int i, int ns, bool sq) {
+        const auto               &Ws  = sq ? W2 : W;
+        const auto               &Wds = sq ? W2d : Wd;
         std::vector<const void *> mp;
         std::vector<int64_t>      md;
         for(int k = 0; k < ns; ++k) {
-            mp.push_back(W[i + k].data());
-            md.insert(md.end(), Wd[i + k].begin(), Wd[i + k].end());
+            mp.push_back(Ws[i + k].data());
+            md.insert(md.end(), Wds[i + k].begin(), Wds[i + k].end());
         }
-        const int64_t dl[3] = {chi[i], chi[i], Wd[i][0]};
-        const int64_t dr[3] = {chi[i + ns], chi[i + ns], Wd[i + ns - 1][1]};
-        if(!hop) hop = std::make_unique<MatVecOp>(DT, ns, mp.data(), md.data(), EL[i].p, dl, ER[i + ns - 1].p, dr, true);
-        else hop->rebind(ns, mp.data(), md.data(), EL[i].p, dl, ER[i + ns - 1].p, dr);
-        MatVecOp     &op = *hop;
+        const int64_t dl[3] = {chi[i], chi[i], Wds[i][0]};
+        const int64_t dr[3] = {chi[i + ns], chi[i + ns], Wds[i + ns - 1][1]};
+        const void   *el = (sq ? EL2 : EL)[i].p, *er = (sq ? ER2 : ER)[i + ns - 1].p;
+        if(!h) h = std::make_unique<MatVecOp>(DT, ns, mp.data(), md.data(), el, dl, er, dr, true);
+        else h->rebind(ns, mp.data(), md.data(), el, dl, er, dr);
+        return *h;
+    }
+
+    // <theta|H|theta> / <theta|theta> on the active sites [i, i+ns).
+    double energy_of(int i, int ns, const T *th) {
+        MatVecOp     &op = bind_op(hop, i, ns, false);
+        const int64_t n  = op.D.n();
+        tmp.reserve(static_cast<size_t>(n) * es() + 64 * sizeof(double));
+        op.multax_device(th, n, tmp.p, n, 1, s, false);
+        double *red = reinterpret_cast<double *>(static_cast<char *>(tmp.p) + static_cast<size_t>(n) * es());
+        dot_device(DT, n, th, tmp.p, red, s);
+        dot_device(DT, n, th, th, red + 2, s);
+        double r[4];
+        DMRG_HIP_CHECK(hipMemcpyAsync(r, red, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+        DMRG_HIP_CHECK(hipStreamSynchronize(s));
+        return r[0] / r[2];
+    }
+
+    // Eigenvector of H_eff on the active sites [i, i+ns) with theta as the start vector; theta <- eigenvector.
+    // Ground state (SA) of H, or in the folded mode the eigenvalue of (H - E_tgt)^2 closest to 0 (the
+    // folded-spectrum executor with primme_closest_abs at target 0, optimize_folded_spectrum.cpp:272-342),
+    // reported with its energy <H>.
+    Solve solve(int i, int ns, T *th) {
+        const auto    t0 = clock::now();
+        MatVecOp     &op = folded ? bind_op(h2op, i, ns, true) : bind_op(hop, i, ns, false);
         const int64_t n  = op.D.n();
         EigsParams    p;
-        p.target      = TARGET_SA;
+        p.target      = folded ? TARGET_CLOSEST : TARGET_SA;
+        p.shift       = 0.0;
         p.tol         = cfg.eigs_tol > 0 ? cfg.eigs_tol : 1e-12;
         p.ncv         = cfg.eigs_ncv;
         p.max_matvecs = 20000;
@@ -230,7 +263,25 @@ struct Fdmrg {
         EigsStats st;
         xbuf.reserve(static_cast<size_t>(n) * es());
         const int64_t mixed_min = cfg.mixed_min_n > 0 ? cfg.mixed_min_n : 131072;
-        if(cfg.mixed && n >= mixed_min) {
+        if(folded && n <= EIG_MAX_SIZE) {
+            // Small folded problems go to the dense solver, as xDMRG picks OptSolver::EIG for problem sizes up to
+            // eig_max_size (xdmrg.cpp:367, optimize_folded_spectrum_eig.cpp:17-36): the folded spectrum's gaps are
+            // tiny relative to its norm, where an iterative solver can settle on the second-closest level.
+            dense.reserve(static_cast<size_t>(n) * n * es());
+            heff_dense(DT, op.D, op.mpo.p, op.envL_p, op.envR_p, dense.p, n, false, s);
+            std::vector<double> w;
+            svd.eigh<T>(static_cast<int>(n), dense.as<T>(), static_cast<int>(n), true, w);
+            int k = 0;
+            for(int j = 1; j < static_cast<int>(n); ++j)
+                if(std::fabs(w[j]) < std::fabs(w[k])) k = j; // closest_abs to the target 0
+            DMRG_HIP_CHECK(hipMemcpyAsync(th, dense.as<T>() + static_cast<int64_t>(k) * n, static_cast<size_t>(n) * es(),
+                                          hipMemcpyDeviceToDevice, s));
+            res->eigensolves += 1;
+            ev = energy_of(i, ns, th);
+            t_eigs += std::chrono::duration<double>(clock::now() - t0).count();
+            return {ev, 0.0};
+        }
+        if(cfg.mixed && !folded && n >= mixed_min) {
             EigsStats s1;
             p.inner_max = 20; // FP32 correction solves in the FP64 phase
             st = eigs_mixed<T, std::conditional_t<std::is_same_v<T, double>, float, cpx<float>>>(op, s, p, th, xbuf.as<T>(), &ev, &rn, &s1);
@@ -243,17 +294,14 @@ struct Fdmrg {
         DMRG_HIP_CHECK(hipMemcpyAsync(th, xbuf.p, static_cast<size_t>(n) * es(), hipMemcpyDeviceToDevice, s));
         res->matvecs += st.matvecs;
         res->eigensolves += 1;
+        if(folded) ev = energy_of(i, ns, th);
         t_eigs += std::chrono::duration<double>(clock::now() - t0).count();
         return {ev, rn};
     }
 
     // <theta|(H - E_shift)^2|theta> on site i (single-site, H^2 envs).
     double expval_sq(int i, const T *th) {
-        const int64_t dl[3] = {chi[i], chi[i], W2d[i][0]}, dr[3] = {chi[i + 1], chi[i + 1], W2d[i][1]};
-        const void   *mp[1] = {W2[i].data()};
-        if(!h2op) h2op = std::make_unique<MatVecOp>(DT, 1, mp, W2d[i].data(), EL2[i].p, dl, ER2[i].p, dr, true);
-        else h2op->rebind(1, mp, W2d[i].data(), EL2[i].p, dl, ER2[i].p, dr);
-        MatVecOp     &op = *h2op;
+        MatVecOp     &op = bind_op(h2op, i, 1, true);
         const int64_t n  = op.D.n();
         tmp.reserve(static_cast<size_t>(n) * es() + 64 * sizeof(double) + 2 * 512 * sizeof(double) + 64);
         op.multax_device(th, n, tmp.p, n, 1, s, false);
@@ -404,7 +452,10 @@ struct Fdmrg {
         DMRG_REQUIRE(cfg.chi_max >= 1, DMRG_ERR_ARG, "chi_max must be positive");
         entropy.assign(L + 1, 0.0);
         terr.assign(L + 1, 0.0);
-        build_mpos(0.0);
+        folded = cfg.xdmrg != 0;
+        // xDMRG: the MPO^2 carries the fixed target shift from the start (set_energy_shift_mpo, xdmrg.cpp:487-501).
+        eshift = folded ? cfg.energy_target / L : 0.0;
+        build_mpos(eshift);
         init_state();
         // Two-site sweeps: grow bonds and converge the energy.
         double E = 0, Eprev = 1e300;
@@ -417,11 +468,14 @@ struct Fdmrg {
             if(std::fabs(E - Eprev) <= 1e-12 * std::max(1.0, std::fabs(E))) break;
             Eprev = E;
         }
-        // Single-site sweeps with H^2 shifted by the current energy; stop at variance < var_tol.
-        eshift = E / L;
-        build_mpos(eshift);
-        // Rebuild the H^2 envs for the current (centre at site 0, right-normalised) state.
-        for(int i = L - 1; i >= 1; --i) env_right(i);
+        // Single-site sweeps with H^2 shifted by the current energy (by the target in xDMRG); stop at
+        // variance < var_tol.
+        if(!folded) {
+            eshift = E / L;
+            build_mpos(eshift);
+            // Rebuild the H^2 envs for the current (centre at site 0, right-normalised) state.
+            for(int i = L - 1; i >= 1; --i) env_right(i);
+        }
         double var = 1e300;
         const int max_sw = cfg.max_sweeps > 0 ? cfg.max_sweeps : 20;
         int       one_sw = 0;

@@ -222,6 +222,16 @@ dmrg_status dmrg_eigs(dmrg_matvec *op, const dmrg_eigs_config *cfg, const void *
 dmrg_status dmrg_optimally_mixed_block(dmrg_matvec *h1, dmrg_matvec *h2, const void *theta, int mode, void *mixed,
                                        double alphas[3], double *opt_val, void *stream);
 
+/* Folded-spectrum executor. Replaces eigs_manager_folded_spectrum + eigs_folded_spectrum_executor
+ * (optimize_folded_spectrum.cpp:235-342): the eigensolve of dmrg_eigs on h2, the operator on the variance envs
+ * with the MPO^2 of H - energy_shift (cfg->target DMRG_TARGET_CLOSEST with cfg->shift = eigv_target is the
+ * reference's primme_closest_abs for OptRitz::SM; cfg->precond = 1 its Jacobi preconditioner), warm-started from
+ * x0, then per returned vector the results extract_results reports (opt_extract_results.cpp:19-90):
+ * energy[q] = <x|H|x>/<x|x> with h, variance[q] = <x|(H - E_shift)^2|x>/<x|x> - (energy - E_shift)^2. */
+dmrg_status dmrg_eigs_folded_spectrum(dmrg_matvec *h, dmrg_matvec *h2, const dmrg_eigs_config *cfg, const void *x0,
+                                      double energy_shift, void *evecs, double *energy, double *variance,
+                                      dmrg_eigs_result *res, void *stream);
+
 /* ---------------------------------------------------------------- finite DMRG sweep (ising_tf_rf) */
 
 #define DMRG_FDMRG_MAX_L 256
@@ -244,6 +254,10 @@ typedef struct dmrg_fdmrg_config {
     int           complex_;        /* 1: complex128 tensors (the model is real; for testing the complex path) */
     uint64_t      seed;            /* initial random product state */
     int64_t       mixed_min_n;     /* <= 0: 131072 (smaller warm-started solves converge faster in FP64 alone) */
+    int           xdmrg;           /* 1: excited-state search (xDMRG): every solve minimises <(H - energy_target)^2> with the
+                                      folded-spectrum executor on the MPO^2 operator (optimize_folded_spectrum.cpp:235-342,
+                                      closest_abs to 0), the shift fixed at the target (xdmrg.cpp:487-501) */
+    double        energy_target;   /* the target energy E_tgt of xdmrg = 1 (absolute, not a density) */
 } dmrg_fdmrg_config;
 
 typedef struct dmrg_fdmrg_result {
@@ -262,7 +276,8 @@ typedef struct dmrg_fdmrg_result {
 } dmrg_fdmrg_result;
 
 void        dmrg_fdmrg_default_config(dmrg_fdmrg_config *cfg);
-/* Replaces fdmrg::run_algorithm (fdmrg.cpp:170-195) for ising_tf_rf: sweeps with all tensors on the GPU. */
+/* Replaces fdmrg::run_algorithm (fdmrg.cpp:170-195) for ising_tf_rf: sweeps with all tensors on the GPU; with
+ * xdmrg = 1 the xdmrg::run_algorithm loop (xdmrg.cpp:297-341) on the folded spectrum around energy_target. */
 dmrg_status dmrg_fdmrg_ising_tf_rf(const dmrg_fdmrg_config *cfg, dmrg_fdmrg_result *res, void *stream);
 
 /* ---------------------------------------------------------------- serving: many independent solves on one GPU */
