@@ -5,8 +5,8 @@
 //
 // Each sweep runs the n(n-1)/2 pairs in n-1 rounds of disjoint pairs (circle method). When W and V fit in
 // LDS, one workgroup runs every sweep with both matrices resident and a barrier between rounds. Larger
-// problems launch one kernel per round with one wavefront per pair; the only host round trip per sweep then
-// reads whether any rotation was applied.
+// problems launch one kernel per round with one workgroup (1 to 4 wavefronts, by column length) per pair; the
+// only host round trip per sweep then reads whether any rotation was applied.
 #include "svd.hpp"
 
 #include <algorithm>
@@ -44,21 +44,23 @@ __device__ inline void round_pair(int np, int r, int i, int &a, int &b) {
     }
 }
 
-// One round: wavefront w of the grid handles pair w. flag[0] is raised when a rotation is applied.
-template <typename T>
-__global__ __launch_bounds__(64) void jacobi_round(int m, int n, int np, int r, T *W, T *V, double tol, unsigned *flag) {
+// One round: workgroup w of the grid (NT threads, a multiple of 64) handles pair w. flag[0] is raised when a
+// rotation is applied. Several wavefronts per pair shorten the dependent walk down long columns.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void jacobi_round(int m, int n, int np, int r, T *W, T *V, double tol, unsigned *flag) {
     int p, q;
     round_pair(np, r, blockIdx.x, p, q);
-    if(p >= n || q >= n) return; // padding player (odd n)
+    if(p >= n || q >= n) return; // padding player (odd n); uniform over the workgroup
     if(p > q) {
         const int t = p;
         p           = q;
         q           = t;
     }
-    T          *wp = W + static_cast<int64_t>(p) * m, *wq = W + static_cast<int64_t>(q) * m;
-    const int   lane = threadIdx.x;
-    double      al = 0, be = 0, gr = 0, gi = 0;
-    for(int i = lane; i < m; i += 64) {
+    T        *wp = W + static_cast<int64_t>(p) * m, *wq = W + static_cast<int64_t>(q) * m;
+    const int tid = threadIdx.x;
+    double    al = 0, be = 0, gr = 0, gi = 0;
+#pragma unroll 4
+    for(int i = tid; i < m; i += NT) {
         if constexpr(jtraits<T>::cx) {
             const T x = wp[i], y = wq[i];
             al += x.re * x.re + x.im * x.im;
@@ -76,17 +78,27 @@ __global__ __launch_bounds__(64) void jacobi_round(int m, int n, int np, int r, 
     be = wave_sum(be);
     gr = wave_sum(gr);
     gi = wave_sum(gi);
+    if constexpr(NT > 64) {
+        __shared__ double red[4][NT / 64];
+        const int         w = tid >> 6;
+        if((tid & 63) == 0) red[0][w] = al, red[1][w] = be, red[2][w] = gr, red[3][w] = gi;
+        __syncthreads();
+        al = be = gr = gi = 0;
+#pragma unroll
+        for(int k = 0; k < NT / 64; ++k) al += red[0][k], be += red[1][k], gr += red[2][k], gi += red[3][k];
+    }
     const double g = sqrt(gr * gr + gi * gi);
-    if(!(g > tol * sqrt(al * be)) || g == 0.0) return;
+    if(!(g > tol * sqrt(al * be)) || g == 0.0) return; // same decision in every thread of the workgroup
     // Align the phase of column q so that the off-diagonal is |g|, then a real rotation zeroes it:
     // t = sign(z) / (|z| + sqrt(1 + z^2)), z = (be - al) / (2 |g|); w_p' = c w_p - s w_q^, w_q' = s w_p + c w_q^.
     const double z  = (be - al) / (2.0 * g);
     const double t  = (z >= 0 ? 1.0 : -1.0) / (fabs(z) + sqrt(1.0 + z * z));
     const double c  = 1.0 / sqrt(1.0 + t * t), s = c * t;
     const double er = gr / g, ei = -gi / g; // e^{-i phi}
-    if(lane == 0) atomicOr(flag, 1u);
+    if(tid == 0) atomicOr(flag, 1u);
     auto rotate = [&](T *x, T *y, int len) {
-        for(int i = lane; i < len; i += 64) {
+#pragma unroll 4
+        for(int i = tid; i < len; i += NT) {
             if constexpr(jtraits<T>::cx) {
                 const T      xv = x[i], yv = y[i];
                 const double yr = yv.re * er - yv.im * ei, yi = yv.re * ei + yv.im * er; // e^{-i phi} y
@@ -312,8 +324,11 @@ void DeviceSvd::jacobi(int m, int n, const T *A, std::vector<double> &S) {
     }
     for(; sweep < 60 && nn > 1; ++sweep) {
         DMRG_HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(unsigned), s));
-        for(int r = 0; r < np - 1; ++r)
-            hipLaunchKernelGGL(jacobi_round<T>, dim3(np / 2), dim3(64), 0, s, mm, nn, np, r, W, V, tol, flag);
+        for(int r = 0; r < np - 1; ++r) {
+            if(mm > 256) hipLaunchKernelGGL((jacobi_round<T, 256>), dim3(np / 2), dim3(256), 0, s, mm, nn, np, r, W, V, tol, flag);
+            else if(mm > 128) hipLaunchKernelGGL((jacobi_round<T, 128>), dim3(np / 2), dim3(128), 0, s, mm, nn, np, r, W, V, tol, flag);
+            else hipLaunchKernelGGL((jacobi_round<T, 64>), dim3(np / 2), dim3(64), 0, s, mm, nn, np, r, W, V, tol, flag);
+        }
         DMRG_HIP_CHECK(hipGetLastError());
         DMRG_HIP_CHECK(hipMemcpyAsync(&hflag, flag, sizeof(unsigned), hipMemcpyDeviceToHost, s));
         DMRG_HIP_CHECK(hipStreamSynchronize(s));
