@@ -3,8 +3,8 @@
 Tolerances: explicit H_eff relative 2-norm error <= 1e-12 against the oracle's literal loops; dense
 eigenvalues <= 1e-12 * |A| against numpy (LAPACK) and the reference's LAPACKE fixtures; singular values
 <= 1e-12 * S[0] against numpy, orthonormality and reconstruction <= 1e-11, for rocSOLVER's gesvd and the
-one-sided Jacobi kernels (both the LDS-resident path for n <= 32 and the launch-per-round path); edge blocks
-exact.
+one-sided Jacobi kernels (the LDS-resident path for n <= 32 and the launch-per-round path with one to four
+wavefronts per pair); edge blocks exact.
 """
 import sys
 from pathlib import Path
@@ -148,7 +148,8 @@ def check_svd(A, res, lim, rank_max):
 
 @pytest.mark.parametrize("method", [0, 3], ids=["gesvd", "jacobi"])
 @pytest.mark.parametrize("cplx", [False, True])
-@pytest.mark.parametrize("m,n", [(1, 1), (1, 7), (9, 1), (2, 2), (30, 30), (31, 31), (50, 32), (33, 70), (64, 40), (37, 90), (300, 200)])
+@pytest.mark.parametrize("m,n", [(1, 1), (1, 7), (9, 1), (2, 2), (30, 30), (31, 31), (50, 32), (33, 70), (64, 40), (37, 90), (300, 200),
+                                 (600, 520), (513, 700)])
 def test_svd_truncate_random(m, n, cplx, method):
     print(f"[case] svd m={m} n={n} cplx={cplx} method={method}", file=sys.stderr, flush=True)
     rng = np.random.default_rng(m * 1000 + n)
