@@ -277,6 +277,74 @@ def section_serve(nreq=32, workers=8, rate=3.0, seed=4):
             "max_variance": max(r["variance"] for r in res)}
 
 
+def folded_problem(th, W, L, R, sigma):
+    """(H - sigma)^2 = H^2 - 2 sigma H + sigma^2 of the isolated workload as one operator: a block-diagonal MPO of
+    the MPO^2 (D^2 channels), -2 sigma W (D channels) and sigma^2 I (one channel), with env blocks stacked the
+    same way (squared blocks, the blocks, the identity). D_total = D^2 + D + 1."""
+    import numpy as np
+
+    W2, L2, R2 = squared_problem(th, W, L, R)
+    D, DD, chi_l, chi_r = W.shape[0], W2.shape[0], L.shape[0], R.shape[0]
+    Dt = DD + D + 1
+    Wt = np.zeros((Dt, Dt, W.shape[2], W.shape[3]), dtype=np.complex128, order="F")
+    Wt[:DD, :DD] = W2
+    Wt[DD:DD + D, DD:DD + D] = -2.0 * sigma * W
+    Wt[Dt - 1, Dt - 1] = sigma * sigma * np.eye(W.shape[2])
+
+    def stack(B, B2, chi):
+        out = np.zeros((chi, chi, Dt), dtype=np.complex128, order="F")
+        out[:, :, :DD], out[:, :, DD:DD + D], out[:, :, Dt - 1] = B2, B, np.eye(chi)
+        return out
+
+    return Wt, stack(L, L2, chi_l), stack(R, R2, chi_r)
+
+
+def section_eigs_h2(chi=512, tol=1e-10, sigma_rel=1e-3):
+    """Eigensolve on the squared operator: the eigenvalue nearest 0 of the shifted H^2, (H - sigma)^2 with
+    sigma = E0 - sigma_rel |H| below the ground state (E0 from an FP64 solve of H), FP64 vs mixed precision;
+    one cold start each. The folded solution is checked against H: its energy against E0 and its variance."""
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import random_heff_problem
+
+    th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=2)
+    h = G.MatVecMPOS([W], L, R, dtype=np.complex128)
+    w, _, info = h.eigs(nev=1, tol=1e-12, seed=7)
+    E0, anorm = float(w[0]), info["anorm"]
+    sigma = E0 - sigma_rel * anorm
+    Wt, Lt, Rt = folded_problem(th, W, L, R, sigma)
+    h2 = G.MatVecMPOS([Wt], Lt, Rt, dtype=np.complex128)
+    h2.eigs(nev=1, tol=1e-6, seed=5, max_matvecs=50)  # warm-up: GEMM plans of the D=31 operator
+    h2.eigs(nev=1, tol=1e-6, seed=5, max_matvecs=50, mixed=True)
+    n = h.rows()
+    out = {"workload": f"(H - sigma)^2 of the chi={chi} cx64 H_eff (seed 2): MPO^2 (25) + H (5) + identity (1) "
+                       f"channels, D_total={Wt.shape[0]}, sigma = E0 - {sigma_rel:g}|H|, tol {tol:g}*|A|, one cold start "
+                       f"per mode", "E0_of_H": E0, "sigma": sigma,
+           "gflop_per_matvec": round(8.0 * (2 * chi ** 3 * Wt.shape[0] * 2 + chi * chi * Wt.shape[0] ** 2 * 4) / 1e9, 2)}
+    dx, dy = G.DeviceArray((n,), np.complex128), G.DeviceArray((n,), np.complex128)
+    for mode in ("fp64", "mixed"):
+        t0 = time.perf_counter()
+        lam, v, i2 = h2.eigs(nev=1, tol=tol, seed=11, mixed=(mode == "mixed"), max_matvecs=8000)
+        dt = time.perf_counter() - t0
+        x = np.ascontiguousarray(v[:, 0] / np.linalg.norm(v[:, 0]))
+        G._lib.call("dmrg_memcpy_htod", dx.ptr, x.ctypes.data_as(__import__("ctypes").c_void_p), x.nbytes)
+        h.MultAx_device(dx, dy)
+        hx = dy.to_numpy()
+        E = float(np.vdot(x, hx).real)
+        out[mode] = {"seconds": round(dt, 3), "solves_per_s": round(1.0 / dt, 3), "converged": i2["nconv"] == 1,
+                     "eigenvalue": float(lam[0]), "matvecs_fp64": i2["matvecs_fp64"], "matvecs_fp32": i2["matvecs_fp32"],
+                     "rel_residual": i2["rnorm"][0] / i2["anorm"], "energy": E, "abs_dE_vs_E0": abs(E - E0),
+                     "variance_H": float(np.linalg.norm(hx - E * x) ** 2)}
+    dx.close()
+    dy.close()
+    h.close()
+    h2.close()
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
+    out["abs_dE_mixed_vs_fp64"] = abs(out["mixed"]["energy"] - out["fp64"]["energy"])
+    return out
+
+
 def section_expansion(chi=512, reps=5):
     """Subspace expansion at chi: the optimally mixed block (8 matvecs: 4 with H over D=5 envs, 4 with the
     variance operator over D^2=25 envs) and the L/R expansion terms (energy block with the MPO, variance block
@@ -544,6 +612,7 @@ def main():
     if not args.quick:
         out["h2_matvec"] = section_h2_matvec(args.chi)
         out["eigs"] = section_eigs(args.chi)
+        out["eigs_h2_folded"] = section_eigs_h2(args.chi)
         out["sweep"] = section_sweep(cpu=not args.no_cpu_baseline)
         out["serve"] = section_serve()
         out["expansion"] = section_expansion(args.chi)
