@@ -113,7 +113,7 @@ class FdmrgResult(C.Structure):
     _fields_ = [("energy", C.c_double), ("variance", C.c_double), ("entropies", C.c_double * (MAX_L + 1)),
                 ("chi", C.c_int64 * (MAX_L + 1)), ("sweeps", C.c_int), ("eigensolves", C.c_int), ("converged", C.c_int),
                 ("matvecs", C.c_int64), ("matvecs_fp32", C.c_int64), ("seconds", C.c_double), ("t_eigs", C.c_double),
-                ("t_env", C.c_double), ("t_svd", C.c_double), ("trunc_error_max", C.c_double)]
+                ("t_env", C.c_double), ("t_svd", C.c_double), ("trunc_error_max", C.c_double), ("gflop", C.c_double)]
 
 
 class Request(C.Structure):
@@ -124,7 +124,8 @@ class Request(C.Structure):
 class RequestResult(C.Structure):
     _fields_ = [("energy", C.c_double), ("variance", C.c_double), ("entropy_mid", C.c_double), ("arrival_s", C.c_double),
                 ("start_s", C.c_double), ("finish_s", C.c_double), ("latency_s", C.c_double), ("worker", C.c_int),
-                ("status", C.c_int), ("sweeps", C.c_int), ("matvecs", C.c_int64), ("footprint_bytes", C.c_int64)]
+                ("status", C.c_int), ("sweeps", C.c_int), ("matvecs", C.c_int64), ("footprint_bytes", C.c_int64),
+                ("gflop", C.c_double)]
 
 
 class ServeConfig(C.Structure):

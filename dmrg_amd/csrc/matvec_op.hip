@@ -132,6 +132,7 @@ void MatVecOp::multax_device(const void *dx, int64_t ldx, void *dy, int64_t ldy,
         matvec(dtype, D, dx, ldx, dy, ldy, nvec, mpo.p, envL_p, envR_p, w, s);
     }
     num_mv += nvec;
+    flops += static_cast<double>(nvec) * D.macs() * (dtype_is_complex(dtype) ? 8.0 : 2.0);
 }
 
 void MatVecOp::multax_host(const void *x, int64_t ldx, void *y, int64_t ldy, int nvec) {

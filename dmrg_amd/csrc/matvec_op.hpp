@@ -68,6 +68,7 @@ struct MatVecOp {
     std::complex<double>               sigma{0.0, 0.0};
     bool                               ready_shift = false;
     int64_t                            num_mv      = 0;
+    double                             flops       = 0.0; // algorithmic FLOP of every matvec applied (8 per complex MAC)
 
     MatVecOp(dmrg_dtype t, int nmpo, const void *const *mpos, const int64_t *mpo_dims, const void *envL_h,
              const int64_t envL_dims[3], const void *envR_h, const int64_t envR_dims[3]);

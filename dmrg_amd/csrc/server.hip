@@ -116,6 +116,7 @@ void serve(const dmrg_serve_config &cfg, const dmrg_request *req, int nreq, dmrg
                 r.entropy_mid = fr.entropies[c.L / 2];
                 r.sweeps      = fr.sweeps;
                 r.matvecs     = fr.matvecs + fr.matvecs_fp32;
+                r.gflop       = fr.gflop;
                 r.status      = fr.converged ? DMRG_OK : DMRG_ERR_NOT_CONVERGED;
             } catch(const Error &e) {
                 r.status = e.status;

@@ -93,6 +93,7 @@ struct Fdmrg {
     bool                                      folded = false; // xDMRG: solves on (H - E_tgt)^2
     dmrg_fdmrg_result                        *res;
     double t_eigs = 0, t_env = 0, t_svd = 0;
+    double env_flop = 0; // algorithmic FLOP of the env updates
 
     Fdmrg(const dmrg_fdmrg_config &c, hipStream_t st, dmrg_fdmrg_result *r) : cfg(c), s(st), L(c.L), svd(st), res(r) {}
 
@@ -160,6 +161,7 @@ struct Fdmrg {
             o.reserve(static_cast<size_t>(D.chiR * D.chiR * D.DR) * es());
             void *w = ws.reserve(static_cast<size_t>(env_ws_elems(D)) * es());
             env_build_block(DT, 'L', D, o.p, in.p, M[i].p, (sq ? W2dev : Wdev)[i].p, w, s);
+            env_flop += D.macs() * (std::is_same_v<T, double> ? 2.0 : 8.0);
         }
         t_env += std::chrono::duration<double>(clock::now() - t0).count();
     }
@@ -172,6 +174,7 @@ struct Fdmrg {
             o.reserve(static_cast<size_t>(D.chiL * D.chiL * D.DL) * es());
             void *w = ws.reserve(static_cast<size_t>(env_ws_elems(D)) * es());
             env_build_block(DT, 'R', D, o.p, in.p, M[i].p, (sq ? W2dev : Wdev)[i].p, w, s);
+            env_flop += D.macs() * (std::is_same_v<T, double> ? 2.0 : 8.0);
         }
         t_env += std::chrono::duration<double>(clock::now() - t0).count();
     }
@@ -504,6 +507,7 @@ struct Fdmrg {
             res->chi[b]       = chi[b];
             res->trunc_error_max = std::max(res->trunc_error_max, terr[b]);
         }
+        res->gflop   = ((hop ? hop->flops : 0.0) + (h2op ? h2op->flops : 0.0) + env_flop) * 1e-9;
         res->t_eigs  = t_eigs;
         res->t_env   = t_env;
         res->t_svd   = t_svd;

@@ -273,6 +273,7 @@ typedef struct dmrg_fdmrg_result {
     double  seconds;
     double  t_eigs, t_env, t_svd;
     double  trunc_error_max;                 /* max over bonds of |S[discarded]|_2 (svd.cpp:257-280) */
+    double  gflop;                           /* algorithmic matrix-core work: matvecs (FP64 and FP32) and env updates */
 } dmrg_fdmrg_result;
 
 void        dmrg_fdmrg_default_config(dmrg_fdmrg_config *cfg);
@@ -300,6 +301,7 @@ typedef struct dmrg_request_result {
     int     sweeps;
     int64_t matvecs;
     int64_t footprint_bytes;
+    double  gflop; /* algorithmic matrix-core work of the request (dmrg_fdmrg_result.gflop) */
 } dmrg_request_result;
 
 typedef struct dmrg_serve_config {
