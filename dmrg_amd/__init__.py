@@ -290,6 +290,37 @@ def serve(requests, workers: int = 4, hbm_budget_bytes: int = 64 << 30, mixed: b
     return res, stats
 
 
+class MatvecGroup:
+    """Matvecs of several operators of uneven shapes as one launch per contraction leg (dmrg_matvec_group_*),
+    optionally replayed from a HIP graph: y_i = H_i x_i for DeviceArrays xs[i], ys[i]. Keep the operators and
+    the vectors alive and unchanged while the group is used."""
+
+    def __init__(self, ops, xs, ys, fp32: bool = False, graph: bool = True):
+        k = len(ops)
+        assert k == len(xs) == len(ys) and k >= 1
+        self._keep = (list(ops), list(xs), list(ys))
+        arr_ops = (C.c_void_p * k)(*[o.h.value for o in ops])
+        arr_x = (C.c_void_p * k)(*[x.ptr.value for x in xs])
+        arr_y = (C.c_void_p * k)(*[y.ptr.value for y in ys])
+        self.h = C.c_void_p()
+        call("dmrg_matvec_group_create", arr_ops, k, arr_x, arr_y, int(fp32), int(graph), C.byref(self.h))
+        self.flop = load().dmrg_matvec_group_flop(self.h)
+
+    def run(self, stream=None) -> None:
+        call("dmrg_matvec_group_run", self.h, stream)
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            load().dmrg_matvec_group_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def optimally_mixed_block(h1: "MatVecMPOS", h2: "MatVecMPOS", theta, mode: int = 3, stream=None):
     """get_optimally_mixed_block_1site (env.cpp:96-239) with h1 the energy operator and h2 the operator on the
     variance envs. mode bit 1 = H1, bit 2 = H2. Returns (mixed theta, (alpha_mps, alpha_h1v, alpha_h2v),

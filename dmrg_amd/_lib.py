@@ -145,6 +145,11 @@ SIGNATURES["dmrg_serve"] = (st, [C.POINTER(ServeConfig), C.POINTER(Request), C.c
 SIGNATURES["dmrg_request_footprint"] = (i64, [C.c_int, C.c_int, C.c_int])
 SIGNATURES["dmrg_eigs_folded_spectrum"] = (st, [vp, vp, C.POINTER(EigsConfig), vp, C.c_double, vp, C.POINTER(C.c_double),
                                                 C.POINTER(C.c_double), C.POINTER(EigsResult), vp])
+SIGNATURES["dmrg_matvec_group_create"] = (st, [C.POINTER(vp), C.c_int, C.POINTER(vp), C.POINTER(vp), C.c_int, C.c_int,
+                                               C.POINTER(vp)])
+SIGNATURES["dmrg_matvec_group_run"] = (st, [vp, vp])
+SIGNATURES["dmrg_matvec_group_flop"] = (C.c_double, [vp])
+SIGNATURES["dmrg_matvec_group_destroy"] = (None, [vp])
 SIGNATURES["dmrg_fdmrg_default_config"] = (None, [C.POINTER(FdmrgConfig)])
 SIGNATURES["dmrg_fdmrg_ising_tf_rf"] = (st, [C.POINTER(FdmrgConfig), C.POINTER(FdmrgResult), vp])
 SIGNATURES["dmrg_eigs_default_config"] = (None, [C.POINTER(EigsConfig)])

@@ -6,6 +6,7 @@
 #include "dense_eig.hpp"
 #include "eigs.hpp"
 #include "expand.hpp"
+#include "group.hpp"
 #include "sweep.hpp"
 #include "dense.hpp"
 #include "svd.hpp"
@@ -524,6 +525,33 @@ dmrg_status dmrg_eigs_folded_spectrum(dmrg_matvec *h, dmrg_matvec *h2, const dmr
         }
     });
 }
+
+dmrg_status dmrg_matvec_group_create(dmrg_matvec *const *ops, int k, const void *const *xs, void *const *ys, int fp32,
+                                     int capture_graph, dmrg_matvec_group **out) {
+    return guard([&] {
+        DMRG_REQUIRE(ops && xs && ys && out && k >= 1, DMRG_ERR_ARG, "bad argument");
+        *out = nullptr;
+        std::vector<MatVecOp *> v(k);
+        for(int i = 0; i < k; ++i) {
+            DMRG_REQUIRE(ops[i], DMRG_ERR_ARG, "null operator");
+            v[i] = reinterpret_cast<MatVecOp *>(ops[i]);
+        }
+        *out = reinterpret_cast<dmrg_matvec_group *>(matvec_group_create(v.data(), k, xs, ys, fp32 != 0, capture_graph != 0));
+    });
+}
+
+dmrg_status dmrg_matvec_group_run(dmrg_matvec_group *group, void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(group, DMRG_ERR_ARG, "null group");
+        reinterpret_cast<MatvecGroup *>(group)->run(static_cast<hipStream_t>(stream));
+    });
+}
+
+double dmrg_matvec_group_flop(const dmrg_matvec_group *group) {
+    return group ? reinterpret_cast<const MatvecGroup *>(group)->flop : 0.0;
+}
+
+void dmrg_matvec_group_destroy(dmrg_matvec_group *group) { delete reinterpret_cast<MatvecGroup *>(group); }
 
 void dmrg_fdmrg_default_config(dmrg_fdmrg_config *cfg) {
     if(!cfg) return;

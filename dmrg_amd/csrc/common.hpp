@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/dmrg_amd.h"
 
@@ -106,6 +107,28 @@ constexpr int SMALLK_MAX_M = 64;
 
 // Launch one GEMM. cfg/ksplit = AUTO/0 pick from the tuning table or the size heuristic.
 void gemm(const GemmDesc &g, hipStream_t stream, TileCfg cfg = TileCfg::AUTO, int ksplit = 0);
+
+// Grouped GEMM: independent GEMMs (batch 1, no split-K) of one tile configuration run as one launch. The
+// descriptors and the tile prefix sums are device arrays filled by grouped_prepare (one group per
+// (swap, aligned) kind of GEMM); they stay valid as long as the GEMMs' buffers do, so a grouped launch can be
+// captured into a graph and replayed. Tile configurations: T64x64, T32x32, T16x128.
+struct GroupedGemm {
+    dmrg_dtype dtype   = DMRG_FP64;
+    TileCfg    cfg     = TileCfg::T64x64;
+    bool       swap    = false;
+    bool       aligned = false;
+    int        ngemm   = 0;
+    int        tiles   = 0;
+    void      *descs   = nullptr; // device, ngemm GemmDesc
+    int       *starts  = nullptr; // device, ngemm + 1 tile offsets
+};
+// Tiles of g under configuration cfg.
+int grouped_tiles(const GemmDesc &g, TileCfg cfg);
+// Builds the launch groups of gs; device storage for the descriptors is allocated with hipMalloc and owned by
+// the returned groups (release with grouped_free).
+std::vector<GroupedGemm> grouped_prepare(const std::vector<GemmDesc> &gs, TileCfg cfg);
+void                     grouped_launch(const GroupedGemm &g, hipStream_t s);
+void                     grouped_free(std::vector<GroupedGemm> &gg);
 int  device_cu_count();
 // Autotuner controls: on/off, a forced plan for every GEMM (cfg < 0 clears it), cache reset and size.
 void set_autotune(int on);

@@ -50,14 +50,10 @@ static GemmDesc make(dmrg_dtype t, int64_t M, int64_t N, int64_t K, int nvec) {
     return g;
 }
 
-void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *out, int64_t ldy, int nvec,
-            const void *mpo, const void *envL, const void *envR, void *ws, hipStream_t s, hipEvent_t *ev) {
+void matvec_descs(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *out, int64_t ldy, int nvec,
+                  const void *mpo, const void *envL, const void *envR, void *ws, GemmDesc leg[3]) {
     const int64_t d = D.d, cL = D.chiL, cR = D.chiR, DL = D.DL, DR = D.DR;
     const size_t  es  = dtype_size(t);
-    auto          mark = [&](int i) {
-        if(ev) DMRG_HIP_CHECK(hipEventRecord(ev[i], s));
-    };
-    mark(0);
     char         *w0  = static_cast<char *>(ws);
     const int64_t nT1 = (cL >= cR) ? d * cR * cL * DL : d * cL * cR * DR; // first intermediate per vector
     const int64_t nT2 = (cL >= cR) ? cR * DR * d * cL : cL * DL * cR * d; // second intermediate per vector
@@ -74,24 +70,20 @@ void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *o
         g1.B = envL, g1.bk = plain(1), g1.bn = plain(cL);
         g1.C = T1, g1.cm = plain(1), g1.cn = plain(d * cR), g1.c_batch = nT1;
         g1.part = PT, g1.part_capacity = PC; // leg 3's partial slabs are free until leg 3
-        gemm(g1, s);
-        mark(1);
+        leg[0] = g1;
         // Leg 2: T2[b,w',s',a'] = sum_{s,w} W[w,w',s,s'] T1[s,b,a',w]  m=(w',s') k=(s,w) n=(b,a')
         GemmDesc g2 = make(t, DR * d, cR * cL, d * DL, nvec);
         g2.A = mpo, g2.am = fused(DL, DL * DR * d, DR), g2.ak = fused(DL * DR, 1, d);
         g2.B = T1, g2.bk = fused(1, d * cR * cL, d), g2.bn = fused(d, d * cR, cR), g2.b_batch = nT1;
         g2.C = T2, g2.cm = fused(cR, cR * DR, DR), g2.cn = fused(1, cR * DR * d, cR), g2.c_batch = nT2;
-        gemm(g2, s);
-        mark(2);
+        leg[1] = g2;
         // Leg 3: out[s',a',b'] = sum_{b,w'} T2[b,w',s',a'] R[b,b',w']   m=(s',a') k=(b,w') n=b'
         GemmDesc g3 = make(t, d * cL, cR, cR * DR, nvec);
         g3.A = T2, g3.am = plain(cR * DR), g3.ak = plain(1), g3.a_batch = nT2;
         g3.B = envR, g3.bk = fused(1, cR * cR, cR), g3.bn = plain(cR);
         g3.C = out, g3.cm = plain(1), g3.cn = plain(d * cL), g3.c_batch = ldy;
         g3.part = PT, g3.part_capacity = PC;
-        if(ev) g3.ev_after_main = ev[3];
-        gemm(g3, s);
-        mark(4);
+        leg[2] = g3;
     } else {
         // Order of contraction_tblis.cpp:285-289 (theta.R first).
         // Leg 1: U[s,a,b',w'] = sum_b theta[s,a,b] R[b,b',w']            m=(s,a) k=b n=(b',w')
@@ -100,25 +92,38 @@ void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *o
         g1.B = envR, g1.bk = plain(1), g1.bn = plain(cR);
         g1.C = T1, g1.cm = plain(1), g1.cn = plain(d * cL), g1.c_batch = nT1;
         g1.part = PT, g1.part_capacity = PC;
-        gemm(g1, s);
-        mark(1);
+        leg[0] = g1;
         // Leg 2: V[a,w,b',s'] = sum_{s,w'} W[w,w',s,s'] U[s,a,b',w']    m=(w,s') k=(s,w') n=(a,b')
         GemmDesc g2 = make(t, DL * d, cL * cR, d * DR, nvec);
         g2.A = mpo, g2.am = fused(1, DL * DR * d, DL), g2.ak = fused(DL * DR, DL, d);
         g2.B = T1, g2.bk = fused(1, d * cL * cR, d), g2.bn = fused(d, d * cL, cL), g2.b_batch = nT1;
         g2.C = T2, g2.cm = fused(cL, cL * DL * cR, DL), g2.cn = fused(1, cL * DL, cL), g2.c_batch = nT2;
-        gemm(g2, s);
-        mark(2);
+        leg[1] = g2;
         // Leg 3: out[s',a',b'] = sum_{a,w} L[a,a',w] V[a,w,b',s']       m=a' k=(a,w) n=(b',s')
         GemmDesc g3 = make(t, cL, cR * d, cL * DL, nvec);
         g3.A = envL, g3.am = plain(cL), g3.ak = fused(1, cL * cL, cL);
         g3.B = T2, g3.bk = plain(1), g3.bn = plain(cL * DL), g3.b_batch = nT2;
         g3.C = out, g3.cm = plain(d), g3.cn = fused(d * cL, 1, cR), g3.c_batch = ldy;
         g3.part = PT, g3.part_capacity = PC;
-        if(ev) g3.ev_after_main = ev[3];
-        gemm(g3, s);
-        mark(4);
+        leg[2] = g3;
     }
+}
+
+void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *out, int64_t ldy, int nvec,
+            const void *mpo, const void *envL, const void *envR, void *ws, hipStream_t s, hipEvent_t *ev) {
+    GemmDesc leg[3];
+    matvec_descs(t, D, theta, ldx, out, ldy, nvec, mpo, envL, envR, ws, leg);
+    auto mark = [&](int i) {
+        if(ev) DMRG_HIP_CHECK(hipEventRecord(ev[i], s));
+    };
+    mark(0);
+    gemm(leg[0], s);
+    mark(1);
+    gemm(leg[1], s);
+    mark(2);
+    if(ev) leg[2].ev_after_main = ev[3];
+    gemm(leg[2], s);
+    mark(4);
 }
 
 int64_t env_ws_elems(const Dims &D) {

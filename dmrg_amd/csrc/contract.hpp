@@ -29,6 +29,9 @@ int64_t matvec_ws_elems(const Dims &D);
 // ws must hold nvec * matvec_ws_elems(D) elements of dtype t.
 // If ev is non-null, ev[0..4] are recorded on s before leg 1, after legs 1 and 2, after the main kernel of
 // leg 3 and after leg 3 (its split-K reduce, when there is one).
+// The three GEMMs of one matvec (legs 1-3 in launch order), as matvec launches them.
+void matvec_descs(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *out, int64_t ldy, int nvec,
+                  const void *mpo, const void *envL, const void *envR, void *ws, GemmDesc leg[3]);
 void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *out, int64_t ldy, int nvec,
             const void *mpo, const void *envL, const void *envR, void *ws, hipStream_t s, hipEvent_t *ev = nullptr);
 

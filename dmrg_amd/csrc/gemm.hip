@@ -224,8 +224,10 @@ struct TileLoader {
     }
 };
 
+// One output tile of g: bx is the tile's position in the launch (before the XCD remap), split the K slice
+// and b the batch index.
 template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN, bool SWAP, bool ALIGNED>
-__global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
+__device__ __forceinline__ void gemm_body(const GemmDesc &g, int bx, int split, int b) {
     using E               = elem_t<R, CPLX>;
     using MF              = mfma_t<R>;
     using acc_t           = typename MF::acc;
@@ -253,16 +255,14 @@ __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
     const int tilesM = (g.M + BM - 1) / BM;
     const int tilesN = (g.N + BN - 1) / BN;
     const int total  = tilesM * tilesN;
-    int       bid    = blockIdx.x;
+    int       bid    = bx;
     if(g.xcd_remap && (total % 8) == 0) bid = (bid % 8) * (total / 8) + bid / 8;
     const int mt = bid % tilesM;
     const int nt = bid / tilesM;
     const int m0 = mt * BM;
     const int n0 = nt * BN;
-    const int b  = blockIdx.z;
 
-    // K range of this workgroup (split-K along blockIdx.y).
-    const int split = blockIdx.y;
+    // K range of this workgroup (split-K slice).
     const int kbeg  = split * g.kchunk;
     const int kend  = min(g.K, kbeg + g.kchunk);
 
@@ -387,6 +387,26 @@ __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
                     }
                 }
             }
+}
+
+template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN, bool SWAP, bool ALIGNED>
+__global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
+    gemm_body<R, CPLX, BM, BN, BK, WM, WN, SWAP, ALIGNED>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Grouped GEMM: one launch for several independent GEMMs of one tile configuration. Workgroup x belongs to
+// the GEMM whose tile range [tile_start[i], tile_start[i + 1]) holds x (no split-K, batch 1 each); the
+// descriptors and the prefix sums live in device memory, so a launch can be captured once and replayed.
+template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN, bool SWAP, bool ALIGNED>
+__global__ __launch_bounds__(64 * WM *WN) void gemm_grouped_kernel(const GemmDesc *descs, const int *tile_start, int ngemm) {
+    const int x  = blockIdx.x;
+    int       lo = 0, hi = ngemm - 1; // last i with tile_start[i] <= x
+    while(lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if(tile_start[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    gemm_body<R, CPLX, BM, BN, BK, WM, WN, SWAP, ALIGNED>(descs[lo], x - tile_start[lo], 0, 0);
 }
 
 // GEMM with tiny K and M (K <= SMALLK_MAX_K, M <= SMALLK_MAX_M): A sits in LDS, every thread streams one column
@@ -672,6 +692,124 @@ static void launch_plan_single(GemmDesc g, const LaunchPlan &p0, hipStream_t str
         case DMRG_CX32: launch_typed<float, true>(g, p.cfg, swap, stream); break;
     }
     DMRG_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------
+// Grouped launches
+
+static int group_bk(dmrg_dtype t, TileCfg cfg) {
+    const int e = static_cast<int>(dtype_size(t));
+    const int bk = e == 4 ? 32 : 16;
+    return (cfg == TileCfg::T16x128 || cfg == TileCfg::T128x64) && e == 16 ? 8 : bk;
+}
+
+int grouped_tiles(const GemmDesc &g, TileCfg cfg) {
+    const CfgShape sh = cfg_shape(cfg);
+    return ((g.M + sh.BM - 1) / sh.BM) * ((g.N + sh.BN - 1) / sh.BN);
+}
+
+std::vector<GroupedGemm> grouped_prepare(const std::vector<GemmDesc> &gs, TileCfg cfg) {
+    DMRG_REQUIRE(cfg == TileCfg::T64x64 || cfg == TileCfg::T32x32 || cfg == TileCfg::T16x128, DMRG_ERR_ARG,
+                 "grouped GEMM: tile configuration not available");
+    const CfgShape sh = cfg_shape(cfg);
+    const int      bk = group_bk(gs.empty() ? DMRG_FP64 : gs[0].dtype, cfg);
+    // (swap, aligned) -> member GEMMs
+    std::vector<GemmDesc> kinds[4];
+    for(GemmDesc g : gs) {
+        if(g.M <= 0 || g.N <= 0) continue;
+        DMRG_REQUIRE(g.batch == 1 && !g.accumulate && g.K > 0, DMRG_ERR_ARG, "grouped GEMM: batch 1, K > 0, no accumulation");
+        DMRG_REQUIRE(g.dtype == gs[0].dtype, DMRG_ERR_ARG, "grouped GEMM: one dtype per group");
+        DMRG_REQUIRE(static_cast<int64_t>(g.M) * g.N < (int64_t(1) << 31), DMRG_ERR_DIMS, "gemm: output too large");
+        g.a_fast_m      = fast_score(g.am) >= fast_score(g.ak) ? 1 : 0;
+        g.b_fast_k      = fast_score(g.bk) >= fast_score(g.bn) ? 1 : 0;
+        g.ksplit        = 1;
+        g.kchunk        = (g.K + 31) / 32 * 32;
+        g.xcd_remap     = 0;
+        g.part          = nullptr;
+        g.part_capacity = 0;
+        g.ev_after_main = nullptr;
+        const bool swap = (g.cm.s0 == 1 && g.cm.c >= 16) && !(g.cn.s0 == 1 && g.cn.c >= 16);
+        const int  es   = static_cast<int>(dtype_size(g.dtype));
+        const bool aligned = separable(g.am, sh.BM) && separable(g.ak, bk) && separable(g.bn, sh.BN) && separable(g.bk, bk) &&
+                             (g.kchunk % bk) == 0 && small_tile_offsets(g.am, sh.BM, g.ak, bk, es) &&
+                             small_tile_offsets(g.bn, sh.BN, g.bk, bk, es);
+        kinds[(swap ? 2 : 0) + (aligned ? 1 : 0)].push_back(g);
+    }
+    std::vector<GroupedGemm> out;
+    for(int k = 0; k < 4; ++k) {
+        if(kinds[k].empty()) continue;
+        GroupedGemm gg;
+        gg.dtype   = kinds[k][0].dtype;
+        gg.cfg     = cfg;
+        gg.swap    = (k & 2) != 0;
+        gg.aligned = (k & 1) != 0;
+        gg.ngemm   = static_cast<int>(kinds[k].size());
+        std::vector<int> st(gg.ngemm + 1, 0);
+        for(int i = 0; i < gg.ngemm; ++i) {
+            const int64_t next = static_cast<int64_t>(st[i]) + grouped_tiles(kinds[k][i], cfg);
+            DMRG_REQUIRE(next < (int64_t(1) << 31), DMRG_ERR_DIMS, "grouped GEMM: too many tiles");
+            st[i + 1] = static_cast<int>(next);
+        }
+        gg.tiles = st[gg.ngemm];
+        if(hipMalloc(&gg.descs, sizeof(GemmDesc) * gg.ngemm) != hipSuccess ||
+           hipMalloc(reinterpret_cast<void **>(&gg.starts), sizeof(int) * (gg.ngemm + 1)) != hipSuccess) {
+            (void) hipGetLastError();
+            out.push_back(gg);
+            grouped_free(out);
+            throw Error(DMRG_ERR_OOM, "grouped GEMM: descriptor allocation failed");
+        }
+        DMRG_HIP_CHECK(hipMemcpy(gg.descs, kinds[k].data(), sizeof(GemmDesc) * gg.ngemm, hipMemcpyHostToDevice));
+        DMRG_HIP_CHECK(hipMemcpy(gg.starts, st.data(), sizeof(int) * (gg.ngemm + 1), hipMemcpyHostToDevice));
+        out.push_back(gg);
+    }
+    return out;
+}
+
+template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN>
+static void grouped_launch_cfg(const GroupedGemm &g, hipStream_t s) {
+    const dim3 grid(g.tiles), block(64 * WM * WN);
+    const auto *d = static_cast<const GemmDesc *>(g.descs);
+    if(g.aligned) {
+        if(g.swap) hipLaunchKernelGGL((gemm_grouped_kernel<R, CPLX, BM, BN, BK, WM, WN, true, true>), grid, block, 0, s, d, g.starts, g.ngemm);
+        else hipLaunchKernelGGL((gemm_grouped_kernel<R, CPLX, BM, BN, BK, WM, WN, false, true>), grid, block, 0, s, d, g.starts, g.ngemm);
+    } else {
+        if(g.swap) hipLaunchKernelGGL((gemm_grouped_kernel<R, CPLX, BM, BN, BK, WM, WN, true, false>), grid, block, 0, s, d, g.starts, g.ngemm);
+        else hipLaunchKernelGGL((gemm_grouped_kernel<R, CPLX, BM, BN, BK, WM, WN, false, false>), grid, block, 0, s, d, g.starts, g.ngemm);
+    }
+}
+
+template <typename R, bool CPLX>
+static void grouped_launch_typed(const GroupedGemm &g, hipStream_t s) {
+    constexpr int E  = static_cast<int>(sizeof(elem_t<R, CPLX>));
+    constexpr int BK = E == 4 ? 32 : 16;
+    constexpr int BH = E == 16 ? 8 : BK;
+    switch(g.cfg) {
+        case TileCfg::T64x64: grouped_launch_cfg<R, CPLX, 64, 64, BK, 2, 2>(g, s); break;
+        case TileCfg::T32x32: grouped_launch_cfg<R, CPLX, 32, 32, BK, 2, 2>(g, s); break;
+        case TileCfg::T16x128: grouped_launch_cfg<R, CPLX, 16, 128, BH, 1, 4>(g, s); break;
+        default: throw Error(DMRG_ERR_ARG, "grouped GEMM: tile configuration not available");
+    }
+}
+
+void grouped_launch(const GroupedGemm &g, hipStream_t s) {
+    if(g.tiles <= 0) return;
+    switch(g.dtype) {
+        case DMRG_FP64: grouped_launch_typed<double, false>(g, s); break;
+        case DMRG_CX64: grouped_launch_typed<double, true>(g, s); break;
+        case DMRG_FP32: grouped_launch_typed<float, false>(g, s); break;
+        case DMRG_CX32: grouped_launch_typed<float, true>(g, s); break;
+    }
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+void grouped_free(std::vector<GroupedGemm> &gg) {
+    for(auto &g : gg) {
+        if(g.descs) (void) hipFree(g.descs);
+        if(g.starts) (void) hipFree(g.starts);
+        g.descs  = nullptr;
+        g.starts = nullptr;
+    }
+    gg.clear();
 }
 
 // Workgroups of configuration t that one CU holds at once (aligned, swapped variant as the representative).

@@ -1,0 +1,86 @@
+// Grouped H_eff matvecs (see group.hpp). The three legs of every operator's matvec are the GEMMs matvec()
+// launches one by one; here leg j of all k operators goes into one grouped launch, so k small problems that
+// each fill a few CUs share the whole GPU in 3 launches (plus at most 3 more when the GEMMs differ in kind).
+#include "group.hpp"
+
+#include <algorithm>
+#include <memory>
+
+namespace dmrg {
+
+MatvecGroup::~MatvecGroup() {
+    if(exec) (void) hipGraphExecDestroy(exec);
+    if(graph) (void) hipGraphDestroy(graph);
+    for(auto &l : legs) grouped_free(l);
+}
+
+void MatvecGroup::run(hipStream_t s) {
+    if(exec) {
+        DMRG_HIP_CHECK(hipGraphLaunch(exec, s));
+        return;
+    }
+    for(auto &l : legs)
+        for(auto &g : l) grouped_launch(g, s);
+}
+
+MatvecGroup *matvec_group_create(MatVecOp *const *ops, int k, const void *const *xs, void *const *ys, bool fp32, bool capture) {
+    DMRG_REQUIRE(k >= 1 && ops && xs && ys, DMRG_ERR_ARG, "matvec group: bad arguments");
+    auto                  grp = std::make_unique<MatvecGroup>();
+    std::vector<GemmDesc> leg[3];
+    grp->ws.resize(k);
+    grp->k = k;
+    const dmrg_dtype t0 = ops[0]->dtype;
+    for(int i = 0; i < k; ++i) {
+        MatVecOp &op = *ops[i];
+        DMRG_REQUIRE(op.dtype == t0, DMRG_ERR_ARG, "matvec group: operators of one dtype");
+        DMRG_REQUIRE(xs[i] && ys[i], DMRG_ERR_ARG, "matvec group: null vector");
+        if(fp32) {
+            op.ensure_fp32(nullptr);
+            DMRG_HIP_CHECK(hipStreamSynchronize(nullptr));
+        }
+        const dmrg_dtype t = fp32 ? dtype_single_of(op.dtype) : op.dtype;
+        grp->ws[i].reserve(static_cast<size_t>(matvec_ws_elems(op.D)) * dtype_size(t));
+        GemmDesc g[3];
+        matvec_descs(t, op.D, xs[i], op.D.n(), ys[i], op.D.n(), 1, fp32 ? op.mpo32.p : op.mpo.p, fp32 ? op.envL32.p : op.envL_p,
+                     fp32 ? op.envR32.p : op.envR_p, grp->ws[i].p, g);
+        for(int j = 0; j < 3; ++j) leg[j].push_back(g[j]);
+        grp->flop += op.D.macs() * (dtype_is_complex(t) ? 8.0 : 2.0);
+    }
+    // Tiles: the MPO leg (M = d D, K = d D) in 16 x 128 tiles while M is small; the big legs in 64 x 64 tiles
+    // when that gives about two waves of workgroups on the GPU, else in 32 x 32 tiles.
+    for(int j = 0; j < 3; ++j) {
+        TileCfg cfg;
+        if(j == 1) {
+            int mmax = 0;
+            for(const auto &g : leg[j]) mmax = std::max(mmax, g.M);
+            cfg = mmax <= 32 ? TileCfg::T16x128 : TileCfg::T32x32;
+        } else {
+            int64_t tiles = 0;
+            for(const auto &g : leg[j]) tiles += grouped_tiles(g, TileCfg::T64x64);
+            cfg = tiles >= 2 * device_cu_count() ? TileCfg::T64x64 : TileCfg::T32x32;
+        }
+        grp->legs[j] = grouped_prepare(leg[j], cfg);
+    }
+    if(capture) {
+        // Captured on a stream of its own at default priority; the graph holds only kernel nodes.
+        hipStream_t cs = nullptr;
+        DMRG_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        try {
+            DMRG_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            for(auto &l : grp->legs)
+                for(auto &g : l) grouped_launch(g, cs);
+            DMRG_HIP_CHECK(hipStreamEndCapture(cs, &grp->graph));
+            DMRG_HIP_CHECK(hipGraphInstantiate(&grp->exec, grp->graph, nullptr, nullptr, 0));
+        } catch(...) {
+            hipGraph_t partial = nullptr;
+            (void) hipStreamEndCapture(cs, &partial); // leave capture mode before the stream goes
+            if(partial) (void) hipGraphDestroy(partial);
+            (void) hipStreamDestroy(cs);
+            throw;
+        }
+        DMRG_HIP_CHECK(hipStreamDestroy(cs));
+    }
+    return grp.release();
+}
+
+} // namespace dmrg

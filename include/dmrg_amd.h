@@ -232,6 +232,23 @@ dmrg_status dmrg_eigs_folded_spectrum(dmrg_matvec *h, dmrg_matvec *h2, const dmr
                                       double energy_shift, void *evecs, double *energy, double *variance,
                                       dmrg_eigs_result *res, void *stream);
 
+/* ---------------------------------------------------------------- grouped matvecs for serving */
+
+/* Matvecs of k operators of uneven shapes (one dtype) as one launch per contraction leg: y_i = H_i x_i.
+ * No counterpart in the reference (its MultAx serves one operator at a time); for request loops that keep many
+ * small operators resident, where one operator's matvec fills only a few CUs. fp32 = 1 runs the FP32 matrix-core
+ * form (xs/ys in the single-precision dtype). capture_graph = 1 records the launches once into a HIP graph that
+ * dmrg_matvec_group_run replays. The group holds pointers into the operators' buffers and into xs/ys: keep
+ * those (and the operators' shift/MPO state) unchanged while the group is used; destroy it first. */
+typedef struct dmrg_matvec_group dmrg_matvec_group;
+dmrg_status dmrg_matvec_group_create(dmrg_matvec *const *ops, int k, const void *const *xs, void *const *ys, int fp32,
+                                     int capture_graph, dmrg_matvec_group **out);
+/* Enqueue one grouped matvec of every operator on stream (stream-ordered; no host synchronisation). */
+dmrg_status dmrg_matvec_group_run(dmrg_matvec_group *group, void *stream);
+/* Algorithmic FLOP of one run (8 per complex MAC, 2 per real MAC). */
+double      dmrg_matvec_group_flop(const dmrg_matvec_group *group);
+void        dmrg_matvec_group_destroy(dmrg_matvec_group *group);
+
 /* ---------------------------------------------------------------- finite DMRG sweep (ising_tf_rf) */
 
 #define DMRG_FDMRG_MAX_L 256
