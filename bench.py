@@ -290,6 +290,56 @@ def section_serve(nreq=32, workers=16, load=0.7, seed=4):
             "max_variance": max(r["variance"] for r in res)}
 
 
+def section_grouped(nops=32, reps=200, seed=6):
+    """Serving-sized operators (chi in {32, 48, 64, 96, 128}, d=2, D=5, complex): their matvecs one after
+    another (3-4 launches each), as one grouped launch per leg, and as a replayed HIP graph of those launches."""
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import random_heff_problem
+
+    rng = np.random.default_rng(seed)
+    chis = [int(rng.choice([32, 48, 64, 96, 128])) for _ in range(nops)]
+    ops, xs, ys = [], [], []
+    for i, chi in enumerate(chis):
+        th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=100 + i)
+        ops.append(G.MatVecMPOS([W], L, R, dtype=np.complex128))
+        xs.append(G.DeviceArray.from_numpy(th.reshape(-1, order="F")))
+        ys.append(G.DeviceArray((th.size,), np.complex128))
+    out = {"workload": f"{nops} operators, chi in {{32,48,64,96,128}} (drawn, seed {seed}), d=2, D=5, cx64, {reps} runs"}
+
+    def timed(fn):
+        for _ in range(5):
+            fn()
+        G._lib.call("dmrg_synchronize")
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        G._lib.call("dmrg_synchronize")
+        return (time.perf_counter() - t0) / reps
+
+    def sequential():
+        for op, x, y in zip(ops, xs, ys):
+            op.MultAx_device(x, y)
+
+    t_seq = timed(sequential)
+    flop = None
+    for name, graph in (("grouped", False), ("grouped_graph", True)):
+        grp = G.MatvecGroup(ops, xs, ys, graph=graph)
+        flop = grp.flop
+        t = timed(grp.run)
+        out[name] = {"ms_per_batch": round(t * 1e3, 4), "matvecs_per_s": round(nops / t, 1), "tflops": round(flop / t / 1e12, 3)}
+        grp.close()
+    out["sequential"] = {"ms_per_batch": round(t_seq * 1e3, 4), "matvecs_per_s": round(nops / t_seq, 1),
+                         "tflops": round(flop / t_seq / 1e12, 3)}
+    out["speedup_graph_vs_sequential"] = round(out["sequential"]["ms_per_batch"] / out["grouped_graph"]["ms_per_batch"], 2)
+    for a in xs + ys:
+        a.close()
+    for op in ops:
+        op.close()
+    return out
+
+
 def folded_problem(th, W, L, R, sigma):
     """(H - sigma)^2 = H^2 - 2 sigma H + sigma^2 of the isolated workload as one operator: a block-diagonal MPO of
     the MPO^2 (D^2 channels), -2 sigma W (D channels) and sigma^2 I (one channel), with env blocks stacked the
@@ -628,6 +678,7 @@ def main():
         out["eigs_h2_folded"] = section_eigs_h2(args.chi)
         out["sweep"] = section_sweep(cpu=not args.no_cpu_baseline)
         out["serve"] = section_serve()
+        out["grouped_matvecs"] = section_grouped()
         out["expansion"] = section_expansion(args.chi)
         out["two_site_chi1024"] = section_two_site()
     if not args.no_cpu_baseline:
