@@ -290,6 +290,37 @@ def serve(requests, workers: int = 4, hbm_budget_bytes: int = 64 << 30, mixed: b
     return res, stats
 
 
+def eigs_batched(ops, tol: float = 1e-12, target: str = "SA", x0s=None, ncv: int = 0, max_matvecs: int = 100000,
+                 seed: int = 12345, stream=None):
+    """Lowest (SA) or highest (LA) eigenpair of each operator, solved together in lockstep (dmrg_eigs_batched).
+    Returns a list of (eigenvalue, eigenvector numpy, result dict)."""
+    k = len(ops)
+    cfg = _lib.EigsConfig()
+    load().dmrg_eigs_default_config(C.byref(cfg))
+    cfg.nev, cfg.ncv, cfg.target = 1, ncv, {"SA": 0, "LA": 1}[target]
+    cfg.tol, cfg.max_matvecs, cfg.seed = tol, max_matvecs, seed
+    outs = [DeviceArray((o.rows(),), o.dtype) for o in ops]
+    ins = [None] * k
+    if x0s is not None:
+        ins = [DeviceArray.from_numpy(np.asarray(x, dtype=o.dtype).reshape(-1, order="F")) if x is not None else None
+               for x, o in zip(x0s, ops)]
+    arr_ops = (C.c_void_p * k)(*[o.h.value for o in ops])
+    arr_x = (C.c_void_p * k)(*[(x.ptr.value if x is not None else None) for x in ins])
+    arr_y = (C.c_void_p * k)(*[y.ptr.value for y in outs])
+    res = (_lib.EigsResult * k)()
+    try:
+        call("dmrg_eigs_batched", arr_ops, k, C.byref(cfg), arr_x, arr_y, res, stream)
+        out = []
+        for i in range(k):
+            info = {f: getattr(res[i], f) for f, _ in res[i]._fields_ if f not in ("eigval", "rnorm")}
+            info["rnorm"] = res[i].rnorm[0]
+            out.append((res[i].eigval[0], outs[i].to_numpy(), info))
+        return out
+    finally:
+        for a in outs + [x for x in ins if x is not None]:
+            a.close()
+
+
 class MatvecGroup:
     """Matvecs of several operators of uneven shapes as one launch per contraction leg (dmrg_matvec_group_*),
     optionally replayed from a HIP graph: y_i = H_i x_i for DeviceArrays xs[i], ys[i]. Keep the operators and

@@ -145,6 +145,8 @@ SIGNATURES["dmrg_serve"] = (st, [C.POINTER(ServeConfig), C.POINTER(Request), C.c
 SIGNATURES["dmrg_request_footprint"] = (i64, [C.c_int, C.c_int, C.c_int])
 SIGNATURES["dmrg_eigs_folded_spectrum"] = (st, [vp, vp, C.POINTER(EigsConfig), vp, C.c_double, vp, C.POINTER(C.c_double),
                                                 C.POINTER(C.c_double), C.POINTER(EigsResult), vp])
+SIGNATURES["dmrg_eigs_batched"] = (st, [C.POINTER(vp), C.c_int, C.POINTER(EigsConfig), C.POINTER(vp), C.POINTER(vp),
+                                         C.POINTER(EigsResult), vp])
 SIGNATURES["dmrg_matvec_group_create"] = (st, [C.POINTER(vp), C.c_int, C.POINTER(vp), C.POINTER(vp), C.c_int, C.c_int,
                                                C.POINTER(vp)])
 SIGNATURES["dmrg_matvec_group_run"] = (st, [vp, vp])

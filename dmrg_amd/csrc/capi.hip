@@ -7,6 +7,9 @@
 #include "eigs.hpp"
 #include "expand.hpp"
 #include "group.hpp"
+#include "batch_eigs.hpp"
+
+#include <chrono>
 #include "sweep.hpp"
 #include "dense.hpp"
 #include "svd.hpp"
@@ -522,6 +525,42 @@ dmrg_status dmrg_eigs_folded_spectrum(dmrg_matvec *h, dmrg_matvec *h2, const dmr
             energy[q]       = r[0] / r[4];
             const double dE = energy[q] - energy_shift;
             variance[q]     = r[2] / r[4] - dE * dE; // <(H - E_shift)^2> - (E - E_shift)^2 (xdmrg.cpp:488-496)
+        }
+    });
+}
+
+dmrg_status dmrg_eigs_batched(dmrg_matvec *const *ops, int k, const dmrg_eigs_config *cfg, const void *const *x0s,
+                              void *const *evecs, dmrg_eigs_result *results, void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(ops && cfg && evecs && results && k >= 1, DMRG_ERR_ARG, "null argument");
+        DMRG_REQUIRE(cfg->nev == 1 && cfg->precond == 0 && cfg->mixed == 0 && (cfg->target == DMRG_TARGET_SA || cfg->target == DMRG_TARGET_LA),
+                     DMRG_ERR_UNSUPPORTED, "batched eigs: nev 1, no preconditioner, FP64, target SA or LA");
+        DMRG_REQUIRE(cfg->tol > 0, DMRG_ERR_ARG, "tol must be positive");
+        EigsParams p;
+        p.nev         = 1;
+        p.ncv         = cfg->ncv;
+        p.target      = cfg->target;
+        p.tol         = std::max(cfg->tol, std::numeric_limits<double>::epsilon());
+        p.max_matvecs = cfg->max_matvecs > 0 ? cfg->max_matvecs : 100000;
+        p.seed        = cfg->seed;
+        std::vector<MatVecOp *> v(k);
+        for(int i = 0; i < k; ++i) {
+            DMRG_REQUIRE(ops[i] && evecs[i], DMRG_ERR_ARG, "null operator or output");
+            v[i] = reinterpret_cast<MatVecOp *>(ops[i]);
+        }
+        const auto                   t0 = std::chrono::steady_clock::now();
+        std::vector<BatchEigsResult> r;
+        eigs_batched(v.data(), k, p, x0s, evecs, r, static_cast<hipStream_t>(stream));
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for(int i = 0; i < k; ++i) {
+            results[i]              = dmrg_eigs_result{};
+            results[i].eigval[0]    = r[i].eigval;
+            results[i].rnorm[0]     = r[i].rnorm;
+            results[i].nconv        = r[i].nconv;
+            results[i].iters        = r[i].iters;
+            results[i].matvecs_fp64 = r[i].matvecs;
+            results[i].anorm        = r[i].anorm;
+            results[i].seconds      = sec;
         }
     });
 }
