@@ -340,6 +340,54 @@ def section_grouped(nops=32, reps=200, seed=6):
     return out
 
 
+def section_batched_eigs(nops=32, tol=1e-10, seed=6):
+    """Ground states of serving-sized operators (chi in {32,...,128}, d=2, D=5, complex): one dmrg_eigs after
+    another against one dmrg_eigs_batched call over all of them (lockstep Davidson, grouped matvecs)."""
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import random_heff_problem
+
+    rng = np.random.default_rng(seed)
+    chis = [int(rng.choice([32, 48, 64, 96, 128])) for _ in range(nops)]
+    ops = []
+    for i, chi in enumerate(chis):
+        th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=200 + i)
+        ops.append(G.MatVecMPOS([W], L, R, dtype=np.complex128))
+    # warm-up (GEMM plans, code loads)
+    for op in ops[:5]:
+        op.eigs(nev=1, tol=tol, seed=3)
+    G.eigs_batched(ops[:4], tol=tol, seed=3)
+    lat, e_seq, mv_seq = [], [], 0
+    t0 = time.perf_counter()
+    for op in ops:
+        t1 = time.perf_counter()
+        w, _, info = op.eigs(nev=1, tol=tol, seed=11)
+        lat.append(time.perf_counter() - t1)
+        e_seq.append(w[0])
+        mv_seq += info["matvecs_fp64"]
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    out = G.eigs_batched(ops, tol=tol, seed=11)
+    t_bat = time.perf_counter() - t0
+    e_bat = [o[0] for o in out]
+    anorm = max(o[2]["anorm"] for o in out)
+    for op in ops:
+        op.close()
+    return {"workload": f"{nops} ground states, chi in {{32,48,64,96,128}} (drawn, seed {seed}), d=2, D=5, cx64, "
+                        f"tol {tol:g}*|H|",
+            "sequential": {"seconds": round(t_seq, 3), "solves_per_s": round(nops / t_seq, 2),
+                           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2), "matvecs": int(mv_seq)},
+            "batched": {"seconds": round(t_bat, 3), "solves_per_s": round(nops / t_bat, 2),
+                        "all_converged": all(o[2]["nconv"] == 1 for o in out),
+                        "matvecs": int(sum(o[2]["matvecs_fp64"] for o in out)),
+                        "steps": int(max(o[2]["iters"] for o in out))},
+            "speedup_batched_vs_sequential": round(t_seq / t_bat, 2),
+            "max_abs_dE_batched_vs_sequential": float(np.max(np.abs(np.array(e_bat) - np.array(e_seq)))),
+            "anorm_max": anorm}
+
+
 def folded_problem(th, W, L, R, sigma):
     """(H - sigma)^2 = H^2 - 2 sigma H + sigma^2 of the isolated workload as one operator: a block-diagonal MPO of
     the MPO^2 (D^2 channels), -2 sigma W (D channels) and sigma^2 I (one channel), with env blocks stacked the
@@ -679,6 +727,7 @@ def main():
         out["sweep"] = section_sweep(cpu=not args.no_cpu_baseline)
         out["serve"] = section_serve()
         out["grouped_matvecs"] = section_grouped()
+        out["batched_eigs"] = section_batched_eigs()
         out["expansion"] = section_expansion(args.chi)
         out["two_site_chi1024"] = section_two_site()
     if not args.no_cpu_baseline:
