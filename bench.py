@@ -372,10 +372,36 @@ def section_batched_eigs(nops=32, tol=1e-10, seed=6):
     t_bat = time.perf_counter() - t0
     e_bat = [o[0] for o in out]
     anorm = max(o[2]["anorm"] for o in out)
+    # Continuous batching: Poisson arrivals at 0.7x the batched rate, served by dmrg_serve_eigs (requests join and
+    # leave the running batch) and, on the same arrival times, by one dmrg_eigs after another in arrival order.
+    rate = 0.7 * nops / t_bat
+    arrivals = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, nops))
+    arrivals -= arrivals[0]
+    res_c, st_c = G.serve_eigs(ops, arrivals=arrivals, tol=tol, seed=11)
+    lat_s, t0 = [], time.perf_counter()
+    for op, a in zip(ops, arrivals):
+        wait = a - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        op.eigs(nev=1, tol=tol, seed=11)
+        lat_s.append(time.perf_counter() - t0 - a)
+    t_fifo = time.perf_counter() - t0
+    e_con = [r[0] for r in res_c]
     for op in ops:
         op.close()
+    cont = {"arrival_rate_per_s": round(rate, 1),
+            "continuous_batching": {"solves_per_s": round(st_c["solves_per_s"], 2),
+                                    "p50_ms": round(st_c["p50_latency_s"] * 1e3, 2),
+                                    "p99_ms": round(st_c["p99_latency_s"] * 1e3, 2),
+                                    "peak_concurrency": st_c["peak_concurrency"],
+                                    "all_converged": st_c["completed"] == nops},
+            "one_at_a_time": {"solves_per_s": round(nops / t_fifo, 2),
+                              "p50_ms": round(float(np.percentile(lat_s, 50)) * 1e3, 2),
+                              "p99_ms": round(float(np.percentile(lat_s, 99)) * 1e3, 2)},
+            "max_abs_dE_vs_sequential": float(np.max(np.abs(np.array(e_con) - np.array(e_seq))))}
     return {"workload": f"{nops} ground states, chi in {{32,48,64,96,128}} (drawn, seed {seed}), d=2, D=5, cx64, "
                         f"tol {tol:g}*|H|",
+            "poisson_arrivals": cont,
             "sequential": {"seconds": round(t_seq, 3), "solves_per_s": round(nops / t_seq, 2),
                            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
                            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2), "matvecs": int(mv_seq)},

@@ -321,6 +321,45 @@ def eigs_batched(ops, tol: float = 1e-12, target: str = "SA", x0s=None, ncv: int
             a.close()
 
 
+def serve_eigs(ops, arrivals=None, hbm_budget_bytes: int = 64 << 30, max_batch: int = 0, tol: float = 1e-12,
+               target: str = "SA", x0s=None, ncv: int = 0, max_matvecs: int = 100000, seed: int = 12345, stream=None):
+    """Eigensolve requests on resident operators with continuous batching (dmrg_serve_eigs): request i arrives
+    arrivals[i] seconds after the call (default: all at once), joins the running batch when the HBM budget and
+    max_batch allow, and leaves when converged. Returns (list of (eigenvalue, eigenvector numpy, info dict), stats)."""
+    k = len(ops)
+    arrivals = [0.0] * k if arrivals is None else [float(a) for a in arrivals]
+    if len(arrivals) != k:
+        raise ValueError("one arrival time per operator")
+    cfg = _lib.EigsConfig()
+    load().dmrg_eigs_default_config(C.byref(cfg))
+    cfg.nev, cfg.ncv, cfg.target = 1, ncv, {"SA": 0, "LA": 1}[target]
+    cfg.tol, cfg.max_matvecs, cfg.seed = tol, max_matvecs, seed
+    outs = [DeviceArray((o.rows(),), o.dtype) for o in ops]
+    ins = [None] * k
+    if x0s is not None:
+        ins = [DeviceArray.from_numpy(np.asarray(x, dtype=o.dtype).reshape(-1, order="F")) if x is not None else None
+               for x, o in zip(x0s, ops)]
+    reqs = (_lib.EigRequest * k)()
+    for i in range(k):
+        reqs[i].op, reqs[i].evec, reqs[i].arrival_s = ops[i].h.value, outs[i].ptr.value, arrivals[i]
+        reqs[i].x0 = ins[i].ptr.value if ins[i] is not None else None
+    res = (_lib.EigRequestResult * k)()
+    st = _lib.ServeStats()
+    try:
+        call("dmrg_serve_eigs", C.byref(cfg), reqs, k, int(hbm_budget_bytes), int(max_batch), res, C.byref(st), stream)
+        out = []
+        for i in range(k):
+            r = res[i].res
+            info = {f: getattr(r, f) for f, _ in r._fields_ if f not in ("eigval", "rnorm")}
+            info["rnorm"] = r.rnorm[0]
+            info.update({f: getattr(res[i], f) for f, _ in res[i]._fields_ if f != "res"})
+            out.append((r.eigval[0], outs[i].to_numpy(), info))
+        return out, {f: getattr(st, f) for f, _ in st._fields_}
+    finally:
+        for a in outs + [x for x in ins if x is not None]:
+            a.close()
+
+
 class MatvecGroup:
     """Matvecs of several operators of uneven shapes as one launch per contraction leg (dmrg_matvec_group_*),
     optionally replayed from a HIP graph: y_i = H_i x_i for DeviceArrays xs[i], ys[i]. Keep the operators and

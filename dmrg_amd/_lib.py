@@ -140,6 +140,15 @@ class ServeStats(C.Structure):
                 ("peak_admitted_bytes", C.c_int64), ("peak_concurrency", C.c_int), ("matvecs", C.c_int64)]
 
 
+class EigRequest(C.Structure):
+    _fields_ = [("op", vp), ("x0", vp), ("evec", vp), ("arrival_s", C.c_double)]
+
+
+class EigRequestResult(C.Structure):
+    _fields_ = [("res", EigsResult), ("arrival_s", C.c_double), ("start_s", C.c_double), ("finish_s", C.c_double),
+                ("latency_s", C.c_double), ("footprint_bytes", C.c_int64)]
+
+
 SIGNATURES["dmrg_serve"] = (st, [C.POINTER(ServeConfig), C.POINTER(Request), C.c_int, C.POINTER(RequestResult),
                                  C.POINTER(ServeStats)])
 SIGNATURES["dmrg_request_footprint"] = (i64, [C.c_int, C.c_int, C.c_int])
@@ -147,6 +156,8 @@ SIGNATURES["dmrg_eigs_folded_spectrum"] = (st, [vp, vp, C.POINTER(EigsConfig), v
                                                 C.POINTER(C.c_double), C.POINTER(EigsResult), vp])
 SIGNATURES["dmrg_eigs_batched"] = (st, [C.POINTER(vp), C.c_int, C.POINTER(EigsConfig), C.POINTER(vp), C.POINTER(vp),
                                          C.POINTER(EigsResult), vp])
+SIGNATURES["dmrg_serve_eigs"] = (st, [C.POINTER(EigsConfig), C.POINTER(EigRequest), C.c_int, C.c_int64, C.c_int,
+                                      C.POINTER(EigRequestResult), C.POINTER(ServeStats), vp])
 SIGNATURES["dmrg_matvec_group_create"] = (st, [C.POINTER(vp), C.c_int, C.POINTER(vp), C.POINTER(vp), C.c_int, C.c_int,
                                                C.POINTER(vp)])
 SIGNATURES["dmrg_matvec_group_run"] = (st, [vp, vp])

@@ -1,23 +1,29 @@
-// Batched lockstep Davidson (see batch_eigs.hpp).
+// Batched Davidson over a changing set of problems (see batch_eigs.hpp).
 //
-// Per iteration, for every active problem p with basis V_p, W_p = H_p V_p of m columns (the same m for all):
-//   host:   Rayleigh-Ritz of the m x m projected matrix -> Ritz coefficients y_p and value theta_p
-//   device: x = V y, r = W y - theta x and their norms            (ritz, one grouped launch + fold)
-//           t = r - V (V^H r), twice (classical Gram-Schmidt x2)  (dot, gemv: grouped launches + folds)
+// Each problem p carries its own basis size m_p and phase: START (expand from its start vector), ITER (Ritz pair,
+// then expand from the residual) or RESTART (Ritz pair only, then a thick restart on the host side). One step of
+// all problems:
+//   host:   Rayleigh-Ritz of each m_p x m_p projected matrix -> Ritz coefficients y_p and value theta_p
+//   device: x = V y, r = W y - theta x and their norms            (ritz: grouped launch + fold)
+//           t = s - V (V^H s), twice, s = r or the start vector     (dot, gemv: grouped launches + folds)
 //           H t for all problems                                  (grouped matvec, replayed graph)
 //           V[:, m] = t / |t|, W[:, m] = H t / |t|               (finish)
 //           new column V^H W[:, m] of the projected matrix         (dot + fold)
 //   one device-to-host copy of every problem's scalars, one synchronisation.
-// A problem whose Ritz residual is below tol |A| leaves the batch with x as its eigenvector.
+// A problem whose Ritz residual is below tol |A| leaves with x as its eigenvector; new problems join between
+// steps (continuous batching), and the grouped launches are rebuilt when the set changes.
 #include "batch_eigs.hpp"
 
 #include "dense_eig.hpp"
 #include "group.hpp"
+#include "vecops.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <chrono>
 #include <numeric>
+#include <thread>
 
 namespace dmrg {
 
@@ -37,12 +43,16 @@ constexpr int IO_HCOL  = IO_C + 2 * (BMAX_M + 1); // new column of the projected
 constexpr int IO_NRM   = IO_HCOL + 2 * (BMAX_M + 1); // |r|^2, |x|^2, |src|^2, |t|^2 pass 1, |t|^2 pass 2
 constexpr int IO_SIZE  = IO_NRM + 8;
 
+enum Phase : int { PH_NONE = 0, PH_START = 1, PH_ITER = 2, PH_RESTART = 3 };
+
 template <typename T>
 struct BProb {
     int64_t n;
     int64_t blk0;
     int     nblk;
-    int     active;
+    int     phase;
+    int     m;
+    int     pad;
     T      *V, *W, *x, *r, *X, *Y;
     double *io;
 };
@@ -95,15 +105,54 @@ __device__ inline void block_store(double (&v)[NV], double *partial) {
     }
 }
 
-// x = V y, r = W y - theta x; partials |r|^2, |x|^2 (columns 0, 1).
+// Stages of the grouped kernels and folds; per problem they fix the column count and the destination.
+enum Stage : int { ST_RITZ = 0, ST_DOT1 = 1, ST_GEMV1 = 2, ST_DOT2 = 3, ST_GEMV2 = 4, ST_HCOL = 5 };
+
 template <typename T>
-__global__ __launch_bounds__(BT) void b_ritz(const BProb<T> *pr, const int *blkmap, int m, double *partial) {
+__device__ inline bool stage_on(const BProb<T> &P, int st) {
+    switch(st) {
+        case ST_RITZ: return P.phase == PH_ITER || P.phase == PH_RESTART;
+        case ST_DOT2:
+        case ST_GEMV2: return P.phase == PH_ITER;
+        default: return P.phase == PH_START || P.phase == PH_ITER;
+    }
+}
+// Basis columns used by a stage (the new column's index for ST_HCOL).
+template <typename T>
+__device__ inline int stage_m(const BProb<T> &P) {
+    return P.phase == PH_START ? 0 : P.m;
+}
+// Source of a dot/gemv stage: the start vector X (START, and pass 2), the residual r (pass 1 of ITER), W[:, m] (HCOL).
+template <typename T>
+__device__ inline const T *stage_src(const BProb<T> &P, int st) {
+    if(st == ST_HCOL) return P.W + static_cast<int64_t>(stage_m(P)) * P.n;
+    if(st == ST_DOT2 || st == ST_GEMV2 || P.phase == PH_START) return P.X;
+    return P.r;
+}
+// Values folded by a stage and where they go in io.
+template <typename T>
+__device__ inline void stage_fold(const BProb<T> &P, int st, int &nval, int &dst) {
+    const int m = stage_m(P);
+    switch(st) {
+        case ST_RITZ: nval = 2, dst = IO_NRM; break;
+        case ST_DOT1: nval = 2 * (m + 1), dst = IO_C; break; // c, then |s|^2 at IO_C + 2m
+        case ST_GEMV1: nval = 1, dst = P.phase == PH_START ? IO_NRM + 4 : IO_NRM + 3; break;
+        case ST_DOT2: nval = 2 * m, dst = IO_C; break;
+        case ST_GEMV2: nval = 1, dst = IO_NRM + 4; break;
+        default: nval = 2 * (m + 1), dst = IO_HCOL; break;
+    }
+}
+
+// x = V y, r = W y - theta x; partials |r|^2, |x|^2.
+template <typename T>
+__global__ __launch_bounds__(BT) void b_ritz(const BProb<T> *pr, const int *blkmap, double *partial) {
     const BProb<T> &P = pr[blkmap[blockIdx.x]];
-    if(!P.active) return;
+    if(!stage_on(P, ST_RITZ)) return;
+    const int         m = P.m;
     __shared__ double ys[2 * BMAX_M + 2];
     for(int j = threadIdx.x; j < 2 * m + 2; j += BT) ys[j] = j < 2 * m ? P.io[IO_Y + j] : P.io[IO_THETA + (j - 2 * m)];
     __syncthreads();
-    const double th = ys[2 * m];
+    const double  th = ys[2 * m];
     const int64_t lb = blockIdx.x - P.blk0;
     double        acc[2] = {0.0, 0.0};
     for(int64_t i = lb * BT + threadIdx.x; i < P.n; i += static_cast<int64_t>(P.nblk) * BT) {
@@ -127,23 +176,18 @@ __global__ __launch_bounds__(BT) void b_ritz(const BProb<T> *pr, const int *blkm
     block_store<2>(acc, partial + static_cast<int64_t>(blockIdx.x) * PST);
 }
 
-// Source vector of a pass: the residual r (src 0), the staging vector X (src 1) or W[:, m] (src 2).
+// Dots of a stage: c_j = V_j^H s for the stage's columns, and |s|^2 after them in ST_DOT1.
 template <typename T>
-__device__ inline const T *src_of(const BProb<T> &P, int src, int m) {
-    return src == 0 ? P.r : src == 1 ? P.X : P.W + static_cast<int64_t>(m) * P.n;
-}
-
-// c_j = V_j^H s for j < mc (partials, complex: slots 2j, 2j+1), and |s|^2 (slots 2 mc, 2 mc + 1) when self.
-template <typename T>
-__global__ __launch_bounds__(BT) void b_dot(const BProb<T> *pr, const int *blkmap, int mc, int src, int msrc, int self,
-                                            double *partial) {
+__global__ __launch_bounds__(BT) void b_dot(const BProb<T> *pr, const int *blkmap, int st, double *partial) {
     const BProb<T> &P = pr[blkmap[blockIdx.x]];
-    if(!P.active) return;
-    const T      *s   = src_of(P, src, msrc);
+    if(!stage_on(P, st)) return;
+    const T      *s   = stage_src(P, st);
+    const int     m   = stage_m(P);
+    const int     mc  = st == ST_HCOL ? m + 1 : m;
+    const int     nc  = mc + (st == ST_DOT1 ? 1 : 0);
     const int64_t lb  = blockIdx.x - P.blk0;
     double       *out = partial + static_cast<int64_t>(blockIdx.x) * PST;
     constexpr int CH  = 8;
-    const int     nc  = mc + (self ? 1 : 0);
     for(int j0 = 0; j0 < nc; j0 += CH) {
         double acc[2 * CH];
 #pragma unroll
@@ -167,15 +211,16 @@ __global__ __launch_bounds__(BT) void b_dot(const BProb<T> *pr, const int *blkma
     }
 }
 
-// X = s - V c (c from io[IO_C]); partial |X|^2 (slot 0).
+// X = s - V c (c from io[IO_C]); partial |X|^2.
 template <typename T>
-__global__ __launch_bounds__(BT) void b_gemv(const BProb<T> *pr, const int *blkmap, int m, int src, double *partial) {
+__global__ __launch_bounds__(BT) void b_gemv(const BProb<T> *pr, const int *blkmap, int st, double *partial) {
     const BProb<T> &P = pr[blkmap[blockIdx.x]];
-    if(!P.active) return;
+    if(!stage_on(P, st)) return;
+    const int         m = stage_m(P);
     __shared__ double cs[2 * BMAX_M];
     for(int j = threadIdx.x; j < 2 * m; j += BT) cs[j] = P.io[IO_C + j];
     __syncthreads();
-    const T      *s  = src_of(P, src, m);
+    const T      *s  = stage_src(P, st);
     const int64_t lb = blockIdx.x - P.blk0;
     double        acc[1] = {0.0};
     for(int64_t i = lb * BT + threadIdx.x; i < P.n; i += static_cast<int64_t>(P.nblk) * BT) {
@@ -197,9 +242,10 @@ __global__ __launch_bounds__(BT) void b_gemv(const BProb<T> *pr, const int *blkm
 
 // V[:, m] = X / |t|, W[:, m] = Y / |t| with |t|^2 = io[IO_NRM + 4].
 template <typename T>
-__global__ __launch_bounds__(BT) void b_finish(const BProb<T> *pr, const int *blkmap, int m) {
+__global__ __launch_bounds__(BT) void b_finish(const BProb<T> *pr, const int *blkmap) {
     const BProb<T> &P = pr[blkmap[blockIdx.x]];
-    if(!P.active) return;
+    if(!stage_on(P, ST_GEMV1)) return;
+    const int     m  = stage_m(P);
     const double  nr = P.io[IO_NRM + 4];
     const double  a  = nr > 0 ? 1.0 / sqrt(nr) : 0.0;
     const int64_t lb = blockIdx.x - P.blk0;
@@ -213,12 +259,15 @@ __global__ __launch_bounds__(BT) void b_finish(const BProb<T> *pr, const int *bl
     }
 }
 
-// io[dst + c] = sum over the problem's workgroups of partial slot c, c < nval, in workgroup order.
+// io[dst + c] = sum over the problem's workgroups of partial slot c, in workgroup order (c < the stage's count).
 template <typename T>
-__global__ __launch_bounds__(BT) void b_fold(const BProb<T> *pr, const double *partial, int nval, int dst) {
+__global__ __launch_bounds__(BT) void b_fold(const BProb<T> *pr, const double *partial, int st) {
     const BProb<T> &P = pr[blockIdx.y];
-    const int       c = blockIdx.x;
-    if(!P.active || c >= nval) return;
+    if(!stage_on(P, st)) return;
+    int nval, dst;
+    stage_fold(P, st, nval, dst);
+    const int c = blockIdx.x;
+    if(c >= nval) return;
     double acc[1] = {0.0};
     for(int b = threadIdx.x; b < P.nblk; b += BT) acc[0] += partial[(P.blk0 + b) * PST + c];
     __shared__ double res[1];
@@ -227,270 +276,324 @@ __global__ __launch_bounds__(BT) void b_fold(const BProb<T> *pr, const double *p
     if(threadIdx.x == 0) P.io[dst + c] = res[0];
 }
 
-// Seeded start vector into X (entries uniform in [-1, 1]).
-__device__ inline uint64_t mix64(uint64_t z) {
-    z += 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
 template <typename T>
-__global__ __launch_bounds__(BT) void b_random(const BProb<T> *pr, const int *blkmap, uint64_t seed) {
-    const BProb<T> &P  = pr[blkmap[blockIdx.x]];
-    const int64_t   lb = blockIdx.x - P.blk0;
-    for(int64_t i = lb * BT + threadIdx.x; i < P.n; i += static_cast<int64_t>(P.nblk) * BT) {
-        const uint64_t h = mix64(seed * 0x100000001B3ull + static_cast<uint64_t>(i));
-        const double   a = static_cast<double>(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-        const double   b = static_cast<double>(mix64(h) >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-        P.X[i]           = mkc<T>(a, bt<T>::cx ? b : 0.0);
-    }
-}
+struct Slot {
+    MatVecOp       *op = nullptr;
+    DevBuf          mem;
+    int64_t         n = 0;
+    int             m = 0, phase = PH_START, mmax = 4, mmin = 1, opt = 0;
+    std::vector<cx> H, Z;
+    std::vector<double> w;
+    double          anorm = 0.0;
+    BatchEigsResult res;
+    void           *evec = nullptr;
+    int             id   = 0;
+    int64_t         footprint = 0;
+    T *V() const { return mem.as<T>(); }
+    cx &Hat(int i, int j) { return H[i + static_cast<size_t>(mmax) * j]; }
+};
+
+struct Finished {
+    int             id;
+    BatchEigsResult res;
+    int64_t         footprint;
+};
 
 template <typename T>
-struct Batch {
-    hipStream_t                  s;
-    std::vector<MatVecOp *>      ops;
-    std::vector<BProb<T>>        hp;   // host copy of the descriptors
-    DevBuf                       dprobs, dblk, dpart, dio, mem;
-    std::vector<int>             blkmap;
-    int                          nblk_total = 0;
-    double                      *hio        = nullptr; // pinned, k * IO_SIZE
-    std::unique_ptr<MatvecGroup> group;
+struct Engine {
+    hipStream_t                           s;
+    EigsParams                            p;
+    std::vector<std::unique_ptr<Slot<T>>> slots;
+    std::vector<BProb<T>>                 hp;
+    DevBuf                                dprobs, dblk, dpart, dio;
+    double                               *hio     = nullptr;
+    size_t                                hio_cap = 0;
+    int                                   nblk_total = 0, maxm = 0, nactive = 0;
+    std::unique_ptr<MatvecGroup>          group;
+    bool                                  dirty = true;
 
-    Batch(hipStream_t st) : s(st) {}
-    ~Batch() {
+    Engine(hipStream_t st, const EigsParams &pp) : s(st), p(pp) {}
+    ~Engine() {
         if(hio) (void) hipHostFree(hio);
     }
-    int k() const { return static_cast<int>(ops.size()); }
+    int k() const { return static_cast<int>(slots.size()); }
 
-    void upload_probs() {
-        DMRG_HIP_CHECK(hipMemcpyAsync(dprobs.p, hp.data(), sizeof(BProb<T>) * hp.size(), hipMemcpyHostToDevice, s));
-        DMRG_HIP_CHECK(hipStreamSynchronize(s)); // hp may change right after
+    static int64_t footprint_of(const MatVecOp &op, int mmax) {
+        return static_cast<int64_t>(op.D.n()) * (2 * mmax + 4) * static_cast<int64_t>(sizeof(T));
+    }
+    static void basis_rule(const EigsParams &p, int64_t n, int &mmax, int &mmin) {
+        // Basis rule of the single-problem solver (solver.cpp:12-15, solver_primme.cpp:374).
+        mmax = p.ncv > 0 ? p.ncv : static_cast<int>(std::ceil(std::log2(static_cast<double>(std::max<int64_t>(n, 2)))));
+        mmax = static_cast<int>(std::clamp<int64_t>(mmax, 4, std::min<int64_t>(n, BMAX_M)));
+        mmin = std::max(1, std::min(mmax / 2 - 1, mmax - 2));
     }
 
-    // Grouped matvec X -> Y of the active problems (rebuilt when the active set changes).
-    void rebuild_group() {
-        std::vector<MatVecOp *> a;
-        std::vector<const void *> xs;
-        std::vector<void *>       ys;
-        for(int p = 0; p < k(); ++p)
-            if(hp[p].active) a.push_back(ops[p]), xs.push_back(hp[p].X), ys.push_back(hp[p].Y);
-        group.reset();
+    void add(MatVecOp *op, const void *x0, void *evec, int id) {
+        DMRG_REQUIRE(op->D.n() >= 2, DMRG_ERR_ARG, "batched eigs: problems of size < 2");
+        auto sl  = std::make_unique<Slot<T>>();
+        sl->op   = op;
+        sl->n    = op->D.n();
+        sl->evec = evec;
+        sl->id   = id;
+        basis_rule(p, sl->n, sl->mmax, sl->mmin);
+        sl->mmax = static_cast<int>(std::min<int64_t>(sl->mmax, sl->n));
+        sl->H.assign(static_cast<size_t>(sl->mmax) * sl->mmax, cx(0.0, 0.0));
+        sl->footprint = footprint_of(*op, sl->mmax);
+        sl->mem.reserve(static_cast<size_t>(sl->n) * (2 * sl->mmax + 4) * sizeof(T));
+        T *X = sl->V() + sl->n * (2 * sl->mmax + 2);
+        if(x0) DMRG_HIP_CHECK(hipMemcpyAsync(X, x0, static_cast<size_t>(sl->n) * sizeof(T), hipMemcpyDeviceToDevice, s));
+        else random_fill<T>(sl->n, X, p.seed, s);
+        slots.push_back(std::move(sl));
+        ++nactive;
+        dirty = true;
+    }
+
+    // Device descriptors, block map, exchange buffers and the grouped matvec of the current set.
+    // Finished problems stay in the set (their matvecs still run inside the grouped launches) until the next
+    // rebuild, which happens when problems join or when half of the set has finished.
+    void rebuild() {
         DMRG_HIP_CHECK(hipStreamSynchronize(s));
-        if(!a.empty()) group.reset(matvec_group_create(a.data(), static_cast<int>(a.size()), xs.data(), ys.data(), false, true));
+        group.reset();
+        slots.erase(std::remove_if(slots.begin(), slots.end(), [](const auto &q) { return q->phase == PH_NONE; }), slots.end());
+        hp.assign(k(), BProb<T>{});
+        std::vector<int> blkmap;
+        int64_t          blk = 0;
+        for(int q = 0; q < k(); ++q) {
+            Slot<T>  &S = *slots[q];
+            BProb<T> &P = hp[q];
+            T        *b = S.V();
+            P.n         = S.n;
+            P.V = b, P.W = b + S.n * S.mmax, P.x = b + S.n * 2 * S.mmax, P.r = P.x + S.n, P.X = P.r + S.n, P.Y = P.X + S.n;
+            P.nblk  = static_cast<int>(std::clamp<int64_t>((S.n + BT * 4 - 1) / (BT * 4), 1, 256));
+            P.blk0  = blk;
+            P.phase = S.phase, P.m = S.m;
+            blk += P.nblk;
+            for(int b2 = 0; b2 < P.nblk; ++b2) blkmap.push_back(q);
+        }
+        nblk_total = static_cast<int>(blk);
+        if(k() == 0) return;
+        dprobs.reserve(sizeof(BProb<T>) * k());
+        dblk.reserve(sizeof(int) * blkmap.size());
+        dpart.reserve(sizeof(double) * PST * blkmap.size());
+        dio.reserve(sizeof(double) * IO_SIZE * k());
+        const size_t need = static_cast<size_t>(IO_SIZE) * k();
+        if(need > hio_cap) {
+            if(hio) (void) hipHostFree(hio);
+            hio = nullptr;
+            DMRG_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&hio), sizeof(double) * need, hipHostMallocDefault));
+            hio_cap = need;
+        }
+        for(int q = 0; q < k(); ++q) hp[q].io = dio.template as<double>() + static_cast<int64_t>(q) * IO_SIZE;
+        DMRG_HIP_CHECK(hipMemcpyAsync(dblk.p, blkmap.data(), sizeof(int) * blkmap.size(), hipMemcpyHostToDevice, s));
+        DMRG_HIP_CHECK(hipStreamSynchronize(s));
+        std::vector<MatVecOp *>   ops(k());
+        std::vector<const void *> xs(k());
+        std::vector<void *>       ys(k());
+        for(int q = 0; q < k(); ++q) ops[q] = slots[q]->op, xs[q] = hp[q].X, ys[q] = hp[q].Y;
+        group.reset(matvec_group_create(ops.data(), k(), xs.data(), ys.data(), false, true));
+        dirty = false;
     }
 
-    void fold(int nval, int dst) {
-        hipLaunchKernelGGL(b_fold<T>, dim3(nval, k()), dim3(BT), 0, s, dprobs.as<BProb<T>>(), dpart.as<double>(), nval, dst);
-    }
-    // t = src - V (V^H src), twice when m > 0, into X; |src|^2 in IO_C + 2m, the final |t|^2 in IO_NRM + 4.
-    void orthogonalize(int m, int src) {
-        const dim3 g(nblk_total), b(BT);
-        auto      *P = dprobs.as<BProb<T>>();
-        auto      *M = dblk.as<int>();
-        double    *Q = dpart.as<double>();
-        hipLaunchKernelGGL(b_dot<T>, g, b, 0, s, P, M, m, src, m, 1, Q);
-        fold(2 * (m + 1), IO_C);
-        hipLaunchKernelGGL(b_gemv<T>, g, b, 0, s, P, M, m, src, Q);
-        fold(1, m > 0 ? IO_NRM + 3 : IO_NRM + 4);
-        if(m > 0) {
-            hipLaunchKernelGGL(b_dot<T>, g, b, 0, s, P, M, m, 1, m, 0, Q);
-            fold(2 * m, IO_C);
-            hipLaunchKernelGGL(b_gemv<T>, g, b, 0, s, P, M, m, 1, Q);
-            fold(1, IO_NRM + 4);
+    void upload_phases() {
+        maxm = 0;
+        for(int q = 0; q < k(); ++q) {
+            hp[q].phase = slots[q]->phase;
+            hp[q].m     = slots[q]->m;
+            maxm        = std::max(maxm, slots[q]->m);
         }
+        DMRG_HIP_CHECK(hipMemcpyAsync(dprobs.p, hp.data(), sizeof(BProb<T>) * hp.size(), hipMemcpyHostToDevice, s));
+    }
+
+    void stage(int st) {
+        auto     *P = dprobs.template as<BProb<T>>();
+        auto     *M = dblk.template as<int>();
+        double   *Q = dpart.template as<double>();
+        const dim3 g(nblk_total), b(BT);
+        switch(st) {
+            case ST_RITZ: hipLaunchKernelGGL(b_ritz<T>, g, b, 0, s, P, M, Q); break;
+            case ST_GEMV1:
+            case ST_GEMV2: hipLaunchKernelGGL(b_gemv<T>, g, b, 0, s, P, M, st, Q); break;
+            default: hipLaunchKernelGGL(b_dot<T>, g, b, 0, s, P, M, st, Q); break;
+        }
+        hipLaunchKernelGGL(b_fold<T>, dim3(2 * (maxm + 2), k()), dim3(BT), 0, s, P, Q, st);
         DMRG_HIP_CHECK(hipGetLastError());
+    }
+
+    // One step of every active problem; returns the problems that finished in it.
+    std::vector<Finished> step() {
+        if(dirty) rebuild();
+        std::vector<Finished> done;
+        if(nactive == 0) return done;
+        // Rayleigh-Ritz of the problems that have a basis.
+        for(int q = 0; q < k(); ++q) {
+            Slot<T> &S = *slots[q];
+            if(S.phase != PH_ITER && S.phase != PH_RESTART) continue;
+            const int       m = S.m;
+            std::vector<cx> Hm(static_cast<size_t>(m) * m);
+            for(int j = 0; j < m; ++j)
+                for(int i = 0; i < m; ++i) Hm[i + static_cast<size_t>(m) * j] = S.Hat(i, j);
+            S.Z.assign(static_cast<size_t>(m) * m, cx(0.0, 0.0));
+            S.w.assign(m, 0.0);
+            if(!hermitian_eig(m, Hm.data(), m, S.w.data(), S.Z.data())) throw Error(DMRG_ERR_NOT_CONVERGED, "batched eigs: Rayleigh-Ritz failed");
+            for(int i = 0; i < m; ++i) S.anorm = std::max(S.anorm, std::fabs(S.w[i]));
+            S.opt      = p.target == TARGET_LA ? m - 1 : 0;
+            double *io = hio + static_cast<size_t>(q) * IO_SIZE;
+            for(int i = 0; i < m; ++i) {
+                io[IO_Y + 2 * i]     = S.Z[i + static_cast<size_t>(m) * S.opt].real();
+                io[IO_Y + 2 * i + 1] = S.Z[i + static_cast<size_t>(m) * S.opt].imag();
+            }
+            io[IO_THETA] = S.w[S.opt];
+        }
+        upload_phases();
+        DMRG_HIP_CHECK(hipMemcpyAsync(dio.p, hio, sizeof(double) * IO_SIZE * k(), hipMemcpyHostToDevice, s));
+        stage(ST_RITZ);
+        stage(ST_DOT1);
+        stage(ST_GEMV1);
+        stage(ST_DOT2);
+        stage(ST_GEMV2);
+        group->run(s);
+        hipLaunchKernelGGL(b_finish<T>, dim3(nblk_total), dim3(BT), 0, s, dprobs.template as<BProb<T>>(), dblk.template as<int>());
+        stage(ST_HCOL);
+        DMRG_HIP_CHECK(hipMemcpyAsync(hio, dio.p, sizeof(double) * IO_SIZE * k(), hipMemcpyDeviceToHost, s));
+        DMRG_HIP_CHECK(hipStreamSynchronize(s));
+
+        for(int q = 0; q < k(); ++q) {
+            Slot<T>      &S  = *slots[q];
+            if(S.phase == PH_NONE) continue;
+            const double *io = hio + static_cast<size_t>(q) * IO_SIZE;
+            auto take_column = [&](int m) {
+                for(int i = 0; i <= m; ++i) S.Hat(i, m) = cx(io[IO_HCOL + 2 * i], io[IO_HCOL + 2 * i + 1]);
+                for(int i = 0; i < m; ++i) S.Hat(m, i) = std::conj(S.Hat(i, m));
+                S.Hat(m, m) = cx(S.Hat(m, m).real(), 0.0);
+                S.res.matvecs += 1;
+            };
+            if(S.phase == PH_START) {
+                take_column(0);
+                S.m     = 1;
+                S.phase = PH_ITER;
+                continue;
+            }
+            const double rn = std::sqrt(std::max(0.0, io[IO_NRM])) / std::sqrt(std::max(1e-300, io[IO_NRM + 1]));
+            S.res.eigval    = S.w[S.opt];
+            S.res.rnorm     = rn;
+            S.res.anorm     = S.anorm;
+            S.res.iters += 1;
+            const bool conv  = rn <= p.tol * std::max(S.anorm, 1e-300);
+            const bool limit = S.res.matvecs >= p.max_matvecs;
+            const bool broke = S.phase == PH_ITER && !(io[IO_NRM + 4] > 1e-26 * std::max(io[IO_C + 2 * S.m], 1e-300));
+            if(conv || limit || broke) {
+                S.res.nconv = conv ? 1 : 0;
+                DMRG_HIP_CHECK(hipMemcpyAsync(S.evec, hp[q].x, static_cast<size_t>(S.n) * sizeof(T), hipMemcpyDeviceToDevice, s));
+                S.phase = PH_NONE;
+                --nactive;
+                done.push_back(Finished{S.id, S.res, S.footprint});
+                continue;
+            }
+            if(S.phase == PH_ITER) {
+                take_column(S.m);
+                if(++S.m == S.mmax) S.phase = PH_RESTART;
+                continue;
+            }
+            // Thick restart to the mmin Ritz vectors nearest the target: V <- V Q, W <- W Q, H <- diag.
+            const int        m = S.m;
+            std::vector<int> ord(m);
+            std::iota(ord.begin(), ord.end(), 0);
+            if(p.target == TARGET_LA) std::reverse(ord.begin(), ord.end());
+            std::vector<T> Qh(static_cast<size_t>(m) * S.mmin);
+            for(int j = 0; j < S.mmin; ++j)
+                for(int i = 0; i < m; ++i) {
+                    const cx z = S.Z[i + static_cast<size_t>(m) * ord[j]];
+                    if constexpr(bt<T>::cx) Qh[i + static_cast<size_t>(m) * j] = T{z.real(), z.imag()};
+                    else Qh[i + static_cast<size_t>(m) * j] = z.real();
+                }
+            DevBuf Qd, tmp;
+            Qd.reserve(Qh.size() * sizeof(T));
+            tmp.reserve(static_cast<size_t>(S.n) * S.mmin * sizeof(T));
+            DMRG_HIP_CHECK(hipMemcpyAsync(Qd.p, Qh.data(), Qh.size() * sizeof(T), hipMemcpyHostToDevice, s));
+            for(T *Bm : {hp[q].V, hp[q].W}) {
+                GemmDesc g;
+                g.dtype = S.op->dtype;
+                g.M = static_cast<int32_t>(S.n), g.N = S.mmin, g.K = m, g.batch = 1;
+                g.A = Bm, g.am = plain(1), g.ak = plain(S.n);
+                g.B = Qd.p, g.bk = plain(1), g.bn = plain(m);
+                g.C = tmp.p, g.cm = plain(1), g.cn = plain(S.n);
+                gemm(g, s);
+                DMRG_HIP_CHECK(hipMemcpyAsync(Bm, tmp.p, static_cast<size_t>(S.n) * S.mmin * sizeof(T), hipMemcpyDeviceToDevice, s));
+            }
+            DMRG_HIP_CHECK(hipStreamSynchronize(s)); // Qh and the temporaries die here
+            std::fill(S.H.begin(), S.H.end(), cx(0.0, 0.0));
+            for(int j = 0; j < S.mmin; ++j) S.Hat(j, j) = cx(S.w[ord[j]], 0.0);
+            S.m     = S.mmin;
+            S.phase = PH_ITER;
+        }
+        if(!done.empty()) {
+            DMRG_HIP_CHECK(hipStreamSynchronize(s)); // eigenvectors in place before the problems are reported
+            if(2 * nactive <= k()) dirty = true;
+        }
+        return done;
     }
 };
 
 template <typename T>
 void batched_impl(MatVecOp *const *ops, int k, const EigsParams &p, const void *const *x0s, void *const *evecs,
                   std::vector<BatchEigsResult> &res, hipStream_t s) {
-    DMRG_REQUIRE(k >= 1, DMRG_ERR_ARG, "batched eigs: no problems");
-    DMRG_REQUIRE(std::max(1, p.nev) == 1 && p.precond == 0 && (p.target == TARGET_SA || p.target == TARGET_LA), DMRG_ERR_UNSUPPORTED,
-                 "batched eigs: one pair per problem at SA or LA, no preconditioner");
-    Batch<T> B(s);
-    B.ops.assign(ops, ops + k);
-    int64_t nmin = INT64_MAX, nmax = 0;
-    for(int q = 0; q < k; ++q) nmin = std::min(nmin, ops[q]->D.n()), nmax = std::max(nmax, ops[q]->D.n());
-    // Basis rule of the single-problem solver (solver.cpp:12-15), common to the batch.
-    int mmax = p.ncv > 0 ? p.ncv : static_cast<int>(std::ceil(std::log2(static_cast<double>(std::max<int64_t>(nmax, 2)))));
-    mmax     = static_cast<int>(std::clamp<int64_t>(mmax, 4, std::min<int64_t>(nmin, BMAX_M)));
-    const int mmin = std::max(1, std::min(mmax / 2 - 1, mmax - 2));
-    DMRG_REQUIRE(nmin >= 2, DMRG_ERR_ARG, "batched eigs: problems of size < 2");
-    mmax = static_cast<int>(std::min<int64_t>(mmax, nmin));
-    // Device storage: per problem V, W (n x mmax), x, r, X, Y (n each).
-    size_t total = 0;
-    for(int q = 0; q < k; ++q) total += static_cast<size_t>(ops[q]->D.n()) * (2 * mmax + 4);
-    B.mem.reserve(total * sizeof(T));
-    B.hp.resize(k);
-    T      *base = B.mem.template as<T>();
-    int64_t blk  = 0;
-    for(int q = 0; q < k; ++q) {
-        const int64_t n = ops[q]->D.n();
-        BProb<T>     &P = B.hp[q];
-        P.n             = n;
-        P.V = base, base += n * mmax;
-        P.W = base, base += n * mmax;
-        P.x = base, base += n;
-        P.r = base, base += n;
-        P.X = base, base += n;
-        P.Y = base, base += n;
-        P.nblk   = static_cast<int>(std::clamp<int64_t>((n + BT * 4 - 1) / (BT * 4), 1, 256));
-        P.blk0   = blk;
-        P.active = 1;
-        blk += P.nblk;
-        for(int b = 0; b < P.nblk; ++b) B.blkmap.push_back(q);
-    }
-    B.nblk_total = static_cast<int>(blk);
-    B.dprobs.reserve(sizeof(BProb<T>) * k);
-    B.dblk.reserve(sizeof(int) * B.blkmap.size());
-    B.dpart.reserve(sizeof(double) * PST * B.blkmap.size());
-    B.dio.reserve(sizeof(double) * IO_SIZE * k);
-    DMRG_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&B.hio), sizeof(double) * IO_SIZE * k, hipHostMallocDefault));
-    std::fill(B.hio, B.hio + static_cast<size_t>(IO_SIZE) * k, 0.0);
-    for(int q = 0; q < k; ++q) B.hp[q].io = B.dio.template as<double>() + static_cast<int64_t>(q) * IO_SIZE;
-    DMRG_HIP_CHECK(hipMemcpyAsync(B.dblk.p, B.blkmap.data(), sizeof(int) * B.blkmap.size(), hipMemcpyHostToDevice, s));
-    B.upload_probs();
-
-    // Start vectors into X: the caller's, else seeded random ones.
-    hipLaunchKernelGGL(b_random<T>, dim3(B.nblk_total), dim3(BT), 0, s, B.dprobs.template as<BProb<T>>(), B.dblk.template as<int>(), p.seed);
-    for(int q = 0; q < k; ++q)
-        if(x0s && x0s[q]) DMRG_HIP_CHECK(hipMemcpyAsync(B.hp[q].X, x0s[q], static_cast<size_t>(B.hp[q].n) * sizeof(T), hipMemcpyDeviceToDevice, s));
-    B.rebuild_group();
-
+    Engine<T> E(s, p);
+    for(int q = 0; q < k; ++q) E.add(ops[q], x0s ? x0s[q] : nullptr, evecs[q], q);
     res.assign(k, BatchEigsResult{});
-    std::vector<std::vector<cx>> H(k, std::vector<cx>(static_cast<size_t>(mmax) * mmax, cx(0.0, 0.0)));
-    std::vector<double>          anorm(k, 0.0);
-    std::vector<std::vector<cx>> Z(k);
-    std::vector<std::vector<double>> w(k);
-    std::vector<int>             opt(k, 0);
-    int                          active = k, group_size = k;
-    const dim3                   G(B.nblk_total), TB(BT);
-    auto                        *dP = B.dprobs.template as<BProb<T>>();
-    auto                        *dM = B.dblk.template as<int>();
-    double                      *dQ = B.dpart.template as<double>();
+    while(E.nactive > 0)
+        for(const auto &d : E.step()) res[d.id] = d.res;
+}
 
-    // Expansion from X (m = 0: the start vector) or from the residual; then matvec and the new column.
-    auto expand = [&](int m, int src) {
-        B.orthogonalize(m, src);
-        B.group->run(s);
-        hipLaunchKernelGGL(b_finish<T>, G, TB, 0, s, dP, dM, m);
-        hipLaunchKernelGGL(b_dot<T>, G, TB, 0, s, dP, dM, m + 1, 2, m, 0, dQ);
-        B.fold(2 * (m + 1), IO_HCOL);
-        DMRG_HIP_CHECK(hipGetLastError());
-    };
-    auto fetch = [&]() {
-        DMRG_HIP_CHECK(hipMemcpyAsync(B.hio, B.dio.p, sizeof(double) * IO_SIZE * k, hipMemcpyDeviceToHost, s));
-        DMRG_HIP_CHECK(hipStreamSynchronize(s));
-    };
-    auto Hat = [&](int q, int i, int j) -> cx & { return H[q][i + static_cast<size_t>(mmax) * j]; };
-    auto take_column = [&](int q, int m) {
-        const double *io = B.hio + static_cast<size_t>(q) * IO_SIZE;
-        for(int i = 0; i <= m; ++i) Hat(q, i, m) = cx(io[IO_HCOL + 2 * i], io[IO_HCOL + 2 * i + 1]);
-        for(int i = 0; i < m; ++i) Hat(q, m, i) = std::conj(Hat(q, i, m));
-        Hat(q, m, m) = cx(Hat(q, m, m).real(), 0.0);
-        res[q].matvecs += 1;
-    };
-
-    expand(0, 1);
-    fetch();
-    for(int q = 0; q < k; ++q) take_column(q, 0);
-    int m = 1;
-    while(active > 0) {
-        // Rayleigh-Ritz per active problem; Ritz coefficients to the device.
-        for(int q = 0; q < k; ++q) {
-            if(!B.hp[q].active) continue;
-            std::vector<cx> Hm(static_cast<size_t>(m) * m);
-            for(int j = 0; j < m; ++j)
-                for(int i = 0; i < m; ++i) Hm[i + static_cast<size_t>(m) * j] = Hat(q, i, j);
-            Z[q].assign(static_cast<size_t>(m) * m, cx(0.0, 0.0));
-            w[q].assign(m, 0.0);
-            if(!hermitian_eig(m, Hm.data(), m, w[q].data(), Z[q].data())) throw Error(DMRG_ERR_NOT_CONVERGED, "batched eigs: Rayleigh-Ritz failed");
-            for(int i = 0; i < m; ++i) anorm[q] = std::max(anorm[q], std::fabs(w[q][i]));
-            opt[q]      = p.target == TARGET_LA ? m - 1 : 0;
-            double *io  = B.hio + static_cast<size_t>(q) * IO_SIZE;
-            for(int i = 0; i < m; ++i) {
-                io[IO_Y + 2 * i]     = Z[q][i + static_cast<size_t>(m) * opt[q]].real();
-                io[IO_Y + 2 * i + 1] = Z[q][i + static_cast<size_t>(m) * opt[q]].imag();
-            }
-            io[IO_THETA] = w[q][opt[q]];
+template <typename T>
+void serve_impl(const std::vector<EigRequest> &req, const EigsParams &p, int64_t budget, int max_batch,
+                std::vector<EigRequestResult> &out, ServeSummary &sum, hipStream_t s) {
+    using clock   = std::chrono::steady_clock;
+    const auto t0 = clock::now();
+    auto       now = [&] { return std::chrono::duration<double>(clock::now() - t0).count(); };
+    Engine<T>  E(s, p);
+    const int  n = static_cast<int>(req.size());
+    out.assign(n, EigRequestResult{});
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return req[a].arrival_s < req[b].arrival_s; });
+    size_t  next = 0;
+    int64_t admitted = 0;
+    int     finished = 0;
+    while(finished < n) {
+        // Admit, in arrival order, the requests that have arrived and fit in the budget and the batch.
+        const double t = now();
+        while(next < order.size()) {
+            const EigRequest &r = req[order[next]];
+            if(r.arrival_s > t) break;
+            int mmax, mmin;
+            Engine<T>::basis_rule(p, r.op->D.n(), mmax, mmin);
+            const int64_t fp = Engine<T>::footprint_of(*r.op, mmax);
+            if((E.nactive > 0 && admitted + fp > budget) || (max_batch > 0 && E.nactive >= max_batch)) break;
+            E.add(r.op, r.x0, r.evec, order[next]);
+            EigRequestResult &o = out[order[next]];
+            o.arrival_s         = r.arrival_s;
+            o.start_s           = t;
+            o.footprint_bytes   = fp;
+            admitted += fp;
+            sum.peak_admitted_bytes = std::max(sum.peak_admitted_bytes, admitted);
+            sum.peak_concurrency    = std::max(sum.peak_concurrency, E.nactive);
+            ++next;
         }
-        DMRG_HIP_CHECK(hipMemcpyAsync(B.dio.p, B.hio, sizeof(double) * IO_SIZE * k, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(b_ritz<T>, G, TB, 0, s, dP, dM, m, dQ);
-        B.fold(2, IO_NRM);
-        const bool restart = m == mmax;
-        if(!restart) expand(m, 0);
-        fetch();
-        // Convergence, expansion, stopping.
-        bool changed = false;
-        for(int q = 0; q < k; ++q) {
-            if(!B.hp[q].active) continue;
-            const double *io = B.hio + static_cast<size_t>(q) * IO_SIZE;
-            const double  rn = std::sqrt(std::max(0.0, io[IO_NRM])) / std::sqrt(std::max(1e-300, io[IO_NRM + 1]));
-            res[q].eigval    = w[q][opt[q]];
-            res[q].anorm     = anorm[q];
-            res[q].rnorm     = rn;
-            res[q].iters += 1;
-            const bool conv  = rn <= p.tol * std::max(anorm[q], 1e-300);
-            const bool limit = res[q].matvecs >= p.max_matvecs;
-            const bool broke = !restart && !(io[IO_NRM + 4] > 1e-26 * std::max(io[IO_C + 2 * m], 1e-300));
-            if(conv || limit || broke) {
-                res[q].nconv = conv ? 1 : 0;
-                DMRG_HIP_CHECK(hipMemcpyAsync(evecs[q], B.hp[q].x, static_cast<size_t>(B.hp[q].n) * sizeof(T), hipMemcpyDeviceToDevice, s));
-                B.hp[q].active = 0;
-                --active;
-                changed = true;
-                continue;
-            }
-            if(!restart) take_column(q, m);
+        if(E.nactive == 0) { // idle until the next arrival
+            const double wait = req[order[next]].arrival_s - now();
+            if(wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.001)));
+            continue;
         }
-        if(active == 0) break;
-        if(restart) {
-            // Thick restart to the mmin Ritz vectors nearest the target: V <- V Q, W <- W Q, H <- diag.
-            for(int q = 0; q < k; ++q) {
-                if(!B.hp[q].active) continue;
-                std::vector<int> ord(m);
-                std::iota(ord.begin(), ord.end(), 0);
-                if(p.target == TARGET_LA) std::reverse(ord.begin(), ord.end());
-                std::vector<T> Qh(static_cast<size_t>(m) * mmin);
-                for(int j = 0; j < mmin; ++j)
-                    for(int i = 0; i < m; ++i) {
-                        const cx z = Z[q][i + static_cast<size_t>(m) * ord[j]];
-                        if constexpr(bt<T>::cx) Qh[i + static_cast<size_t>(m) * j] = T{z.real(), z.imag()};
-                        else Qh[i + static_cast<size_t>(m) * j] = z.real();
-                    }
-                DevBuf Qd, tmp;
-                Qd.reserve(Qh.size() * sizeof(T));
-                tmp.reserve(static_cast<size_t>(B.hp[q].n) * mmin * sizeof(T));
-                DMRG_HIP_CHECK(hipMemcpyAsync(Qd.p, Qh.data(), Qh.size() * sizeof(T), hipMemcpyHostToDevice, s));
-                for(T *Bm : {B.hp[q].V, B.hp[q].W}) {
-                    GemmDesc g;
-                    g.dtype = ops[q]->dtype;
-                    g.M = static_cast<int32_t>(B.hp[q].n), g.N = mmin, g.K = m, g.batch = 1;
-                    g.A = Bm, g.am = plain(1), g.ak = plain(B.hp[q].n);
-                    g.B = Qd.p, g.bk = plain(1), g.bn = plain(m);
-                    g.C = tmp.p, g.cm = plain(1), g.cn = plain(B.hp[q].n);
-                    gemm(g, s);
-                    DMRG_HIP_CHECK(hipMemcpyAsync(Bm, tmp.p, static_cast<size_t>(B.hp[q].n) * mmin * sizeof(T), hipMemcpyDeviceToDevice, s));
-                }
-                DMRG_HIP_CHECK(hipStreamSynchronize(s)); // Qh and the temporaries die here
-                std::fill(H[q].begin(), H[q].end(), cx(0.0, 0.0));
-                for(int j = 0; j < mmin; ++j) Hat(q, j, j) = cx(w[q][ord[j]], 0.0);
-            }
-            m = mmin;
-        } else {
-            ++m;
-        }
-        if(changed) {
-            B.upload_probs();
-            if(2 * active <= group_size) {
-                B.rebuild_group();
-                group_size = active;
-            }
+        for(const auto &d : E.step()) {
+            EigRequestResult &o = out[d.id];
+            o.res               = d.res;
+            o.finish_s          = now();
+            o.latency_s         = o.finish_s - o.arrival_s;
+            admitted -= d.footprint;
+            ++finished;
         }
     }
-    DMRG_HIP_CHECK(hipStreamSynchronize(s));
+    sum.makespan_s = now();
 }
 
 } // namespace
@@ -498,11 +601,25 @@ void batched_impl(MatVecOp *const *ops, int k, const EigsParams &p, const void *
 void eigs_batched(MatVecOp *const *ops, int k, const EigsParams &p, const void *const *x0s, void *const *evecs,
                   std::vector<BatchEigsResult> &res, hipStream_t s) {
     DMRG_REQUIRE(ops && evecs && k >= 1, DMRG_ERR_ARG, "batched eigs: bad arguments");
+    DMRG_REQUIRE(std::max(1, p.nev) == 1 && p.precond == 0 && (p.target == TARGET_SA || p.target == TARGET_LA), DMRG_ERR_UNSUPPORTED,
+                 "batched eigs: one pair per problem at SA or LA, no preconditioner");
     const dmrg_dtype t = ops[0]->dtype;
     for(int q = 0; q < k; ++q) DMRG_REQUIRE(ops[q] && ops[q]->dtype == t && evecs[q], DMRG_ERR_ARG, "batched eigs: one dtype, outputs");
     if(t == DMRG_FP64) batched_impl<double>(ops, k, p, x0s, evecs, res, s);
     else if(t == DMRG_CX64) batched_impl<cpx<double>>(ops, k, p, x0s, evecs, res, s);
     else throw Error(DMRG_ERR_ARG, "batched eigs: FP64 or CX64 operators");
+}
+
+void serve_eigs(const std::vector<EigRequest> &req, const EigsParams &p, int64_t budget, int max_batch,
+                std::vector<EigRequestResult> &out, ServeSummary &sum, hipStream_t s) {
+    DMRG_REQUIRE(!req.empty(), DMRG_ERR_ARG, "serve eigs: no requests");
+    DMRG_REQUIRE(std::max(1, p.nev) == 1 && p.precond == 0 && (p.target == TARGET_SA || p.target == TARGET_LA), DMRG_ERR_UNSUPPORTED,
+                 "serve eigs: one pair per problem at SA or LA, no preconditioner");
+    const dmrg_dtype t = req[0].op->dtype;
+    for(const auto &r : req) DMRG_REQUIRE(r.op && r.op->dtype == t && r.evec, DMRG_ERR_ARG, "serve eigs: one dtype, outputs");
+    if(t == DMRG_FP64) serve_impl<double>(req, p, budget, max_batch, out, sum, s);
+    else if(t == DMRG_CX64) serve_impl<cpx<double>>(req, p, budget, max_batch, out, sum, s);
+    else throw Error(DMRG_ERR_ARG, "serve eigs: FP64 or CX64 operators");
 }
 
 } // namespace dmrg

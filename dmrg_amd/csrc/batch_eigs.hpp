@@ -19,8 +19,30 @@ struct BatchEigsResult {
 
 // x0s[i]: optional start vector (device, may be null); evecs[i]: device output (n_i elements). p.nev must be 1,
 // p.target SA or LA, p.precond 0. Converged problems leave the batch; the grouped launches are rebuilt when
-// the active set has halved.
+// the set changes.
 void eigs_batched(MatVecOp *const *ops, int k, const EigsParams &p, const void *const *x0s, void *const *evecs,
                   std::vector<BatchEigsResult> &res, hipStream_t s);
+
+// Continuous batching of eigensolve requests: a request joins the running batch between steps once it has
+// arrived (arrival_s, seconds after the call) and its working memory fits in the budget (and the batch has
+// fewer than max_batch problems, when max_batch > 0); it leaves when converged.
+struct EigRequest {
+    MatVecOp   *op   = nullptr;
+    const void *x0   = nullptr;
+    void       *evec = nullptr;
+    double      arrival_s = 0.0;
+};
+struct EigRequestResult {
+    BatchEigsResult res;
+    double          arrival_s = 0, start_s = 0, finish_s = 0, latency_s = 0;
+    int64_t         footprint_bytes = 0;
+};
+struct ServeSummary {
+    double  makespan_s          = 0;
+    int64_t peak_admitted_bytes = 0;
+    int     peak_concurrency    = 0;
+};
+void serve_eigs(const std::vector<EigRequest> &req, const EigsParams &p, int64_t budget, int max_batch,
+                std::vector<EigRequestResult> &out, ServeSummary &sum, hipStream_t s);
 
 } // namespace dmrg

@@ -529,20 +529,39 @@ dmrg_status dmrg_eigs_folded_spectrum(dmrg_matvec *h, dmrg_matvec *h2, const dmr
     });
 }
 
+namespace {
+dmrg::EigsParams batched_params(const dmrg_eigs_config *cfg) {
+    DMRG_REQUIRE(cfg->nev == 1 && cfg->precond == 0 && cfg->mixed == 0 && (cfg->target == DMRG_TARGET_SA || cfg->target == DMRG_TARGET_LA),
+                 DMRG_ERR_UNSUPPORTED, "batched eigs: nev 1, no preconditioner, FP64, target SA or LA");
+    DMRG_REQUIRE(cfg->tol > 0, DMRG_ERR_ARG, "tol must be positive");
+    DMRG_REQUIRE(cfg->ncv <= 0 || (cfg->ncv >= 4 && cfg->ncv <= 64), DMRG_ERR_ARG, "batched eigs: ncv in [4, 64] or 0");
+    EigsParams p;
+    p.nev         = 1;
+    p.ncv         = cfg->ncv;
+    p.target      = cfg->target;
+    p.tol         = std::max(cfg->tol, std::numeric_limits<double>::epsilon());
+    p.max_matvecs = cfg->max_matvecs > 0 ? cfg->max_matvecs : 100000;
+    p.seed        = cfg->seed;
+    return p;
+}
+dmrg_eigs_result batched_result(const BatchEigsResult &r, double sec) {
+    dmrg_eigs_result o{};
+    o.eigval[0]    = r.eigval;
+    o.rnorm[0]     = r.rnorm;
+    o.nconv        = r.nconv;
+    o.iters        = r.iters;
+    o.matvecs_fp64 = r.matvecs;
+    o.anorm        = r.anorm;
+    o.seconds      = sec;
+    return o;
+}
+} // namespace
+
 dmrg_status dmrg_eigs_batched(dmrg_matvec *const *ops, int k, const dmrg_eigs_config *cfg, const void *const *x0s,
                               void *const *evecs, dmrg_eigs_result *results, void *stream) {
     return guard([&] {
         DMRG_REQUIRE(ops && cfg && evecs && results && k >= 1, DMRG_ERR_ARG, "null argument");
-        DMRG_REQUIRE(cfg->nev == 1 && cfg->precond == 0 && cfg->mixed == 0 && (cfg->target == DMRG_TARGET_SA || cfg->target == DMRG_TARGET_LA),
-                     DMRG_ERR_UNSUPPORTED, "batched eigs: nev 1, no preconditioner, FP64, target SA or LA");
-        DMRG_REQUIRE(cfg->tol > 0, DMRG_ERR_ARG, "tol must be positive");
-        EigsParams p;
-        p.nev         = 1;
-        p.ncv         = cfg->ncv;
-        p.target      = cfg->target;
-        p.tol         = std::max(cfg->tol, std::numeric_limits<double>::epsilon());
-        p.max_matvecs = cfg->max_matvecs > 0 ? cfg->max_matvecs : 100000;
-        p.seed        = cfg->seed;
+        const EigsParams p = batched_params(cfg);
         std::vector<MatVecOp *> v(k);
         for(int i = 0; i < k; ++i) {
             DMRG_REQUIRE(ops[i] && evecs[i], DMRG_ERR_ARG, "null operator or output");
@@ -552,16 +571,59 @@ dmrg_status dmrg_eigs_batched(dmrg_matvec *const *ops, int k, const dmrg_eigs_co
         std::vector<BatchEigsResult> r;
         eigs_batched(v.data(), k, p, x0s, evecs, r, static_cast<hipStream_t>(stream));
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        for(int i = 0; i < k; ++i) {
-            results[i]              = dmrg_eigs_result{};
-            results[i].eigval[0]    = r[i].eigval;
-            results[i].rnorm[0]     = r[i].rnorm;
-            results[i].nconv        = r[i].nconv;
-            results[i].iters        = r[i].iters;
-            results[i].matvecs_fp64 = r[i].matvecs;
-            results[i].anorm        = r[i].anorm;
-            results[i].seconds      = sec;
+        for(int i = 0; i < k; ++i) results[i] = batched_result(r[i], sec);
+    });
+}
+
+dmrg_status dmrg_serve_eigs(const dmrg_eigs_config *cfg, const dmrg_eig_request *reqs, int nreq, int64_t hbm_budget_bytes,
+                            int max_batch, dmrg_eig_request_result *out, dmrg_serve_stats *stats, void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(cfg && reqs && out && stats && nreq >= 1, DMRG_ERR_ARG, "null argument");
+        DMRG_REQUIRE(hbm_budget_bytes > 0, DMRG_ERR_ARG, "serve eigs: budget must be positive");
+        const EigsParams        p = batched_params(cfg);
+        std::vector<EigRequest> rq(nreq);
+        for(int i = 0; i < nreq; ++i) {
+            DMRG_REQUIRE(reqs[i].op && reqs[i].evec && reqs[i].arrival_s >= 0, DMRG_ERR_ARG, "serve eigs: bad request");
+            rq[i].op        = reinterpret_cast<MatVecOp *>(reqs[i].op);
+            rq[i].x0        = reqs[i].x0;
+            rq[i].evec      = reqs[i].evec;
+            rq[i].arrival_s = reqs[i].arrival_s;
         }
+        std::vector<EigRequestResult> r;
+        ServeSummary                  sum;
+        serve_eigs(rq, p, hbm_budget_bytes, max_batch, r, sum, static_cast<hipStream_t>(stream));
+        std::vector<double> lat(nreq);
+        double              mean = 0;
+        int                 conv = 0;
+        for(int i = 0; i < nreq; ++i) {
+            out[i]                 = dmrg_eig_request_result{};
+            out[i].res             = batched_result(r[i].res, r[i].finish_s - r[i].start_s);
+            out[i].arrival_s       = r[i].arrival_s;
+            out[i].start_s         = r[i].start_s;
+            out[i].finish_s        = r[i].finish_s;
+            out[i].latency_s       = r[i].latency_s;
+            out[i].footprint_bytes = r[i].footprint_bytes;
+            lat[i]                 = r[i].latency_s;
+            mean += lat[i] / nreq;
+            conv += r[i].res.nconv > 0;
+        }
+        std::sort(lat.begin(), lat.end());
+        auto pct = [&](double q) { // linear interpolation between order statistics, as dmrg_serve
+            const double pos = q * (lat.size() - 1);
+            const size_t lo  = static_cast<size_t>(std::floor(pos)), hi = std::min(lat.size() - 1, lo + 1);
+            return lat[lo] + (pos - lo) * (lat[hi] - lat[lo]);
+        };
+        *stats                     = dmrg_serve_stats{};
+        stats->makespan_s          = sum.makespan_s;
+        stats->solves_per_s        = nreq / std::max(sum.makespan_s, 1e-12);
+        stats->p50_latency_s       = pct(0.5);
+        stats->p99_latency_s       = pct(0.99);
+        stats->mean_latency_s      = mean;
+        stats->completed           = conv;
+        stats->failed              = nreq - conv;
+        stats->peak_admitted_bytes = sum.peak_admitted_bytes;
+        stats->peak_concurrency    = sum.peak_concurrency;
+        for(int i = 0; i < nreq; ++i) stats->matvecs += r[i].res.matvecs;
     });
 }
 

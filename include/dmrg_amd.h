@@ -359,6 +359,31 @@ dmrg_status dmrg_serve(const dmrg_serve_config *cfg, const dmrg_request *reqs, i
 /* Estimated HBM footprint (bytes) that admission charges for a request. */
 int64_t     dmrg_request_footprint(int L, int chi_max, int complex_);
 
+/* Continuous batching of eigensolve requests on resident operators: each request joins the running batch of
+ * dmrg_eigs_batched between two Davidson steps once it has arrived (arrival_s, real time from the call) and its
+ * working vectors (n (2 ncv + 4) elements) fit in hbm_budget_bytes beside those of the running requests, and
+ * while fewer than max_batch problems run (max_batch <= 0: no limit); it leaves when converged (or at
+ * max_matvecs). Requests are admitted in arrival order. Each problem advances exactly as it would alone
+ * (its own basis size, phase and restarts), so a request's result does not depend on what else runs. No
+ * counterpart in the reference (one eigensolve per call, solver_primme.cpp:297-517): for request loops. cfg as in
+ * dmrg_eigs_batched; the operators must stay alive until the call returns. stats: latency percentiles over all
+ * requests; completed counts converged ones. */
+typedef struct dmrg_eig_request {
+    dmrg_matvec *op;
+    const void  *x0;        /* device start vector or NULL (seeded random start) */
+    void        *evec;      /* device output, n elements */
+    double       arrival_s; /* arrival time relative to the start of dmrg_serve_eigs */
+} dmrg_eig_request;
+
+typedef struct dmrg_eig_request_result {
+    dmrg_eigs_result res;
+    double           arrival_s, start_s, finish_s, latency_s; /* latency = finish - arrival */
+    int64_t          footprint_bytes;
+} dmrg_eig_request_result;
+
+dmrg_status dmrg_serve_eigs(const dmrg_eigs_config *cfg, const dmrg_eig_request *reqs, int nreq, int64_t hbm_budget_bytes,
+                            int max_batch, dmrg_eig_request_result *out, dmrg_serve_stats *stats, void *stream);
+
 /* Host dense Hermitian eigensolver used for the eigensolver's Rayleigh-Ritz step (the role LAPACK ?heevd
  * plays inside PRIMME). A, Z: m x m column-major complex interleaved; w ascending. */
 dmrg_status dmrg_host_heev(int m, const double *A, double *w, double *Z);

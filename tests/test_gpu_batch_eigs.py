@@ -92,3 +92,62 @@ def test_batched_single_problem_and_errors():
     finally:
         for op in opsa + opsb:
             op.close()
+
+
+def overlap(a, b):
+    return abs(np.vdot(a, b)) / (np.linalg.norm(a) * np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_serve_eigs_alone_vs_mix(cplx):
+    """Continuous batching: requests arriving over time, joining and leaving a running batch (at most 3 at once),
+    give the eigenpairs each request gets when solved alone (eigenvalues within 1e-10 |A|, eigenvectors with
+    overlap 1 - 1e-9), and the dense ones (the single-problem solver's for the two larger operators)."""
+    print(f"[case] serve_eigs alone-vs-mix cplx={cplx}", file=sys.stderr, flush=True)
+    shapes = SHAPES + [(2, 32, 40, 5, 5), (2, 48, 48, 5, 5)]
+    probs, ops = build(cplx, shapes, seed=40)
+    try:
+        alone = [G.eigs_batched([op], tol=1e-12)[0] for op in ops]
+        arrivals = [0.004 * i for i in range(len(ops))]
+        mix, stats = G.serve_eigs(ops, arrivals=arrivals, max_batch=3, tol=1e-12)
+        assert stats["completed"] == len(ops) and stats["failed"] == 0, stats
+        assert 1 <= stats["peak_concurrency"] <= 3
+        assert stats["p50_latency_s"] <= stats["p99_latency_s"]
+        for i, ((th, W, L, R), (la, va, ia), (lm, vm, im)) in enumerate(zip(probs, alone, mix)):
+            if i < len(SHAPES):
+                wd = np.linalg.eigvalsh(dense(th, W, L, R))
+                ref, anorm = wd[0], np.abs(wd).max()
+            else:
+                w1, _, i1 = ops[i].eigs(nev=1, tol=1e-12)
+                ref, anorm = w1[0], i1["anorm"]
+            assert im["nconv"] == 1
+            assert abs(lm - la) <= 1e-10 * anorm and abs(lm - ref) <= 1e-10 * anorm, (lm, la, ref)
+            assert overlap(va, vm) >= 1 - 1e-9
+            assert im["start_s"] >= im["arrival_s"] - 1e-9 and im["finish_s"] >= im["start_s"]
+            assert abs(im["latency_s"] - (im["finish_s"] - im["arrival_s"])) <= 1e-12
+    finally:
+        for op in ops:
+            op.close()
+
+
+def test_serve_eigs_budget_admission():
+    """The HBM budget bounds the working memory of the running requests: with room for two of the largest
+    requests at a time, no more than that is ever admitted, and every request still completes."""
+    print("[case] serve_eigs budget", file=sys.stderr, flush=True)
+    shapes = [(2, 48, 48, 5, 5)] * 3 + [(2, 16, 16, 5, 5)] * 3
+    probs, ops = build(True, shapes, seed=60)
+    try:
+        out, stats = G.serve_eigs(ops, tol=1e-12)  # all at t = 0, unlimited budget
+        fp = [info["footprint_bytes"] for _, _, info in out]
+        assert stats["peak_concurrency"] == len(ops) and stats["peak_admitted_bytes"] == sum(fp)
+        budget = 2 * max(fp)
+        out2, stats2 = G.serve_eigs(ops, hbm_budget_bytes=budget, tol=1e-12)
+        assert stats2["completed"] == len(ops)
+        assert stats2["peak_admitted_bytes"] <= budget and stats2["peak_concurrency"] < len(ops)
+        for (l1, v1, _), (l2, v2, _) in zip(out, out2):
+            assert abs(l1 - l2) <= 1e-10 * max(1.0, abs(l1)) and overlap(v1, v2) >= 1 - 1e-9
+        with pytest.raises(G.DmrgError):
+            G.serve_eigs(ops[:1], hbm_budget_bytes=0)
+    finally:
+        for op in ops:
+            op.close()
