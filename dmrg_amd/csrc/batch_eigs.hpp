@@ -1,7 +1,7 @@
 // Batched eigensolver for many small H_eff problems: k independent operators (uneven shapes, one dtype) are
-// advanced in lockstep Davidson steps (the method of the single-problem solver: generalized Davidson with thick
-// restart, one wanted pair per problem), so that one iteration of all k problems costs a fixed number of
-// launches and one host synchronisation: the matvecs of all problems run as one grouped launch per contraction
+// advanced together in Davidson steps (the method of the single-problem solver: generalized Davidson with thick
+// restart, one wanted pair per problem; each problem keeps its own basis size), so that one iteration of all k
+// problems costs a fixed number of launches and one host synchronisation: the matvecs of all problems run as one grouped launch per contraction
 // leg (replayed from a HIP graph), and the block-vector work of all problems runs in grouped kernels whose
 // workgroups each serve one problem.
 #pragma once

@@ -233,9 +233,9 @@ dmrg_status dmrg_eigs_folded_spectrum(dmrg_matvec *h, dmrg_matvec *h2, const dmr
                                       dmrg_eigs_result *res, void *stream);
 
 /* Batched eigensolves: the lowest (DMRG_TARGET_SA) or highest (LA) eigenpair of each of k operators of uneven
- * shapes (one dtype, FP64 or CX64), advanced together in lockstep Davidson steps: per step one grouped matvec of
- * all operators (replayed from a HIP graph), grouped block-vector kernels and one host synchronisation for the
- * whole batch; a converged problem leaves the batch. No counterpart in the reference (eig::solver::eigs solves
+ * shapes (one dtype, FP64 or CX64), advanced together in Davidson steps (each problem with its own basis size and
+ * restarts): per step one grouped matvec of all operators (replayed from a HIP graph), grouped block-vector
+ * kernels and one host synchronisation for the whole batch; a converged problem leaves the batch. No counterpart in the reference (eig::solver::eigs solves
  * one operator, solver_primme.cpp:297-517; the method and the basis rule are the same as dmrg_eigs): for request
  * loops with many small resident problems. cfg: nev 1, precond 0, mixed 0, target SA or LA; tol, ncv, max_matvecs,
  * seed as in dmrg_eigs. x0s: NULL or k device start vectors (entries may be NULL); evecs: k device outputs. */
