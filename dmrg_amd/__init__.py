@@ -270,9 +270,10 @@ def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_m
 
 def serve(requests, workers: int = 4, hbm_budget_bytes: int = 64 << 30, mixed: bool = False, complex_: bool = False,
           var_tol: float = 1e-12, eigs_tol: float = 1e-12, max_sweeps: int = 20, two_site_sweeps: int = 10,
-          J: float = 1.0, h_tran: float = 1.0):
+          J: float = 1.0, h_tran: float = 1.0, batch_eigs: bool = False):
     """Run independent ising_tf_rf ground-state requests as they arrive. requests: list of dicts with keys
-    L, chi_max, h_rand, arrival_s, seed. Returns (per-request results, stats)."""
+    L, chi_max, h_rand, arrival_s, seed. With batch_eigs the requests' FP64 energy solves run together in one
+    batched eigensolver (grouped matvec launches). Returns (per-request results, stats)."""
     n = len(requests)
     fields = [np.ascontiguousarray(np.asarray(r["h_rand"], dtype=np.float64)) for r in requests]
     arr = (_lib.Request * max(n, 1))()
@@ -281,7 +282,7 @@ def serve(requests, workers: int = 4, hbm_budget_bytes: int = 64 << 30, mixed: b
         arr[i].h_rand = fields[i].ctypes.data_as(C.POINTER(C.c_double))
         arr[i].arrival_s, arr[i].seed = float(r.get("arrival_s", 0.0)), int(r.get("seed", 0))
     cfg = _lib.ServeConfig(workers, hbm_budget_bytes, int(mixed), int(complex_), var_tol, eigs_tol, max_sweeps,
-                           two_site_sweeps, J, h_tran)
+                           two_site_sweeps, J, h_tran, int(batch_eigs))
     out = (_lib.RequestResult * max(n, 1))()
     st = _lib.ServeStats()
     call("dmrg_serve", C.byref(cfg), arr, n, out, C.byref(st))
@@ -450,6 +451,30 @@ class MatVecMPOS:
         call("dmrg_matvec_multax", self.h, x2.ctypes.data_as(C.c_void_p), x2.shape[0], y.ctypes.data_as(C.c_void_p),
              y.shape[0], x2.shape[1])
         return y.reshape(x.shape, order="F")
+
+    def set_b(self, mpos, envL, envR) -> None:
+        """The B operator of MatVecMPOS(mpos, enva, envb) (matvec_mpos.h:116-120), same layout as the constructor."""
+        mpos = [np.asfortranarray(np.asarray(m, dtype=np.complex128)) for m in mpos]
+        envL = np.asfortranarray(np.asarray(envL, dtype=np.complex128))
+        envR = np.asfortranarray(np.asarray(envR, dtype=np.complex128))
+        ptrs = (C.c_void_p * len(mpos))(*[m.ctypes.data for m in mpos])
+        dims = (C.c_int64 * (4 * len(mpos)))(*[int(x) for m in mpos for x in m.shape])
+        call("dmrg_matvec_set_b", self.h, len(mpos), ptrs, dims, envL.ctypes.data_as(C.c_void_p), _dims(envL.shape),
+             envR.ctypes.data_as(C.c_void_p), _dims(envR.shape))
+
+    def MultBx(self, x: np.ndarray, out: np.ndarray | None = None) -> np.ndarray:
+        """MultBx (matvec_mpos.cpp:986-1019) on host arrays; with no B operator set, out (zeros when not given) is
+        returned unwritten, as the reference leaves its output untouched."""
+        x = np.asfortranarray(np.asarray(x, dtype=self.dtype))
+        x2 = x.reshape(self.rows(), -1, order="F")
+        y = np.zeros_like(x2, order="F") if out is None else np.asfortranarray(out).reshape(x2.shape, order="F")
+        call("dmrg_matvec_multbx", self.h, x2.ctypes.data_as(C.c_void_p), x2.shape[0], y.ctypes.data_as(C.c_void_p),
+             y.shape[0], x2.shape[1])
+        return y.reshape(x.shape, order="F")
+
+    def MultBx_device(self, dx: DeviceArray, dy: DeviceArray, nvec: int = 1, ld: int | None = None, stream=None) -> None:
+        ld = ld or self.rows()
+        call("dmrg_matvec_multbx_device", self.h, dx.ptr, ld, dy.ptr, ld, nvec, stream)
 
     def MultAx_device(self, dx: DeviceArray, dy: DeviceArray, nvec: int = 1, ld: int | None = None, stream=None,
                       fp32: bool = False) -> None:

@@ -64,6 +64,9 @@ SIGNATURES: dict[str, tuple] = {
     "dmrg_matvec_shape_mps": (st, [vp, i64p]),
     "dmrg_matvec_num_mv": (i64, [vp]),
     "dmrg_matvec_multax": (st, [vp, vp, i64, vp, i64, C.c_int]),
+    "dmrg_matvec_set_b": (st, [vp, C.c_int, C.POINTER(vp), i64p, vp, i64p, vp, i64p]),
+    "dmrg_matvec_multbx": (st, [vp, vp, i64, vp, i64, C.c_int]),
+    "dmrg_matvec_multbx_device": (st, [vp, vp, i64, vp, i64, C.c_int, vp]),
     "dmrg_matvec_multax_device": (st, [vp, vp, i64, vp, i64, C.c_int, vp]),
     "dmrg_matvec_multax_device_fp32": (st, [vp, vp, i64, vp, i64, C.c_int, vp]),
     "dmrg_matvec_set_shift": (st, [vp, C.c_double, C.c_double]),
@@ -131,13 +134,14 @@ class RequestResult(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("workers", C.c_int), ("hbm_budget_bytes", C.c_int64), ("mixed", C.c_int), ("complex_", C.c_int),
                 ("var_tol", C.c_double), ("eigs_tol", C.c_double), ("max_sweeps", C.c_int), ("two_site_sweeps", C.c_int),
-                ("J", C.c_double), ("h_tran", C.c_double)]
+                ("J", C.c_double), ("h_tran", C.c_double), ("batch_eigs", C.c_int)]
 
 
 class ServeStats(C.Structure):
     _fields_ = [("makespan_s", C.c_double), ("solves_per_s", C.c_double), ("p50_latency_s", C.c_double),
                 ("p99_latency_s", C.c_double), ("mean_latency_s", C.c_double), ("completed", C.c_int), ("failed", C.c_int),
-                ("peak_admitted_bytes", C.c_int64), ("peak_concurrency", C.c_int), ("matvecs", C.c_int64)]
+                ("peak_admitted_bytes", C.c_int64), ("peak_concurrency", C.c_int), ("matvecs", C.c_int64),
+                ("batch_steps", C.c_int64), ("peak_batch", C.c_int)]
 
 
 class EigRequest(C.Structure):

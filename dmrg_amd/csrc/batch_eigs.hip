@@ -22,6 +22,12 @@
 #include <cmath>
 #include <memory>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <map>
 #include <numeric>
 #include <thread>
 
@@ -276,10 +282,51 @@ __global__ __launch_bounds__(BT) void b_fold(const BProb<T> *pr, const double *p
     if(threadIdx.x == 0) P.io[dst + c] = res[0];
 }
 
+// Device memory recycled by size class (powers of two and 1.5 times them, from 4 KiB): blocks return to the pool
+// instead of hipFree, which would synchronise the whole device (and with it every other stream's work).
+struct DevPool {
+    std::map<size_t, std::vector<void *>> free_;
+    std::vector<void *>                   all_;
+    DevPool() = default;
+    DevPool(const DevPool &) = delete;
+    DevPool &operator=(const DevPool &) = delete;
+    ~DevPool() {
+        for(void *p : all_) (void) hipFree(p);
+    }
+    static size_t cls(size_t b) {
+        for(int k = 12; k < 62; ++k) {
+            const size_t p2 = size_t(1) << k;
+            if(p2 >= b) return p2;
+            if(p2 + p2 / 2 >= b) return p2 + p2 / 2;
+        }
+        return b;
+    }
+    void *get(size_t b, size_t &c) {
+        c       = cls(b);
+        auto &f = free_[c];
+        if(!f.empty()) {
+            void *p = f.back();
+            f.pop_back();
+            return p;
+        }
+        void *p = nullptr;
+        if(hipMalloc(&p, c) != hipSuccess) {
+            (void) hipGetLastError();
+            throw Error(DMRG_ERR_OOM, "batched eigs: device allocation of " + std::to_string(c) + " bytes failed");
+        }
+        all_.push_back(p);
+        return p;
+    }
+    void put(void *p, size_t c) {
+        if(p) free_[c].push_back(p);
+    }
+};
+
 template <typename T>
 struct Slot {
     MatVecOp       *op = nullptr;
-    DevBuf          mem;
+    void           *mem = nullptr, *ws = nullptr; // pooled: vectors (layout in Engine::add), matvec workspace
+    size_t          mem_cls = 0, ws_cls = 0;
     int64_t         n = 0;
     int             m = 0, phase = PH_START, mmax = 4, mmin = 1, opt = 0;
     std::vector<cx> H, Z;
@@ -289,7 +336,9 @@ struct Slot {
     void           *evec = nullptr;
     int             id   = 0;
     int64_t         footprint = 0;
-    T *V() const { return mem.as<T>(); }
+    T *V() const { return static_cast<T *>(mem); }
+    T *restart_tmp() const { return V() + n * (2 * mmax + 4); }                 // n x mmin
+    T *restart_q() const { return restart_tmp() + n * mmin; }                    // mmax x mmin
     cx &Hat(int i, int j) { return H[i + static_cast<size_t>(mmax) * j]; }
 };
 
@@ -311,15 +360,30 @@ struct Engine {
     int                                   nblk_total = 0, maxm = 0, nactive = 0;
     std::unique_ptr<MatvecGroup>          group;
     bool                                  dirty = true;
+    bool                                  capture; // record the grouped matvec into a HIP graph at each rebuild
+    bool                                  eager_retire = false; // rebuild as soon as a problem finishes (its operator may change)
+    DevPool                               pool;
+    DevBuf                                gstore[3]; // grouped GEMM descriptors, one store per leg
+    int64_t                               steps = 0, rebuilds = 0;
 
-    Engine(hipStream_t st, const EigsParams &pp) : s(st), p(pp) {}
+    Engine(hipStream_t st, const EigsParams &pp, bool capture_graph = true) : s(st), p(pp), capture(capture_graph) {}
     ~Engine() {
+        group.reset();
+        if(s) (void) hipStreamSynchronize(s);
+        for(auto &q : slots) release(*q);
         if(hio) (void) hipHostFree(hio);
+    }
+    void release(Slot<T> &S) {
+        pool.put(S.mem, S.mem_cls);
+        pool.put(S.ws, S.ws_cls);
+        S.mem = S.ws = nullptr;
     }
     int k() const { return static_cast<int>(slots.size()); }
 
-    static int64_t footprint_of(const MatVecOp &op, int mmax) {
-        return static_cast<int64_t>(op.D.n()) * (2 * mmax + 4) * static_cast<int64_t>(sizeof(T));
+    // Vectors: V, W (n x mmax), x, r, X, Y, restart temporaries (n x mmin, mmax x mmin); plus the matvec workspace.
+    static int64_t vec_elems(int64_t n, int mmax, int mmin) { return n * (2 * mmax + 4) + n * mmin + int64_t(mmax) * mmin; }
+    static int64_t footprint_of(const MatVecOp &op, int mmax, int mmin) {
+        return vec_elems(op.D.n(), mmax, mmin) * static_cast<int64_t>(sizeof(T)) + static_cast<int64_t>(matvec_group_ws_bytes(op, false));
     }
     static void basis_rule(const EigsParams &p, int64_t n, int &mmax, int &mmin) {
         // Basis rule of the single-problem solver (solver.cpp:12-15, solver_primme.cpp:374).
@@ -338,8 +402,14 @@ struct Engine {
         basis_rule(p, sl->n, sl->mmax, sl->mmin);
         sl->mmax = static_cast<int>(std::min<int64_t>(sl->mmax, sl->n));
         sl->H.assign(static_cast<size_t>(sl->mmax) * sl->mmax, cx(0.0, 0.0));
-        sl->footprint = footprint_of(*op, sl->mmax);
-        sl->mem.reserve(static_cast<size_t>(sl->n) * (2 * sl->mmax + 4) * sizeof(T));
+        sl->footprint = footprint_of(*op, sl->mmax, sl->mmin);
+        sl->mem       = pool.get(static_cast<size_t>(vec_elems(sl->n, sl->mmax, sl->mmin)) * sizeof(T), sl->mem_cls);
+        try {
+            sl->ws = pool.get(matvec_group_ws_bytes(*op, false), sl->ws_cls);
+        } catch(...) {
+            release(*sl);
+            throw;
+        }
         T *X = sl->V() + sl->n * (2 * sl->mmax + 2);
         if(x0) DMRG_HIP_CHECK(hipMemcpyAsync(X, x0, static_cast<size_t>(sl->n) * sizeof(T), hipMemcpyDeviceToDevice, s));
         else random_fill<T>(sl->n, X, p.seed, s);
@@ -354,6 +424,9 @@ struct Engine {
     void rebuild() {
         DMRG_HIP_CHECK(hipStreamSynchronize(s));
         group.reset();
+        ++rebuilds;
+        for(auto &q : slots)
+            if(q->phase == PH_NONE) release(*q);
         slots.erase(std::remove_if(slots.begin(), slots.end(), [](const auto &q) { return q->phase == PH_NONE; }), slots.end());
         hp.assign(k(), BProb<T>{});
         std::vector<int> blkmap;
@@ -388,9 +461,15 @@ struct Engine {
         DMRG_HIP_CHECK(hipStreamSynchronize(s));
         std::vector<MatVecOp *>   ops(k());
         std::vector<const void *> xs(k());
-        std::vector<void *>       ys(k());
-        for(int q = 0; q < k(); ++q) ops[q] = slots[q]->op, xs[q] = hp[q].X, ys[q] = hp[q].Y;
-        group.reset(matvec_group_create(ops.data(), k(), xs.data(), ys.data(), false, true));
+        std::vector<void *>       ys(k()), wss(k());
+        for(int q = 0; q < k(); ++q) ops[q] = slots[q]->op, xs[q] = hp[q].X, ys[q] = hp[q].Y, wss[q] = slots[q]->ws;
+        GroupStorage gs;
+        gs.wss = wss.data();
+        for(int j = 0; j < 3; ++j) {
+            gs.stores[j]      = gstore[j].reserve(grouped_store_bytes(k()));
+            gs.store_bytes[j] = gstore[j].bytes;
+        }
+        group.reset(matvec_group_create(ops.data(), k(), xs.data(), ys.data(), false, capture, &gs));
         dirty = false;
     }
 
@@ -422,6 +501,7 @@ struct Engine {
     // One step of every active problem; returns the problems that finished in it.
     std::vector<Finished> step() {
         if(dirty) rebuild();
+        ++steps;
         std::vector<Finished> done;
         if(nactive == 0) return done;
         // Rayleigh-Ritz of the problems that have a basis.
@@ -506,21 +586,19 @@ struct Engine {
                     if constexpr(bt<T>::cx) Qh[i + static_cast<size_t>(m) * j] = T{z.real(), z.imag()};
                     else Qh[i + static_cast<size_t>(m) * j] = z.real();
                 }
-            DevBuf Qd, tmp;
-            Qd.reserve(Qh.size() * sizeof(T));
-            tmp.reserve(static_cast<size_t>(S.n) * S.mmin * sizeof(T));
-            DMRG_HIP_CHECK(hipMemcpyAsync(Qd.p, Qh.data(), Qh.size() * sizeof(T), hipMemcpyHostToDevice, s));
+            T *Qd = S.restart_q(), *tmp = S.restart_tmp();
+            DMRG_HIP_CHECK(hipMemcpyAsync(Qd, Qh.data(), Qh.size() * sizeof(T), hipMemcpyHostToDevice, s));
             for(T *Bm : {hp[q].V, hp[q].W}) {
                 GemmDesc g;
                 g.dtype = S.op->dtype;
                 g.M = static_cast<int32_t>(S.n), g.N = S.mmin, g.K = m, g.batch = 1;
                 g.A = Bm, g.am = plain(1), g.ak = plain(S.n);
-                g.B = Qd.p, g.bk = plain(1), g.bn = plain(m);
-                g.C = tmp.p, g.cm = plain(1), g.cn = plain(S.n);
+                g.B = Qd, g.bk = plain(1), g.bn = plain(m);
+                g.C = tmp, g.cm = plain(1), g.cn = plain(S.n);
                 gemm(g, s);
-                DMRG_HIP_CHECK(hipMemcpyAsync(Bm, tmp.p, static_cast<size_t>(S.n) * S.mmin * sizeof(T), hipMemcpyDeviceToDevice, s));
+                DMRG_HIP_CHECK(hipMemcpyAsync(Bm, tmp, static_cast<size_t>(S.n) * S.mmin * sizeof(T), hipMemcpyDeviceToDevice, s));
             }
-            DMRG_HIP_CHECK(hipStreamSynchronize(s)); // Qh and the temporaries die here
+            DMRG_HIP_CHECK(hipStreamSynchronize(s)); // Qh dies here
             std::fill(S.H.begin(), S.H.end(), cx(0.0, 0.0));
             for(int j = 0; j < S.mmin; ++j) S.Hat(j, j) = cx(S.w[ord[j]], 0.0);
             S.m     = S.mmin;
@@ -528,7 +606,7 @@ struct Engine {
         }
         if(!done.empty()) {
             DMRG_HIP_CHECK(hipStreamSynchronize(s)); // eigenvectors in place before the problems are reported
-            if(2 * nactive <= k()) dirty = true;
+            if(eager_retire || 2 * nactive <= k()) dirty = true;
         }
         return done;
     }
@@ -567,7 +645,7 @@ void serve_impl(const std::vector<EigRequest> &req, const EigsParams &p, int64_t
             if(r.arrival_s > t) break;
             int mmax, mmin;
             Engine<T>::basis_rule(p, r.op->D.n(), mmax, mmin);
-            const int64_t fp = Engine<T>::footprint_of(*r.op, mmax);
+            const int64_t fp = Engine<T>::footprint_of(*r.op, mmax, mmin);
             if((E.nactive > 0 && admitted + fp > budget) || (max_batch > 0 && E.nactive >= max_batch)) break;
             E.add(r.op, r.x0, r.evec, order[next]);
             EigRequestResult &o = out[order[next]];
@@ -594,9 +672,134 @@ void serve_impl(const std::vector<EigRequest> &req, const EigsParams &p, int64_t
         }
     }
     sum.makespan_s = now();
+    sum.steps      = E.steps;
 }
 
 } // namespace
+
+struct EigService::Impl {
+    struct Req {
+        MatVecOp       *op;
+        const void     *x0;
+        void           *evec;
+        BatchEigsResult res;
+        bool            done   = false;
+        int             status = DMRG_OK;
+        std::string     err;
+    };
+    dmrg_dtype              dtype;
+    EigsParams              p;
+    mutable std::mutex      mu;
+    std::condition_variable cv, cv_done;
+    std::deque<Req *>       queue;
+    bool                    stop = false;
+    Stats                   st;
+    std::thread             th;
+
+    template <typename T>
+    void run() {
+        hipStream_t s = nullptr;
+        if(hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+        std::unordered_map<int, Req *> live;
+        int                            next_id = 0;
+        auto fail_all = [&](int status, const std::string &msg) {
+            std::lock_guard<std::mutex> lk(mu);
+            for(auto &kv : live) kv.second->status = status, kv.second->err = msg, kv.second->done = true;
+            live.clear();
+            cv_done.notify_all();
+        };
+        auto E = std::make_unique<Engine<T>>(s, p, false);
+        E->eager_retire = true;
+        while(true) {
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return stop || !queue.empty() || E->nactive > 0; });
+                if(stop && queue.empty() && E->nactive == 0) break;
+                while(!queue.empty()) {
+                    Req *r = queue.front();
+                    queue.pop_front();
+                    try {
+                        E->add(r->op, r->x0, r->evec, next_id);
+                        live[next_id++] = r;
+                    } catch(const Error &e) {
+                        r->status = e.status, r->err = e.what(), r->done = true;
+                        cv_done.notify_all();
+                    }
+                }
+                st.peak_batch = std::max(st.peak_batch, E->nactive);
+            }
+            try {
+                const auto done = E->step();
+                std::lock_guard<std::mutex> lk(mu);
+                st.steps    = E->steps;
+                st.rebuilds = E->rebuilds;
+                for(const auto &d : done) {
+                    auto it = live.find(d.id);
+                    if(it == live.end()) continue;
+                    it->second->res  = d.res;
+                    it->second->done = true;
+                    live.erase(it);
+                    ++st.solves;
+                }
+                if(!done.empty()) cv_done.notify_all();
+            } catch(const Error &e) {
+                fail_all(e.status, e.what());
+                E.reset();
+                E = std::make_unique<Engine<T>>(s, p, false);
+                E->eager_retire = true;
+            } catch(const std::exception &e) {
+                fail_all(DMRG_ERR_HIP, e.what());
+                E.reset();
+                E = std::make_unique<Engine<T>>(s, p, false);
+                E->eager_retire = true;
+            }
+        }
+        E.reset();
+        if(s) (void) hipStreamDestroy(s);
+    }
+};
+
+EigService::EigService(dmrg_dtype t, const EigsParams &p) : impl_(std::make_unique<Impl>()) {
+    DMRG_REQUIRE(t == DMRG_FP64 || t == DMRG_CX64, DMRG_ERR_ARG, "eigensolve service: FP64 or CX64 operators");
+    DMRG_REQUIRE(std::max(1, p.nev) == 1 && p.precond == 0 && (p.target == TARGET_SA || p.target == TARGET_LA), DMRG_ERR_UNSUPPORTED,
+                 "eigensolve service: one pair per problem at SA or LA, no preconditioner");
+    impl_->dtype = t;
+    impl_->p     = p;
+    Impl *im     = impl_.get();
+    if(t == DMRG_FP64) impl_->th = std::thread([im] { im->run<double>(); });
+    else impl_->th = std::thread([im] { im->run<cpx<double>>(); });
+}
+
+EigService::~EigService() {
+    {
+        std::lock_guard<std::mutex> lk(impl_->mu);
+        impl_->stop = true;
+    }
+    impl_->cv.notify_all();
+    if(impl_->th.joinable()) impl_->th.join();
+}
+
+BatchEigsResult EigService::solve(MatVecOp *op, const void *x0, void *evec) {
+    DMRG_REQUIRE(op && evec && op->dtype == impl_->dtype, DMRG_ERR_ARG, "eigensolve service: bad problem");
+    Impl::Req r{op, x0, evec, {}};
+    {
+        std::lock_guard<std::mutex> lk(impl_->mu);
+        DMRG_REQUIRE(!impl_->stop, DMRG_ERR_ARG, "eigensolve service: stopped");
+        impl_->queue.push_back(&r);
+    }
+    impl_->cv.notify_all();
+    {
+        std::unique_lock<std::mutex> lk(impl_->mu);
+        impl_->cv_done.wait(lk, [&] { return r.done; });
+    }
+    if(r.status != DMRG_OK) throw Error(static_cast<dmrg_status>(r.status), r.err);
+    return r.res;
+}
+
+EigService::Stats EigService::stats() const {
+    std::lock_guard<std::mutex> lk(impl_->mu);
+    return impl_->st;
+}
 
 void eigs_batched(MatVecOp *const *ops, int k, const EigsParams &p, const void *const *x0s, void *const *evecs,
                   std::vector<BatchEigsResult> &res, hipStream_t s) {

@@ -7,6 +7,7 @@
 #pragma once
 #include "eigs.hpp"
 
+#include <memory>
 #include <vector>
 
 namespace dmrg {
@@ -41,8 +42,32 @@ struct ServeSummary {
     double  makespan_s          = 0;
     int64_t peak_admitted_bytes = 0;
     int     peak_concurrency    = 0;
+    int64_t steps               = 0;
 };
 void serve_eigs(const std::vector<EigRequest> &req, const EigsParams &p, int64_t budget, int max_batch,
                 std::vector<EigRequestResult> &out, ServeSummary &sum, hipStream_t s);
+
+// Eigensolve batcher shared by concurrent callers (the sweep workers of a serving loop): solve() blocks the
+// calling thread while its problem runs, on the service's own engine thread and stream, in one batch with the
+// problems of the other callers; problems join between steps and leave as soon as they converge. The operator
+// must not change until solve() returns; x0 must be complete on the device when solve() is called (the caller
+// has synchronised the stream that wrote it); evec is complete on the device when solve() returns.
+class EigService {
+  public:
+    EigService(dmrg_dtype t, const EigsParams &p);
+    ~EigService();
+    EigService(const EigService &)            = delete;
+    EigService &operator=(const EigService &) = delete;
+    BatchEigsResult solve(MatVecOp *op, const void *x0, void *evec);
+    struct Stats {
+        int64_t steps = 0, rebuilds = 0, solves = 0;
+        int     peak_batch = 0;
+    };
+    Stats stats() const;
+    struct Impl;
+
+  private:
+    std::unique_ptr<Impl> impl_;
+};
 
 } // namespace dmrg

@@ -370,6 +370,36 @@ dmrg_status dmrg_matvec_create(dmrg_matvec **op, dmrg_dtype dtype, int nmpo, con
 dmrg_status dmrg_matvec_destroy(dmrg_matvec *op) {
     return guard([&] { delete reinterpret_cast<MatVecOp *>(op); });
 }
+dmrg_status dmrg_matvec_set_b(dmrg_matvec *op, int nmpo, const void *const *mpos, const int64_t *mpo_dims, const void *envL,
+                              const int64_t envL_dims[3], const void *envR, const int64_t envR_dims[3]) {
+    return guard([&] {
+        DMRG_REQUIRE(op, DMRG_ERR_ARG, "null op");
+        auto *a = reinterpret_cast<MatVecOp *>(op);
+        auto  b = std::make_unique<MatVecOp>(a->dtype, nmpo, mpos, mpo_dims, envL, envL_dims, envR, envR_dims);
+        DMRG_REQUIRE(b->D.d == a->D.d && b->D.chiL == a->D.chiL && b->D.chiR == a->D.chiR, DMRG_ERR_DIMS,
+                     "B operator: theta shape differs from the A operator's");
+        a->opB = std::move(b);
+    });
+}
+dmrg_status dmrg_matvec_multbx(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec) {
+    return guard([&] {
+        DMRG_REQUIRE(op && x && y && nvec >= 0, DMRG_ERR_ARG, "bad argument");
+        auto *a = reinterpret_cast<MatVecOp *>(op);
+        if(!a->opB) return;
+        a->opB->multax_host(x, ldx, y, ldy, nvec);
+        a->num_mv += nvec;
+    });
+}
+dmrg_status dmrg_matvec_multbx_device(dmrg_matvec *op, const void *dx, int64_t ldx, void *dy, int64_t ldy, int nvec,
+                                      void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(op && dx && dy && nvec >= 0, DMRG_ERR_ARG, "bad argument");
+        auto *a = reinterpret_cast<MatVecOp *>(op);
+        if(!a->opB) return;
+        a->opB->multax_device(dx, ldx, dy, ldy, nvec, static_cast<hipStream_t>(stream), false);
+        a->num_mv += nvec;
+    });
+}
 int64_t dmrg_matvec_rows(const dmrg_matvec *op) { return op ? reinterpret_cast<const MatVecOp *>(op)->D.n() : -1; }
 dmrg_status dmrg_matvec_shape_mps(const dmrg_matvec *op, int64_t shape[3]) {
     return guard([&] {
@@ -623,6 +653,8 @@ dmrg_status dmrg_serve_eigs(const dmrg_eigs_config *cfg, const dmrg_eig_request 
         stats->failed              = nreq - conv;
         stats->peak_admitted_bytes = sum.peak_admitted_bytes;
         stats->peak_concurrency    = sum.peak_concurrency;
+        stats->batch_steps         = sum.steps;
+        stats->peak_batch          = sum.peak_concurrency;
         for(int i = 0; i < nreq; ++i) stats->matvecs += r[i].res.matvecs;
     });
 }

@@ -121,12 +121,17 @@ struct GroupedGemm {
     int        tiles   = 0;
     void      *descs   = nullptr; // device, ngemm GemmDesc
     int       *starts  = nullptr; // device, ngemm + 1 tile offsets
+    bool       owned   = true;    // descs/starts allocated by grouped_prepare (else inside a caller's store)
 };
 // Tiles of g under configuration cfg.
 int grouped_tiles(const GemmDesc &g, TileCfg cfg);
 // Builds the launch groups of gs; device storage for the descriptors is allocated with hipMalloc and owned by
-// the returned groups (release with grouped_free).
-std::vector<GroupedGemm> grouped_prepare(const std::vector<GemmDesc> &gs, TileCfg cfg);
+// the returned groups (release with grouped_free). With store (a device buffer of store_bytes, grown by the
+// caller to at least grouped_store_bytes(gs.size())), the descriptors are written into it instead and nothing
+// is allocated; the store must not be in use by queued launches.
+std::vector<GroupedGemm> grouped_prepare(const std::vector<GemmDesc> &gs, TileCfg cfg, void *store = nullptr,
+                                         size_t store_bytes = 0);
+size_t                   grouped_store_bytes(size_t ngemm);
 void                     grouped_launch(const GroupedGemm &g, hipStream_t s);
 void                     grouped_free(std::vector<GroupedGemm> &gg);
 int  device_cu_count();

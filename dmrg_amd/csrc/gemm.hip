@@ -708,7 +708,15 @@ int grouped_tiles(const GemmDesc &g, TileCfg cfg) {
     return ((g.M + sh.BM - 1) / sh.BM) * ((g.N + sh.BN - 1) / sh.BN);
 }
 
-std::vector<GroupedGemm> grouped_prepare(const std::vector<GemmDesc> &gs, TileCfg cfg) {
+// Four launch kinds at most, each with ngemm descriptors and ngemm + 1 starts, 256-byte aligned.
+size_t grouped_store_bytes(size_t ngemm) {
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    return 4 * (al(sizeof(GemmDesc) * ngemm) + al(sizeof(int) * (ngemm + 1)));
+}
+
+std::vector<GroupedGemm> grouped_prepare(const std::vector<GemmDesc> &gs, TileCfg cfg, void *store, size_t store_bytes) {
+    DMRG_REQUIRE(!store || store_bytes >= grouped_store_bytes(gs.size()), DMRG_ERR_ARG, "grouped GEMM: descriptor store too small");
+    size_t store_off = 0;
     DMRG_REQUIRE(cfg == TileCfg::T64x64 || cfg == TileCfg::T32x32 || cfg == TileCfg::T16x128, DMRG_ERR_ARG,
                  "grouped GEMM: tile configuration not available");
     const CfgShape sh = cfg_shape(cfg);
@@ -751,8 +759,16 @@ std::vector<GroupedGemm> grouped_prepare(const std::vector<GemmDesc> &gs, TileCf
             st[i + 1] = static_cast<int>(next);
         }
         gg.tiles = st[gg.ngemm];
-        if(hipMalloc(&gg.descs, sizeof(GemmDesc) * gg.ngemm) != hipSuccess ||
-           hipMalloc(reinterpret_cast<void **>(&gg.starts), sizeof(int) * (gg.ngemm + 1)) != hipSuccess) {
+        if(store) {
+            auto al    = [](size_t b) { return (b + 255) / 256 * 256; };
+            char *base = static_cast<char *>(store);
+            gg.owned   = false;
+            gg.descs   = base + store_off;
+            store_off += al(sizeof(GemmDesc) * gg.ngemm);
+            gg.starts = reinterpret_cast<int *>(base + store_off);
+            store_off += al(sizeof(int) * (gg.ngemm + 1));
+        } else if(hipMalloc(&gg.descs, sizeof(GemmDesc) * gg.ngemm) != hipSuccess ||
+                  hipMalloc(reinterpret_cast<void **>(&gg.starts), sizeof(int) * (gg.ngemm + 1)) != hipSuccess) {
             (void) hipGetLastError();
             out.push_back(gg);
             grouped_free(out);
@@ -804,8 +820,8 @@ void grouped_launch(const GroupedGemm &g, hipStream_t s) {
 
 void grouped_free(std::vector<GroupedGemm> &gg) {
     for(auto &g : gg) {
-        if(g.descs) (void) hipFree(g.descs);
-        if(g.starts) (void) hipFree(g.starts);
+        if(g.owned && g.descs) (void) hipFree(g.descs);
+        if(g.owned && g.starts) (void) hipFree(g.starts);
         g.descs  = nullptr;
         g.starts = nullptr;
     }

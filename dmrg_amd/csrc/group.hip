@@ -23,7 +23,13 @@ void MatvecGroup::run(hipStream_t s) {
         for(auto &g : l) grouped_launch(g, s);
 }
 
-MatvecGroup *matvec_group_create(MatVecOp *const *ops, int k, const void *const *xs, void *const *ys, bool fp32, bool capture) {
+size_t matvec_group_ws_bytes(const MatVecOp &op, bool fp32) {
+    const dmrg_dtype t = fp32 ? dtype_single_of(op.dtype) : op.dtype;
+    return static_cast<size_t>(matvec_ws_elems(op.D)) * dtype_size(t);
+}
+
+MatvecGroup *matvec_group_create(MatVecOp *const *ops, int k, const void *const *xs, void *const *ys, bool fp32, bool capture,
+                                 const GroupStorage *storage) {
     DMRG_REQUIRE(k >= 1 && ops && xs && ys, DMRG_ERR_ARG, "matvec group: bad arguments");
     auto                  grp = std::make_unique<MatvecGroup>();
     std::vector<GemmDesc> leg[3];
@@ -38,11 +44,17 @@ MatvecGroup *matvec_group_create(MatVecOp *const *ops, int k, const void *const 
             op.ensure_fp32(nullptr);
             DMRG_HIP_CHECK(hipStreamSynchronize(nullptr));
         }
-        const dmrg_dtype t = fp32 ? dtype_single_of(op.dtype) : op.dtype;
-        grp->ws[i].reserve(static_cast<size_t>(matvec_ws_elems(op.D)) * dtype_size(t));
+        const dmrg_dtype t  = fp32 ? dtype_single_of(op.dtype) : op.dtype;
+        void            *wp = nullptr;
+        if(storage && storage->wss) {
+            wp = storage->wss[i];
+            DMRG_REQUIRE(wp, DMRG_ERR_ARG, "matvec group: null workspace");
+        } else {
+            wp = grp->ws[i].reserve(matvec_group_ws_bytes(op, fp32));
+        }
         GemmDesc g[3];
         matvec_descs(t, op.D, xs[i], op.D.n(), ys[i], op.D.n(), 1, fp32 ? op.mpo32.p : op.mpo.p, fp32 ? op.envL32.p : op.envL_p,
-                     fp32 ? op.envR32.p : op.envR_p, grp->ws[i].p, g);
+                     fp32 ? op.envR32.p : op.envR_p, wp, g);
         for(int j = 0; j < 3; ++j) leg[j].push_back(g[j]);
         grp->flop += op.D.macs() * (dtype_is_complex(t) ? 8.0 : 2.0);
     }
@@ -59,7 +71,7 @@ MatvecGroup *matvec_group_create(MatVecOp *const *ops, int k, const void *const 
             for(const auto &g : leg[j]) tiles += grouped_tiles(g, TileCfg::T64x64);
             cfg = tiles >= 2 * device_cu_count() ? TileCfg::T64x64 : TileCfg::T32x32;
         }
-        grp->legs[j] = grouped_prepare(leg[j], cfg);
+        grp->legs[j] = storage ? grouped_prepare(leg[j], cfg, storage->stores[j], storage->store_bytes[j]) : grouped_prepare(leg[j], cfg);
     }
     if(capture) {
         // Captured on a stream of its own at default priority; the graph holds only kernel nodes.

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <complex>
+#include <memory>
 #include <vector>
 
 namespace dmrg {
@@ -102,6 +103,9 @@ struct MatVecOp {
     void    block_pc_build(double shift, hipStream_t s);
     // y = P x for one vector in the operator's dtype (device buffers of n elements).
     void    block_pc_apply(const void *x, void *y, hipStream_t s);
+    // Second operator for MultBx (mpos_B, envL_B, envR_B; matvec_mpos.h:47), null when none was given.
+    std::unique_ptr<MatVecOp> opB;
+
     ~MatVecOp();
     void time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms[4], hipStream_t s);
 };

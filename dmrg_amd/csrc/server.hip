@@ -8,9 +8,12 @@
 // scale-out is one OS process per disorder seed (slurm/batches_fdmrg.py:8-27); here they share the GPU.
 #include "sweep.hpp"
 
+#include "batch_eigs.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <memory>
 #include <condition_variable>
 #include <mutex>
 #include <thread>
@@ -45,6 +48,16 @@ void serve(const dmrg_serve_config &cfg, const dmrg_request *req, int nreq, dmrg
     std::vector<int> order(nreq);
     for(int i = 0; i < nreq; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return req[a].arrival_s < req[b].arrival_s; });
+
+    // Shared eigensolve batcher: the energy solves of all running requests advance together.
+    std::unique_ptr<EigService> svc;
+    if(cfg.batch_eigs) {
+        EigsParams p;
+        p.target      = TARGET_SA;
+        p.tol         = cfg.eigs_tol > 0 ? cfg.eigs_tol : 1e-12;
+        p.max_matvecs = 20000;
+        svc           = std::make_unique<EigService>(cfg.complex_ ? DMRG_CX64 : DMRG_FP64, p);
+    }
 
     std::mutex              mu;
     std::condition_variable cv;
@@ -110,7 +123,7 @@ void serve(const dmrg_serve_config &cfg, const dmrg_request *req, int nreq, dmrg
             c.seed            = req[idx].seed;
             try {
                 dmrg_fdmrg_result fr;
-                fdmrg_ising(c, fr, s);
+                fdmrg_ising(c, fr, s, svc.get());
                 r.energy      = fr.energy;
                 r.variance    = fr.variance;
                 r.entropy_mid = fr.entropies[c.L / 2];
@@ -168,6 +181,11 @@ void serve(const dmrg_serve_config &cfg, const dmrg_request *req, int nreq, dmrg
     st.solves_per_s        = st.makespan_s > 0 ? st.completed / st.makespan_s : 0;
     st.peak_admitted_bytes = peak;
     st.peak_concurrency    = peak_running;
+    if(svc) {
+        const EigService::Stats bs = svc->stats();
+        st.batch_steps             = bs.steps;
+        st.peak_batch              = bs.peak_batch;
+    }
 }
 
 } // namespace dmrg

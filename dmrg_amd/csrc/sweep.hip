@@ -7,6 +7,8 @@
 //     with H^2 built from energy-shifted MPOs as set_energy_shift_mpo does (MpoSite.cpp:57-69).
 #include "sweep.hpp"
 
+#include "batch_eigs.hpp"
+
 #include "dense.hpp"
 
 #include "eigs.hpp"
@@ -92,6 +94,7 @@ struct Fdmrg {
     double                                    eshift = 0.0;
     bool                                      folded = false; // xDMRG: solves on (H - E_tgt)^2
     dmrg_fdmrg_result                        *res;
+    EigService                               *svc = nullptr; // shared batcher for the iterative FP64 energy solves
     double t_eigs = 0, t_env = 0, t_svd = 0;
     double env_flop = 0; // algorithmic FLOP of the env updates
 
@@ -284,7 +287,15 @@ struct Fdmrg {
             t_eigs += std::chrono::duration<double>(clock::now() - t0).count();
             return {ev, 0.0};
         }
-        if(cfg.mixed && !folded && n >= mixed_min) {
+        if(svc && !folded && !(cfg.mixed && n >= mixed_min)) {
+            // Batched with the solves of the other requests (same method, basis rule and tolerance).
+            DMRG_HIP_CHECK(hipStreamSynchronize(s)); // theta and the operator's blocks are complete
+            const BatchEigsResult r = svc->solve(&op, th, xbuf.p);
+            ev = r.eigval, rn = r.rnorm;
+            st.matvecs = r.matvecs;
+            op.num_mv += r.matvecs;
+            op.flops += static_cast<double>(r.matvecs) * op.D.macs() * (dtype_is_complex(op.dtype) ? 8.0 : 2.0);
+        } else if(cfg.mixed && !folded && n >= mixed_min) {
             EigsStats s1;
             p.inner_max = 20; // FP32 correction solves in the FP64 phase
             st = eigs_mixed<T, std::conditional_t<std::is_same_v<T, double>, float, cpx<float>>>(op, s, p, th, xbuf.as<T>(), &ev, &rn, &s1);
@@ -517,13 +528,15 @@ struct Fdmrg {
 
 } // namespace
 
-void fdmrg_ising(const dmrg_fdmrg_config &cfg, dmrg_fdmrg_result &res, hipStream_t s) {
+void fdmrg_ising(const dmrg_fdmrg_config &cfg, dmrg_fdmrg_result &res, hipStream_t s, EigService *svc) {
     res = dmrg_fdmrg_result{};
     if(cfg.complex_) {
         Fdmrg<cpx<double>> f(cfg, s, &res);
+        f.svc = svc;
         f.run();
     } else {
         Fdmrg<double> f(cfg, s, &res);
+        f.svc = svc;
         f.run();
     }
 }

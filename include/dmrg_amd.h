@@ -122,6 +122,17 @@ dmrg_status dmrg_matvec_create(dmrg_matvec **op, dmrg_dtype dtype, int nmpo, con
                                const int64_t *mpo_dims, const void *envL, const int64_t envL_dims[3],
                                const void *envR, const int64_t envR_dims[3]);
 dmrg_status dmrg_matvec_destroy(dmrg_matvec *op);
+/* The B operator of MatVecMPOS(mpos, enva, envb) (matvec_mpos.h:116-120, members mpos_B, envL_B, envR_B): MPOs
+ * and env blocks in the layout of dmrg_matvec_create, copied to the device; its theta shape must equal op's.
+ * Replaces the envb half of that constructor (matvec_mpos.cpp:40-130). Setting it again replaces it. */
+dmrg_status dmrg_matvec_set_b(dmrg_matvec *op, int nmpo, const void *const *mpos, const int64_t *mpo_dims, const void *envL,
+                              const int64_t envL_dims[3], const void *envR, const int64_t envR_dims[3]);
+/* MultBx on host buffers (matvec_mpos.cpp:986-1019): y[:, i] = B x[:, i] for i < nvec; like the reference it
+ * returns without writing y when no B operator was set, and counts nvec products in num_mv. */
+dmrg_status dmrg_matvec_multbx(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec);
+/* MultBx on device buffers, stream-ordered. */
+dmrg_status dmrg_matvec_multbx_device(dmrg_matvec *op, const void *dx, int64_t ldx, void *dy, int64_t ldy, int nvec,
+                                      void *stream);
 /* MatVecMPOS::rows(): n = prod(d) * chiL * chiR. */
 int64_t     dmrg_matvec_rows(const dmrg_matvec *op);
 /* MatVecMPOS::get_shape_mps(): {prod d, chiL, chiR}. */
@@ -339,6 +350,8 @@ typedef struct dmrg_serve_config {
     double  var_tol, eigs_tol;
     int     max_sweeps, two_site_sweeps;
     double  J, h_tran;
+    int     batch_eigs; /* 1: the requests' iterative FP64 energy solves run together in one batch (grouped matvec
+                           launches, one synchronisation per Davidson step for all of them); 0: each on its own stream */
 } dmrg_serve_config;
 
 typedef struct dmrg_serve_stats {
@@ -349,6 +362,8 @@ typedef struct dmrg_serve_stats {
     int64_t peak_admitted_bytes;
     int     peak_concurrency;
     int64_t matvecs;
+    int64_t batch_steps; /* batched eigensolver steps (one grouped matvec + one synchronisation each); 0 unbatched */
+    int     peak_batch;  /* most eigensolves in one batched step */
 } dmrg_serve_stats;
 
 /* Runs the requests as they arrive (real time), admitting each when a worker and HBM budget are free, and

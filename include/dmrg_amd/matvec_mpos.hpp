@@ -91,6 +91,19 @@ class MatVecMPOS {
         check(dmrg_matvec_create(&op_, dtype_of<T>(), static_cast<int>(p.size()), p.data(), d.data(), envL, dl, envR, dr),
               "MatVecMPOS");
     }
+    // MatVecMPOS(mpos, enva, envb) (matvec_mpos.h:116-120): the A operator as above plus the B operator of MultBx.
+    MatVecMPOS(const std::vector<const std::complex<double> *> &mpos, const std::vector<std::array<long, 4>> &mpo_dims,
+               const std::complex<double> *envL, std::array<long, 3> envL_dims, const std::complex<double> *envR,
+               std::array<long, 3> envR_dims, const std::complex<double> *envL_B, std::array<long, 3> envL_B_dims,
+               const std::complex<double> *envR_B, std::array<long, 3> envR_B_dims)
+        : MatVecMPOS(mpos, mpo_dims, envL, envL_dims, envR, envR_dims) {
+        std::vector<const void *> p(mpos.begin(), mpos.end());
+        std::vector<int64_t>      d;
+        for(const auto &a : mpo_dims) d.insert(d.end(), a.begin(), a.end());
+        const int64_t dl[3] = {envL_B_dims[0], envL_B_dims[1], envL_B_dims[2]};
+        const int64_t dr[3] = {envR_B_dims[0], envR_B_dims[1], envR_B_dims[2]};
+        check(dmrg_matvec_set_b(op_, static_cast<int>(p.size()), p.data(), d.data(), envL_B, dl, envR_B, dr), "MatVecMPOS");
+    }
     ~MatVecMPOS() {
         if(op_) dmrg_matvec_destroy(op_);
     }
@@ -120,6 +133,12 @@ class MatVecMPOS {
         const dmrg_status s  = dmrg_matvec_multax(op_, x, *ldx, y, *ldy, *blockSize);
         t_multAx->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         *err = (s == DMRG_OK) ? 0 : -1;
+    }
+
+    // MultBx (matvec_mpos.cpp:986-1019): the B operator; mps_out is left as it is when there is none.
+    void MultBx(T *mps_in, T *mps_out) { check(dmrg_matvec_multbx(op_, mps_in, rows(), mps_out, rows(), 1), "MultBx"); }
+    void MultBx(void *x, int *ldx, void *y, int *ldy, int *blockSize, primme_params *, int *err) {
+        *err = dmrg_matvec_multbx(op_, x, *ldx, y, *ldy, *blockSize) == DMRG_OK ? 0 : -1;
     }
 
     // Jacobi preconditioner, PRIMME applyPreconditioner callback (matvec_mpos.cpp:1333-1361 with

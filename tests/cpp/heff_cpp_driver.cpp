@@ -43,6 +43,7 @@ static int solver_requirements(M &matrix, void *x, int ld, void *y, int nvec, do
 
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <vector>
 
 using cx = std::complex<double>;
@@ -104,6 +105,22 @@ static int run(FILE *in, FILE *out, const int64_t h[8]) {
     fwrite(ypc.data(), sizeof(T), ypc.size(), out);
     fwrite(ysh.data(), sizeof(T), ysh.size(), out);
     fwrite(&npc, sizeof(int64_t), 1, out);
+    // MultBx: without a B operator the output is left as it is; with B built on the same blocks it equals MultAx.
+    {
+        std::vector<T> yb(n, T(7));
+        op.MultBx(x.data(), yb.data());
+        for(const T &v : yb)
+            if(v != T(7)) return 13;
+        dmrg_amd::MatVecMPOS<T> ab({mpo.data()}, {{DL, DR, d, d}}, eL.data(), {cL, cL, DL}, eR.data(), {cR, cR, DR}, eL.data(),
+                                   {cL, cL, DL}, eR.data(), {cR, cR, DR});
+        std::vector<T> ya(n), yb2(n);
+        ab.MultAx(x.data(), ya.data());
+        ab.MultBx(x.data(), yb2.data());
+        double dmax = 0, amax = 0;
+        for(int64_t i = 0; i < n; ++i) dmax = std::max(dmax, std::abs(ya[i] - yb2[i])), amax = std::max(amax, std::abs(ya[i]));
+        if(dmax > 1e-13 * amax) return 14;
+        if(ab.num_mv() != 2) return 15;
+    }
     // Error path: a dimension mismatch surfaces as dmrg_amd::error with DMRG_ERR_DIMS.
     try {
         dmrg_amd::contraction::matrix_vector_product<T>(y2.data(), x.data(), {d, cL + 1, cR}, mpo_t.data(), {DL, DR, d, d},

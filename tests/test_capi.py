@@ -61,12 +61,12 @@ def test_struct_layouts_match_header(tmp_path):
     from dmrg_amd import _lib
     src = tmp_path / "sizes.c"
     names = ["dmrg_eigs_config", "dmrg_eigs_result", "dmrg_fdmrg_config", "dmrg_fdmrg_result", "dmrg_request",
-             "dmrg_request_result", "dmrg_serve_config", "dmrg_serve_stats"]
+             "dmrg_request_result", "dmrg_serve_config", "dmrg_serve_stats", "dmrg_eig_request", "dmrg_eig_request_result"]
     src.write_text('#include <stdio.h>\n#include "dmrg_amd.h"\nint main(void){\n' +
                    "".join(f'printf("%zu\\n", sizeof({n}));\n' for n in names) + "return 0;}\n")
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
     sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     mirrors = [_lib.EigsConfig, _lib.EigsResult, _lib.FdmrgConfig, _lib.FdmrgResult, _lib.Request, _lib.RequestResult,
-               _lib.ServeConfig, _lib.ServeStats]
+               _lib.ServeConfig, _lib.ServeStats, _lib.EigRequest, _lib.EigRequestResult]
     assert sizes == [C.sizeof(m) for m in mirrors]

@@ -259,3 +259,34 @@ def test_autotuner_caches_plans():
     assert _lib.load().dmrg_autotune_entries() == n
     np.testing.assert_array_equal(a, b)  # same plan -> bitwise identical
     assert rel(a, O.matrix_vector_product(th, W, L, R)) <= TOL64
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128])
+def test_matvec_op_multbx(dtype):
+    """MultBx (matvec_mpos.cpp:986-1019): without a B operator the output is left as it is; with B set on other env
+    blocks (the variance blocks of MatVecMPOS(mpos, enva, envb)) it is the oracle matvec on those blocks, counted
+    in num_mv; a B operator of another theta shape is DMRG_ERR_DIMS."""
+    print(f"[case] multbx {np.dtype(dtype).name}", file=sys.stderr, flush=True)
+    th, W, L, R = random_heff_problem(2, 12, 10, 4, 4, seed=31, complex_=True)
+    _, W2, L2, R2 = random_heff_problem(2, 12, 10, 3, 5, seed=32, complex_=True)
+    if dtype == np.float64:
+        th, W, L, R, W2, L2, R2 = (np.ascontiguousarray(a.real) for a in (th, W, L, R, W2, L2, R2))
+    op = G.MatVecMPOS([W], L, R, dtype=dtype)
+    try:
+        x = th.reshape(-1, order="F").astype(dtype)
+        y0 = np.full_like(x, 3.0)
+        assert np.array_equal(op.MultBx(x, out=y0.copy()), y0)
+        op.set_b([W2], L2, R2)
+        nmv = op.num_mv
+        yb = op.MultBx(x)
+        ref = O.matrix_vector_product(th.astype(dtype), W2.astype(dtype), L2.astype(dtype), R2.astype(dtype)).reshape(-1, order="F")
+        assert np.linalg.norm(yb - ref) <= 1e-12 * np.linalg.norm(ref)
+        assert op.num_mv == nmv + 1
+        ya = op.MultAx(x)
+        refa = O.matrix_vector_product(th.astype(dtype), W.astype(dtype), L.astype(dtype), R.astype(dtype)).reshape(-1, order="F")
+        assert np.linalg.norm(ya - refa) <= 1e-12 * np.linalg.norm(refa)
+        _, W3, L3, R3 = random_heff_problem(2, 12, 11, 3, 3, seed=33, complex_=True)
+        with pytest.raises(DmrgError):
+            op.set_b([W3], L3, R3)
+    finally:
+        op.close()

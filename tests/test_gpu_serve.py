@@ -51,3 +51,22 @@ def test_serve_rejects_oversized_request():
     q = make_requests()[0]
     with pytest.raises(G.DmrgError):
         G.serve([q], workers=1, hbm_budget_bytes=1 << 20)
+
+
+@pytest.mark.parametrize("complex_", [False, True])
+def test_serve_batched_eigensolves_match_alone_and_exact(complex_):
+    """Requests whose energy solves run together in the shared batched eigensolver (grouped matvec launches,
+    problems joining and leaving between Davidson steps) give the energies and entropies of the same requests
+    run alone without batching (|dE| <= 1e-10 |E|, mid-chain entropy within 1e-8) and the free-fermion energy."""
+    import sys
+    print(f"[case] serve batched eigs complex={complex_}", file=sys.stderr, flush=True)
+    reqs = make_requests()
+    res, st = G.serve(reqs, workers=4, batch_eigs=True, complex_=complex_)
+    assert st["completed"] == len(reqs) and st["failed"] == 0, (st, res)
+    assert st["batch_steps"] > 0 and st["peak_batch"] >= 2, st
+    for r, q in zip(res, reqs):
+        e0 = O.ising_free_fermion_ground_energy(1.0, 1.0, q["h_rand"])
+        assert abs(r["energy"] - e0) <= 1e-9 * abs(e0), (r, e0)
+        alone, _ = G.serve([dict(q, arrival_s=0.0)], workers=1, complex_=complex_)
+        assert abs(alone[0]["energy"] - r["energy"]) <= 1e-10 * abs(e0)
+        assert abs(alone[0]["entropy_mid"] - r["entropy_mid"]) <= 1e-8
