@@ -141,7 +141,8 @@ class ServeStats(C.Structure):
     _fields_ = [("makespan_s", C.c_double), ("solves_per_s", C.c_double), ("p50_latency_s", C.c_double),
                 ("p99_latency_s", C.c_double), ("mean_latency_s", C.c_double), ("completed", C.c_int), ("failed", C.c_int),
                 ("peak_admitted_bytes", C.c_int64), ("peak_concurrency", C.c_int), ("matvecs", C.c_int64),
-                ("batch_steps", C.c_int64), ("peak_batch", C.c_int)]
+                ("batch_steps", C.c_int64), ("peak_batch", C.c_int), ("batch_step_s", C.c_double),
+                ("batch_rebuild_s", C.c_double), ("batch_rr_s", C.c_double)]
 
 
 class EigRequest(C.Structure):

@@ -365,6 +365,7 @@ struct Engine {
     DevPool                               pool;
     DevBuf                                gstore[3]; // grouped GEMM descriptors, one store per leg
     int64_t                               steps = 0, rebuilds = 0;
+    double                                t_rebuild = 0, t_rr = 0, t_step = 0; // host seconds: rebuilds, Rayleigh-Ritz, whole steps
 
     Engine(hipStream_t st, const EigsParams &pp, bool capture_graph = true) : s(st), p(pp), capture(capture_graph) {}
     ~Engine() {
@@ -500,8 +501,20 @@ struct Engine {
 
     // One step of every active problem; returns the problems that finished in it.
     std::vector<Finished> step() {
-        if(dirty) rebuild();
+        using clk     = std::chrono::steady_clock;
+        const auto t0 = clk::now();
+        auto       since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
+        if(dirty) {
+            rebuild();
+            t_rebuild += since(t0);
+        }
         ++steps;
+        struct StepTimer {
+            double &acc;
+            clk::time_point t;
+            ~StepTimer() { acc += std::chrono::duration<double>(clk::now() - t).count(); }
+        } step_timer{t_step, t0};
+        const auto t_rr0 = clk::now();
         std::vector<Finished> done;
         if(nactive == 0) return done;
         // Rayleigh-Ritz of the problems that have a basis.
@@ -524,6 +537,7 @@ struct Engine {
             }
             io[IO_THETA] = S.w[S.opt];
         }
+        t_rr += since(t_rr0);
         upload_phases();
         DMRG_HIP_CHECK(hipMemcpyAsync(dio.p, hio, sizeof(double) * IO_SIZE * k(), hipMemcpyHostToDevice, s));
         stage(ST_RITZ);
@@ -731,8 +745,11 @@ struct EigService::Impl {
             try {
                 const auto done = E->step();
                 std::lock_guard<std::mutex> lk(mu);
-                st.steps    = E->steps;
-                st.rebuilds = E->rebuilds;
+                st.steps     = E->steps;
+                st.rebuilds  = E->rebuilds;
+                st.t_rebuild = E->t_rebuild;
+                st.t_rr      = E->t_rr;
+                st.t_step    = E->t_step;
                 for(const auto &d : done) {
                     auto it = live.find(d.id);
                     if(it == live.end()) continue;

@@ -62,6 +62,7 @@ class EigService {
     struct Stats {
         int64_t steps = 0, rebuilds = 0, solves = 0;
         int     peak_batch = 0;
+        double  t_rebuild = 0, t_rr = 0, t_step = 0; // engine-thread seconds: rebuilds, Rayleigh-Ritz, whole steps
     };
     Stats stats() const;
     struct Impl;

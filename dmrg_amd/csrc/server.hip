@@ -185,6 +185,9 @@ void serve(const dmrg_serve_config &cfg, const dmrg_request *req, int nreq, dmrg
         const EigService::Stats bs = svc->stats();
         st.batch_steps             = bs.steps;
         st.peak_batch              = bs.peak_batch;
+        st.batch_step_s            = bs.t_step;
+        st.batch_rebuild_s         = bs.t_rebuild;
+        st.batch_rr_s              = bs.t_rr;
     }
 }
 

@@ -364,6 +364,8 @@ typedef struct dmrg_serve_stats {
     int64_t matvecs;
     int64_t batch_steps; /* batched eigensolver steps (one grouped matvec + one synchronisation each); 0 unbatched */
     int     peak_batch;  /* most eigensolves in one batched step */
+    double  batch_step_s, batch_rebuild_s, batch_rr_s; /* batcher thread: time in steps, in rebuilds of the
+                                                          grouped launches, in the host Rayleigh-Ritz solves */
 } dmrg_serve_stats;
 
 /* Runs the requests as they arrive (real time), admitting each when a worker and HBM budget are free, and
