@@ -677,6 +677,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="matvec only (no eigensolve, sweep or serving sections)")
     ap.add_argument("--two-site", action="store_true", help="only the two-site chi=1024 section, then exit")
     ap.add_argument("--profile-matvec", action="store_true", help="only the chi=512 FP64 matvec steps, then exit")
+    ap.add_argument("--fp32", action="store_true", help="with --profile-matvec: the FP32 matrix-core matvec instead")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="one 256 MiB device-to-device copy (scales the HBM byte counters of a profiler run), then exit")
     ap.add_argument("--profile-eigs", choices=("fp64", "mixed"),
@@ -688,8 +689,9 @@ def main():
         print(json.dumps(section_two_site()))
         return
     if args.profile_matvec:
-        # Only the chi=512 FP64 matvec steps (for profiler passes); pair with DMRG_TUNE_CACHE to reuse plans.
-        r = gpu_matvec_bench(args.chi, args.steps, args.warmup, legs_nrep=1)
+        # Only the chi=512 matvec steps (FP64, or FP32 with --fp32) for profiler passes; pair with DMRG_TUNE_CACHE
+        # to reuse plans.
+        r = gpu_matvec_bench(args.chi, args.steps, args.warmup, fp32=args.fp32, legs_nrep=1)
         print(json.dumps({"profile_matvec_ms": round(r["ms_per_step"], 5), "legs_ms": r["legs_ms"]}))
         return
     if args.profile_eigs:
