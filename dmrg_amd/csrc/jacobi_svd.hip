@@ -4,9 +4,10 @@
 // Jacobi keeps high relative accuracy for small singular values.
 //
 // Each sweep runs the n(n-1)/2 pairs in n-1 rounds of disjoint pairs (circle method). When W and V fit in
-// LDS, one workgroup runs every sweep with both matrices resident and a barrier between rounds. Larger
-// problems launch one kernel per round with one workgroup (1 to 4 wavefronts, by column length) per pair; the
-// only host round trip per sweep then reads whether any rotation was applied.
+// LDS, one workgroup runs every sweep with both matrices resident and a barrier between rounds. Mid-size
+// problems launch one kernel per round with one workgroup (1 to 4 wavefronts, by column length) per pair; large
+// ones pair blocks of 16 columns instead (block one-sided Jacobi, three launches per round of block pairs). The
+// only host round trip per sweep reads whether any rotation was applied.
 #include "svd.hpp"
 
 #include <algorithm>
@@ -113,6 +114,242 @@ __global__ __launch_bounds__(NT) void jacobi_round(int m, int n, int np, int r, 
     };
     rotate(wp, wq, m);
     rotate(V + static_cast<int64_t>(p) * n, V + static_cast<int64_t>(q) * n, n);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Block one-sided Jacobi: columns in blocks of BB = B2 / 2; each round pairs the blocks (circle method) and,
+// for every block pair, (1) forms the Gram matrix G = X^H X of its B2 columns X (partial sums over row chunks,
+// bj_gram), (2) diagonalises G with cyclic two-sided Jacobi in LDS, accumulating Q (bj_eig; the 2 x 2 rotation
+// and its threshold are those of jacobi_round, computed from the same inner products), (3) rotates X <- X Q and
+// V[:, cols] <- V[:, cols] Q (bj_update). A sweep meets every column pair, as the scalar sweep does, in
+// n / BB - 1 rounds of 3 launches instead of n - 1 launches.
+
+template <typename T>
+__device__ inline T jconj(const T &a) {
+    if constexpr(jtraits<T>::cx) return T{a.re, -a.im};
+    else return a;
+}
+template <typename T>
+__device__ inline T jmul(const T &a, const T &b) {
+    if constexpr(jtraits<T>::cx) return T{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    else return a * b;
+}
+template <typename T>
+__device__ inline T jzero() {
+    if constexpr(jtraits<T>::cx) return T{0.0, 0.0};
+    else return 0.0;
+}
+template <typename T>
+__device__ inline T jscale(double a, const T &b) {
+    if constexpr(jtraits<T>::cx) return T{a * b.re, a * b.im};
+    else return a * b;
+}
+template <typename T>
+__device__ inline T jadd(const T &a, const T &b) {
+    if constexpr(jtraits<T>::cx) return T{a.re + b.re, a.im + b.im};
+    else return a + b;
+}
+template <typename T>
+__device__ inline T jreal(double a) {
+    if constexpr(jtraits<T>::cx) return T{a, 0.0};
+    else return a;
+}
+template <typename T>
+__device__ inline double jre(const T &a) {
+    if constexpr(jtraits<T>::cx) return a.re;
+    else return a;
+}
+
+// Column of slot k (< 2 BB) of the block pair (P, Q): slots [0, BB) hold block P, [BB, 2 BB) block Q; -1 past
+// the last column (ragged last block).
+template <int BB>
+__device__ inline int bj_col(int n, int P, int Q, int k) {
+    const int c = k < BB ? P * BB + k : Q * BB + (k - BB);
+    return c < n ? c : -1;
+}
+
+constexpr int BJ_THREADS = 256;
+constexpr int BJ_ROWS    = 64; // rows per LDS tile in the Gram kernel
+
+// Partial Gram matrices: workgroup (pair, chunk) sums conj(X[r, i]) X[r, j] over its rows into
+// part[(pair * ks + chunk)][i + B2 j].
+template <typename T, int B2>
+__global__ __launch_bounds__(BJ_THREADS) void bj_gram(int m, int n, int nbp, int r, const T *W, int ks, int rows_per, T *part) {
+    constexpr int BB = B2 / 2;
+    int           P, Q;
+    round_pair(nbp, r, blockIdx.x, P, Q);
+    if(P * BB >= n || Q * BB >= n) return; // padding block (uniform)
+    __shared__ T   Xs[BJ_ROWS][B2 + 1];
+    __shared__ int cols[B2];
+    const int      tid = threadIdx.x;
+    if(tid < B2) cols[tid] = bj_col<BB>(n, P, Q, tid);
+    const int r0 = blockIdx.y * rows_per, r1 = min(m, r0 + rows_per);
+    constexpr int JPT = B2 * B2 / BJ_THREADS; // entries per thread
+    const int     i = tid % B2, jb = tid / B2;
+    T             acc[JPT];
+#pragma unroll
+    for(int u = 0; u < JPT; ++u) acc[u] = jzero<T>();
+    __syncthreads();
+    for(int t0 = r0; t0 < r1; t0 += BJ_ROWS) {
+        for(int idx = tid; idx < BJ_ROWS * B2; idx += BJ_THREADS) {
+            const int row = idx % BJ_ROWS, k = idx / BJ_ROWS, c = cols[k];
+            Xs[row][k]    = (c >= 0 && t0 + row < r1) ? W[static_cast<int64_t>(c) * m + t0 + row] : jzero<T>();
+        }
+        __syncthreads();
+        for(int row = 0; row < BJ_ROWS; ++row) {
+            const T xi = jconj(Xs[row][i]);
+#pragma unroll
+            for(int u = 0; u < JPT; ++u) acc[u] = jadd(acc[u], jmul(xi, Xs[row][jb + u * (BJ_THREADS / B2)]));
+        }
+        __syncthreads();
+    }
+    T *out = part + (static_cast<int64_t>(blockIdx.x) * ks + blockIdx.y) * B2 * B2;
+#pragma unroll
+    for(int u = 0; u < JPT; ++u) out[i + B2 * (jb + u * (BJ_THREADS / B2))] = acc[u];
+}
+
+// Sums the partial Gram matrices of a pair (fixed order), diagonalises G by cyclic Jacobi sweeps in LDS and
+// writes the accumulated rotation Q (B2 x B2, column-major) and whether any rotation was applied.
+template <typename T, int B2>
+__global__ __launch_bounds__(BJ_THREADS) void bj_eig(int n, int nbp, int r, int ks, const T *part, T *qbuf, int *rotated,
+                                                      double tol, int max_inner, unsigned *flag) {
+    constexpr int BB = B2 / 2, NP = B2 / 2;
+    int           P, Q;
+    round_pair(nbp, r, blockIdx.x, P, Q);
+    const int tid = threadIdx.x;
+    if(P * BB >= n || Q * BB >= n) {
+        if(tid == 0) rotated[blockIdx.x] = 0;
+        return;
+    }
+    __shared__ T      G[B2][B2 + 1], Qs[B2][B2 + 1];
+    __shared__ double rc[NP], rs[NP];
+    __shared__ T      re[NP];
+    __shared__ int    rp[NP], rq[NP], ron[NP];
+    __shared__ int    any_sweep, any_total;
+    const T          *src = part + static_cast<int64_t>(blockIdx.x) * ks * B2 * B2;
+    for(int idx = tid; idx < B2 * B2; idx += BJ_THREADS) {
+        const int i = idx % B2, j = idx / B2;
+        T         a = jzero<T>();
+        for(int c = 0; c < ks; ++c) a = jadd(a, src[static_cast<int64_t>(c) * B2 * B2 + idx]);
+        G[i][j]  = a;
+        Qs[i][j] = jreal<T>(i == j ? 1.0 : 0.0);
+    }
+    if(tid == 0) any_total = 0;
+    __syncthreads();
+    // Hermitian: keep the upper triangle, real diagonal.
+    for(int idx = tid; idx < B2 * B2; idx += BJ_THREADS) {
+        const int i = idx % B2, j = idx / B2;
+        if(i > j) G[i][j] = jconj(G[j][i]);
+        else if(i == j) G[i][i] = jreal<T>(jre(G[i][i]));
+    }
+    __syncthreads();
+    for(int sw = 0; sw < max_inner; ++sw) {
+        if(tid == 0) any_sweep = 0;
+        __syncthreads();
+        for(int rr = 0; rr < B2 - 1; ++rr) {
+            if(tid < NP) {
+                int p, q;
+                round_pair(B2, rr, tid, p, q);
+                if(p > q) {
+                    const int t = p;
+                    p           = q;
+                    q           = t;
+                }
+                rp[tid]         = p;
+                rq[tid]         = q;
+                const double al = jre(G[p][p]), be = jre(G[q][q]);
+                const T      gm = G[p][q];
+                double       gr, gi;
+                if constexpr(jtraits<T>::cx) gr = gm.re, gi = gm.im;
+                else gr = gm, gi = 0.0;
+                const double g  = sqrt(gr * gr + gi * gi);
+                const bool   on = g > tol * sqrt(al * be) && g != 0.0;
+                ron[tid]        = on ? 1 : 0;
+                if(on) {
+                    const double z = (be - al) / (2.0 * g);
+                    const double t = (z >= 0 ? 1.0 : -1.0) / (fabs(z) + sqrt(1.0 + z * z));
+                    const double c = 1.0 / sqrt(1.0 + t * t);
+                    rc[tid]        = c;
+                    rs[tid]        = c * t;
+                    if constexpr(jtraits<T>::cx) re[tid] = T{gr / g, -gi / g}; // e^{-i phi}
+                    else re[tid] = gr / g;
+                    any_sweep = 1;
+                }
+            }
+            __syncthreads();
+            // rows p, q of G <- J^H G
+            for(int k = tid; k < NP * B2; k += BJ_THREADS) {
+                const int u = k / B2, col = k % B2;
+                if(!ron[u]) continue;
+                const int    p = rp[u], q = rq[u];
+                const double c = rc[u], sn = rs[u];
+                const T      ec = jconj(re[u]);
+                const T      a = G[p][col], b = jmul(ec, G[q][col]);
+                G[p][col]      = jadd(jscale(c, a), jscale(-sn, b));
+                G[q][col]      = jadd(jscale(sn, a), jscale(c, b));
+            }
+            __syncthreads();
+            // columns p, q of G and Q <- . J
+            for(int k = tid; k < 2 * NP * B2; k += BJ_THREADS) {
+                const int u = (k / B2) % NP, row = k % B2;
+                if(!ron[u]) continue;
+                const int    p = rp[u], q = rq[u];
+                const double c = rc[u], sn = rs[u];
+                const bool onG = k < NP * B2;
+                T(*M)[B2 + 1]  = onG ? G : Qs;
+                const T a = M[row][p], b = jmul(re[u], M[row][q]);
+                const T np_ = jadd(jscale(c, a), jscale(-sn, b)), nq_ = jadd(jscale(sn, a), jscale(c, b));
+                // the rotation annihilates G[p][q] and G[q][p]: store the exact zero there
+                M[row][p] = (onG && row == q) ? jzero<T>() : np_;
+                M[row][q] = (onG && row == p) ? jzero<T>() : nq_;
+            }
+            __syncthreads();
+        }
+        const int done = !any_sweep;
+        if(tid == 0 && any_sweep) any_total = 1;
+        __syncthreads();
+        if(done) break;
+    }
+    if(tid == 0) {
+        rotated[blockIdx.x] = any_total;
+        if(any_total) atomicOr(flag, 1u);
+    }
+    if(any_total) {
+        T *qo = qbuf + static_cast<int64_t>(blockIdx.x) * B2 * B2;
+        for(int idx = tid; idx < B2 * B2; idx += BJ_THREADS) qo[idx] = Qs[idx % B2][idx / B2];
+    }
+}
+
+// X <- X Q on the pair's columns of W (chunks [0, cw)) and of V (chunks [cw, ...)), one row per thread.
+template <typename T, int B2>
+__global__ __launch_bounds__(BJ_THREADS) void bj_update(int m, int n, int nbp, int r, T *W, T *V, const T *qbuf, const int *rotated, int cw) {
+    constexpr int BB = B2 / 2;
+    int           P, Q;
+    round_pair(nbp, r, blockIdx.x, P, Q);
+    if(P * BB >= n || Q * BB >= n || !rotated[blockIdx.x]) return;
+    __shared__ T   Qs[B2][B2];
+    __shared__ int cols[B2];
+    const int      tid = threadIdx.x;
+    const T       *qi  = qbuf + static_cast<int64_t>(blockIdx.x) * B2 * B2;
+    for(int idx = tid; idx < B2 * B2; idx += BJ_THREADS) Qs[idx % B2][idx / B2] = qi[idx];
+    if(tid < B2) cols[tid] = bj_col<BB>(n, P, Q, tid);
+    __syncthreads();
+    const bool on_w = static_cast<int>(blockIdx.y) < cw;
+    T         *M    = on_w ? W : V;
+    const int  ld = on_w ? m : n, rows = on_w ? m : n;
+    const int  row = (on_w ? blockIdx.y : blockIdx.y - cw) * BJ_THREADS + tid;
+    if(row >= rows) return;
+    T x[B2];
+#pragma unroll
+    for(int k = 0; k < B2; ++k) x[k] = cols[k] >= 0 ? M[static_cast<int64_t>(cols[k]) * ld + row] : jzero<T>();
+#pragma unroll 4
+    for(int j = 0; j < B2; ++j) {
+        if(cols[j] < 0) continue;
+        T y = jzero<T>();
+#pragma unroll
+        for(int k = 0; k < B2; ++k) y = jadd(y, jmul(x[k], Qs[k][j]));
+        M[static_cast<int64_t>(cols[j]) * ld + row] = y;
+    }
 }
 
 template <int G>
@@ -277,6 +514,13 @@ __global__ void assemble_kernel(int m, int n, const int *perm, const double *S, 
 // LDS budget of the single-workgroup path (gfx950 has 160 KiB per CU).
 constexpr size_t JACOBI_LDS_BYTES = 152 * 1024;
 constexpr int    JACOBI_LDS_MAX_N = 32;
+// Block path from this many columns on (measured on gfx950, complex n x n: equal at 64, scalar rounds 5 % faster at
+// 256, block rounds 1.2x faster at 512 and 3.3x at 2048); B2 columns (two blocks) per inner problem.
+constexpr int JACOBI_BLOCK_MIN_N = 384;
+constexpr int JACOBI_BLOCK_B2    = 32;
+// Inner Jacobi sweeps per block pair and round: one sweep leaves the outer sweep count unchanged (the outer
+// iteration finishes what an inexact inner solve leaves) at a fraction of the cost of converging each one.
+constexpr int JACOBI_BLOCK_INNER = 1;
 
 unsigned grid_of(int64_t total) { return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 8192))); }
 
@@ -321,6 +565,35 @@ void DeviceSvd::jacobi(int m, int n, const T *A, std::vector<double> &S) {
         DMRG_HIP_CHECK(hipStreamSynchronize(s));
         DMRG_REQUIRE(hs < 60, DMRG_ERR_NOT_CONVERGED, "Jacobi SVD: no convergence in 60 sweeps");
         sweep = 60; // skip the per-round path below
+    }
+    // Block path: column blocks of JACOBI_BLOCK_B2 / 2, three launches per round of block pairs.
+    if(!resident && nn >= JACOBI_BLOCK_MIN_N) {
+        constexpr int B2     = JACOBI_BLOCK_B2;
+        const int     nb     = (nn + B2 / 2 - 1) / (B2 / 2);
+        const int     nbp    = nb + (nb & 1);
+        const int     npairs = nbp / 2;
+        const int     ks     = static_cast<int>(std::clamp<int64_t>((512 + npairs - 1) / npairs, 1, std::max(1, mm / BJ_ROWS)));
+        const int     rows_per = ((mm + ks - 1) / ks + BJ_ROWS - 1) / BJ_ROWS * BJ_ROWS;
+        const int     ksn      = (mm + rows_per - 1) / rows_per; // chunks actually holding rows
+        const int     cw = (mm + BJ_THREADS - 1) / BJ_THREADS, cv = (nn + BJ_THREADS - 1) / BJ_THREADS;
+        bj_part.reserve(static_cast<size_t>(npairs) * ksn * B2 * B2 * es);
+        bj_q.reserve(static_cast<size_t>(npairs) * B2 * B2 * es);
+        bj_rot.reserve(static_cast<size_t>(npairs) * sizeof(int));
+        T   *part = bj_part.as<T>(), *qb = bj_q.as<T>();
+        int *rot  = bj_rot.as<int>();
+        for(; sweep < 60; ++sweep) {
+            DMRG_HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(unsigned), s));
+            for(int r = 0; r < nbp - 1; ++r) {
+                hipLaunchKernelGGL((bj_gram<T, B2>), dim3(npairs, ksn), dim3(BJ_THREADS), 0, s, mm, nn, nbp, r, W, ksn, rows_per, part);
+                hipLaunchKernelGGL((bj_eig<T, B2>), dim3(npairs), dim3(BJ_THREADS), 0, s, nn, nbp, r, ksn, part, qb, rot, tol, JACOBI_BLOCK_INNER, flag);
+                hipLaunchKernelGGL((bj_update<T, B2>), dim3(npairs, cw + cv), dim3(BJ_THREADS), 0, s, mm, nn, nbp, r, W, V, qb, rot, cw);
+            }
+            DMRG_HIP_CHECK(hipGetLastError());
+            DMRG_HIP_CHECK(hipMemcpyAsync(&hflag, flag, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+            DMRG_HIP_CHECK(hipStreamSynchronize(s));
+            if(hflag == 0) break;
+        }
+        sweep = 60; // skip the per-pair rounds below
     }
     for(; sweep < 60 && nn > 1; ++sweep) {
         DMRG_HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(unsigned), s));

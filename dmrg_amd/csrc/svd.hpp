@@ -40,6 +40,7 @@ class DeviceSvd {
     hipStream_t s;
     void       *handle = nullptr;
     DevBuf      a, sv, u, vt, e, info, res;
+    DevBuf      bj_part, bj_q, bj_rot; // block Jacobi: partial Gram matrices, rotations, per-pair flags
     int         k_ = 0;
 };
 

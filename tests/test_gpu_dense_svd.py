@@ -149,7 +149,7 @@ def check_svd(A, res, lim, rank_max):
 @pytest.mark.parametrize("method", [0, 3], ids=["gesvd", "jacobi"])
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("m,n", [(1, 1), (1, 7), (9, 1), (2, 2), (30, 30), (31, 31), (50, 32), (33, 70), (64, 40), (37, 90), (300, 200),
-                                 (600, 520), (513, 700)])
+                                 (600, 520), (513, 700), (640, 448)])
 def test_svd_truncate_random(m, n, cplx, method):
     print(f"[case] svd m={m} n={n} cplx={cplx} method={method}", file=sys.stderr, flush=True)
     rng = np.random.default_rng(m * 1000 + n)
@@ -188,3 +188,16 @@ def test_svd_rank_deficient_and_rejected_routines():
         with pytest.raises(DmrgError) as e:
             G.svd_truncate(A, 1e-14, 0, method)
         assert e.value.status == 7
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_svd_block_jacobi_rank_deficient(cplx):
+    """Block pairs with exactly dependent and zero columns (500 x 420 of rank 30, 20 zero columns): the block
+    one-sided Jacobi path returns the same singular values and a valid factorisation as gesvd."""
+    print(f"[case] block jacobi rank deficient cplx={cplx}", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(23)
+    X = rng.standard_normal((500, 30)) + (1j * rng.standard_normal((500, 30)) if cplx else 0)
+    Y = rng.standard_normal((30, 400)) + (1j * rng.standard_normal((30, 400)) if cplx else 0)
+    A = np.asfortranarray(np.hstack([X @ Y, np.zeros((500, 20))]))
+    for method in (0, 3):
+        check_svd(A, G.svd_truncate(A, 1e-10, 0, method), 1e-10, 0)
