@@ -125,6 +125,14 @@ struct GroupedGemm {
 };
 // Tiles of g under configuration cfg.
 int grouped_tiles(const GemmDesc &g, TileCfg cfg);
+
+// Block one-sided Jacobi SVD steps (FP64 or CX64) on W (m x npad, ld m) and V (n x npad, ld n), npad a multiple
+// of 16, for the block pairs of circle-method round r over nbp blocks of 16 columns: partial Gram matrices
+// part[(pair ks + chunk)][32 x 32] over row chunks of rows_per rows (a multiple of 32), and X <- X qbuf[pair] on
+// both matrices for the pairs flagged in rotated.
+void bj_gram_mfma(dmrg_dtype t, int m, int npad, int nbp, int r, const void *W, int ks, int rows_per, void *part, hipStream_t s);
+void bj_update_mfma(dmrg_dtype t, int m, int n, int npad, int nbp, int r, void *W, void *V, const void *qbuf, const int *rotated,
+                    hipStream_t s);
 // Builds the launch groups of gs; device storage for the descriptors is allocated with hipMalloc and owned by
 // the returned groups (release with grouped_free). With store (a device buffer of store_bytes, grown by the
 // caller to at least grouped_store_bytes(gs.size())), the descriptors are written into it instead and nothing

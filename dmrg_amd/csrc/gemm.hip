@@ -1027,4 +1027,100 @@ void gemm(const GemmDesc &gin, hipStream_t stream, TileCfg cfg, int ksplit) {
     launch_plan(g, p, stream);
 }
 
+// ---------------------------------------------------------------------------------------
+// Block one-sided Jacobi SVD steps on the matrix cores. Blocks of 16 columns of W (m x npad, ld m) and V
+// (n x npad, ld n) are paired by the circle method (round r over nbp players, pair = blockIdx.x, the lower block
+// first); a pair's 32 columns are X = [block P, block Q], addressed as one index map with a gap between the two
+// blocks. The descriptors are built in the kernel, so a round costs one launch per step and no host work.
+
+namespace {
+
+__device__ inline void bj_block_pair(int nbp, int r, int i, int &P, int &Q) {
+    if(i == 0) {
+        P = r % (nbp - 1);
+        Q = nbp - 1;
+    } else {
+        P = (r + i) % (nbp - 1);
+        Q = (r - i + (nbp - 1)) % (nbp - 1);
+    }
+    if(P > Q) {
+        const int t = P;
+        P           = Q;
+        Q           = t;
+    }
+}
+
+constexpr uint32_t BJ_PLAIN = 0x7fffffffu;
+
+// Partial Gram matrices G = X^H X over the row chunks [y rows_per, (y + 1) rows_per): part[(pair ks + y)][i + 32 j].
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(256) void bj_gram_kernel(int m, int nb, int nbp, int r, const void *W, int ks, int rows_per, void *part) {
+    int P, Q;
+    bj_block_pair(nbp, r, blockIdx.x, P, Q);
+    if(Q >= nb) return; // the padding block of an odd block count
+    using E = elem_t<R, CPLX>;
+    GemmDesc      g;
+    const int64_t gap = static_cast<int64_t>(Q - P) * 16 * m;
+    g.M = 32, g.N = 32, g.K = m, g.batch = 1;
+    g.A     = static_cast<const E *>(W) + static_cast<int64_t>(P) * 16 * m;
+    g.am    = Idx2{m, gap, 16u};
+    g.ak    = Idx2{1, 0, BJ_PLAIN};
+    g.conjA = CPLX ? 1 : 0;
+    g.B = g.A, g.bk = g.ak, g.bn = g.am;
+    g.a_fast_m = 0, g.b_fast_k = 1, g.xcd_remap = 0;
+    g.ksplit = ks, g.kchunk = rows_per;
+    g.part   = static_cast<E *>(part) + static_cast<int64_t>(blockIdx.x) * ks * 32 * 32;
+    g.C      = g.part;
+    g.cm     = Idx2{1, 0, BJ_PLAIN};
+    g.cn     = Idx2{32, 0, BJ_PLAIN};
+    gemm_body<R, CPLX, 32, 32, 16, 2, 2, false, false>(g, 0, blockIdx.y, 0);
+}
+
+// X <- X Q in place (Q: qbuf[pair], 32 x 32 column-major) for the pairs whose inner solve rotated: row tiles
+// [0, tw) of W, then those of V. A workgroup owns its 64 rows of all 32 columns and reads them (both K stages)
+// before it writes them.
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(256) void bj_update_kernel(int m, int n, int nb, int nbp, int r, void *W, void *V, const void *qbuf,
+                                                        const int *rotated, int tw) {
+    int P, Q;
+    bj_block_pair(nbp, r, blockIdx.x, P, Q);
+    if(Q >= nb || !rotated[blockIdx.x]) return;
+    using E = elem_t<R, CPLX>;
+    const bool    on_w = static_cast<int>(blockIdx.y) < tw;
+    const int     ld   = on_w ? m : n;
+    E            *base = static_cast<E *>(on_w ? W : V) + static_cast<int64_t>(P) * 16 * ld;
+    const int64_t gap  = static_cast<int64_t>(Q - P) * 16 * ld;
+    GemmDesc      g;
+    g.M = ld, g.N = 32, g.K = 32, g.batch = 1;
+    g.A  = base;
+    g.am = Idx2{1, 0, BJ_PLAIN};
+    g.ak = Idx2{ld, gap, 16u};
+    g.B  = static_cast<const E *>(qbuf) + static_cast<int64_t>(blockIdx.x) * 32 * 32;
+    g.bk = Idx2{1, 0, BJ_PLAIN};
+    g.bn = Idx2{32, 0, BJ_PLAIN};
+    g.C  = base;
+    g.cm = Idx2{1, 0, BJ_PLAIN};
+    g.cn = Idx2{ld, gap, 16u};
+    g.a_fast_m = 1, g.b_fast_k = 1, g.xcd_remap = 0, g.ksplit = 1, g.kchunk = 32;
+    gemm_body<R, CPLX, 64, 32, 16, 2, 2, false, false>(g, on_w ? blockIdx.y : blockIdx.y - tw, 0, 0);
+}
+
+} // namespace
+
+void bj_gram_mfma(dmrg_dtype t, int m, int npad, int nbp, int r, const void *W, int ks, int rows_per, void *part, hipStream_t s) {
+    const dim3 grid(nbp / 2, ks), block(256);
+    const int  nb = npad / 16;
+    if(t == DMRG_CX64) hipLaunchKernelGGL((bj_gram_kernel<double, true>), grid, block, 0, s, m, nb, nbp, r, W, ks, rows_per, part);
+    else hipLaunchKernelGGL((bj_gram_kernel<double, false>), grid, block, 0, s, m, nb, nbp, r, W, ks, rows_per, part);
+}
+
+void bj_update_mfma(dmrg_dtype t, int m, int n, int npad, int nbp, int r, void *W, void *V, const void *qbuf, const int *rotated,
+                    hipStream_t s) {
+    const int  tw = (m + 63) / 64, tv = (n + 63) / 64, nb = npad / 16;
+    const dim3 grid(nbp / 2, tw + tv), block(256);
+    if(t == DMRG_CX64)
+        hipLaunchKernelGGL((bj_update_kernel<double, true>), grid, block, 0, s, m, n, nb, nbp, r, W, V, qbuf, rotated, tw);
+    else hipLaunchKernelGGL((bj_update_kernel<double, false>), grid, block, 0, s, m, n, nb, nbp, r, W, V, qbuf, rotated, tw);
+}
+
 } // namespace dmrg

@@ -117,12 +117,13 @@ __global__ __launch_bounds__(NT) void jacobi_round(int m, int n, int np, int r, 
 }
 
 // ------------------------------------------------------------------------------------------------------------
-// Block one-sided Jacobi: columns in blocks of BB = B2 / 2; each round pairs the blocks (circle method) and,
-// for every block pair, (1) forms the Gram matrix G = X^H X of its B2 columns X (partial sums over row chunks,
-// bj_gram), (2) diagonalises G with cyclic two-sided Jacobi in LDS, accumulating Q (bj_eig; the 2 x 2 rotation
-// and its threshold are those of jacobi_round, computed from the same inner products), (3) rotates X <- X Q and
-// V[:, cols] <- V[:, cols] Q (bj_update). A sweep meets every column pair, as the scalar sweep does, in
-// n / BB - 1 rounds of 3 launches instead of n - 1 launches.
+// Block one-sided Jacobi: columns in blocks of BB = B2 / 2 (W and V padded with zero columns to whole blocks);
+// each round pairs the blocks (circle method) and, for every block pair, (1) forms the Gram matrix G = X^H X of
+// its B2 columns X on the matrix cores (partial sums over row chunks, bj_gram_mfma), (2) runs cyclic two-sided
+// Jacobi on G in LDS, accumulating Q (bj_eig; the 2 x 2 rotation and its threshold are those of jacobi_round,
+// computed from the same inner products), (3) rotates X <- X Q and V[:, cols] <- V[:, cols] Q on the matrix cores
+// (bj_update_mfma). A sweep meets every column pair, as the scalar sweep does, in n / BB - 1 rounds of 3
+// launches instead of n - 1 launches.
 
 template <typename T>
 __device__ inline T jconj(const T &a) {
@@ -160,53 +161,8 @@ __device__ inline double jre(const T &a) {
     else return a;
 }
 
-// Column of slot k (< 2 BB) of the block pair (P, Q): slots [0, BB) hold block P, [BB, 2 BB) block Q; -1 past
-// the last column (ragged last block).
-template <int BB>
-__device__ inline int bj_col(int n, int P, int Q, int k) {
-    const int c = k < BB ? P * BB + k : Q * BB + (k - BB);
-    return c < n ? c : -1;
-}
-
 constexpr int BJ_THREADS = 256;
-constexpr int BJ_ROWS    = 64; // rows per LDS tile in the Gram kernel
-
-// Partial Gram matrices: workgroup (pair, chunk) sums conj(X[r, i]) X[r, j] over its rows into
-// part[(pair * ks + chunk)][i + B2 j].
-template <typename T, int B2>
-__global__ __launch_bounds__(BJ_THREADS) void bj_gram(int m, int n, int nbp, int r, const T *W, int ks, int rows_per, T *part) {
-    constexpr int BB = B2 / 2;
-    int           P, Q;
-    round_pair(nbp, r, blockIdx.x, P, Q);
-    if(P * BB >= n || Q * BB >= n) return; // padding block (uniform)
-    __shared__ T   Xs[BJ_ROWS][B2 + 1];
-    __shared__ int cols[B2];
-    const int      tid = threadIdx.x;
-    if(tid < B2) cols[tid] = bj_col<BB>(n, P, Q, tid);
-    const int r0 = blockIdx.y * rows_per, r1 = min(m, r0 + rows_per);
-    constexpr int JPT = B2 * B2 / BJ_THREADS; // entries per thread
-    const int     i = tid % B2, jb = tid / B2;
-    T             acc[JPT];
-#pragma unroll
-    for(int u = 0; u < JPT; ++u) acc[u] = jzero<T>();
-    __syncthreads();
-    for(int t0 = r0; t0 < r1; t0 += BJ_ROWS) {
-        for(int idx = tid; idx < BJ_ROWS * B2; idx += BJ_THREADS) {
-            const int row = idx % BJ_ROWS, k = idx / BJ_ROWS, c = cols[k];
-            Xs[row][k]    = (c >= 0 && t0 + row < r1) ? W[static_cast<int64_t>(c) * m + t0 + row] : jzero<T>();
-        }
-        __syncthreads();
-        for(int row = 0; row < BJ_ROWS; ++row) {
-            const T xi = jconj(Xs[row][i]);
-#pragma unroll
-            for(int u = 0; u < JPT; ++u) acc[u] = jadd(acc[u], jmul(xi, Xs[row][jb + u * (BJ_THREADS / B2)]));
-        }
-        __syncthreads();
-    }
-    T *out = part + (static_cast<int64_t>(blockIdx.x) * ks + blockIdx.y) * B2 * B2;
-#pragma unroll
-    for(int u = 0; u < JPT; ++u) out[i + B2 * (jb + u * (BJ_THREADS / B2))] = acc[u];
-}
+constexpr int BJ_ROWS    = 64; // granularity of the Gram kernel's row chunks
 
 // Sums the partial Gram matrices of a pair (fixed order), diagonalises G by cyclic Jacobi sweeps in LDS and
 // writes the accumulated rotation Q (B2 x B2, column-major) and whether any rotation was applied.
@@ -317,38 +273,6 @@ __global__ __launch_bounds__(BJ_THREADS) void bj_eig(int n, int nbp, int r, int 
     if(any_total) {
         T *qo = qbuf + static_cast<int64_t>(blockIdx.x) * B2 * B2;
         for(int idx = tid; idx < B2 * B2; idx += BJ_THREADS) qo[idx] = Qs[idx % B2][idx / B2];
-    }
-}
-
-// X <- X Q on the pair's columns of W (chunks [0, cw)) and of V (chunks [cw, ...)), one row per thread.
-template <typename T, int B2>
-__global__ __launch_bounds__(BJ_THREADS) void bj_update(int m, int n, int nbp, int r, T *W, T *V, const T *qbuf, const int *rotated, int cw) {
-    constexpr int BB = B2 / 2;
-    int           P, Q;
-    round_pair(nbp, r, blockIdx.x, P, Q);
-    if(P * BB >= n || Q * BB >= n || !rotated[blockIdx.x]) return;
-    __shared__ T   Qs[B2][B2];
-    __shared__ int cols[B2];
-    const int      tid = threadIdx.x;
-    const T       *qi  = qbuf + static_cast<int64_t>(blockIdx.x) * B2 * B2;
-    for(int idx = tid; idx < B2 * B2; idx += BJ_THREADS) Qs[idx % B2][idx / B2] = qi[idx];
-    if(tid < B2) cols[tid] = bj_col<BB>(n, P, Q, tid);
-    __syncthreads();
-    const bool on_w = static_cast<int>(blockIdx.y) < cw;
-    T         *M    = on_w ? W : V;
-    const int  ld = on_w ? m : n, rows = on_w ? m : n;
-    const int  row = (on_w ? blockIdx.y : blockIdx.y - cw) * BJ_THREADS + tid;
-    if(row >= rows) return;
-    T x[B2];
-#pragma unroll
-    for(int k = 0; k < B2; ++k) x[k] = cols[k] >= 0 ? M[static_cast<int64_t>(cols[k]) * ld + row] : jzero<T>();
-#pragma unroll 4
-    for(int j = 0; j < B2; ++j) {
-        if(cols[j] < 0) continue;
-        T y = jzero<T>();
-#pragma unroll
-        for(int k = 0; k < B2; ++k) y = jadd(y, jmul(x[k], Qs[k][j]));
-        M[static_cast<int64_t>(cols[j]) * ld + row] = y;
     }
 }
 
@@ -514,9 +438,10 @@ __global__ void assemble_kernel(int m, int n, const int *perm, const double *S, 
 // LDS budget of the single-workgroup path (gfx950 has 160 KiB per CU).
 constexpr size_t JACOBI_LDS_BYTES = 152 * 1024;
 constexpr int    JACOBI_LDS_MAX_N = 32;
-// Block path from this many columns on (measured on gfx950, complex n x n: equal at 64, scalar rounds 5 % faster at
-// 256, block rounds 1.2x faster at 512 and 3.3x at 2048); B2 columns (two blocks) per inner problem.
-constexpr int JACOBI_BLOCK_MIN_N = 384;
+// Block path from this many columns on (measured on gfx950, complex n x n, ms per SVD, per-pair rounds / block
+// rounds: 64: 2.86 / 2.4, 128: 7.4 / 5.8, 256: 19.5 / 14.4, 512: 61 / 35, 2048: 1325 / 289); B2 columns (two
+// blocks) per inner problem.
+constexpr int JACOBI_BLOCK_MIN_N = 64;
 constexpr int JACOBI_BLOCK_B2    = 32;
 // Inner Jacobi sweeps per block pair and round: one sweep leaves the outer sweep count unchanged (the outer
 // iteration finishes what an inexact inner solve leaves) at a fraction of the cost of converging each one.
@@ -533,8 +458,12 @@ void DeviceSvd::jacobi(int m, int n, const T *A, std::vector<double> &S) {
     const int  mm = tr ? n : m, nn = tr ? m : n;
     k_            = nn;
     const size_t es = sizeof(T);
-    a.reserve(static_cast<size_t>(mm) * nn * es);    // W
-    e.reserve(static_cast<size_t>(nn) * nn * es);    // V (n x n)
+    const size_t lds_need = (static_cast<size_t>(mm) * nn + static_cast<size_t>(nn) * nn) * es;
+    const bool   resident = nn > 1 && nn <= JACOBI_LDS_MAX_N && lds_need <= JACOBI_LDS_BYTES;
+    const bool   blocked  = !resident && nn >= JACOBI_BLOCK_MIN_N;
+    const int    npad     = blocked ? (nn + 15) / 16 * 16 : nn; // whole 16-column blocks (zero columns)
+    a.reserve(static_cast<size_t>(mm) * npad * es);  // W (m x npad)
+    e.reserve(static_cast<size_t>(nn) * npad * es);  // V (n x npad)
     sv.reserve(static_cast<size_t>(nn) * sizeof(double));
     info.reserve(sizeof(unsigned) * 2 + static_cast<size_t>(nn) * sizeof(int));
     res.reserve(static_cast<size_t>(nn) * sizeof(double));
@@ -543,16 +472,19 @@ void DeviceSvd::jacobi(int m, int n, const T *A, std::vector<double> &S) {
     else DMRG_HIP_CHECK(hipMemcpyAsync(W, A, static_cast<size_t>(m) * n * es, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(identity_kernel<T>, dim3(grid_of(static_cast<int64_t>(nn) * nn)), dim3(256), 0, s, nn, V);
     DMRG_HIP_CHECK(hipGetLastError());
+    if(npad > nn) {
+        DMRG_HIP_CHECK(hipMemsetAsync(W + static_cast<int64_t>(mm) * nn, 0, static_cast<size_t>(mm) * (npad - nn) * es, s));
+        DMRG_HIP_CHECK(hipMemsetAsync(V + static_cast<int64_t>(nn) * nn, 0, static_cast<size_t>(nn) * (npad - nn) * es, s));
+    }
     unsigned    *flag = info.as<unsigned>();
     const int    np   = nn + (nn & 1);
     const double tol  = std::sqrt(static_cast<double>(mm)) * 2.3e-16; // gesvj's threshold, sqrt(m) eps
     unsigned     hflag = 0;
     int          sweep = 0;
-    const size_t lds   = (static_cast<size_t>(mm) * nn + static_cast<size_t>(nn) * nn) * es;
+    const size_t lds   = lds_need;
     // A round costs about the same few microseconds inside one workgroup as one launch per round over the
     // whole chip (measured on gfx950), so the resident path pays off only while one workgroup takes a whole
     // round with one wavefront per pair and the launches it saves dominate.
-    const bool resident = nn > 1 && nn <= JACOBI_LDS_MAX_N && lds <= JACOBI_LDS_BYTES;
     if(resident) {
         int *dsweeps = reinterpret_cast<int *>(flag + 1);
         // n <= 32: at most 16 pairs a round, one wavefront (64 lanes) each
@@ -567,15 +499,16 @@ void DeviceSvd::jacobi(int m, int n, const T *A, std::vector<double> &S) {
         sweep = 60; // skip the per-round path below
     }
     // Block path: column blocks of JACOBI_BLOCK_B2 / 2, three launches per round of block pairs.
-    if(!resident && nn >= JACOBI_BLOCK_MIN_N) {
-        constexpr int B2     = JACOBI_BLOCK_B2;
-        const int     nb     = (nn + B2 / 2 - 1) / (B2 / 2);
+    if(blocked) {
+        constexpr int    B2     = JACOBI_BLOCK_B2;
+        static_assert(B2 == 32, "the matrix-core Gram and update kernels take pairs of 16-column blocks");
+        const dmrg_dtype dt     = jtraits<T>::cx ? DMRG_CX64 : DMRG_FP64;
+        const int        nb     = npad / (B2 / 2);
         const int     nbp    = nb + (nb & 1);
         const int     npairs = nbp / 2;
         const int     ks     = static_cast<int>(std::clamp<int64_t>((512 + npairs - 1) / npairs, 1, std::max(1, mm / BJ_ROWS)));
         const int     rows_per = ((mm + ks - 1) / ks + BJ_ROWS - 1) / BJ_ROWS * BJ_ROWS;
         const int     ksn      = (mm + rows_per - 1) / rows_per; // chunks actually holding rows
-        const int     cw = (mm + BJ_THREADS - 1) / BJ_THREADS, cv = (nn + BJ_THREADS - 1) / BJ_THREADS;
         bj_part.reserve(static_cast<size_t>(npairs) * ksn * B2 * B2 * es);
         bj_q.reserve(static_cast<size_t>(npairs) * B2 * B2 * es);
         bj_rot.reserve(static_cast<size_t>(npairs) * sizeof(int));
@@ -584,9 +517,9 @@ void DeviceSvd::jacobi(int m, int n, const T *A, std::vector<double> &S) {
         for(; sweep < 60; ++sweep) {
             DMRG_HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(unsigned), s));
             for(int r = 0; r < nbp - 1; ++r) {
-                hipLaunchKernelGGL((bj_gram<T, B2>), dim3(npairs, ksn), dim3(BJ_THREADS), 0, s, mm, nn, nbp, r, W, ksn, rows_per, part);
-                hipLaunchKernelGGL((bj_eig<T, B2>), dim3(npairs), dim3(BJ_THREADS), 0, s, nn, nbp, r, ksn, part, qb, rot, tol, JACOBI_BLOCK_INNER, flag);
-                hipLaunchKernelGGL((bj_update<T, B2>), dim3(npairs, cw + cv), dim3(BJ_THREADS), 0, s, mm, nn, nbp, r, W, V, qb, rot, cw);
+                bj_gram_mfma(dt, mm, npad, nbp, r, W, ksn, rows_per, part, s);
+                hipLaunchKernelGGL((bj_eig<T, B2>), dim3(npairs), dim3(BJ_THREADS), 0, s, npad, nbp, r, ksn, part, qb, rot, tol, JACOBI_BLOCK_INNER, flag);
+                bj_update_mfma(dt, mm, nn, npad, nbp, r, W, V, qb, rot, s);
             }
             DMRG_HIP_CHECK(hipGetLastError());
             DMRG_HIP_CHECK(hipMemcpyAsync(&hflag, flag, sizeof(unsigned), hipMemcpyDeviceToHost, s));
