@@ -360,7 +360,9 @@ struct Engine {
     int                                   nblk_total = 0, maxm = 0, nactive = 0;
     std::unique_ptr<MatvecGroup>          group;
     bool                                  dirty = true;
-    bool                                  capture; // record the grouped matvec into a HIP graph at each rebuild
+    bool                                  capture; // record each step's launches into a HIP graph at each rebuild
+    hipGraph_t                            step_graph = nullptr;
+    hipGraphExec_t                        step_exec  = nullptr;
     bool                                  eager_retire = false; // rebuild as soon as a problem finishes (its operator may change)
     DevPool                               pool;
     DevBuf                                gstore[3]; // grouped GEMM descriptors, one store per leg
@@ -369,6 +371,7 @@ struct Engine {
 
     Engine(hipStream_t st, const EigsParams &pp, bool capture_graph = true) : s(st), p(pp), capture(capture_graph) {}
     ~Engine() {
+        drop_step_graph();
         group.reset();
         if(s) (void) hipStreamSynchronize(s);
         for(auto &q : slots) release(*q);
@@ -424,6 +427,7 @@ struct Engine {
     // rebuild, which happens when problems join or when half of the set has finished.
     void rebuild() {
         DMRG_HIP_CHECK(hipStreamSynchronize(s));
+        drop_step_graph();
         group.reset();
         ++rebuilds;
         for(auto &q : slots)
@@ -470,7 +474,9 @@ struct Engine {
             gs.stores[j]      = gstore[j].reserve(grouped_store_bytes(k()));
             gs.store_bytes[j] = gstore[j].bytes;
         }
-        group.reset(matvec_group_create(ops.data(), k(), xs.data(), ys.data(), false, capture, &gs));
+        // The grouped matvec goes into the step graph as plain kernel nodes.
+        group.reset(matvec_group_create(ops.data(), k(), xs.data(), ys.data(), false, false, &gs));
+        if(capture) capture_step();
         dirty = false;
     }
 
@@ -484,19 +490,60 @@ struct Engine {
         DMRG_HIP_CHECK(hipMemcpyAsync(dprobs.p, hp.data(), sizeof(BProb<T>) * hp.size(), hipMemcpyHostToDevice, s));
     }
 
-    void stage(int st) {
+    // fold_rows: blocks per problem of the fold (enough for 2 (m + 2) values; the surplus returns at once).
+    void stage(int st, hipStream_t cs, int fold_rows) {
         auto     *P = dprobs.template as<BProb<T>>();
         auto     *M = dblk.template as<int>();
         double   *Q = dpart.template as<double>();
         const dim3 g(nblk_total), b(BT);
         switch(st) {
-            case ST_RITZ: hipLaunchKernelGGL(b_ritz<T>, g, b, 0, s, P, M, Q); break;
+            case ST_RITZ: hipLaunchKernelGGL(b_ritz<T>, g, b, 0, cs, P, M, Q); break;
             case ST_GEMV1:
-            case ST_GEMV2: hipLaunchKernelGGL(b_gemv<T>, g, b, 0, s, P, M, st, Q); break;
-            default: hipLaunchKernelGGL(b_dot<T>, g, b, 0, s, P, M, st, Q); break;
+            case ST_GEMV2: hipLaunchKernelGGL(b_gemv<T>, g, b, 0, cs, P, M, st, Q); break;
+            default: hipLaunchKernelGGL(b_dot<T>, g, b, 0, cs, P, M, st, Q); break;
         }
-        hipLaunchKernelGGL(b_fold<T>, dim3(2 * (maxm + 2), k()), dim3(BT), 0, s, P, Q, st);
+        hipLaunchKernelGGL(b_fold<T>, dim3(fold_rows, k()), dim3(BT), 0, cs, P, Q, st);
         DMRG_HIP_CHECK(hipGetLastError());
+    }
+
+    // The device part of a step: every launch reads the problems' phase, basis size and scalars from device
+    // memory, so the sequence is the same for every step between two rebuilds.
+    void device_step(hipStream_t cs, int fold_rows) {
+        stage(ST_RITZ, cs, fold_rows);
+        stage(ST_DOT1, cs, fold_rows);
+        stage(ST_GEMV1, cs, fold_rows);
+        stage(ST_DOT2, cs, fold_rows);
+        stage(ST_GEMV2, cs, fold_rows);
+        group->run(cs);
+        hipLaunchKernelGGL(b_finish<T>, dim3(nblk_total), dim3(BT), 0, cs, dprobs.template as<BProb<T>>(), dblk.template as<int>());
+        stage(ST_HCOL, cs, fold_rows);
+    }
+
+    void drop_step_graph() {
+        if(step_exec) (void) hipGraphExecDestroy(step_exec);
+        if(step_graph) (void) hipGraphDestroy(step_graph);
+        step_exec  = nullptr;
+        step_graph = nullptr;
+    }
+
+    // Records device_step into a HIP graph (kernel nodes only), replayed by every step until the next rebuild.
+    void capture_step() {
+        hipStream_t cs = nullptr;
+        DMRG_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        try {
+            DMRG_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            device_step(cs, 2 * (BMAX_M + 2));
+            DMRG_HIP_CHECK(hipStreamEndCapture(cs, &step_graph));
+            DMRG_HIP_CHECK(hipGraphInstantiate(&step_exec, step_graph, nullptr, nullptr, 0));
+        } catch(...) {
+            hipGraph_t partial = nullptr;
+            (void) hipStreamEndCapture(cs, &partial); // leave capture mode before the stream goes
+            if(partial) (void) hipGraphDestroy(partial);
+            (void) hipStreamDestroy(cs);
+            drop_step_graph();
+            throw;
+        }
+        DMRG_HIP_CHECK(hipStreamDestroy(cs));
     }
 
     // One step of every active problem; returns the problems that finished in it.
@@ -540,14 +587,8 @@ struct Engine {
         t_rr += since(t_rr0);
         upload_phases();
         DMRG_HIP_CHECK(hipMemcpyAsync(dio.p, hio, sizeof(double) * IO_SIZE * k(), hipMemcpyHostToDevice, s));
-        stage(ST_RITZ);
-        stage(ST_DOT1);
-        stage(ST_GEMV1);
-        stage(ST_DOT2);
-        stage(ST_GEMV2);
-        group->run(s);
-        hipLaunchKernelGGL(b_finish<T>, dim3(nblk_total), dim3(BT), 0, s, dprobs.template as<BProb<T>>(), dblk.template as<int>());
-        stage(ST_HCOL);
+        if(step_exec) DMRG_HIP_CHECK(hipGraphLaunch(step_exec, s));
+        else device_step(s, 2 * (maxm + 2));
         DMRG_HIP_CHECK(hipMemcpyAsync(hio, dio.p, sizeof(double) * IO_SIZE * k(), hipMemcpyDeviceToHost, s));
         DMRG_HIP_CHECK(hipStreamSynchronize(s));
 
