@@ -27,6 +27,7 @@ constexpr int CG_CHECK = 4; // host look at the CG state every CG_CHECK steps
 template <typename T32>
 CorrectionSolver<T32>::CorrectionSolver(MatVecOp &o, hipStream_t st) : op(&o), s(st), n(o.D.n()) {
     scratch.reserve(static_cast<size_t>(vec_scratch_doubles()) * sizeof(double));
+    vec_scratch_init(scratch.as<double>(), s);
     dev.reserve((8 + CG_STATE) * sizeof(double));
     DMRG_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host), CG_STATE * sizeof(double), hipHostMallocDefault));
 }
@@ -76,6 +77,7 @@ int CorrectionSolver<T32>::solve(const T64 *x, const T64 *r, double theta, doubl
 template <typename T>
 Davidson<T>::Davidson(MatVecOp &o, hipStream_t st) : op(&o), s(st), n(o.D.n()) {
     scratch.reserve(static_cast<size_t>(vec_scratch_doubles()) * sizeof(double));
+    vec_scratch_init(scratch.as<double>(), s);
     small.reserve(SB_SIZE * sizeof(double));
     DMRG_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host), SB_SIZE * sizeof(double), hipHostMallocDefault));
 }

@@ -42,6 +42,7 @@ void mixed_block_impl(MatVecOp &h1, MatVecOp &h2, const T *theta, int mode, T *m
     HVb.reserve(3 * static_cast<size_t>(n) * sizeof(T));
     dev.reserve(64 * sizeof(double));
     scr.reserve(static_cast<size_t>(vec_scratch_doubles()) * sizeof(double));
+    vec_scratch_init(scr.as<double>(), s);
     T      *V = Vb.as<T>(), *HV = HVb.as<T>();
     double *d = dev.as<double>(), *sc = scr.as<double>();
     auto    col = [&](T *B, int j) { return B + static_cast<int64_t>(j) * n; };

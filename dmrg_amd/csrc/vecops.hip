@@ -55,6 +55,39 @@ static int grid_for(int64_t n) {
     return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(VEC_MAX_BLOCKS, (n + VT_THREADS - 1) / VT_THREADS)));
 }
 
+// ------------------------------------------------------------------------------------------- fused folds
+// A reduction over few workgroups folds its partial sums in its last workgroup instead of a second launch:
+// every workgroup publishes its partials (release fence) and counts itself in; the one that counts last
+// reads all partials past the non-coherent L1 and sums them in workgroup order (the order of the separate
+// fold kernel's operands, so the result stays deterministic), then resets the counter for the next use.
+constexpr int FUSE_MAX_BLOCKS = 128; // more partials than this per value are folded by a separate launch
+constexpr int FUSE_MAX_VALUES = VT_THREADS;
+
+static unsigned *scratch_counter(double *scratch) { return reinterpret_cast<unsigned *>(scratch + vec_scratch_doubles() - 8); }
+
+void vec_scratch_init(double *scratch, hipStream_t s) {
+    DMRG_HIP_CHECK(hipMemsetAsync(scratch_counter(scratch), 0, 8 * sizeof(double), s));
+}
+
+// partial: [block][nv]; value v goes to out[v], or self_out[v & 1] for the column v / 2 == j_self (when given).
+__device__ inline void fold_in_last_block(const double *partial, int nv, double *out, int j_self, double *self_out, unsigned *counter) {
+    __shared__ int last;
+    __threadfence();
+    __syncthreads();
+    if(threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1 ? 1 : 0;
+    __syncthreads();
+    if(!last) return;
+    __threadfence();
+    for(int v = threadIdx.x; v < nv; v += blockDim.x) {
+        double acc = 0.0;
+        for(unsigned b = 0; b < gridDim.x; ++b)
+            acc += __hip_atomic_load(partial + static_cast<int64_t>(b) * nv + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        double *o = ((v >> 1) == j_self && self_out) ? self_out + (v & 1) : out + v;
+        *o        = acc;
+    }
+    if(threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ------------------------------------------------------------------------------------------- multidot
 constexpr int MD_CH        = 16;  // basis columns per pass (32 double accumulators per thread)
 constexpr int MD_PER_THREAD = 4;  // rows per thread and pass at full grid: amortises the block reduction
@@ -93,7 +126,8 @@ __device__ inline bool rs_owner(int lane) {
 // Columns j < m are V[:, j]; column m (when with_self) is x itself, so x^H x comes out of the same pass.
 template <typename T>
 __global__ __launch_bounds__(VT_THREADS) void multidot_partial(int64_t n, int m, int mc, const T *V, int64_t ldv, const T *x,
-                                                               const int *skip, double *partial) {
+                                                               const int *skip, double *partial, unsigned *counter, double *out,
+                                                               int m_out, double *self_out) {
     if(skip && *skip == 0) return;
     constexpr int     NV = 2 * MD_CH;
     __shared__ double red[NV][VT_THREADS / 64];
@@ -130,6 +164,7 @@ __global__ __launch_bounds__(VT_THREADS) void multidot_partial(int64_t n, int m,
         }
         __syncthreads();
     }
+    if(counter) fold_in_last_block(partial, 2 * mc, out, m_out, self_out, counter);
 }
 
 static int md_grid(int64_t n) {
@@ -161,8 +196,14 @@ void multidot(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_o
     if(m <= 0) return;
     DMRG_REQUIRE(m <= VEC_MAX_M, DMRG_ERR_ARG, "multidot: too many columns");
     const int g = md_grid(n);
-    hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, m, V, ldv, x, nullptr, scratch);
-    hipLaunchKernelGGL(fold_columns, dim3(m), dim3(VT_THREADS), 0, s, g, m, scratch, d_out, -1, nullptr, nullptr);
+    if(g <= FUSE_MAX_BLOCKS && 2 * m <= FUSE_MAX_VALUES) {
+        hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, m, V, ldv, x, nullptr, scratch,
+                           scratch_counter(scratch), d_out, -1, nullptr);
+    } else {
+        hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, m, V, ldv, x, nullptr, scratch, nullptr,
+                           nullptr, -1, nullptr);
+        hipLaunchKernelGGL(fold_columns, dim3(m), dim3(VT_THREADS), 0, s, g, m, scratch, d_out, -1, nullptr, nullptr);
+    }
     DMRG_HIP_CHECK(hipGetLastError());
 }
 
@@ -173,8 +214,14 @@ void multidot_x(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d
     if(mc <= 0) return;
     DMRG_REQUIRE(mc <= VEC_MAX_M, DMRG_ERR_ARG, "multidot: too many columns");
     const int g = md_grid(n);
-    hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, mc, V, ldv, x, skip, scratch);
-    hipLaunchKernelGGL(fold_columns, dim3(mc), dim3(VT_THREADS), 0, s, g, mc, scratch, d_out, d_self ? m : -1, d_self, skip);
+    if(g <= FUSE_MAX_BLOCKS && 2 * mc <= FUSE_MAX_VALUES) {
+        hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, mc, V, ldv, x, skip, scratch,
+                           scratch_counter(scratch), d_out, d_self ? m : -1, d_self);
+    } else {
+        hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, mc, V, ldv, x, skip, scratch, nullptr,
+                           nullptr, -1, nullptr);
+        hipLaunchKernelGGL(fold_columns, dim3(mc), dim3(VT_THREADS), 0, s, g, mc, scratch, d_out, d_self ? m : -1, d_self, skip);
+    }
     DMRG_HIP_CHECK(hipGetLastError());
 }
 
@@ -203,7 +250,8 @@ __global__ __launch_bounds__(VT_THREADS) void gemv_update_kernel(int64_t n, int 
 template <typename T>
 __global__ __launch_bounds__(VT_THREADS) void gemv_update_norm_kernel(int64_t n, int m, const T *V, int64_t ldv, const double *h,
                                                                       double alpha, double beta, const T *x, T *y,
-                                                                      const int *skip, double *partial) {
+                                                                      const int *skip, double *partial, unsigned *counter,
+                                                                      double *out) {
     if(skip && *skip == 0) return;
     __shared__ double hs[2 * VEC_MAX_M];
     __shared__ double red[2][VT_THREADS / 64];
@@ -231,6 +279,7 @@ __global__ __launch_bounds__(VT_THREADS) void gemv_update_norm_kernel(int64_t n,
         partial[2 * blockIdx.x]     = acc[0];
         partial[2 * blockIdx.x + 1] = 0.0;
     }
+    if(counter) fold_in_last_block(partial, 2, out, -1, nullptr, counter);
 }
 
 // flag = 1 when |t|^2 after a projection fell below eta2 times its value before (the DGKS test for a
@@ -253,8 +302,14 @@ void gemv_update_norm(int64_t n, int m, const T *V, int64_t ldv, const double *d
                       double *d_nrm2, const int *skip, double *scratch, hipStream_t s) {
     DMRG_REQUIRE(m <= VEC_MAX_M, DMRG_ERR_ARG, "gemv: too many columns");
     const int g = grid_for(n);
-    hipLaunchKernelGGL(gemv_update_norm_kernel<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, V, ldv, d_h, alpha, beta, x, y, skip, scratch);
-    hipLaunchKernelGGL(fold_columns, dim3(1), dim3(VT_THREADS), 0, s, g, 1, scratch, d_nrm2, -1, nullptr, skip);
+    if(g <= FUSE_MAX_BLOCKS) {
+        hipLaunchKernelGGL(gemv_update_norm_kernel<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, V, ldv, d_h, alpha, beta, x, y, skip, scratch,
+                           scratch_counter(scratch), d_nrm2);
+    } else {
+        hipLaunchKernelGGL(gemv_update_norm_kernel<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, V, ldv, d_h, alpha, beta, x, y, skip, scratch,
+                           nullptr, nullptr);
+        hipLaunchKernelGGL(fold_columns, dim3(1), dim3(VT_THREADS), 0, s, g, 1, scratch, d_nrm2, -1, nullptr, skip);
+    }
     DMRG_HIP_CHECK(hipGetLastError());
 }
 
@@ -266,7 +321,7 @@ void dgks_flag(const double *d_pre, const double *d_post, double eta2, int *d_fl
 // ------------------------------------------------------------------------------------------- Ritz vector + residual
 template <typename T>
 __global__ __launch_bounds__(VT_THREADS) void ritz_kernel(int64_t n, int m, const T *V, const T *W, int64_t ldv, const double *y,
-                                                          double theta, T *x, T *r, double *partial) {
+                                                          double theta, T *x, T *r, double *partial, unsigned *counter, double *out) {
     __shared__ double ys[2 * VEC_MAX_M];
     __shared__ double red[2][VT_THREADS / 64];
     for(int j = threadIdx.x; j < 2 * m; j += VT_THREADS) ys[j] = y[j];
@@ -295,6 +350,7 @@ __global__ __launch_bounds__(VT_THREADS) void ritz_kernel(int64_t n, int m, cons
         partial[2 * blockIdx.x]     = acc[0];
         partial[2 * blockIdx.x + 1] = acc[1];
     }
+    if(counter) fold_in_last_block(partial, 2, out, -1, nullptr, counter);
 }
 
 template <typename T>
@@ -302,8 +358,13 @@ void ritz_residual(int64_t n, int m, const T *V, const T *W, int64_t ld, const d
                    double *d_out, double *scratch, hipStream_t s) {
     DMRG_REQUIRE(m <= VEC_MAX_M, DMRG_ERR_ARG, "ritz: too many columns");
     const int g = grid_for(n);
-    hipLaunchKernelGGL(ritz_kernel<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, V, W, ld, d_y, theta, x, r, scratch);
-    hipLaunchKernelGGL(fold_columns, dim3(1), dim3(VT_THREADS), 0, s, g, 1, scratch, d_out, -1, nullptr, nullptr);
+    if(g <= FUSE_MAX_BLOCKS) {
+        hipLaunchKernelGGL(ritz_kernel<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, V, W, ld, d_y, theta, x, r, scratch,
+                           scratch_counter(scratch), d_out);
+    } else {
+        hipLaunchKernelGGL(ritz_kernel<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, V, W, ld, d_y, theta, x, r, scratch, nullptr, nullptr);
+        hipLaunchKernelGGL(fold_columns, dim3(1), dim3(VT_THREADS), 0, s, g, 1, scratch, d_out, -1, nullptr, nullptr);
+    }
     DMRG_HIP_CHECK(hipGetLastError());
 }
 

@@ -35,10 +35,13 @@ struct vtraits<cpx<float>> {
     static constexpr dmrg_dtype dt = DMRG_CX32;
 };
 
-// Scratch for reductions: partial sums of up to VEC_MAX_BLOCKS blocks x VEC_MAX_M columns (complex).
+// Scratch for reductions: partial sums of up to VEC_MAX_BLOCKS blocks x VEC_MAX_M columns (complex), then a
+// workgroup counter (last 8 doubles) for reductions whose last workgroup folds the partial sums itself. Call
+// vec_scratch_init after (re)allocating a scratch buffer: the counter must start at zero (it resets itself).
 constexpr int VEC_MAX_BLOCKS = 2048; // enough waves in flight to stream the basis at HBM rate
 constexpr int VEC_MAX_M      = 256;
-constexpr int64_t vec_scratch_doubles() { return 2LL * VEC_MAX_BLOCKS * VEC_MAX_M + 4 * VEC_MAX_M; }
+constexpr int64_t vec_scratch_doubles() { return 2LL * VEC_MAX_BLOCKS * VEC_MAX_M + 4 * VEC_MAX_M + 8; }
+void vec_scratch_init(double *scratch, hipStream_t s);
 
 // out[j] = sum_i conj(V[i, j]) x[i], j < m (complex double, out on device, 2*m doubles).
 template <typename T>
