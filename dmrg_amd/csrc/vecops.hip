@@ -78,14 +78,26 @@ __device__ inline void fold_in_last_block(const double *partial, int nv, double 
     __syncthreads();
     if(!last) return;
     __threadfence();
-    for(int v = threadIdx.x; v < nv; v += blockDim.x) {
+    // Thread t sums value t % nv over the workgroups b = t / nv, t / nv + per, ...; then thread v adds the per
+    // sums of value v in order. nv <= blockDim.x.
+    __shared__ double red[VT_THREADS];
+    const int         per = blockDim.x / nv;
+    const int         t   = threadIdx.x;
+    if(t < per * nv) {
+        const int v = t % nv;
+        double    a = 0.0;
+        for(unsigned b = t / nv; b < gridDim.x; b += per)
+            a += __hip_atomic_load(partial + static_cast<int64_t>(b) * nv + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        red[t] = a;
+    }
+    __syncthreads();
+    if(t < nv) {
         double acc = 0.0;
-        for(unsigned b = 0; b < gridDim.x; ++b)
-            acc += __hip_atomic_load(partial + static_cast<int64_t>(b) * nv + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        double *o = ((v >> 1) == j_self && self_out) ? self_out + (v & 1) : out + v;
+        for(int k = 0; k < per; ++k) acc += red[t + k * nv];
+        double *o = ((t >> 1) == j_self && self_out) ? self_out + (t & 1) : out + t;
         *o        = acc;
     }
-    if(threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if(t == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ------------------------------------------------------------------------------------------- multidot
