@@ -125,6 +125,8 @@ struct GroupedGemm {
 };
 // Tiles of g under configuration cfg.
 int grouped_tiles(const GemmDesc &g, TileCfg cfg);
+// Whether GEMM plans are tuned by timing (dmrg_autotune_enable).
+int autotune_enabled();
 
 // Block one-sided Jacobi SVD steps (FP64 or CX64) on W (m x npad, ld m) and V (n x npad, ld n), npad a multiple
 // of 16, for the block pairs of circle-method round r over nbp blocks of 16 columns: partial Gram matrices

@@ -968,6 +968,7 @@ static void append_tune_cache_locked(const std::string &key, const LaunchPlan &p
 }
 
 void set_autotune(int on) { g_autotune = on; }
+int  autotune_enabled() { return g_autotune; }
 void set_plan_override(int cfg, int ks, int nsplit, int ks2) {
     g_override_cfg    = cfg;
     g_override_ks     = ks;

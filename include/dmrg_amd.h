@@ -260,7 +260,9 @@ dmrg_status dmrg_eigs_batched(dmrg_matvec *const *ops, int k, const dmrg_eigs_co
  * small operators resident, where one operator's matvec fills only a few CUs. fp32 = 1 runs the FP32 matrix-core
  * form (xs/ys in the single-precision dtype). capture_graph = 1 records the launches once into a HIP graph that
  * dmrg_matvec_group_run replays. The group holds pointers into the operators' buffers and into xs/ys: keep
- * those (and the operators' shift/MPO state) unchanged while the group is used; destroy it first. */
+ * those (and the operators' shift/MPO state) unchanged while the group is used; destroy it first. With
+ * dmrg_autotune_enable on (the default), creating a group of a new shape class times the tile configurations of
+ * each leg on these operands (so ys receives H_i x_i during creation) and caches the fastest. */
 typedef struct dmrg_matvec_group dmrg_matvec_group;
 dmrg_status dmrg_matvec_group_create(dmrg_matvec *const *ops, int k, const void *const *xs, void *const *ys, int fp32,
                                      int capture_graph, dmrg_matvec_group **out);
