@@ -734,9 +734,9 @@ void serve_impl(const std::vector<EigRequest> &req, const EigsParams &p, int64_t
 
 struct EigService::Impl {
     struct Req {
-        MatVecOp       *op;
-        const void     *x0;
-        void           *evec;
+        MatVecOp       *op   = nullptr;
+        const void     *x0   = nullptr;
+        void           *evec = nullptr;
         BatchEigsResult res;
         bool            done   = false;
         int             status = DMRG_OK;
@@ -839,7 +839,10 @@ EigService::~EigService() {
 
 BatchEigsResult EigService::solve(MatVecOp *op, const void *x0, void *evec) {
     DMRG_REQUIRE(op && evec && op->dtype == impl_->dtype, DMRG_ERR_ARG, "eigensolve service: bad problem");
-    Impl::Req r{op, x0, evec, {}};
+    Impl::Req r;
+    r.op   = op;
+    r.x0   = x0;
+    r.evec = evec;
     {
         std::lock_guard<std::mutex> lk(impl_->mu);
         DMRG_REQUIRE(!impl_->stop, DMRG_ERR_ARG, "eigensolve service: stopped");

@@ -107,6 +107,34 @@ static DeviceSvd &solver_for(hipStream_t s) {
 
 using namespace dmrg;
 
+namespace {
+dmrg::EigsParams batched_params(const dmrg_eigs_config *cfg) {
+    DMRG_REQUIRE(cfg->nev == 1 && cfg->precond == 0 && cfg->mixed == 0 && (cfg->target == DMRG_TARGET_SA || cfg->target == DMRG_TARGET_LA),
+                 DMRG_ERR_UNSUPPORTED, "batched eigs: nev 1, no preconditioner, FP64, target SA or LA");
+    DMRG_REQUIRE(cfg->tol > 0, DMRG_ERR_ARG, "tol must be positive");
+    DMRG_REQUIRE(cfg->ncv <= 0 || (cfg->ncv >= 4 && cfg->ncv <= 64), DMRG_ERR_ARG, "batched eigs: ncv in [4, 64] or 0");
+    EigsParams p;
+    p.nev         = 1;
+    p.ncv         = cfg->ncv;
+    p.target      = cfg->target;
+    p.tol         = std::max(cfg->tol, std::numeric_limits<double>::epsilon());
+    p.max_matvecs = cfg->max_matvecs > 0 ? cfg->max_matvecs : 100000;
+    p.seed        = cfg->seed;
+    return p;
+}
+dmrg_eigs_result batched_result(const BatchEigsResult &r, double sec) {
+    dmrg_eigs_result o{};
+    o.eigval[0]    = r.eigval;
+    o.rnorm[0]     = r.rnorm;
+    o.nconv        = r.nconv;
+    o.iters        = r.iters;
+    o.matvecs_fp64 = r.matvecs;
+    o.anorm        = r.anorm;
+    o.seconds      = sec;
+    return o;
+}
+} // namespace
+
 extern "C" {
 
 const char *dmrg_last_error(void) { return g_last_error.c_str(); }
@@ -559,33 +587,6 @@ dmrg_status dmrg_eigs_folded_spectrum(dmrg_matvec *h, dmrg_matvec *h2, const dmr
     });
 }
 
-namespace {
-dmrg::EigsParams batched_params(const dmrg_eigs_config *cfg) {
-    DMRG_REQUIRE(cfg->nev == 1 && cfg->precond == 0 && cfg->mixed == 0 && (cfg->target == DMRG_TARGET_SA || cfg->target == DMRG_TARGET_LA),
-                 DMRG_ERR_UNSUPPORTED, "batched eigs: nev 1, no preconditioner, FP64, target SA or LA");
-    DMRG_REQUIRE(cfg->tol > 0, DMRG_ERR_ARG, "tol must be positive");
-    DMRG_REQUIRE(cfg->ncv <= 0 || (cfg->ncv >= 4 && cfg->ncv <= 64), DMRG_ERR_ARG, "batched eigs: ncv in [4, 64] or 0");
-    EigsParams p;
-    p.nev         = 1;
-    p.ncv         = cfg->ncv;
-    p.target      = cfg->target;
-    p.tol         = std::max(cfg->tol, std::numeric_limits<double>::epsilon());
-    p.max_matvecs = cfg->max_matvecs > 0 ? cfg->max_matvecs : 100000;
-    p.seed        = cfg->seed;
-    return p;
-}
-dmrg_eigs_result batched_result(const BatchEigsResult &r, double sec) {
-    dmrg_eigs_result o{};
-    o.eigval[0]    = r.eigval;
-    o.rnorm[0]     = r.rnorm;
-    o.nconv        = r.nconv;
-    o.iters        = r.iters;
-    o.matvecs_fp64 = r.matvecs;
-    o.anorm        = r.anorm;
-    o.seconds      = sec;
-    return o;
-}
-} // namespace
 
 dmrg_status dmrg_eigs_batched(dmrg_matvec *const *ops, int k, const dmrg_eigs_config *cfg, const void *const *x0s,
                               void *const *evecs, dmrg_eigs_result *results, void *stream) {
