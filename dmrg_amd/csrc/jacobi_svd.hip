@@ -487,10 +487,17 @@ void DeviceSvd::jacobi(int m, int n, const T *A, std::vector<double> &S) {
     // round with one wavefront per pair and the launches it saves dominate.
     if(resident) {
         int *dsweeps = reinterpret_cast<int *>(flag + 1);
-        // n <= 32: at most 16 pairs a round, one wavefront (64 lanes) each
-        DMRG_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&jacobi_lds<T, 64>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           static_cast<int>(JACOBI_LDS_BYTES)));
-        hipLaunchKernelGGL((jacobi_lds<T, 64>), dim3(1), dim3(1024), lds, s, mm, nn, np, W, V, tol, 60, dsweeps);
+        // n <= 32: at most 16 pairs a round; 16 lanes a pair for columns up to 128 (4 wavefronts share each
+        // round's barrier), one wavefront a pair beyond
+        if(mm <= 128) {
+            DMRG_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&jacobi_lds<T, 16>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               static_cast<int>(JACOBI_LDS_BYTES)));
+            hipLaunchKernelGGL((jacobi_lds<T, 16>), dim3(1), dim3(256), lds, s, mm, nn, np, W, V, tol, 60, dsweeps);
+        } else {
+            DMRG_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&jacobi_lds<T, 64>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               static_cast<int>(JACOBI_LDS_BYTES)));
+            hipLaunchKernelGGL((jacobi_lds<T, 64>), dim3(1), dim3(1024), lds, s, mm, nn, np, W, V, tol, 60, dsweeps);
+        }
         DMRG_HIP_CHECK(hipGetLastError());
         int hs = 0;
         DMRG_HIP_CHECK(hipMemcpyAsync(&hs, dsweeps, sizeof(int), hipMemcpyDeviceToHost, s));
