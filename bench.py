@@ -300,6 +300,55 @@ def section_serve(nreq=32, workers=16, load=0.7, seed=4):
             "max_variance": max(r["variance"] for r in res), "batched_eigensolves": batched}
 
 
+def serve_stream(nreq=512, workers=16, load=0.7, seed=4, chunk=128, mixed=True):
+    """The serving configuration at its full size: nreq ising_tf_rf requests (L in {16,20,24,32}, chi_max in
+    {64,128,256}, w in {0.5,2,6}, seeds 100..), drawn as in section_serve (its first 32 are the same), with
+    mixed precision for the problems large enough for it. Run as consecutive serve calls of `chunk` requests
+    (a progress line after each): first all queued at t = 0 (capacity = requests / summed makespans), then
+    Poisson arrivals at `load` x capacity; latencies are pooled over the chunks."""
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import ising_fields
+
+    rng = np.random.default_rng(seed)
+    shapes = [(int(rng.choice([16, 20, 24, 32])), int(rng.choice([64, 128, 256])), float(rng.choice([0.5, 2.0, 6.0])))
+              for _ in range(nreq)]
+    gaps = rng.exponential(1.0, nreq)
+
+    def run(rate, tag):
+        lat, makespan, done, gflop, var = [], 0.0, 0, 0.0, 0.0
+        for c0 in range(0, nreq, chunk):
+            reqs, t = [], 0.0
+            for i in range(c0, min(nreq, c0 + chunk)):
+                L, chi, w = shapes[i]
+                if rate:
+                    t += float(gaps[i]) / rate
+                reqs.append({"L": L, "chi_max": chi, "h_rand": ising_fields(L, w, seed=100 + i),
+                             "arrival_s": t - (float(gaps[c0]) / rate if rate else 0.0), "seed": 100 + i})
+            res, st = G.serve(reqs, workers=workers, hbm_budget_bytes=128 << 30, mixed=mixed)
+            makespan += st["makespan_s"]
+            done += st["completed"]
+            lat += [r["latency_s"] for r in res]
+            gflop += sum(r["gflop"] for r in res)
+            var = max(var, max(r["variance"] for r in res))
+            print(f"[serve {tag}] requests {c0}..{min(nreq, c0 + chunk) - 1}: {st['completed']} done in "
+                  f"{st['makespan_s']:.1f} s", file=sys.stderr, flush=True)
+        return lat, makespan, done, gflop, var
+
+    _, ms_cap, _, _, _ = run(None, "capacity")
+    capacity = nreq / ms_cap
+    rate = load * capacity
+    lat, ms, done, gflop, var = run(rate, "load")
+    return {"serve_stream": f"{nreq} requests, L in {{16,20,24,32}}, chi_max in {{64,128,256}}, w in {{0.5,2,6}}, "
+                            f"seeds 100-{100 + nreq - 1}, {workers} workers, {'mixed' if mixed else 'FP64'}, "
+                            f"chunks of {chunk}",
+            "capacity_solves_per_s": round(capacity, 3), "arrival_rate_per_s": round(rate, 3),
+            "solves_per_s": round(nreq / ms, 3), "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 3), "completed": done,
+            "total_gflop": round(gflop, 1), "max_variance": var}
+
+
 def section_grouped(nops=32, reps=200, seed=6):
     """Serving-sized operators (chi in {32, 48, 64, 96, 128}, d=2, D=5, complex): their matvecs one after
     another (3-4 launches each), as one grouped launch per leg, and as a replayed HIP graph of those launches."""
@@ -678,6 +727,8 @@ def main():
     ap.add_argument("--two-site", action="store_true", help="only the two-site chi=1024 section, then exit")
     ap.add_argument("--profile-matvec", action="store_true", help="only the chi=512 FP64 matvec steps, then exit")
     ap.add_argument("--fp32", action="store_true", help="with --profile-matvec: the FP32 matrix-core matvec instead")
+    ap.add_argument("--serve-stream", type=int, metavar="N",
+                    help="only the serving stream of N requests (mixed precision, capacity then Poisson pass), then exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="one 256 MiB device-to-device copy (scales the HBM byte counters of a profiler run), then exit")
     ap.add_argument("--profile-eigs", choices=("fp64", "mixed"),
@@ -699,6 +750,9 @@ def main():
         return
     if args.profile_svd:
         print(json.dumps(profile_svd(args.profile_svd)))
+        return
+    if args.serve_stream:
+        print(json.dumps(serve_stream(args.serve_stream)))
         return
     if args.copy_calibration:
         import dmrg_amd as G
