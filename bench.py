@@ -250,7 +250,7 @@ def section_sweep(cpu=True):
     return out
 
 
-def section_serve(nreq=32, workers=16, load=0.7, seed=4):
+def section_serve(nreq=32, workers=16, load=0.7, seed=4, mixed=True):
     """Serving stream of independent ising_tf_rf realisations: L in {16,20,24,32}, chi_max in {64,128,256},
     h_i ~ U[-w,w] with w in {0.5,2,6}, seeds 100... A first pass with every request queued at t = 0 measures
     the capacity (solves/s at saturation); the second pass offers Poisson arrivals at `load` x that capacity and
@@ -274,13 +274,14 @@ def section_serve(nreq=32, workers=16, load=0.7, seed=4):
             out.append({"L": L, "chi_max": chi, "h_rand": ising_fields(L, w, seed=100 + i), "arrival_s": t, "seed": 100 + i})
         return out
 
-    _, cap = G.serve(requests(None), workers=workers, hbm_budget_bytes=128 << 30)
+    _, cap = G.serve(requests(None), workers=workers, hbm_budget_bytes=128 << 30, mixed=mixed)
     capacity = cap["solves_per_s"]
     rate = load * capacity
-    res, st = G.serve(requests(rate), workers=workers, hbm_budget_bytes=128 << 30)
+    res, st = G.serve(requests(rate), workers=workers, hbm_budget_bytes=128 << 30, mixed=mixed)
     gflop = sum(r["gflop"] for r in res)
-    # Same saturation pass with the requests' energy solves batched into one lockstep eigensolver.
-    res_b, cap_b = G.serve(requests(None), workers=workers, hbm_budget_bytes=128 << 30, batch_eigs=True)
+    # Same saturation pass with the requests' FP64 energy solves batched into one eigensolver (the mixed-precision
+    # solves of large problems stay with their request).
+    res_b, cap_b = G.serve(requests(None), workers=workers, hbm_budget_bytes=128 << 30, batch_eigs=True, mixed=mixed)
     batched = {"capacity_solves_per_s": round(cap_b["solves_per_s"], 3),
                "p50_latency_s_at_saturation": round(cap_b["p50_latency_s"], 3),
                "p50_latency_s_at_saturation_unbatched": round(cap["p50_latency_s"], 3),
@@ -290,7 +291,8 @@ def section_serve(nreq=32, workers=16, load=0.7, seed=4):
                "rebuild_s": round(cap_b["batch_rebuild_s"], 2),
                "max_abs_dE_vs_unbatched": float(max(abs(a["energy"] - b["energy"]) for a, b in zip(res_b, res)))}
     return {"workload": f"{nreq} requests, L in {{16,20,24,32}}, chi_max in {{64,128,256}}, w in {{0.5,2,6}}, "
-                        f"{workers} workers, FP64; Poisson arrivals at {load:g} x the measured capacity",
+                        f"{workers} workers, {'mixed precision (n >= 131072)' if mixed else 'FP64'}; Poisson arrivals "
+                        f"at {load:g} x the measured capacity",
             "capacity_solves_per_s": round(capacity, 3), "arrival_rate_per_s": round(rate, 3),
             "solves_per_s": round(st["solves_per_s"], 3), "p50_latency_s": round(st["p50_latency_s"], 3),
             "p99_latency_s": round(st["p99_latency_s"], 3), "mean_latency_s": round(st["mean_latency_s"], 3),
