@@ -380,10 +380,11 @@ int64_t     dmrg_request_footprint(int L, int chi_max, int complex_);
 
 /* Continuous batching of eigensolve requests on resident operators: each request joins the running batch of
  * dmrg_eigs_batched between two Davidson steps once it has arrived (arrival_s, real time from the call) and its
- * working vectors (n (2 ncv + 4) elements) fit in hbm_budget_bytes beside those of the running requests, and
- * while fewer than max_batch problems run (max_batch <= 0: no limit); it leaves when converged (or at
- * max_matvecs). Requests are admitted in arrival order. Each problem advances exactly as it would alone
- * (its own basis size, phase and restarts), so a request's result does not depend on what else runs. No
+ * working memory (basis and image vectors, restart temporaries and matvec workspace: footprint_bytes) fits in
+ * hbm_budget_bytes beside that of the running requests, and while fewer than max_batch problems run
+ * (max_batch <= 0: no limit); it leaves when converged (or at max_matvecs). Requests are admitted in arrival
+ * order. Each problem advances as it would alone (its own basis size, phase and restarts), so a request's
+ * result is the one it gets alone up to rounding (the grouped products' tile shapes depend on the set). No
  * counterpart in the reference (one eigensolve per call, solver_primme.cpp:297-517): for request loops. cfg as in
  * dmrg_eigs_batched; the operators must stay alive until the call returns. stats: latency percentiles over all
  * requests; completed counts converged ones. */
