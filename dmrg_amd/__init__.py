@@ -246,7 +246,7 @@ def svd_truncate(A, trunc_lim: float = 1e-14, rank_max: int = 0, method: int = 3
 def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_max: int = 64, max_sweeps: int = 20,
                       two_site_sweeps: int = 10, var_tol: float = 1e-12, trunc_lim: float = 1e-14, eigs_tol: float = 1e-12,
                       mixed: bool = False, complex_: bool = False, seed: int = 0, mixed_min_n: int = 0,
-                      xdmrg: bool = False, energy_target: float = 0.0, stream=None) -> dict:
+                      xdmrg: bool = False, energy_target: float = 0.0, eig_max_size: int = 0, stream=None) -> dict:
     """Finite DMRG ground state of the random transverse-field Ising chain on the GPU
     (fdmrg::run_algorithm with ising_tf_rf, fdmrg.cpp:170-376), or with xdmrg the eigenstate found by minimising
     <(H - energy_target)^2> (xdmrg.cpp:297-341, folded spectrum). Returns energies, variance, entropies, timings."""
@@ -260,6 +260,7 @@ def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_m
     cfg.var_tol, cfg.trunc_lim, cfg.eigs_tol = var_tol, trunc_lim, eigs_tol
     cfg.mixed, cfg.complex_, cfg.seed, cfg.mixed_min_n = int(mixed), int(complex_), seed, mixed_min_n
     cfg.xdmrg, cfg.energy_target = int(xdmrg), float(energy_target)
+    cfg.eig_max_size = int(eig_max_size)
     res = _lib.FdmrgResult()
     call("dmrg_fdmrg_ising_tf_rf", C.byref(cfg), C.byref(res), stream)
     out = {k: getattr(res, k) for k, _ in res._fields_ if k not in ("entropies", "chi")}

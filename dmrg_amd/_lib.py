@@ -109,7 +109,7 @@ class FdmrgConfig(C.Structure):
                 ("chi_max", C.c_int), ("max_sweeps", C.c_int), ("two_site_sweeps", C.c_int), ("var_tol", C.c_double),
                 ("trunc_lim", C.c_double), ("eigs_tol", C.c_double), ("eigs_ncv", C.c_int), ("mixed", C.c_int),
                 ("complex_", C.c_int), ("seed", C.c_uint64), ("mixed_min_n", C.c_int64), ("xdmrg", C.c_int),
-                ("energy_target", C.c_double)]
+                ("energy_target", C.c_double), ("eig_max_size", C.c_int64)]
 
 
 class FdmrgResult(C.Structure):

@@ -287,6 +287,18 @@ struct Fdmrg {
             t_eigs += std::chrono::duration<double>(clock::now() - t0).count();
             return {ev, 0.0};
         }
+        if(!folded && cfg.eig_max_size > 0 && n <= cfg.eig_max_size) {
+            // The reference's dense branch for small problems (OptSolver::EIG when problem_size <= eig_max_size,
+            // fdmrg.cpp:307, optimize_energy_eig.cpp:18-58): explicit H_eff, lowest eigenpair.
+            dense.reserve(static_cast<size_t>(n) * n * es());
+            heff_dense(DT, op.D, op.mpo.p, op.envL_p, op.envR_p, dense.p, n, false, s);
+            std::vector<double> w;
+            svd.eigh<T>(static_cast<int>(n), dense.as<T>(), static_cast<int>(n), true, w);
+            DMRG_HIP_CHECK(hipMemcpyAsync(th, dense.as<T>(), static_cast<size_t>(n) * es(), hipMemcpyDeviceToDevice, s));
+            res->eigensolves += 1;
+            t_eigs += std::chrono::duration<double>(clock::now() - t0).count();
+            return {w[0], 0.0};
+        }
         if(svc && !folded && !(cfg.mixed && n >= mixed_min)) {
             // Batched with the solves of the other requests (same method, basis rule and tolerance).
             DMRG_HIP_CHECK(hipStreamSynchronize(s)); // theta and the operator's blocks are complete

@@ -298,6 +298,10 @@ typedef struct dmrg_fdmrg_config {
                                       folded-spectrum executor on the MPO^2 operator (optimize_folded_spectrum.cpp:235-342,
                                       closest_abs to 0), the shift fixed at the target (xdmrg.cpp:487-501) */
     double        energy_target;   /* the target energy E_tgt of xdmrg = 1 (absolute, not a density) */
+    int64_t       eig_max_size;    /* ground-state solves of problems up to this size use the dense solver, as the
+                                      reference's OptSolver::EIG for problem_size <= eig_max_size (fdmrg.cpp:307,
+                                      AlgorithmFinite.cpp:318; the reference's default is 4096); 0 (the default):
+                                      the iterative solver at every size, which is faster on the GPU */
 } dmrg_fdmrg_config;
 
 typedef struct dmrg_fdmrg_result {

@@ -45,3 +45,19 @@ def test_sweep_mixed_matches_fp64():
     assert abs(rmx["energy"] - r64["energy"]) <= 1e-10 * abs(r64["energy"])
     assert rmx["variance"] < 1e-12 and rmx["matvecs_fp32"] > 0
     np.testing.assert_allclose(rmx["entropies"], r64["entropies"], atol=1e-8)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_sweep_dense_branch_matches_iterative(cplx):
+    """eig_max_size = 4096 (the reference's setting: dense solves for problems up to 4096, here every solve of an
+    L=10, chi=32 chain) gives the energy and entropies of the iterative sweep and the free-fermion closed form."""
+    L = 10
+    h = ising_fields(L, 1.0, seed=5)
+    rit = G.fdmrg_ising_tf_rf(L, h, chi_max=32, complex_=cplx)
+    rde = G.fdmrg_ising_tf_rf(L, h, chi_max=32, complex_=cplx, eig_max_size=4096)
+    e0 = O.ising_free_fermion_ground_energy(1.0, 1.0, h)
+    assert abs(rde["energy"] - e0) <= 1e-10 * abs(e0), (rde["energy"], e0)
+    assert abs(rde["energy"] - rit["energy"]) <= 1e-10 * abs(e0)
+    assert rde["variance"] < 1e-12 and rde["matvecs"] < rit["matvecs"]
+    np.testing.assert_allclose(rde["entropies"], rit["entropies"], atol=1e-8)
+    np.testing.assert_allclose(rde["entropies"], O.ising_free_fermion_entropies(1.0, 1.0, h), atol=1e-8)
