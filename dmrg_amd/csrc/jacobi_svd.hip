@@ -440,8 +440,8 @@ constexpr size_t JACOBI_LDS_BYTES = 152 * 1024;
 constexpr int    JACOBI_LDS_MAX_N = 32;
 // Block path from this many columns on (measured on gfx950, complex n x n, ms per SVD, per-pair rounds / block
 // rounds: 64: 2.86 / 2.4, 128: 7.4 / 5.8, 256: 19.5 / 14.4, 512: 61 / 35, 2048: 1325 / 289); B2 columns (two
-// blocks) per inner problem.
-constexpr int JACOBI_BLOCK_MIN_N = 64;
+// blocks) per inner problem. From 33 columns there are at least 3 blocks, so every round has work.
+constexpr int JACOBI_BLOCK_MIN_N = 33;
 constexpr int JACOBI_BLOCK_B2    = 32;
 // Inner Jacobi sweeps per block pair and round: one sweep leaves the outer sweep count unchanged (the outer
 // iteration finishes what an inexact inner solve leaves) at a fraction of the cost of converging each one.
