@@ -149,7 +149,7 @@ def check_svd(A, res, lim, rank_max):
 @pytest.mark.parametrize("method", [0, 3], ids=["gesvd", "jacobi"])
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("m,n", [(1, 1), (1, 7), (9, 1), (2, 2), (30, 30), (31, 31), (50, 32), (33, 70), (64, 40), (37, 90), (300, 200),
-                                 (600, 520), (513, 700), (640, 448)])
+                                 (600, 520), (513, 700), (640, 448), (600, 30), (20, 1200)])
 def test_svd_truncate_random(m, n, cplx, method):
     print(f"[case] svd m={m} n={n} cplx={cplx} method={method}", file=sys.stderr, flush=True)
     rng = np.random.default_rng(m * 1000 + n)
