@@ -392,7 +392,9 @@ struct Engine {
     static void basis_rule(const EigsParams &p, int64_t n, int &mmax, int &mmin) {
         // Basis rule of the single-problem solver (solver.cpp:12-15, solver_primme.cpp:374).
         mmax = p.ncv > 0 ? p.ncv : static_cast<int>(std::ceil(std::log2(static_cast<double>(std::max<int64_t>(n, 2)))));
-        mmax = static_cast<int>(std::clamp<int64_t>(mmax, 4, std::min<int64_t>(n, BMAX_M)));
+        // At least 4, at most min(n, BMAX_M): for n < 4 the upper bound wins, so that add() and the admission
+        // footprint of serve_impl use the same basis size.
+        mmax = static_cast<int>(std::min<int64_t>(std::max(mmax, 4), std::min<int64_t>(n, BMAX_M)));
         mmin = std::max(1, std::min(mmax / 2 - 1, mmax - 2));
     }
 
@@ -404,7 +406,6 @@ struct Engine {
         sl->evec = evec;
         sl->id   = id;
         basis_rule(p, sl->n, sl->mmax, sl->mmin);
-        sl->mmax = static_cast<int>(std::min<int64_t>(sl->mmax, sl->n));
         sl->H.assign(static_cast<size_t>(sl->mmax) * sl->mmax, cx(0.0, 0.0));
         sl->footprint = footprint_of(*op, sl->mmax, sl->mmin);
         sl->mem       = pool.get(static_cast<size_t>(vec_elems(sl->n, sl->mmax, sl->mmin)) * sizeof(T), sl->mem_cls);
@@ -689,6 +690,13 @@ void serve_impl(const std::vector<EigRequest> &req, const EigsParams &p, int64_t
     std::vector<int> order(n);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return req[a].arrival_s < req[b].arrival_s; });
+    // A request that cannot fit in the budget even alone is an error of the whole call, as in dmrg_serve.
+    for(int i = 0; i < n; ++i) {
+        int mmax, mmin;
+        Engine<T>::basis_rule(p, req[i].op->D.n(), mmax, mmin);
+        DMRG_REQUIRE(Engine<T>::footprint_of(*req[i].op, mmax, mmin) <= budget, DMRG_ERR_OOM,
+                     "request " + std::to_string(i) + " does not fit in the HBM budget");
+    }
     size_t  next = 0;
     int64_t admitted = 0;
     int     finished = 0;
@@ -701,7 +709,7 @@ void serve_impl(const std::vector<EigRequest> &req, const EigsParams &p, int64_t
             int mmax, mmin;
             Engine<T>::basis_rule(p, r.op->D.n(), mmax, mmin);
             const int64_t fp = Engine<T>::footprint_of(*r.op, mmax, mmin);
-            if((E.nactive > 0 && admitted + fp > budget) || (max_batch > 0 && E.nactive >= max_batch)) break;
+            if(admitted + fp > budget || (max_batch > 0 && E.nactive >= max_batch)) break;
             E.add(r.op, r.x0, r.evec, order[next]);
             EigRequestResult &o = out[order[next]];
             o.arrival_s         = r.arrival_s;

@@ -646,7 +646,7 @@ dmrg_status dmrg_serve_eigs(const dmrg_eigs_config *cfg, const dmrg_eig_request 
         };
         *stats                     = dmrg_serve_stats{};
         stats->makespan_s          = sum.makespan_s;
-        stats->solves_per_s        = nreq / std::max(sum.makespan_s, 1e-12);
+        stats->solves_per_s        = conv / std::max(sum.makespan_s, 1e-12); // converged solves, as dmrg_serve
         stats->p50_latency_s       = pct(0.5);
         stats->p99_latency_s       = pct(0.99);
         stats->mean_latency_s      = mean;

@@ -1,7 +1,7 @@
 // Host dense Hermitian eigensolver for the eigensolver's projected (Rayleigh-Ritz) problems, m <= a few
 // hundred. Householder reduction of the Hermitian matrix to a complex tridiagonal one, a diagonal unitary
-// scaling that makes the off-diagonal real, then implicit-shift QL iterations on the real symmetric
-// tridiagonal matrix with the rotations accumulated into the eigenvector matrix.
+// scaling that makes the off-diagonal real, then implicit symmetric QR steps (Golub & Van Loan) on the real
+// symmetric tridiagonal matrix with the rotations accumulated into the eigenvector matrix.
 #include "dense_eig.hpp"
 
 #include <algorithm>
@@ -13,57 +13,60 @@ namespace dmrg {
 
 using cx = std::complex<double>;
 
-// QL with implicit Wilkinson shifts on the symmetric tridiagonal (d, e); e[i] couples i and i+1.
-// Rotations are applied to the columns of Z (m x m, column-major, complex).
-static bool tridiag_ql(int m, std::vector<double> &d, std::vector<double> &e, std::vector<cx> &Z) {
+// Symmetric tridiagonal eigenproblem by the implicit symmetric QR algorithm with Wilkinson shifts: the
+// deflation loop of Golub & Van Loan, Matrix Computations (4th ed., 2013), Algorithm 8.3.3, whose step is
+// their Algorithm 8.3.2 (one implicit QR step, a chase of the bulge down the unreduced block by Givens
+// rotations). d: diagonal (m), e[i]: the coupling of rows i and i+1 (m - 1 used). The rotations are
+// accumulated into the columns of Z (m x m, column-major, complex): Z <- Z G_k.
+static bool tridiag_qr(int m, std::vector<double> &d, std::vector<double> &e, std::vector<cx> &Z) {
     if(m <= 1) return true;
     e.resize(m);
-    e[m - 1] = 0.0;
-    for(int l = 0; l < m; ++l) {
-        int iter = 0;
-        while(true) {
-            int mm = l;
-            for(; mm < m - 1; ++mm) {
-                const double dd = std::fabs(d[mm]) + std::fabs(d[mm + 1]);
-                if(std::fabs(e[mm]) <= std::numeric_limits<double>::epsilon() * dd) break;
+    const double eps      = std::numeric_limits<double>::epsilon();
+    const long   max_step = 60L * m; // total implicit steps; typically about 2 per eigenvalue
+    long         steps    = 0;
+    auto rotate_columns = [&](int k, double c, double s) { // Z(:, k:k+1) <- Z(:, k:k+1) [c s; -s c]
+        cx *zk = Z.data() + static_cast<size_t>(m) * k, *zk1 = zk + m;
+        for(int i = 0; i < m; ++i) {
+            const cx a = zk[i], b = zk1[i];
+            zk[i]      = c * a - s * b;
+            zk1[i]     = s * a + c * b;
+        }
+    };
+    int hi = m - 1; // the trailing rows hi+1.. are converged
+    while(hi > 0) {
+        // Deflation test of Alg. 8.3.3: a coupling below eps (|d_i| + |d_i+1|) is set to zero.
+        for(int i = 0; i < hi; ++i)
+            if(std::fabs(e[i]) <= eps * (std::fabs(d[i]) + std::fabs(d[i + 1]))) e[i] = 0.0;
+        if(e[hi - 1] == 0.0) {
+            --hi;
+            continue;
+        }
+        int lo = hi - 1; // the unreduced block lo..hi ending at hi
+        while(lo > 0 && e[lo - 1] != 0.0) --lo;
+        if(++steps > max_step) return false;
+        // Wilkinson shift: the eigenvalue of the trailing 2 x 2 block [d_hi-1 e; e d_hi] closer to d_hi.
+        const double t  = 0.5 * (d[hi - 1] - d[hi]);
+        const double b2 = e[hi - 1] * e[hi - 1];
+        const double mu = d[hi] - b2 / (t + std::copysign(std::hypot(t, e[hi - 1]), t));
+        // Bulge chase. G_k acts on rows/columns (k, k+1): T <- G_k^T T G_k with G_k = [c s; -s c] chosen so
+        // that G_k^T [x; z] = [r; 0], x the entry to keep and z the entry to annihilate (the shifted first
+        // column for k = lo, then the bulge at (k+1, k-1)).
+        double x = d[lo] - mu, z = e[lo];
+        for(int k = lo; k < hi; ++k) {
+            const double r = std::hypot(x, z);
+            double       c = 1.0, s = 0.0;
+            if(r > 0.0) c = x / r, s = -z / r;
+            if(k > lo) e[k - 1] = r;
+            const double p = d[k], q = d[k + 1], b = e[k];
+            d[k]     = c * c * p - 2.0 * c * s * b + s * s * q;
+            d[k + 1] = s * s * p + 2.0 * c * s * b + c * c * q;
+            e[k]     = c * s * (p - q) + (c * c - s * s) * b;
+            rotate_columns(k, c, s);
+            if(k + 1 < hi) { // the rotation moves the bulge to (k+2, k)
+                x = e[k];
+                z = -s * e[k + 1];
+                e[k + 1] *= c;
             }
-            if(mm == l) break;
-            if(++iter > 60) return false;
-            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = std::hypot(g, 1.0);
-            g        = d[mm] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
-            double s = 1.0, c = 1.0, p = 0.0;
-            int    i = mm - 1;
-            bool   early = false;
-            for(; i >= l; --i) {
-                double f = s * e[i];
-                double b = c * e[i];
-                r        = std::hypot(f, g);
-                e[i + 1] = r;
-                if(r == 0.0) {
-                    d[i + 1] -= p;
-                    e[mm] = 0.0;
-                    early = true;
-                    break;
-                }
-                s        = f / r;
-                c        = g / r;
-                g        = d[i + 1] - p;
-                r        = (d[i] - g) * s + 2.0 * c * b;
-                p        = s * r;
-                d[i + 1] = g + p;
-                g        = c * r - b;
-                for(int k = 0; k < m; ++k) {
-                    const cx zi1 = Z[k + static_cast<size_t>(m) * (i + 1)];
-                    const cx zi  = Z[k + static_cast<size_t>(m) * i];
-                    Z[k + static_cast<size_t>(m) * (i + 1)] = s * zi + c * zi1;
-                    Z[k + static_cast<size_t>(m) * i]       = c * zi - s * zi1;
-                }
-            }
-            if(early) continue;
-            d[l] -= p;
-            e[l]  = g;
-            e[mm] = 0.0;
         }
     }
     return true;
@@ -126,7 +129,7 @@ bool hermitian_eig(int m, const cx *A, int lda, double *w, cx *Z_out) {
     }
     for(int j = 0; j < m; ++j)
         for(int i = 0; i < m; ++i) Q[i + static_cast<size_t>(m) * j] *= ph[j];
-    if(!tridiag_ql(m, d, e, Q)) return false;
+    if(!tridiag_qr(m, d, e, Q)) return false;
     // Sort ascending.
     std::vector<int> idx(m);
     std::iota(idx.begin(), idx.end(), 0);

@@ -125,6 +125,10 @@ MatvecGroup *matvec_group_create(MatVecOp *const *ops, int k, const void *const 
         for(int j = 0; j < 3; ++j) leg[j].push_back(g[j]);
         grp->flop += op.D.macs() * (dtype_is_complex(t) ? 8.0 : 2.0);
     }
+    // The tuner launches on a stream of its own, reading xs and writing ys and the workspaces: in the standalone
+    // path (no caller-owned storage) every stream's work on them must be finished first. The batched engine owns
+    // its vectors and synchronises its stream before it rebuilds a group.
+    if(!storage && autotune_enabled()) DMRG_HIP_CHECK(hipDeviceSynchronize());
     // Tiles: the MPO leg (M = d D, K = d D) in 16 x 128 tiles while M is small; the big legs in 64 x 64 tiles
     // when that gives about two waves of workgroups on the GPU, else in 32 x 32 tiles.
     for(int j = 0; j < 3; ++j) {

@@ -262,7 +262,9 @@ dmrg_status dmrg_eigs_batched(dmrg_matvec *const *ops, int k, const dmrg_eigs_co
  * dmrg_matvec_group_run replays. The group holds pointers into the operators' buffers and into xs/ys: keep
  * those (and the operators' shift/MPO state) unchanged while the group is used; destroy it first. With
  * dmrg_autotune_enable on (the default), creating a group of a new shape class times the tile configurations of
- * each leg on these operands (so ys receives H_i x_i during creation) and caches the fastest. */
+ * each leg on these operands (so ys receives H_i x_i during creation) and caches the fastest. Creation
+ * synchronises the device before it tunes, so xs must be written (by work already enqueued on any stream)
+ * when it is called, and no other work may use ys meanwhile. */
 typedef struct dmrg_matvec_group dmrg_matvec_group;
 dmrg_status dmrg_matvec_group_create(dmrg_matvec *const *ops, int k, const void *const *xs, void *const *ys, int fp32,
                                      int capture_graph, dmrg_matvec_group **out);
@@ -390,8 +392,10 @@ int64_t     dmrg_request_footprint(int L, int chi_max, int complex_);
  * order. Each problem advances as it would alone (its own basis size, phase and restarts), so a request's
  * result is the one it gets alone up to rounding (the grouped products' tile shapes depend on the set). No
  * counterpart in the reference (one eigensolve per call, solver_primme.cpp:297-517): for request loops. cfg as in
- * dmrg_eigs_batched; the operators must stay alive until the call returns. stats: latency percentiles over all
- * requests; completed counts converged ones. */
+ * dmrg_eigs_batched; the operators must stay alive until the call returns. A request whose footprint alone
+ * exceeds hbm_budget_bytes fails the whole call with DMRG_ERR_OOM before any work starts (as dmrg_serve).
+ * stats: latency percentiles over all requests; completed counts converged ones and solves_per_s is
+ * completed / makespan (as dmrg_serve). */
 typedef struct dmrg_eig_request {
     dmrg_matvec *op;
     const void  *x0;        /* device start vector or NULL (seeded random start) */

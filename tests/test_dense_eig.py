@@ -45,3 +45,32 @@ def test_heev_degenerate_and_diagonal():
     w, Z = heev(A)
     np.testing.assert_allclose(w, np.sort(lam), atol=1e-13)
     np.testing.assert_allclose(A @ Z, Z * w, atol=1e-13)
+
+
+def test_heev_wilkinson_and_graded():
+    """Wilkinson's W21+ (pairs of eigenvalues equal to ~1e-14, the classic hard case for the shift strategy), a
+    block-diagonal matrix whose tridiagonal form splits, and a graded spectrum at m = 256, against numpy."""
+    n = 21
+    W = np.diag(np.abs(np.arange(n) - 10).astype(float)) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1)
+    w, Z = heev(W)
+    np.testing.assert_allclose(w, np.linalg.eigvalsh(W), atol=1e-13)
+    np.testing.assert_allclose(W @ Z, Z * w, atol=1e-12)
+    np.testing.assert_allclose(Z.conj().T @ Z, np.eye(n), atol=1e-12)
+    B = np.zeros((12, 12), dtype=complex)
+    rng = np.random.default_rng(3)
+    for s in (slice(0, 5), slice(5, 6), slice(6, 12)):
+        k = s.stop - s.start
+        X = rng.standard_normal((k, k)) + 1j * rng.standard_normal((k, k))
+        B[s, s] = X + X.conj().T
+    w, Z = heev(B)
+    np.testing.assert_allclose(w, np.linalg.eigvalsh(B), atol=1e-12 * np.abs(w).max())
+    np.testing.assert_allclose(B @ Z, Z * w, atol=1e-11 * np.abs(w).max())
+    m = 256
+    Q, _ = np.linalg.qr(rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m)))
+    lam = np.sign(rng.standard_normal(m)) * np.logspace(0, -10, m)
+    A = (Q * lam) @ Q.conj().T
+    A = 0.5 * (A + A.conj().T)
+    w, Z = heev(A)
+    np.testing.assert_allclose(w, np.linalg.eigvalsh(A), atol=1e-13)
+    np.testing.assert_allclose(A @ Z, Z * w, atol=1e-12)
+    np.testing.assert_allclose(Z.conj().T @ Z, np.eye(m), atol=1e-12)

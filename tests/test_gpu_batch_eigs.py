@@ -144,10 +144,18 @@ def test_serve_eigs_budget_admission():
         out2, stats2 = G.serve_eigs(ops, hbm_budget_bytes=budget, tol=1e-12)
         assert stats2["completed"] == len(ops)
         assert stats2["peak_admitted_bytes"] <= budget and stats2["peak_concurrency"] < len(ops)
+        for st in (stats, stats2):  # throughput counts converged solves, as dmrg_serve
+            assert abs(st["solves_per_s"] * st["makespan_s"] - st["completed"]) <= 1e-6 * st["completed"]
         for (l1, v1, _), (l2, v2, _) in zip(out, out2):
             assert abs(l1 - l2) <= 1e-10 * max(1.0, abs(l1)) and overlap(v1, v2) >= 1 - 1e-9
         with pytest.raises(G.DmrgError):
             G.serve_eigs(ops[:1], hbm_budget_bytes=0)
+        # One request larger than the whole budget fails the call up front (DMRG_ERR_OOM), even when it would
+        # arrive into an empty batch.
+        small = min(fp)
+        with pytest.raises(G.DmrgError) as ei:
+            G.serve_eigs([ops[3], ops[0]], hbm_budget_bytes=small, tol=1e-12)
+        assert ei.value.status == 3
     finally:
         for op in ops:
             op.close()
