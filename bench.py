@@ -104,29 +104,12 @@ def load_traffic(name: str):
     return None
 
 
-def squared_problem(th, W, L, R):
-    """Exact H^2 of the isolated workload: MPO^2 of MpoSite.cpp:57-69 and squared env blocks
-    L2[:, :, w1 + D w2] = L[:, :, w1] L[:, :, w2] (R likewise), so that H2 theta = H (H theta)."""
-    import numpy as np
-
-    D = W.shape[0]
-    t = np.tensordot(W, W.conj(), axes=([3], [2])).transpose(0, 3, 1, 4, 2, 5)
-    W2 = np.asfortranarray(t.reshape(D * D, D * D, W.shape[2], W.shape[3], order="F"))
-    L2 = np.empty((L.shape[0], L.shape[0], D * D), dtype=L.dtype, order="F")
-    R2 = np.empty((R.shape[0], R.shape[0], D * D), dtype=R.dtype, order="F")
-    for a in range(D):
-        for b in range(D):
-            L2[:, :, a + D * b] = L[:, :, a] @ L[:, :, b]
-            R2[:, :, a + D * b] = R[:, :, a] @ R[:, :, b]
-    return W2, L2, R2
-
-
 def section_h2_matvec(chi=512, steps=30):
     """Isolated H^2 matvec (D^2 = 25 squared MPO and squared env blocks), FP64 and FP32 matrix cores."""
     import numpy as np
 
     import dmrg_amd as G
-    from dmrg_amd.workloads import heff_flops, random_heff_problem
+    from dmrg_amd.workloads import heff_flops, random_heff_problem, squared_problem
 
     th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=2)
     W2, L2, R2 = squared_problem(th, W, L, R)
@@ -475,28 +458,6 @@ def section_batched_eigs(nops=32, tol=1e-10, seed=6):
             "anorm_max": anorm}
 
 
-def folded_problem(th, W, L, R, sigma):
-    """(H - sigma)^2 = H^2 - 2 sigma H + sigma^2 of the isolated workload as one operator: a block-diagonal MPO of
-    the MPO^2 (D^2 channels), -2 sigma W (D channels) and sigma^2 I (one channel), with env blocks stacked the
-    same way (squared blocks, the blocks, the identity). D_total = D^2 + D + 1."""
-    import numpy as np
-
-    W2, L2, R2 = squared_problem(th, W, L, R)
-    D, DD, chi_l, chi_r = W.shape[0], W2.shape[0], L.shape[0], R.shape[0]
-    Dt = DD + D + 1
-    Wt = np.zeros((Dt, Dt, W.shape[2], W.shape[3]), dtype=np.complex128, order="F")
-    Wt[:DD, :DD] = W2
-    Wt[DD:DD + D, DD:DD + D] = -2.0 * sigma * W
-    Wt[Dt - 1, Dt - 1] = sigma * sigma * np.eye(W.shape[2])
-
-    def stack(B, B2, chi):
-        out = np.zeros((chi, chi, Dt), dtype=np.complex128, order="F")
-        out[:, :, :DD], out[:, :, DD:DD + D], out[:, :, Dt - 1] = B2, B, np.eye(chi)
-        return out
-
-    return Wt, stack(L, L2, chi_l), stack(R, R2, chi_r)
-
-
 def section_eigs_h2(chi=512, tol=1e-10, sigma_rel=1e-3):
     """Eigensolve on the squared operator: the eigenvalue nearest 0 of the shifted H^2, (H - sigma)^2 with
     sigma = E0 - sigma_rel |H| below the ground state (E0 from an FP64 solve of H), FP64 vs mixed precision;
@@ -504,7 +465,7 @@ def section_eigs_h2(chi=512, tol=1e-10, sigma_rel=1e-3):
     import numpy as np
 
     import dmrg_amd as G
-    from dmrg_amd.workloads import random_heff_problem
+    from dmrg_amd.workloads import folded_problem, random_heff_problem
 
     th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=2)
     h = G.MatVecMPOS([W], L, R, dtype=np.complex128)

@@ -293,14 +293,15 @@ def serve(requests, workers: int = 4, hbm_budget_bytes: int = 64 << 30, mixed: b
 
 
 def eigs_batched(ops, tol: float = 1e-12, target: str = "SA", x0s=None, ncv: int = 0, max_matvecs: int = 100000,
-                 seed: int = 12345, stream=None):
-    """Lowest (SA) or highest (LA) eigenpair of each operator, solved together in lockstep (dmrg_eigs_batched).
-    Returns a list of (eigenvalue, eigenvector numpy, result dict)."""
+                 seed: int = 12345, mixed: bool = False, stream=None):
+    """Lowest (SA) or highest (LA) eigenpair of each operator, solved together in lockstep (dmrg_eigs_batched);
+    mixed: an FP32 phase on the FP32 matrix cores, then FP64 to tol. Returns a list of (eigenvalue, eigenvector
+    numpy, result dict)."""
     k = len(ops)
     cfg = _lib.EigsConfig()
     load().dmrg_eigs_default_config(C.byref(cfg))
     cfg.nev, cfg.ncv, cfg.target = 1, ncv, {"SA": 0, "LA": 1}[target]
-    cfg.tol, cfg.max_matvecs, cfg.seed = tol, max_matvecs, seed
+    cfg.tol, cfg.max_matvecs, cfg.seed, cfg.mixed = tol, max_matvecs, seed, int(mixed)
     outs = [DeviceArray((o.rows(),), o.dtype) for o in ops]
     ins = [None] * k
     if x0s is not None:
@@ -324,7 +325,8 @@ def eigs_batched(ops, tol: float = 1e-12, target: str = "SA", x0s=None, ncv: int
 
 
 def serve_eigs(ops, arrivals=None, hbm_budget_bytes: int = 64 << 30, max_batch: int = 0, tol: float = 1e-12,
-               target: str = "SA", x0s=None, ncv: int = 0, max_matvecs: int = 100000, seed: int = 12345, stream=None):
+               target: str = "SA", x0s=None, ncv: int = 0, max_matvecs: int = 100000, seed: int = 12345,
+               mixed: bool = False, stream=None):
     """Eigensolve requests on resident operators with continuous batching (dmrg_serve_eigs): request i arrives
     arrivals[i] seconds after the call (default: all at once), joins the running batch when the HBM budget and
     max_batch allow, and leaves when converged. Returns (list of (eigenvalue, eigenvector numpy, info dict), stats)."""
@@ -335,7 +337,7 @@ def serve_eigs(ops, arrivals=None, hbm_budget_bytes: int = 64 << 30, max_batch: 
     cfg = _lib.EigsConfig()
     load().dmrg_eigs_default_config(C.byref(cfg))
     cfg.nev, cfg.ncv, cfg.target = 1, ncv, {"SA": 0, "LA": 1}[target]
-    cfg.tol, cfg.max_matvecs, cfg.seed = tol, max_matvecs, seed
+    cfg.tol, cfg.max_matvecs, cfg.seed, cfg.mixed = tol, max_matvecs, seed, int(mixed)
     outs = [DeviceArray((o.rows(),), o.dtype) for o in ops]
     ins = [None] * k
     if x0s is not None:
@@ -496,6 +498,10 @@ class MatVecMPOS:
     def set_jcb_block_size(self, size: int) -> None:
         """set_jcbMaxBlockSize: 1 = Jacobi, > 1 = block-Jacobi preconditioner in MultPc."""
         call("dmrg_matvec_set_jcb_block_size", self.h, size)
+
+    def set_factorization(self, factorization: str) -> None:
+        """MatVecMPOS::factorization of the block-Jacobi blocks: 'LLT', 'LDLT', 'LU' (default) or 'QR'."""
+        call("dmrg_matvec_set_factorization", self.h, {"LLT": 1, "LDLT": 2, "LU": 3, "QR": 4}[factorization])
 
     def MultPc(self, x: np.ndarray, shifts) -> np.ndarray:
         """Preconditioner callback on host vectors (MultPc, matvec_mpos.cpp:1333-1361): column i is

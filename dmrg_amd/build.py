@@ -44,18 +44,22 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-CPP_DRIVER_SRC = PKG.parent / "tests" / "cpp" / "heff_cpp_driver.cpp"
-CPP_DRIVER_BIN = PKG.parent / "tests" / "cpp" / "bin" / "heff_cpp_driver"
+CPP_DIR = PKG.parent / "tests" / "cpp"
+# Host-only C++ programs exercising include/dmrg_amd/matvec_mpos.hpp against the built library.
+CPP_DRIVERS = ["heff_cpp_driver", "solver_templates"]
+CPP_DRIVER_SRC = CPP_DIR / "heff_cpp_driver.cpp"
+CPP_DRIVER_BIN = CPP_DIR / "bin" / "heff_cpp_driver"
 
 
 def build_cpp_driver() -> Path:
-    """Host-only C++ program exercising include/dmrg_amd/matvec_mpos.hpp against the built library."""
-    CPP_DRIVER_BIN.parent.mkdir(parents=True, exist_ok=True)
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{INCLUDE}", str(CPP_DRIVER_SRC), f"-L{PKG}", "-ldmrg_amd",
-           "-Wl,-rpath,$ORIGIN/../../../dmrg_amd", "-o", str(CPP_DRIVER_BIN)]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"C++ driver build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    """Builds every C++ driver under tests/cpp (g++, linked against libdmrg_amd.so)."""
+    (CPP_DIR / "bin").mkdir(parents=True, exist_ok=True)
+    for name in CPP_DRIVERS:
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{INCLUDE}", str(CPP_DIR / f"{name}.cpp"), f"-L{PKG}",
+               "-ldmrg_amd", "-Wl,-rpath,$ORIGIN/../../../dmrg_amd", "-o", str(CPP_DIR / "bin" / name)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"C++ driver build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return CPP_DRIVER_BIN
 
 

@@ -11,7 +11,8 @@
 //           new column V^H W[:, m] of the projected matrix         (dot + fold)
 //   one device-to-host copy of every problem's scalars, one synchronisation.
 // A problem whose Ritz residual is below tol |A| leaves with x as its eigenvector; new problems join between
-// steps (continuous batching), and the grouped launches are rebuilt when the set changes.
+// steps (continuous batching), and the grouped launches are rebuilt when the set changes. The engine runs in FP64
+// or, as the first phase of the mixed-precision batch (MixedEngine), in FP32.
 #include "batch_eigs.hpp"
 
 #include "dense_eig.hpp"
@@ -63,15 +64,29 @@ struct BProb {
     double *io;
 };
 
+// Element traits: complex or real, the real type, and whether the engine runs in single precision (FP32 vectors
+// and FP32 matrix-core matvecs; every reduction still accumulates in double).
 template <typename T>
 struct bt;
 template <>
 struct bt<double> {
-    static constexpr bool cx = false;
+    static constexpr bool cx = false, single = false;
+    using R                  = double;
 };
 template <>
 struct bt<cpx<double>> {
-    static constexpr bool cx = true;
+    static constexpr bool cx = true, single = false;
+    using R                  = double;
+};
+template <>
+struct bt<float> {
+    static constexpr bool cx = false, single = true;
+    using R                  = float;
+};
+template <>
+struct bt<cpx<float>> {
+    static constexpr bool cx = true, single = true;
+    using R                  = float;
 };
 
 template <typename T>
@@ -81,8 +96,9 @@ __device__ inline void ldc(const T &v, double &re, double &im) {
 }
 template <typename T>
 __device__ inline T mkc(double re, double im) {
-    if constexpr(bt<T>::cx) return T{re, im};
-    else return re;
+    using R = typename bt<T>::R;
+    if constexpr(bt<T>::cx) return T{static_cast<R>(re), static_cast<R>(im)};
+    else return static_cast<T>(re);
 }
 
 __device__ inline double wsum64(double v) {
@@ -332,6 +348,8 @@ struct Slot {
     std::vector<cx> H, Z;
     std::vector<double> w;
     double          anorm = 0.0;
+    double          best_rn = 1e300; // stagnation rule: smallest residual seen and steps since it dropped by 10%
+    int             since_best = 0;
     BatchEigsResult res;
     void           *evec = nullptr;
     int             id   = 0;
@@ -387,7 +405,8 @@ struct Engine {
     // Vectors: V, W (n x mmax), x, r, X, Y, restart temporaries (n x mmin, mmax x mmin); plus the matvec workspace.
     static int64_t vec_elems(int64_t n, int mmax, int mmin) { return n * (2 * mmax + 4) + n * mmin + int64_t(mmax) * mmin; }
     static int64_t footprint_of(const MatVecOp &op, int mmax, int mmin) {
-        return vec_elems(op.D.n(), mmax, mmin) * static_cast<int64_t>(sizeof(T)) + static_cast<int64_t>(matvec_group_ws_bytes(op, false));
+        return vec_elems(op.D.n(), mmax, mmin) * static_cast<int64_t>(sizeof(T)) +
+               static_cast<int64_t>(matvec_group_ws_bytes(op, bt<T>::single));
     }
     static void basis_rule(const EigsParams &p, int64_t n, int &mmax, int &mmin) {
         // Basis rule of the single-problem solver (solver.cpp:12-15, solver_primme.cpp:374).
@@ -398,19 +417,22 @@ struct Engine {
         mmin = std::max(1, std::min(mmax / 2 - 1, mmax - 2));
     }
 
-    void add(MatVecOp *op, const void *x0, void *evec, int id) {
+    // anorm0: an estimate of |A| carried over from an earlier phase (0: none).
+    void add(MatVecOp *op, const void *x0, void *evec, int id, double anorm0 = 0.0) {
         DMRG_REQUIRE(op->D.n() >= 2, DMRG_ERR_ARG, "batched eigs: problems of size < 2");
+        DMRG_REQUIRE(!op->chain, DMRG_ERR_UNSUPPORTED, "batched eigs: operators with more than 5 MPOs (applied in sequence)");
         auto sl  = std::make_unique<Slot<T>>();
         sl->op   = op;
         sl->n    = op->D.n();
         sl->evec = evec;
         sl->id   = id;
+        sl->anorm = anorm0;
         basis_rule(p, sl->n, sl->mmax, sl->mmin);
         sl->H.assign(static_cast<size_t>(sl->mmax) * sl->mmax, cx(0.0, 0.0));
         sl->footprint = footprint_of(*op, sl->mmax, sl->mmin);
         sl->mem       = pool.get(static_cast<size_t>(vec_elems(sl->n, sl->mmax, sl->mmin)) * sizeof(T), sl->mem_cls);
         try {
-            sl->ws = pool.get(matvec_group_ws_bytes(*op, false), sl->ws_cls);
+            sl->ws = pool.get(matvec_group_ws_bytes(*op, bt<T>::single), sl->ws_cls);
         } catch(...) {
             release(*sl);
             throw;
@@ -476,7 +498,7 @@ struct Engine {
             gs.store_bytes[j] = gstore[j].bytes;
         }
         // The grouped matvec goes into the step graph as plain kernel nodes.
-        group.reset(matvec_group_create(ops.data(), k(), xs.data(), ys.data(), false, false, &gs));
+        group.reset(matvec_group_create(ops.data(), k(), xs.data(), ys.data(), bt<T>::single, false, &gs));
         if(capture) capture_step();
         dirty = false;
     }
@@ -617,7 +639,10 @@ struct Engine {
             const bool conv  = rn <= p.tol * std::max(S.anorm, 1e-300);
             const bool limit = S.res.matvecs >= p.max_matvecs;
             const bool broke = S.phase == PH_ITER && !(io[IO_NRM + 4] > 1e-26 * std::max(io[IO_C + 2 * S.m], 1e-300));
-            if(conv || limit || broke) {
+            if(rn < 0.9 * S.best_rn) S.best_rn = rn, S.since_best = 0;
+            else ++S.since_best;
+            const bool stagnated = p.stagnation_window > 0 && S.since_best >= p.stagnation_window;
+            if(conv || limit || broke || stagnated) {
                 S.res.nconv = conv ? 1 : 0;
                 DMRG_HIP_CHECK(hipMemcpyAsync(S.evec, hp[q].x, static_cast<size_t>(S.n) * sizeof(T), hipMemcpyDeviceToDevice, s));
                 S.phase = PH_NONE;
@@ -639,14 +664,15 @@ struct Engine {
             for(int j = 0; j < S.mmin; ++j)
                 for(int i = 0; i < m; ++i) {
                     const cx z = S.Z[i + static_cast<size_t>(m) * ord[j]];
-                    if constexpr(bt<T>::cx) Qh[i + static_cast<size_t>(m) * j] = T{z.real(), z.imag()};
-                    else Qh[i + static_cast<size_t>(m) * j] = z.real();
+                    using R = typename bt<T>::R;
+                    if constexpr(bt<T>::cx) Qh[i + static_cast<size_t>(m) * j] = T{static_cast<R>(z.real()), static_cast<R>(z.imag())};
+                    else Qh[i + static_cast<size_t>(m) * j] = static_cast<R>(z.real());
                 }
             T *Qd = S.restart_q(), *tmp = S.restart_tmp();
             DMRG_HIP_CHECK(hipMemcpyAsync(Qd, Qh.data(), Qh.size() * sizeof(T), hipMemcpyHostToDevice, s));
             for(T *Bm : {hp[q].V, hp[q].W}) {
                 GemmDesc g;
-                g.dtype = S.op->dtype;
+                g.dtype = bt<T>::single ? dtype_single_of(S.op->dtype) : S.op->dtype;
                 g.M = static_cast<int32_t>(S.n), g.N = S.mmin, g.K = m, g.batch = 1;
                 g.A = Bm, g.am = plain(1), g.ak = plain(S.n);
                 g.B = Qd, g.bk = plain(1), g.bn = plain(m);
@@ -668,23 +694,117 @@ struct Engine {
     }
 };
 
-template <typename T>
+// Mixed precision over the batch: each problem first runs in an FP32 engine (FP32 vectors, FP32 matrix-core grouped
+// matvecs, double reductions) to 5e-6 |A| or until its residual stagnates (no 10% drop over 25 steps), the
+// residual floor of FP32 arithmetic being ~1e-6 |A|; its FP32 Ritz vector then starts it in an FP64 engine that
+// converges it to tol (the two phases of the single-problem mixed solver, eigs_mixed, without the correction
+// solves). The two engines step one after the other on the same stream, so a problem moves to the FP64 phase as
+// soon as its FP32 phase ends while the others go on.
+template <typename T64, typename T32>
+struct MixedEngine {
+    using Lo = Engine<T32>;
+    using Hi = Engine<T64>;
+    struct Rec {
+        MatVecOp       *op   = nullptr;
+        void           *evec = nullptr;
+        void           *x32 = nullptr, *y32 = nullptr;
+        size_t          cx32 = 0, cy32 = 0;
+        BatchEigsResult res1;
+        int64_t         footprint = 0;
+    };
+    hipStream_t                  s;
+    EigsParams                   p;
+    Lo                           lo;
+    Hi                           hi;
+    DevPool                      pool; // FP32 start vectors and FP32 phase results
+    std::unordered_map<int, Rec> rec;
+    int                          nactive = 0;
+    int64_t                      steps   = 0;
+
+    static EigsParams phase1(const EigsParams &p) {
+        EigsParams p1        = p;
+        p1.tol               = std::max(p.tol, 5e-6);
+        p1.stagnation_window = 25;
+        return p1;
+    }
+    static EigsParams phase2(const EigsParams &p) {
+        EigsParams p2        = p;
+        p2.seed              = p.seed + 1;
+        p2.stagnation_window = 0;
+        return p2;
+    }
+    MixedEngine(hipStream_t st, const EigsParams &pp, bool capture_graph = true)
+        : s(st), p(pp), lo(st, phase1(pp), capture_graph), hi(st, phase2(pp), capture_graph) {}
+
+    static void basis_rule(const EigsParams &p, int64_t n, int &mmax, int &mmin) { Hi::basis_rule(p, n, mmax, mmin); }
+    // Both phases' vectors (the engines keep their pooled memory) and the FP32 staging vectors.
+    static int64_t footprint_of(const MatVecOp &op, int mmax, int mmin) {
+        return Hi::footprint_of(op, mmax, mmin) + Lo::footprint_of(op, mmax, mmin) + 2 * op.D.n() * static_cast<int64_t>(sizeof(T32));
+    }
+
+    void add(MatVecOp *op, const void *x0, void *evec, int id) {
+        Rec r;
+        r.op   = op;
+        r.evec = evec;
+        const int64_t n = op->D.n();
+        int           mmax, mmin;
+        basis_rule(p, n, mmax, mmin);
+        r.footprint = footprint_of(*op, mmax, mmin);
+        r.y32       = pool.get(static_cast<size_t>(n) * sizeof(T32), r.cy32);
+        if(x0) {
+            r.x32 = pool.get(static_cast<size_t>(n) * sizeof(T32), r.cx32);
+            convert<T32, T64>(n, static_cast<const T64 *>(x0), static_cast<T32 *>(r.x32), s);
+        }
+        lo.add(op, r.x32, r.y32, id);
+        rec[id] = r;
+        ++nactive;
+    }
+
+    std::vector<Finished> step() {
+        std::vector<Finished> done;
+        if(lo.nactive > 0)
+            for(const auto &d : lo.step()) { // FP32 phase over: its Ritz vector starts the FP64 phase
+                Rec          &r = rec.at(d.id);
+                const int64_t n = r.op->D.n();
+                convert<T64, T32>(n, static_cast<const T32 *>(r.y32), static_cast<T64 *>(r.evec), s);
+                r.res1 = d.res;
+                hi.add(r.op, r.evec, r.evec, d.id, d.res.anorm);
+                pool.put(r.x32, r.cx32); // later users of these blocks are ordered after the conversion on s
+                pool.put(r.y32, r.cy32);
+                r.x32 = r.y32 = nullptr;
+            }
+        if(hi.nactive > 0)
+            for(const auto &d : hi.step()) {
+                auto it = rec.find(d.id);
+                Finished f{d.id, d.res, it->second.footprint};
+                f.res.matvecs_fp32 = it->second.res1.matvecs;
+                f.res.iters += it->second.res1.iters;
+                rec.erase(it);
+                --nactive;
+                done.push_back(f);
+            }
+        steps = lo.steps + hi.steps;
+        return done;
+    }
+};
+
+template <typename T, typename Eng = Engine<T>>
 void batched_impl(MatVecOp *const *ops, int k, const EigsParams &p, const void *const *x0s, void *const *evecs,
                   std::vector<BatchEigsResult> &res, hipStream_t s) {
-    Engine<T> E(s, p);
+    Eng E(s, p);
     for(int q = 0; q < k; ++q) E.add(ops[q], x0s ? x0s[q] : nullptr, evecs[q], q);
     res.assign(k, BatchEigsResult{});
     while(E.nactive > 0)
         for(const auto &d : E.step()) res[d.id] = d.res;
 }
 
-template <typename T>
+template <typename T, typename Eng = Engine<T>>
 void serve_impl(const std::vector<EigRequest> &req, const EigsParams &p, int64_t budget, int max_batch,
                 std::vector<EigRequestResult> &out, ServeSummary &sum, hipStream_t s) {
     using clock   = std::chrono::steady_clock;
     const auto t0 = clock::now();
     auto       now = [&] { return std::chrono::duration<double>(clock::now() - t0).count(); };
-    Engine<T>  E(s, p);
+    Eng        E(s, p);
     const int  n = static_cast<int>(req.size());
     out.assign(n, EigRequestResult{});
     std::vector<int> order(n);
@@ -693,8 +813,8 @@ void serve_impl(const std::vector<EigRequest> &req, const EigsParams &p, int64_t
     // A request that cannot fit in the budget even alone is an error of the whole call, as in dmrg_serve.
     for(int i = 0; i < n; ++i) {
         int mmax, mmin;
-        Engine<T>::basis_rule(p, req[i].op->D.n(), mmax, mmin);
-        DMRG_REQUIRE(Engine<T>::footprint_of(*req[i].op, mmax, mmin) <= budget, DMRG_ERR_OOM,
+        Eng::basis_rule(p, req[i].op->D.n(), mmax, mmin);
+        DMRG_REQUIRE(Eng::footprint_of(*req[i].op, mmax, mmin) <= budget, DMRG_ERR_OOM,
                      "request " + std::to_string(i) + " does not fit in the HBM budget");
     }
     size_t  next = 0;
@@ -707,8 +827,8 @@ void serve_impl(const std::vector<EigRequest> &req, const EigsParams &p, int64_t
             const EigRequest &r = req[order[next]];
             if(r.arrival_s > t) break;
             int mmax, mmin;
-            Engine<T>::basis_rule(p, r.op->D.n(), mmax, mmin);
-            const int64_t fp = Engine<T>::footprint_of(*r.op, mmax, mmin);
+            Eng::basis_rule(p, r.op->D.n(), mmax, mmin);
+            const int64_t fp = Eng::footprint_of(*r.op, mmax, mmin);
             if(admitted + fp > budget || (max_batch > 0 && E.nactive >= max_batch)) break;
             E.add(r.op, r.x0, r.evec, order[next]);
             EigRequestResult &o = out[order[next]];
@@ -871,27 +991,39 @@ EigService::Stats EigService::stats() const {
 }
 
 void eigs_batched(MatVecOp *const *ops, int k, const EigsParams &p, const void *const *x0s, void *const *evecs,
-                  std::vector<BatchEigsResult> &res, hipStream_t s) {
+                  std::vector<BatchEigsResult> &res, hipStream_t s, bool mixed) {
     DMRG_REQUIRE(ops && evecs && k >= 1, DMRG_ERR_ARG, "batched eigs: bad arguments");
     DMRG_REQUIRE(std::max(1, p.nev) == 1 && p.precond == 0 && (p.target == TARGET_SA || p.target == TARGET_LA), DMRG_ERR_UNSUPPORTED,
                  "batched eigs: one pair per problem at SA or LA, no preconditioner");
     const dmrg_dtype t = ops[0]->dtype;
     for(int q = 0; q < k; ++q) DMRG_REQUIRE(ops[q] && ops[q]->dtype == t && evecs[q], DMRG_ERR_ARG, "batched eigs: one dtype, outputs");
-    if(t == DMRG_FP64) batched_impl<double>(ops, k, p, x0s, evecs, res, s);
-    else if(t == DMRG_CX64) batched_impl<cpx<double>>(ops, k, p, x0s, evecs, res, s);
-    else throw Error(DMRG_ERR_ARG, "batched eigs: FP64 or CX64 operators");
+    if(t == DMRG_FP64) {
+        if(mixed) batched_impl<double, MixedEngine<double, float>>(ops, k, p, x0s, evecs, res, s);
+        else batched_impl<double>(ops, k, p, x0s, evecs, res, s);
+    } else if(t == DMRG_CX64) {
+        if(mixed) batched_impl<cpx<double>, MixedEngine<cpx<double>, cpx<float>>>(ops, k, p, x0s, evecs, res, s);
+        else batched_impl<cpx<double>>(ops, k, p, x0s, evecs, res, s);
+    } else {
+        throw Error(DMRG_ERR_ARG, "batched eigs: FP64 or CX64 operators");
+    }
 }
 
 void serve_eigs(const std::vector<EigRequest> &req, const EigsParams &p, int64_t budget, int max_batch,
-                std::vector<EigRequestResult> &out, ServeSummary &sum, hipStream_t s) {
+                std::vector<EigRequestResult> &out, ServeSummary &sum, hipStream_t s, bool mixed) {
     DMRG_REQUIRE(!req.empty(), DMRG_ERR_ARG, "serve eigs: no requests");
     DMRG_REQUIRE(std::max(1, p.nev) == 1 && p.precond == 0 && (p.target == TARGET_SA || p.target == TARGET_LA), DMRG_ERR_UNSUPPORTED,
                  "serve eigs: one pair per problem at SA or LA, no preconditioner");
     const dmrg_dtype t = req[0].op->dtype;
     for(const auto &r : req) DMRG_REQUIRE(r.op && r.op->dtype == t && r.evec, DMRG_ERR_ARG, "serve eigs: one dtype, outputs");
-    if(t == DMRG_FP64) serve_impl<double>(req, p, budget, max_batch, out, sum, s);
-    else if(t == DMRG_CX64) serve_impl<cpx<double>>(req, p, budget, max_batch, out, sum, s);
-    else throw Error(DMRG_ERR_ARG, "serve eigs: FP64 or CX64 operators");
+    if(t == DMRG_FP64) {
+        if(mixed) serve_impl<double, MixedEngine<double, float>>(req, p, budget, max_batch, out, sum, s);
+        else serve_impl<double>(req, p, budget, max_batch, out, sum, s);
+    } else if(t == DMRG_CX64) {
+        if(mixed) serve_impl<cpx<double>, MixedEngine<cpx<double>, cpx<float>>>(req, p, budget, max_batch, out, sum, s);
+        else serve_impl<cpx<double>>(req, p, budget, max_batch, out, sum, s);
+    } else {
+        throw Error(DMRG_ERR_ARG, "serve eigs: FP64 or CX64 operators");
+    }
 }
 
 } // namespace dmrg

@@ -15,14 +15,16 @@ namespace dmrg {
 struct BatchEigsResult {
     double  eigval = 0.0, rnorm = 0.0, anorm = 0.0;
     int     nconv = 0, iters = 0;
-    int64_t matvecs = 0;
+    int64_t matvecs = 0;      // FP64 matvecs
+    int64_t matvecs_fp32 = 0; // FP32 matvecs of the mixed-precision batch's first phase
 };
 
 // x0s[i]: optional start vector (device, may be null); evecs[i]: device output (n_i elements). p.nev must be 1,
 // p.target SA or LA, p.precond 0. Converged problems leave the batch; the grouped launches are rebuilt when
 // the set changes.
+// mixed: FP32 phase (FP32 vectors and matrix-core matvecs) to 5e-6 |A| or stagnation, then FP64 to p.tol.
 void eigs_batched(MatVecOp *const *ops, int k, const EigsParams &p, const void *const *x0s, void *const *evecs,
-                  std::vector<BatchEigsResult> &res, hipStream_t s);
+                  std::vector<BatchEigsResult> &res, hipStream_t s, bool mixed = false);
 
 // Continuous batching of eigensolve requests: a request joins the running batch between steps once it has
 // arrived (arrival_s, seconds after the call) and its working memory fits in the budget (and the batch has
@@ -45,7 +47,7 @@ struct ServeSummary {
     int64_t steps               = 0;
 };
 void serve_eigs(const std::vector<EigRequest> &req, const EigsParams &p, int64_t budget, int max_batch,
-                std::vector<EigRequestResult> &out, ServeSummary &sum, hipStream_t s);
+                std::vector<EigRequestResult> &out, ServeSummary &sum, hipStream_t s, bool mixed = false);
 
 // Eigensolve batcher shared by concurrent callers (the sweep workers of a serving loop): solve() blocks the
 // calling thread while its problem runs, on the service's own engine thread and stream, in one batch with the

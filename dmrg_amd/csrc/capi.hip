@@ -109,8 +109,8 @@ using namespace dmrg;
 
 namespace {
 dmrg::EigsParams batched_params(const dmrg_eigs_config *cfg) {
-    DMRG_REQUIRE(cfg->nev == 1 && cfg->precond == 0 && cfg->mixed == 0 && (cfg->target == DMRG_TARGET_SA || cfg->target == DMRG_TARGET_LA),
-                 DMRG_ERR_UNSUPPORTED, "batched eigs: nev 1, no preconditioner, FP64, target SA or LA");
+    DMRG_REQUIRE(cfg->nev == 1 && cfg->precond == 0 && (cfg->target == DMRG_TARGET_SA || cfg->target == DMRG_TARGET_LA),
+                 DMRG_ERR_UNSUPPORTED, "batched eigs: nev 1, no preconditioner, target SA or LA");
     DMRG_REQUIRE(cfg->tol > 0, DMRG_ERR_ARG, "tol must be positive");
     DMRG_REQUIRE(cfg->ncv <= 0 || (cfg->ncv >= 4 && cfg->ncv <= 64), DMRG_ERR_ARG, "batched eigs: ncv in [4, 64] or 0");
     EigsParams p;
@@ -129,6 +129,7 @@ dmrg_eigs_result batched_result(const BatchEigsResult &r, double sec) {
     o.nconv        = r.nconv;
     o.iters        = r.iters;
     o.matvecs_fp64 = r.matvecs;
+    o.matvecs_fp32 = r.matvecs_fp32;
     o.anorm        = r.anorm;
     o.seconds      = sec;
     return o;
@@ -478,6 +479,13 @@ dmrg_status dmrg_matvec_set_jcb_block_size(dmrg_matvec *op, int64_t size) {
     });
 }
 
+dmrg_status dmrg_matvec_set_factorization(dmrg_matvec *op, int factorization) {
+    return guard([&] {
+        DMRG_REQUIRE(op, DMRG_ERR_ARG, "null operator");
+        reinterpret_cast<MatVecOp *>(op)->set_factorization(factorization);
+    });
+}
+
 dmrg_status dmrg_matvec_precond(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec,
                                 const double *shifts) {
     return guard([&] {
@@ -600,7 +608,7 @@ dmrg_status dmrg_eigs_batched(dmrg_matvec *const *ops, int k, const dmrg_eigs_co
         }
         const auto                   t0 = std::chrono::steady_clock::now();
         std::vector<BatchEigsResult> r;
-        eigs_batched(v.data(), k, p, x0s, evecs, r, static_cast<hipStream_t>(stream));
+        eigs_batched(v.data(), k, p, x0s, evecs, r, static_cast<hipStream_t>(stream), cfg->mixed != 0);
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         for(int i = 0; i < k; ++i) results[i] = batched_result(r[i], sec);
     });
@@ -622,7 +630,7 @@ dmrg_status dmrg_serve_eigs(const dmrg_eigs_config *cfg, const dmrg_eig_request 
         }
         std::vector<EigRequestResult> r;
         ServeSummary                  sum;
-        serve_eigs(rq, p, hbm_budget_bytes, max_batch, r, sum, static_cast<hipStream_t>(stream));
+        serve_eigs(rq, p, hbm_budget_bytes, max_batch, r, sum, static_cast<hipStream_t>(stream), cfg->mixed != 0);
         std::vector<double> lat(nreq);
         double              mean = 0;
         int                 conv = 0;

@@ -107,6 +107,17 @@ constexpr int SMALLK_MAX_M = 64;
 
 // Launch one GEMM. cfg/ksplit = AUTO/0 pick from the tuning table or the size heuristic.
 void gemm(const GemmDesc &g, hipStream_t stream, TileCfg cfg = TileCfg::AUTO, int ksplit = 0);
+// While an object of this type lives, gemm() on this thread never times candidate plans (no events, no
+// synchronisation): it launches a plan already tuned for the signature, else the size heuristic's.
+struct NoTuneScope {
+    NoTuneScope();
+    ~NoTuneScope();
+    NoTuneScope(const NoTuneScope &)            = delete;
+    NoTuneScope &operator=(const NoTuneScope &) = delete;
+
+  private:
+    bool prev_;
+};
 
 // Grouped GEMM: independent GEMMs (batch 1, no split-K) of one tile configuration run as one launch. The
 // descriptors and the tile prefix sums are device arrays filled by grouped_prepare (one group per

@@ -37,6 +37,7 @@ Dims check_matvec_dims(const int64_t mps[3], const int64_t mpo[4], const int64_t
 constexpr int64_t MAX_KSPLIT = 8;
 
 int64_t matvec_ws_elems(const Dims &D) { return D.d * D.chiL * D.chiR * (D.DL + D.DR) + MAX_KSPLIT * D.n(); }
+int64_t matvec_ws_elems_nopart(const Dims &D) { return D.d * D.chiL * D.chiR * (D.DL + D.DR); }
 
 static GemmDesc make(dmrg_dtype t, int64_t M, int64_t N, int64_t K, int nvec) {
     DMRG_REQUIRE(M < (int64_t(1) << 31) && N < (int64_t(1) << 31) && K < (int64_t(1) << 31), DMRG_ERR_DIMS,
@@ -51,7 +52,7 @@ static GemmDesc make(dmrg_dtype t, int64_t M, int64_t N, int64_t K, int nvec) {
 }
 
 void matvec_descs(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *out, int64_t ldy, int nvec,
-                  const void *mpo, const void *envL, const void *envR, void *ws, GemmDesc leg[3]) {
+                  const void *mpo, const void *envL, const void *envR, void *ws, GemmDesc leg[3], bool partials) {
     const int64_t d = D.d, cL = D.chiL, cR = D.chiR, DL = D.DL, DR = D.DR;
     const size_t  es  = dtype_size(t);
     char         *w0  = static_cast<char *>(ws);
@@ -59,8 +60,8 @@ void matvec_descs(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, v
     const int64_t nT2 = (cL >= cR) ? cR * DR * d * cL : cL * DL * cR * d; // second intermediate per vector
     void         *T1  = w0;
     void         *T2  = w0 + static_cast<size_t>(nvec) * nT1 * es;
-    void         *PT  = w0 + static_cast<size_t>(nvec) * (nT1 + nT2) * es; // split-K partials (leg 1, then leg 3)
-    const int64_t PC  = MAX_KSPLIT * D.n() * nvec;
+    void         *PT  = partials ? w0 + static_cast<size_t>(nvec) * (nT1 + nT2) * es : nullptr; // split-K partials
+    const int64_t PC  = partials ? MAX_KSPLIT * D.n() * nvec : 0;                            // (leg 1, then leg 3)
 
     if(cL >= cR) {
         // Order of contraction_tblis.cpp:279-283 (theta.L first).
@@ -110,9 +111,9 @@ void matvec_descs(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, v
 }
 
 void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *out, int64_t ldy, int nvec,
-            const void *mpo, const void *envL, const void *envR, void *ws, hipStream_t s, hipEvent_t *ev) {
+            const void *mpo, const void *envL, const void *envR, void *ws, hipStream_t s, hipEvent_t *ev, bool partials) {
     GemmDesc leg[3];
-    matvec_descs(t, D, theta, ldx, out, ldy, nvec, mpo, envL, envR, ws, leg);
+    matvec_descs(t, D, theta, ldx, out, ldy, nvec, mpo, envL, envR, ws, leg, partials);
     auto mark = [&](int i) {
         if(ev) DMRG_HIP_CHECK(hipEventRecord(ev[i], s));
     };
@@ -124,6 +125,64 @@ void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *o
     if(ev) leg[2].ev_after_main = ev[3];
     gemm(leg[2], s);
     mark(4);
+}
+
+// ------------------------------------------------------------------------------------------
+// MPO-chain matvec (more than 5 MPOs, contraction_tblis.cpp:307-420): theta.L, then one MPO at a time over its
+// physical index and its left bond, then .R. The intermediates keep one layout, X[s, b, a', w] with the fused
+// physical index s = (s_0, ..., s_{n-1}) (site 0 fastest); MPO i turns s_i into s_i' and w_i into w_{i+1}, so the
+// sites before i hold outputs and the sites after it inputs, as the reference's reshapes do (:380-406).
+
+double chain_macs(const Dims &D, int nmpo, const ChainMpo *mpos) {
+    double m = static_cast<double>(D.d) * D.chiL * D.chiL * D.chiR * D.DL + static_cast<double>(D.d) * D.chiL * D.chiR * D.chiR * D.DR;
+    for(int i = 0; i < nmpo; ++i) m += static_cast<double>(D.d) * mpos[i].d * D.chiL * D.chiR * mpos[i].DL * mpos[i].DR;
+    return m;
+}
+
+int64_t chain_ws_elems(const Dims &D, int nmpo, const ChainMpo *mpos) {
+    int64_t maxD = std::max(D.DL, D.DR);
+    for(int i = 0; i < nmpo; ++i) maxD = std::max({maxD, mpos[i].DL, mpos[i].DR});
+    return 2 * D.d * D.chiL * D.chiR * maxD + MAX_KSPLIT * D.n();
+}
+
+void matvec_chain(dmrg_dtype t, const Dims &D, int nmpo, const ChainMpo *mpos, const void *theta, int64_t ldx, void *out,
+                  int64_t ldy, int nvec, const void *envL, const void *envR, void *ws, hipStream_t s) {
+    const int64_t d = D.d, cL = D.chiL, cR = D.chiR;
+    int64_t       maxD = std::max(D.DL, D.DR);
+    for(int i = 0; i < nmpo; ++i) maxD = std::max({maxD, mpos[i].DL, mpos[i].DR});
+    const size_t  es = dtype_size(t);
+    const int64_t nX = d * cR * cL * maxD; // one intermediate per vector
+    char         *w0 = static_cast<char *>(ws);
+    void         *X[2] = {w0, w0 + static_cast<size_t>(nvec) * nX * es};
+    void         *PT   = w0 + static_cast<size_t>(2 * nvec) * nX * es;
+    const int64_t PC   = MAX_KSPLIT * D.n() * nvec;
+    // theta.L (contraction_tblis.cpp:358): X0[s,b,a',w] = sum_a theta[s,a,b] L[a,a',w]     m=(s,b) k=a n=(a',w)
+    GemmDesc g1 = make(t, d * cR, cL * D.DL, cL, nvec);
+    g1.A = theta, g1.am = fused(1, d * cL, d), g1.ak = plain(d), g1.a_batch = ldx;
+    g1.B = envL, g1.bk = plain(1), g1.bn = plain(cL);
+    g1.C = X[0], g1.cm = plain(1), g1.cn = plain(d * cR), g1.c_batch = nX;
+    g1.part = PT, g1.part_capacity = PC;
+    gemm(g1, s);
+    // MPO i (:380-406): X'[pre,s_i',post,b,a',w'] = sum_{s_i,w} W_i[w,w',s_i,s_i'] X[pre,s_i,post,b,a',w]
+    //   m=(s_i',w') k=(s_i,w) n=(pre,(post,b,a')); pre spans the P = d_0...d_{i-1} outputs already made.
+    int64_t P = 1;
+    for(int i = 0; i < nmpo; ++i) {
+        const ChainMpo &w  = mpos[i];
+        const int64_t   di = w.d, DLi = w.DL, DRi = w.DR;
+        GemmDesc        g  = make(t, di * DRi, (d / di) * cR * cL, di * DLi, nvec);
+        g.A = w.W, g.am = fused(DLi * DRi * di, DLi, di), g.ak = fused(DLi * DRi, 1, di);
+        g.B = X[i & 1], g.bk = fused(P, d * cR * cL, di), g.bn = fused(1, P * di, P), g.b_batch = nX;
+        g.C = X[(i + 1) & 1], g.cm = fused(P, d * cR * cL, di), g.cn = fused(1, P * di, P), g.c_batch = nX;
+        gemm(g, s);
+        P *= di;
+    }
+    // .R (:417-419): out[s',a',b'] = sum_{b,w'} X[s',b,a',w'] R[b,b',w']     m=(s',a') k=(b,w') n=b'
+    GemmDesc g3 = make(t, d * cL, cR, cR * D.DR, nvec);
+    g3.A = X[nmpo & 1], g3.am = fused(1, d * cR, d), g3.ak = fused(d, d * cR * cL, cR), g3.a_batch = nX;
+    g3.B = envR, g3.bk = fused(1, cR * cR, cR), g3.bn = plain(cR);
+    g3.C = out, g3.cm = plain(1), g3.cn = plain(d * cL), g3.c_batch = ldy;
+    g3.part = PT, g3.part_capacity = PC;
+    gemm(g3, s);
 }
 
 int64_t env_ws_elems(const Dims &D) {

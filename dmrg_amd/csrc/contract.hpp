@@ -30,10 +30,26 @@ int64_t matvec_ws_elems(const Dims &D);
 // If ev is non-null, ev[0..4] are recorded on s before leg 1, after legs 1 and 2, after the main kernel of
 // leg 3 and after leg 3 (its split-K reduce, when there is one).
 // The three GEMMs of one matvec (legs 1-3 in launch order), as matvec launches them.
+// partials = false: ws holds only the two intermediates (nvec * matvec_ws_elems_nopart elements) and no leg is
+// split over K.
 void matvec_descs(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *out, int64_t ldy, int nvec,
-                  const void *mpo, const void *envL, const void *envR, void *ws, GemmDesc leg[3]);
+                  const void *mpo, const void *envL, const void *envR, void *ws, GemmDesc leg[3], bool partials = true);
 void matvec(dmrg_dtype t, const Dims &D, const void *theta, int64_t ldx, void *out, int64_t ldy, int nvec,
-            const void *mpo, const void *envL, const void *envR, void *ws, hipStream_t s, hipEvent_t *ev = nullptr);
+            const void *mpo, const void *envL, const void *envR, void *ws, hipStream_t s, hipEvent_t *ev = nullptr,
+            bool partials = true);
+int64_t matvec_ws_elems_nopart(const Dims &D);
+
+// One MPO of a chain applied one at a time: device array (DL, DR, d, d) in the matvec's dtype.
+struct ChainMpo {
+    const void *W  = nullptr;
+    int64_t     DL = 0, DR = 0, d = 0;
+};
+// MPO-chain matvec (contraction_tblis.cpp:307-420): D.d is the product of the chain's physical dims, D.DL the
+// first MPO's left bond and D.DR the last one's right bond. ws holds nvec * chain_ws_elems elements.
+double  chain_macs(const Dims &D, int nmpo, const ChainMpo *mpos);
+int64_t chain_ws_elems(const Dims &D, int nmpo, const ChainMpo *mpos);
+void    matvec_chain(dmrg_dtype t, const Dims &D, int nmpo, const ChainMpo *mpos, const void *theta, int64_t ldx, void *out,
+                     int64_t ldy, int nvec, const void *envL, const void *envR, void *ws, hipStream_t s);
 
 // Environment update (EnvBase::build_block). For side 'L' the output is (chiR, chiR, DR); for 'R' (chiL, chiL, DL).
 int64_t env_ws_elems(const Dims &D);

@@ -984,6 +984,10 @@ int tuned_entries() {
     return static_cast<int>(g_tuned.size());
 }
 
+static thread_local bool g_tls_no_tune = false;
+NoTuneScope::NoTuneScope() : prev_(g_tls_no_tune) { g_tls_no_tune = true; }
+NoTuneScope::~NoTuneScope() { g_tls_no_tune = prev_; }
+
 void gemm(const GemmDesc &gin, hipStream_t stream, TileCfg cfg, int ksplit) {
     if(gin.M <= 0 || gin.N <= 0 || gin.batch <= 0) return;
     GemmDesc g = gin;
@@ -1003,7 +1007,7 @@ void gemm(const GemmDesc &gin, hipStream_t stream, TileCfg cfg, int ksplit) {
         // add up) and while the stream is being captured into a graph.
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         DMRG_HIP_CHECK(hipStreamIsCapturing(stream, &cs));
-        const bool can_tune = g_autotune && !g.accumulate && cs == hipStreamCaptureStatusNone;
+        const bool can_tune = g_autotune && !g.accumulate && cs == hipStreamCaptureStatusNone && !g_tls_no_tune;
         const std::string key = tune_key(g);
         bool              found = false;
         {

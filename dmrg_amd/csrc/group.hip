@@ -106,6 +106,7 @@ MatvecGroup *matvec_group_create(MatVecOp *const *ops, int k, const void *const 
     for(int i = 0; i < k; ++i) {
         MatVecOp &op = *ops[i];
         DMRG_REQUIRE(op.dtype == t0, DMRG_ERR_ARG, "matvec group: operators of one dtype");
+        DMRG_REQUIRE(!op.chain, DMRG_ERR_UNSUPPORTED, "matvec group: operators with more than 5 MPOs (applied in sequence)");
         DMRG_REQUIRE(xs[i] && ys[i], DMRG_ERR_ARG, "matvec group: null vector");
         if(fp32) {
             op.ensure_fp32(nullptr);

@@ -60,14 +60,7 @@ MatVecOp::MatVecOp(dmrg_dtype t, int nmpo, const void *const *mpos, const int64_
                    const int64_t envL_dims[3], const void *envR_h, const int64_t envR_dims[3])
     : dtype(t), nmpo_in(nmpo) {
     DMRG_REQUIRE(envL_h && envR_h && envL_dims && envR_dims, DMRG_ERR_ARG, "null env");
-    int64_t md[4];
-    mpo_host = merge_mpos(nmpo, reinterpret_cast<const cx *const *>(mpos), mpo_dims, md);
-    const int64_t mps_dims[3] = {md[2], envL_dims[0], envR_dims[0]};
-    D                         = check_matvec_dims(mps_dims, md, envL_dims, envR_dims);
-    if(t == DMRG_FP64) {
-        // MatVecMPOS<fp64> requires a real MPO (matvec_mpos.cpp:59); env blocks are cast with .real().
-        for(const auto &v : mpo_host) DMRG_REQUIRE(v.imag() == 0.0, DMRG_ERR_ARG, "mpo is not real");
-    }
+    load_mpos(nmpo, mpos, mpo_dims, envL_dims, envR_dims);
     upload_mpo();
     upload_as<double>(envL, static_cast<const cx *>(envL_h), D.chiL * D.chiL * D.DL, t);
     upload_as<double>(envR, static_cast<const cx *>(envR_h), D.chiR * D.chiR * D.DR, t);
@@ -84,12 +77,7 @@ MatVecOp::MatVecOp(dmrg_dtype t, int nmpo, const void *const *mpos, const int64_
 void MatVecOp::rebind(int nmpo, const void *const *mpos, const int64_t *mpo_dims, const void *envL_d,
                       const int64_t envL_dims[3], const void *envR_d, const int64_t envR_dims[3]) {
     DMRG_REQUIRE(envL_d && envR_d && envL_dims && envR_dims, DMRG_ERR_ARG, "null env");
-    int64_t md[4];
-    mpo_host = merge_mpos(nmpo, reinterpret_cast<const cx *const *>(mpos), mpo_dims, md);
-    const int64_t mps_dims[3] = {md[2], envL_dims[0], envR_dims[0]};
-    D                         = check_matvec_dims(mps_dims, md, envL_dims, envR_dims);
-    if(dtype == DMRG_FP64)
-        for(const auto &v : mpo_host) DMRG_REQUIRE(v.imag() == 0.0, DMRG_ERR_ARG, "mpo is not real");
+    load_mpos(nmpo, mpos, mpo_dims, envL_dims, envR_dims);
     nmpo_in     = nmpo;
     sigma       = cx(0.0, 0.0);
     ready_shift = false;
@@ -98,8 +86,53 @@ void MatVecOp::rebind(int nmpo, const void *const *mpos, const int64_t *mpo_dims
     envR_p = envR_d;
 }
 
+void MatVecOp::load_mpos(int nmpo, const void *const *mpos, const int64_t *mpo_dims, const int64_t envL_dims[3],
+                         const int64_t envR_dims[3]) {
+    int64_t md[4];
+    mpo_host = merge_mpos(nmpo, reinterpret_cast<const cx *const *>(mpos), mpo_dims, md);
+    const int64_t mps_dims[3] = {md[2], envL_dims[0], envR_dims[0]};
+    D                         = check_matvec_dims(mps_dims, md, envL_dims, envR_dims);
+    // MatVecMPOS<fp64> requires real MPOs (matvec_mpos.cpp:59); env blocks are cast with .real().
+    auto require_real = [&](const cx *v, size_t n) {
+        if(dtype != DMRG_FP64) return;
+        for(size_t i = 0; i < n; ++i) DMRG_REQUIRE(v[i].imag() == 0.0, DMRG_ERR_ARG, "mpo is not real");
+    };
+    require_real(mpo_host.data(), mpo_host.size());
+    chain = nmpo > 5; // matvec_mpos.cpp:104-123: at most 5 MPOs are merged, longer lists applied in sequence
+    chain_host.clear();
+    chain_dims.clear();
+    if(chain) {
+        for(int i = 0; i < nmpo; ++i) {
+            const int64_t *dm = mpo_dims + 4 * i;
+            const cx      *src = static_cast<const cx *>(mpos[i]);
+            chain_host.emplace_back(src, src + dm[0] * dm[1] * dm[2] * dm[3]);
+            chain_dims.insert(chain_dims.end(), dm, dm + 4);
+            require_real(src, chain_host.back().size());
+        }
+        chain_dev.resize(nmpo);
+        chain_dev32.resize(nmpo);
+    }
+}
+
+double MatVecOp::macs() const {
+    if(!chain) return D.macs();
+    const auto v = chain_view(false);
+    return chain_macs(D, static_cast<int>(v.size()), v.data());
+}
+
+std::vector<ChainMpo> MatVecOp::chain_view(bool fp32) const {
+    std::vector<ChainMpo> v(chain_host.size());
+    for(size_t i = 0; i < v.size(); ++i) {
+        v[i].W  = fp32 ? chain_dev32[i].p : chain_dev[i].p;
+        v[i].DL = chain_dims[4 * i], v[i].DR = chain_dims[4 * i + 1], v[i].d = chain_dims[4 * i + 2];
+    }
+    return v;
+}
+
 void MatVecOp::upload_mpo() {
     upload_as<double>(mpo, mpo_host.data(), static_cast<int64_t>(mpo_host.size()), dtype);
+    for(size_t i = 0; i < chain_host.size(); ++i)
+        upload_as<double>(chain_dev[i], chain_host[i].data(), static_cast<int64_t>(chain_host[i].size()), dtype);
     have32     = false;
     diag_valid = false;
     pc_ready   = false;
@@ -115,6 +148,11 @@ void MatVecOp::ensure_fp32(hipStream_t s) {
     convert_f64_to_f32(nm, mpo.as<double>(), mpo32.as<float>(), s);
     convert_f64_to_f32(nl, static_cast<const double *>(envL_p), envL32.as<float>(), s);
     convert_f64_to_f32(nr, static_cast<const double *>(envR_p), envR32.as<float>(), s);
+    for(size_t i = 0; i < chain_host.size(); ++i) {
+        const int64_t nc = static_cast<int64_t>(chain_host[i].size()) * cf;
+        chain_dev32[i].reserve(nc * sizeof(float));
+        convert_f64_to_f32(nc, chain_dev[i].as<double>(), chain_dev32[i].as<float>(), s);
+    }
     have32 = true;
 }
 
@@ -122,17 +160,20 @@ void MatVecOp::multax_device(const void *dx, int64_t ldx, void *dy, int64_t ldy,
     if(nvec == 0) return;
     const int64_t n = D.n();
     DMRG_REQUIRE(ldx >= n && ldy >= n, DMRG_ERR_ARG, "leading dimension smaller than rows()");
-    if(fp32) {
-        ensure_fp32(s);
-        const dmrg_dtype t32 = dtype_single_of(dtype);
-        void *w = ws32.reserve(static_cast<size_t>(nvec) * matvec_ws_elems(D) * dtype_size(t32));
-        matvec(t32, D, dx, ldx, dy, ldy, nvec, mpo32.p, envL32.p, envR32.p, w, s);
+    if(fp32) ensure_fp32(s);
+    const dmrg_dtype t  = fp32 ? dtype_single_of(dtype) : dtype;
+    DevBuf          &wb = fp32 ? ws32 : ws;
+    if(chain) {
+        const auto v = chain_view(fp32);
+        void      *w = wb.reserve(static_cast<size_t>(nvec) * chain_ws_elems(D, static_cast<int>(v.size()), v.data()) * dtype_size(t));
+        matvec_chain(t, D, static_cast<int>(v.size()), v.data(), dx, ldx, dy, ldy, nvec, fp32 ? envL32.p : envL_p,
+                     fp32 ? envR32.p : envR_p, w, s);
     } else {
-        void *w = ws.reserve(static_cast<size_t>(nvec) * matvec_ws_elems(D) * dtype_size(dtype));
-        matvec(dtype, D, dx, ldx, dy, ldy, nvec, mpo.p, envL_p, envR_p, w, s);
+        void *w = wb.reserve(static_cast<size_t>(nvec) * matvec_ws_elems(D) * dtype_size(t));
+        matvec(t, D, dx, ldx, dy, ldy, nvec, fp32 ? mpo32.p : mpo.p, fp32 ? envL32.p : envL_p, fp32 ? envR32.p : envR_p, w, s);
     }
     num_mv += nvec;
-    flops += static_cast<double>(nvec) * D.macs() * (dtype_is_complex(dtype) ? 8.0 : 2.0);
+    flops += static_cast<double>(nvec) * macs() * (dtype_is_complex(dtype) ? 8.0 : 2.0);
 }
 
 void MatVecOp::multax_host(const void *x, int64_t ldx, void *y, int64_t ldy, int nvec) {
@@ -148,13 +189,29 @@ void MatVecOp::multax_host(const void *x, int64_t ldx, void *y, int64_t ldy, int
 }
 
 void MatVecOp::set_shift(cx shift) {
-    // matvec_mpos.cpp:1437-1471: applied once (readyShift), on the merged MPO's bottom-left corner.
+    // matvec_mpos.cpp:1437-1471: applied once (readyShift); (sigma - shift) / N_mpo times the identity is added to
+    // the bottom-left corner W[DL-1, 0] of each of the N_mpo MPOs the operator holds: the one merged MPO of a list
+    // of up to 5, or every MPO of a longer chain.
     if(ready_shift) return;
     if(sigma == shift) return;
-    const cx delta = sigma - shift; // one merged MPO: shift per MPO = shift / 1
-    for(int64_t s = 0; s < D.d; ++s) {
-        cx &v = mpo_host[(D.DL - 1) + D.DL * (0 + D.DR * (s + D.d * s))];
-        v += (dtype == DMRG_FP64) ? cx(delta.real(), 0.0) : delta;
+    auto add_corner = [&](std::vector<cx> &W, int64_t DL, int64_t DR, int64_t d, cx delta) {
+        for(int64_t s = 0; s < d; ++s) {
+            cx &v = W[(DL - 1) + DL * (0 + DR * (s + d * s))];
+            v += (dtype == DMRG_FP64) ? cx(delta.real(), 0.0) : delta;
+        }
+    };
+    if(chain) {
+        const int n     = static_cast<int>(chain_host.size());
+        const cx  delta = (sigma - shift) / static_cast<double>(n);
+        std::vector<const void *> ptrs(n);
+        for(int i = 0; i < n; ++i) {
+            add_corner(chain_host[i], chain_dims[4 * i], chain_dims[4 * i + 1], chain_dims[4 * i + 2], delta);
+            ptrs[i] = chain_host[i].data();
+        }
+        int64_t md[4];
+        mpo_host = merge_mpos(n, reinterpret_cast<const cx *const *>(ptrs.data()), chain_dims.data(), md);
+    } else {
+        add_corner(mpo_host, D.DL, D.DR, D.d, sigma - shift);
     }
     sigma = shift;
     upload_mpo();
@@ -205,6 +262,7 @@ void MatVecOp::precond_host(const void *x, int64_t ldx, void *y, int64_t ldy, in
 }
 
 void MatVecOp::time_legs(const void *dx, void *dy, int nrep, bool fp32, float ms[4], hipStream_t s) {
+    DMRG_REQUIRE(!chain, DMRG_ERR_UNSUPPORTED, "leg timing of an operator with an MPO chain");
     hipEvent_t ev[5];
     for(auto &e : ev) DMRG_HIP_CHECK(hipEventCreate(&e));
     double acc[4] = {0, 0, 0, 0};

@@ -66,6 +66,12 @@ struct MatVecOp {
     bool                               have32 = false;
     DevBuf                             ws, ws32, xbuf, ybuf, diagws, diagc;
     bool                               diag_valid = false; // diagc holds the current operator's diagonal
+    // More than 5 MPOs (matvec_mpos.cpp:120-123): the matvec applies them one at a time (matvec_chain); the merged
+    // MPO above still serves the diagonal, the dense matrix and the block preconditioner.
+    bool                               chain = false;
+    std::vector<std::vector<std::complex<double>>> chain_host; // per MPO, cx64, shift applied
+    std::vector<int64_t>               chain_dims;            // 4 per MPO
+    std::vector<DevBuf>                chain_dev, chain_dev32;
     std::complex<double>               sigma{0.0, 0.0};
     bool                               ready_shift = false;
     int64_t                            num_mv      = 0;
@@ -81,7 +87,13 @@ struct MatVecOp {
     void rebind(int nmpo, const void *const *mpos, const int64_t *mpo_dims, const void *envL_d, const int64_t envL_dims[3],
                 const void *envR_d, const int64_t envR_dims[3]);
 
+    // Multiply-accumulates of one matvec (the chain's own count when the MPOs are applied one at a time).
+    double macs() const;
+    std::vector<ChainMpo> chain_view(bool fp32) const;
     void upload_mpo();
+    // Merged MPO, dims, and (for more than 5 MPOs) the chain copies, from host cx64 data.
+    void load_mpos(int nmpo, const void *const *mpos, const int64_t *mpo_dims, const int64_t envL_dims[3],
+                   const int64_t envR_dims[3]);
     void ensure_fp32(hipStream_t s);
     void multax_device(const void *dx, int64_t ldx, void *dy, int64_t ldy, int nvec, hipStream_t s, bool fp32);
     void multax_host(const void *x, int64_t ldx, void *y, int64_t ldy, int nvec);
@@ -97,7 +109,9 @@ struct MatVecOp {
     int64_t jcb      = 1;
     bool    pc_ready = false;
     double  pc_shift = 0.0;
-    DevBuf  pcinv, pcx, pcy, pcpiv, pcinfo;
+    DevBuf  pcinv, pcx, pcy, pcpiv, pcinfo, pcchol, pcneg;
+    int     pc_fact = 3; // eig::Factorization of the blocks: 1 LLT (sign rule of the reference), 3 LU (also LDLT, QR)
+    void    set_factorization(int f);
     void   *blas = nullptr; // rocblas_handle, created on first use
     void    set_jcb(int64_t size);
     void    block_pc_build(double shift, hipStream_t s);

@@ -306,7 +306,7 @@ struct Fdmrg {
             ev = r.eigval, rn = r.rnorm;
             st.matvecs = r.matvecs;
             op.num_mv += r.matvecs;
-            op.flops += static_cast<double>(r.matvecs) * op.D.macs() * (dtype_is_complex(op.dtype) ? 8.0 : 2.0);
+            op.flops += static_cast<double>(r.matvecs) * op.macs() * (dtype_is_complex(op.dtype) ? 8.0 : 2.0);
         } else if(cfg.mixed && !folded && n >= mixed_min) {
             EigsStats s1;
             p.inner_max = 20; // FP32 correction solves in the FP64 phase

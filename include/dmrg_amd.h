@@ -166,6 +166,11 @@ dmrg_status dmrg_matvec_get_diagonal(dmrg_matvec *op, void *ddiag, void *stream)
  * dmrg_matvec_precond, > 1 the block-Jacobi one (inverses of the diagonal blocks of (H_eff - shift) of this size,
  * built at the next dmrg_matvec_precond call with its first shift, and again after set_shift or a size change). */
 dmrg_status dmrg_matvec_set_jcb_block_size(dmrg_matvec *op, int64_t size);
+/* The MatVecMPOS::factorization of the block-Jacobi blocks (matvec_mpos.cpp:1143-1240), as eig::Factorization
+ * values: 1 LLT, 2 LDLT, 3 LU (the default), 4 QR; others DMRG_ERR_UNSUPPORTED. LDLT, LU and QR solve every
+ * nonsingular block exactly and share the LU inverse; LLT keeps the reference's sign rule: a block with
+ * Re B(0,0) <= 0 whose negation is positive definite is applied as -B^-1, every other block as B^-1. */
+dmrg_status dmrg_matvec_set_factorization(dmrg_matvec *op, int factorization);
 dmrg_status dmrg_matvec_precond(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec,
                                 const double *shifts);
 
@@ -248,8 +253,11 @@ dmrg_status dmrg_eigs_folded_spectrum(dmrg_matvec *h, dmrg_matvec *h2, const dmr
  * restarts): per step one grouped matvec of all operators (replayed from a HIP graph), grouped block-vector
  * kernels and one host synchronisation for the whole batch; a converged problem leaves the batch. No counterpart in the reference (eig::solver::eigs solves
  * one operator, solver_primme.cpp:297-517; the method and the basis rule are the same as dmrg_eigs): for request
- * loops with many small resident problems. cfg: nev 1, precond 0, mixed 0, target SA or LA; tol, ncv, max_matvecs,
- * seed as in dmrg_eigs. x0s: NULL or k device start vectors (entries may be NULL); evecs: k device outputs. */
+ * loops with many small resident problems. cfg: nev 1, precond 0, target SA or LA; tol, ncv, max_matvecs,
+ * seed as in dmrg_eigs. cfg->mixed = 1: each problem first runs in an FP32 batch (FP32 vectors, grouped FP32
+ * matrix-core matvecs, double reductions) to 5e-6 |A| or until its residual stops dropping by 10% over 25 steps,
+ * then its FP32 Ritz vector starts it in the FP64 batch, which converges it to tol; results report both matvec
+ * counts. x0s: NULL or k device start vectors (entries may be NULL); evecs: k device outputs. */
 dmrg_status dmrg_eigs_batched(dmrg_matvec *const *ops, int k, const dmrg_eigs_config *cfg, const void *const *x0s,
                               void *const *evecs, dmrg_eigs_result *results, void *stream);
 

@@ -11,19 +11,25 @@
 //                               (source/tools/common/contraction.h:31-36) with host pointers.
 //   dmrg_amd::contraction::expectation_value<Scalar> (contraction_tblis.cpp:30-97).
 //
-// MatVecMPOS<T> also carries the members eig::solver::eigs_primme<MatrixProductType> reads
-// (solver_primme.cpp:297-517): Scalar, can_shift / can_shift_invert / can_precondition, set_shift,
-// isReadyShift, isReadyFactorOp, preconditioner, set_jcbMaxBlockSize, rows, num_mv / num_op / num_pc,
-// t_multAx / t_multOPv / t_multPc, and the Jacobi MultPc callback (optimize_energy.cpp:34-42), so that it can
-// stand in for the reference's operator type in that template. Shift-invert (FactorOP / MultOPv) is not
-// provided: isReadyFactorOp() is false, which keeps PRIMME on MultAx.
+// MatVecMPOS<T> also carries the members the reference's solver templates read, with the same types:
+//   eig::solver::eigs_primme<MatrixProductType> (solver_primme.cpp:297-517): Scalar, can_shift /
+//     can_shift_invert / can_precondition, set_shift, isReadyShift, isReadyFactorOp, preconditioner,
+//     set_jcbMaxBlockSize(std::optional<long>), rows, num_op, t_multOPv, and the MultPc callback of
+//     preconditioner_jacobi (optimize_energy.cpp:34-42; optimize_folded_spectrum.cpp:31-39);
+//   eig::solver_arpack<MatrixType> (solver_arpack.cpp:77-244, 320-380): set_mode, set_side, get_form, get_side,
+//     rows, MultAx(T*, T*) (also as the member pointer &MatrixType::MultAx) and the fields num_mv, num_op.
+// Block-Jacobi MultPc (jcbMaxBlockSize > 1) runs on the GPU with the factorization chosen by the field
+// factorization (LLT, or LU for every other choice; matvec_mpos.cpp:1086-1331). Shift-invert (FactorOP /
+// MultOPv) is not provided: isReadyFactorOp() is false, which keeps PRIMME and ARPACK on MultAx.
 #pragma once
 #include "../dmrg_amd.h"
 
+#include <algorithm>
 #include <array>
 #include <chrono>
 #include <complex>
 #include <memory>
+#include <optional>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -49,14 +55,26 @@ constexpr dmrg_dtype dtype_of() {
     return std::is_same_v<T, double> ? DMRG_FP64 : DMRG_CX64;
 }
 
-// Accepts assignments from any enumeration (eig::Preconditioner, eig::Factorization ...) and keeps the value.
+// Accepts assignments from any enumeration (eig::Preconditioner, eig::Factorization, eig::Form ...), keeps the
+// value and compares equal to the enumerator of that value, so that the reference's expressions
+// (matrix.preconditioner = eig::Preconditioner::JACOBI; matrix.get_form() != Form::SYMM) compile unchanged.
 struct EnumValue {
     int value = 0;
-    template <typename E>
+    EnumValue() = default;
+    explicit EnumValue(int v) : value(v) {}
+    template <typename E, typename = std::enable_if_t<std::is_enum_v<E>>>
+    EnumValue(E e) : value(static_cast<int>(e)) {}
+    template <typename E, typename = std::enable_if_t<std::is_enum_v<E>>>
     EnumValue &operator=(E e) {
         value = static_cast<int>(e);
         return *this;
     }
+    template <typename E, typename = std::enable_if_t<std::is_enum_v<E>>>
+    bool operator==(E e) const { return value == static_cast<int>(e); }
+    template <typename E, typename = std::enable_if_t<std::is_enum_v<E>>>
+    bool operator!=(E e) const { return value != static_cast<int>(e); }
+    template <typename E, typename = std::enable_if_t<std::is_enum_v<E>>>
+    operator E() const { return static_cast<E>(value); }
 };
 
 // Accumulating wall-clock timer with the get_time() the reference's tid::ur timers expose.
@@ -73,8 +91,8 @@ class MatVecMPOS {
     static constexpr bool can_shift_invert  = false;
     static constexpr bool can_precondition  = true;
     EnumValue             preconditioner;   // eig::Preconditioner: NONE (0) or JACOBI
-    EnumValue             factorization;    // accepted and ignored (no shift-invert)
-    long                  num_op = 0, num_pc = 0;
+    EnumValue             factorization;    // eig::Factorization of the block-Jacobi blocks: LLT (1), else LU
+    long                  num_mv = 0, num_op = 0, num_pc = 0; // matvec_mpos.h:139-141 (num_mv: vectors applied)
     std::unique_ptr<Timer> t_multAx  = std::make_unique<Timer>();
     std::unique_ptr<Timer> t_multOPv = std::make_unique<Timer>();
     std::unique_ptr<Timer> t_multPc  = std::make_unique<Timer>();
@@ -110,20 +128,22 @@ class MatVecMPOS {
     MatVecMPOS(const MatVecMPOS &)            = delete;
     MatVecMPOS &operator=(const MatVecMPOS &) = delete;
 
-    long rows() const { return static_cast<long>(dmrg_matvec_rows(op_)); }
-    long cols() const { return rows(); }
+    // matvec_mpos.h:127-128: int, as Arpack++ and PRIMME take it.
+    int rows() const { return static_cast<int>(dmrg_matvec_rows(op_)); }
+    int cols() const { return rows(); }
     std::array<long, 3> get_shape_mps() const {
         int64_t s[3];
         check(dmrg_matvec_shape_mps(op_, s), "get_shape_mps");
         return {static_cast<long>(s[0]), static_cast<long>(s[1]), static_cast<long>(s[2])};
     }
-    long num_mv() const { return static_cast<long>(dmrg_matvec_num_mv(op_)); }
-
-    // MultAx(T*, T*) (matvec_mpos.cpp:963-974): one vector, host buffers of rows() elements.
+    // MultAx(T*, T*) (matvec_mpos.cpp:963-974): one vector, host buffers of rows() elements. Also the member
+    // function Arpack++ calls through &MatrixType::MultAx (solver_arpack.cpp:109,142,169) and by reverse
+    // communication (:336-340).
     void MultAx(T *mps_in, T *mps_out) {
         const auto t0 = std::chrono::steady_clock::now();
         check(dmrg_matvec_multax(op_, mps_in, rows(), mps_out, rows(), 1), "MultAx");
         t_multAx->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        num_mv = static_cast<long>(dmrg_matvec_num_mv(op_));
     }
 
     // PRIMME matrixMatvec callback (matvec_mpos.cpp:976-984): blockSize columns at strides ldx / ldy.
@@ -132,8 +152,18 @@ class MatVecMPOS {
         const auto        t0 = std::chrono::steady_clock::now();
         const dmrg_status s  = dmrg_matvec_multax(op_, x, *ldx, y, *ldy, *blockSize);
         t_multAx->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        num_mv = static_cast<long>(dmrg_matvec_num_mv(op_));
         *err = (s == DMRG_OK) ? 0 : -1;
     }
+
+    // Arpack++ glue (solver_arpack.cpp:90-91,108): the problem form and eigenvector side are recorded; the
+    // operator is the same for every setting (matvec_mpos.cpp:1475-1481).
+    template <typename Form>
+    void set_mode(Form f) { form_ = f; }
+    template <typename Side>
+    void set_side(Side s) { side_ = s; }
+    const EnumValue &get_form() const { return form_; }
+    const EnumValue &get_side() const { return side_; }
 
     // MultBx (matvec_mpos.cpp:986-1019): the B operator; mps_out is left as it is when there is none.
     void MultBx(T *mps_in, T *mps_out) { check(dmrg_matvec_multbx(op_, mps_in, rows(), mps_out, rows(), 1), "MultBx"); }
@@ -153,15 +183,38 @@ class MatVecMPOS {
             *err = 0;
             return;
         }
-        const auto        t0 = std::chrono::steady_clock::now();
-        const dmrg_status s  = dmrg_matvec_precond(op_, x, ldx, y, ldy, blockSize, shifts);
+        const auto t0 = std::chrono::steady_clock::now();
+        if(factorization.value != applied_factorization_) {
+            const dmrg_status sf = dmrg_matvec_set_factorization(op_, factorization.value);
+            if(sf != DMRG_OK) {
+                *err = -1;
+                return;
+            }
+            applied_factorization_ = factorization.value;
+        }
+        const dmrg_status s = dmrg_matvec_precond(op_, x, ldx, y, ldy, blockSize, shifts);
         t_multPc->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         num_pc += blockSize;
         *err = (s == DMRG_OK) ? 0 : -1;
     }
-    // Block sizes above 1 select the reference's block-Jacobi preconditioner; this operator applies the
-    // diagonal (size 1) form for every setting.
-    void set_jcbMaxBlockSize(long size) { jcb_max_block_size_ = size; }
+    // set_jcbMaxBlockSize (matvec_mpos.cpp:1483-1505): 1 selects the Jacobi preconditioner, larger sizes the
+    // block-Jacobi one. The size is clamped to [1, rows()] and then lowered in steps of 2 (down to half) to the
+    // first size that divides rows() or leaves a last block of more than 80% of the others, keeping the one with
+    // the largest remainder seen, so that the blocks are of nearly equal size. No value: unchanged.
+    void set_jcbMaxBlockSize(std::optional<long> size) {
+        if(!size.has_value()) return;
+        const long n   = rows();
+        long       jcb = std::clamp(size.value(), 1l, n);
+        long       best_rem = n % jcb, best = jcb;
+        for(long cand = jcb; cand >= jcb / 2 && cand >= 1; cand -= 2) {
+            const long rem = n % cand;
+            if(rem > best_rem || rem == 0) best_rem = rem, best = cand;
+            if(rem == 0 || rem > 4 * cand / 5) break;
+        }
+        if(best != jcb) jcb = std::clamp(best, jcb / 2, n);
+        check(dmrg_matvec_set_jcb_block_size(op_, jcb), "set_jcbMaxBlockSize");
+        jcb_max_block_size_ = jcb;
+    }
     long get_jcbMaxBlockSize() const { return jcb_max_block_size_; }
 
     // set_shift(sigma) (matvec_mpos.cpp:1433-1472).
@@ -177,9 +230,12 @@ class MatVecMPOS {
     dmrg_matvec *handle() { return op_; }
 
   private:
-    dmrg_matvec *op_                 = nullptr;
-    long         jcb_max_block_size_ = 1;
-    bool         ready_shift_        = false;
+    dmrg_matvec *op_                    = nullptr;
+    long         jcb_max_block_size_    = 1;
+    bool         ready_shift_           = false;
+    int          applied_factorization_ = 3; // the library's default, eig::Factorization::LU
+    EnumValue    form_;                      // eig::Form::SYMM
+    EnumValue    side_{1};                   // eig::Side::R, the reference's default (matvec_mpos.h:48)
 };
 
 namespace contraction {

@@ -237,3 +237,41 @@ def entanglement_entropy(schmidt) -> float:
     p = np.abs(np.asarray(schmidt)) ** 2
     p = p[p > 0]
     return float(-np.sum(p * np.log(p)))
+
+
+def matrix_vector_product_chain(mps, mpos, envL, envR):
+    """H_eff . mps for a list of MPOs applied one at a time (contraction_tblis.cpp:307-420, the path MatVecMPOS
+    takes for more than 5 MPOs, matvec_mpos.cpp:120-123). mpos unshuffled, each (DL_i, DR_i, d_i, d_i).
+
+    t carries the indices (s_cur, s_rest, b, a', s_done, w): theta.envL first (:358), then MPO i contracts s_cur
+    and w (idx {0,5},{0,2} of the {2,3,0,1}-shuffled MPO, :398) and the result is reshaped so that the next site's
+    input comes first and this site's output joins s_done as its slowest part (:392-399); finally .envR over
+    (b, w') and shuffle {1,0,2} (:417-419)."""
+    ds = [m.shape[2] for m in mpos]
+    d = int(np.prod(ds))
+    if mps.shape[0] != d or mps.shape[1] != envL.shape[0] or mps.shape[2] != envR.shape[0] \
+            or envL.shape[2] != mpos[0].shape[0] or envR.shape[2] != mpos[-1].shape[1]:
+        raise ValueError("Dimension mismatch")
+    cL, cR = envL.shape[0], envR.shape[0]
+    t = np.tensordot(mps, envL, axes=([1], [0]))                                 # (s, b, a', w)
+    t = t.reshape(ds[0], d // ds[0], cR, cL, 1, envL.shape[2], order="F")          # (s_0, rest, b, a', 1, w)
+    for i, W in enumerate(mpos):
+        shf = W.transpose(2, 3, 0, 1)                                              # (s, s', wL, wR)
+        t = np.tensordot(t, shf, axes=([0, 5], [0, 2]))                            # (rest, b, a', done, s', wR)
+        rest, done = t.shape[0], t.shape[3]
+        nxt = ds[i + 1] if i + 1 < len(ds) else 1
+        t = t.reshape(nxt, rest // nxt, cR, cL, done * W.shape[3], W.shape[1], order="F")
+    t = t.reshape(cR, cL, d, mpos[-1].shape[1], order="F")                         # (b, a', s', w')
+    t = np.tensordot(t, envR, axes=([0, 3], [0, 2]))                               # (a', s', b')
+    return np.ascontiguousarray(t.transpose(1, 0, 2))
+
+
+def shift_mpos(mpos, sigma_old, sigma_new):
+    """MatVecMPOS::set_shift (matvec_mpos.cpp:1437-1471): (sigma_old - sigma_new)/N times the identity added to the
+    bottom-left block W[DL-1, 0] of each of the N MPOs the operator holds."""
+    out = []
+    for W in mpos:
+        W = np.array(W, dtype=np.complex128, copy=True)
+        W[W.shape[0] - 1, 0] += (sigma_old - sigma_new) / len(mpos) * np.eye(W.shape[2])
+        out.append(W)
+    return out

@@ -86,7 +86,7 @@ static int run(FILE *in, FILE *out, const int64_t h[8]) {
     const T ev = dmrg_amd::contraction::expectation_value<T>(x.data(), {d, cL, cR}, mpo_t.data(), {DL, DR, d, d}, eL_t.data(),
                                                              {cL, cL, DL}, eR_t.data(), {cR, cR, DR});
     const double  evr[2] = {std::real(cx(ev)), std::imag(cx(ev))};
-    const int64_t nmv    = op.num_mv();
+    const int64_t nmv    = op.num_mv;
     fwrite(y.data(), sizeof(T), y.size(), out);
     fwrite(y1.data(), sizeof(T), y1.size(), out);
     fwrite(y2.data(), sizeof(T), y2.size(), out);
@@ -119,7 +119,7 @@ static int run(FILE *in, FILE *out, const int64_t h[8]) {
         double dmax = 0, amax = 0;
         for(int64_t i = 0; i < n; ++i) dmax = std::max(dmax, std::abs(ya[i] - yb2[i])), amax = std::max(amax, std::abs(ya[i]));
         if(dmax > 1e-13 * amax) return 14;
-        if(ab.num_mv() != 2) return 15;
+        if(ab.num_mv != 2) return 15;
     }
     // Error path: a dimension mismatch surfaces as dmrg_amd::error with DMRG_ERR_DIMS.
     try {

@@ -159,3 +159,55 @@ def test_serve_eigs_budget_admission():
     finally:
         for op in ops:
             op.close()
+
+
+@pytest.mark.parametrize("target", ["SA", "LA"])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_batched_mixed_matches_dense_and_fp64(cplx, target):
+    """Mixed-precision batch (FP32 phase on the FP32 matrix cores, then FP64): the dense eigenvalues within
+    1e-10 |A|, residuals at the FP64 tolerance, eigenvectors equal to the FP64 batch's (overlap 1 - 1e-9), and
+    both phases' matvecs counted."""
+    print(f"[case] batched mixed cplx={cplx} target={target}", file=sys.stderr, flush=True)
+    shapes = SHAPES + [(2, 48, 40, 5, 5), (2, 64, 64, 5, 5)]
+    probs, ops = build(cplx, shapes, seed=70)
+    try:
+        mix = G.eigs_batched(ops, tol=1e-12, target=target, mixed=True)
+        f64 = G.eigs_batched(ops, tol=1e-12, target=target)
+        for (th, W, L, R), (lam, vec, info), (l64, v64, i64) in zip(probs, mix, f64):
+            n = vec.shape[0]
+            if n <= 4096:
+                H = dense(th, W, L, R)
+                wd = np.linalg.eigvalsh(H)
+                ref, anorm = (wd[0] if target == "SA" else wd[-1]), np.abs(wd).max()
+                assert np.linalg.norm(H @ vec - lam * vec) <= 1e-11 * anorm
+            else:
+                ref, anorm = l64, i64["anorm"]
+            assert info["nconv"] == 1, info
+            assert abs(lam - ref) <= 1e-10 * anorm and abs(lam - l64) <= 1e-10 * anorm, (lam, ref, l64)
+            assert overlap(vec, v64) >= 1 - 1e-9
+            assert info["matvecs_fp32"] > 0 and info["matvecs_fp64"] > 0, info
+    finally:
+        for op in ops:
+            op.close()
+
+
+def test_serve_eigs_mixed_alone_vs_mix():
+    """Continuous batching in mixed precision: requests arriving over time (at most 3 at once) give the eigenpairs
+    they get alone in mixed precision and the FP64 ones (eigenvalues within 1e-10 |A|, overlap 1 - 1e-9)."""
+    print("[case] serve_eigs mixed alone-vs-mix", file=sys.stderr, flush=True)
+    shapes = SHAPES[:5] + [(2, 32, 40, 5, 5), (2, 48, 48, 5, 5)]
+    probs, ops = build(True, shapes, seed=90)
+    try:
+        alone = [G.eigs_batched([op], tol=1e-12, mixed=True)[0] for op in ops]
+        f64 = [G.eigs_batched([op], tol=1e-12)[0] for op in ops]
+        arrivals = [0.003 * i for i in range(len(ops))]
+        mix, stats = G.serve_eigs(ops, arrivals=arrivals, max_batch=3, tol=1e-12, mixed=True)
+        assert stats["completed"] == len(ops) and stats["failed"] == 0, stats
+        for (la, va, ia), (lm, vm, im), (l64, v64, i64) in zip(alone, mix, f64):
+            anorm = i64["anorm"]
+            assert im["nconv"] == 1 and im["matvecs_fp32"] > 0
+            assert abs(lm - la) <= 1e-10 * anorm and abs(lm - l64) <= 1e-10 * anorm, (lm, la, l64)
+            assert overlap(va, vm) >= 1 - 1e-9 and overlap(vm, v64) >= 1 - 1e-9
+    finally:
+        for op in ops:
+            op.close()

@@ -88,3 +88,66 @@ def heff_diagonal(W, L, R):
 def test_diagonal_helper_matches_brute_force():
     th, W, L, R = random_heff_problem(2, 3, 4, 3, 2, seed=9, complex_=True)
     assert np.allclose(heff_diagonal(W, L, R), np.diag(O.dense_heff(W, L, R)), rtol=0, atol=1e-13)
+
+
+def jcb_rule(size, n):
+    """MatVecMPOS::set_jcbMaxBlockSize (matvec_mpos.cpp:1483-1505), restated in Python."""
+    jcb = min(max(size, 1), n)
+    newsize = jcb
+    best = (n % newsize, newsize)
+    while newsize >= jcb // 2:
+        rem = n % newsize
+        if rem > best[0] or rem == 0:
+            best = (rem, newsize)
+        if rem == 0 or rem > 4 * newsize // 5:
+            break
+        newsize -= 2
+    if best[1] != jcb:
+        jcb = min(max(best[1], jcb // 2), n)
+    return jcb
+
+
+def test_jcb_rule_cases():
+    assert jcb_rule(64, 240) == 62 and jcb_rule(1, 240) == 1 and jcb_rule(60, 240) == 60 and jcb_rule(1000, 240) == 240
+
+
+SOLVER_BIN = Path(__file__).resolve().parent / "cpp" / "bin" / "solver_templates"
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_cpp_adaptor_in_solver_templates(tmp_path, cplx):
+    """The adaptor compiled inside stand-ins of eig::solver::eigs_primme and eig::solver_arpack (their member
+    expressions on the operator: set_jcbMaxBlockSize(std::optional<long>), preconditioner, num_op, t_multOPv;
+    set_mode, set_side, get_form, &MatrixType::MultAx, reverse communication, the num_mv field). Each stand-in's
+    Lanczos reaches the lowest eigenvalue of the dense H_eff (1e-9 |H|); num_mv counts the vectors applied; the
+    block size follows the reference's rule (64 -> 62 for n = 240); the PRIMME preconditioner callback with
+    factorization = LLT applies the block rule of the reference."""
+    assert SOLVER_BIN.exists(), "build first (__graft_entry__.build())"
+    d, cL, cR, DL, DR = 2, 12, 10, 3, 3
+    th, W, L, R = random_heff_problem(d, cL, cR, DL, DR, seed=8, complex_=cplx)
+    n = d * cL * cR
+    dt = np.complex128 if cplx else np.float64
+    x = np.random.default_rng(3).standard_normal(n) + (1j * np.random.default_rng(4).standard_normal(n) if cplx else 0)
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        np.array([int(cplx), d, cL, cR, DL, DR, 64], dtype=np.int64).tofile(f)
+        for a in (W, L, R):
+            np.asfortranarray(a.astype(np.complex128)).ravel(order="F").tofile(f)
+        x.astype(dt).tofile(f)
+    r = subprocess.run([str(SOLVER_BIN), str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    raw = np.fromfile(out, dtype=np.uint8)
+    E = np.frombuffer(raw[:24].tobytes(), dtype=np.float64)
+    k = np.frombuffer(raw[24:72].tobytes(), dtype=np.int64)
+    ypc = np.frombuffer(raw[72:72 + n * np.dtype(dt).itemsize].tobytes(), dtype=dt)
+    H = O.dense_heff(W.astype(dt), L.astype(dt), R.astype(dt))
+    w = np.linalg.eigvalsh(0.5 * (H + H.conj().T))
+    scale = np.abs(w).max()
+    for e in E:
+        assert abs(e - w[0]) <= 1e-9 * scale, (E, w[0])
+    mv_stub, mv_field, mv_arpack, jcb, steps_primme, steps_arpack = (int(v) for v in k)
+    assert mv_stub == mv_field == steps_primme and mv_arpack == steps_arpack
+    assert jcb == jcb_rule(64, n) == 62
+    from test_gpu_precond import llt_rule_reference
+    ref = llt_rule_reference(H, jcb, 0.5, x.astype(dt).reshape(n, 1))[:, 0]
+    assert np.linalg.norm(ypc - ref) <= 1e-10 * np.linalg.norm(ref)

@@ -290,3 +290,86 @@ def test_matvec_op_multbx(dtype):
             op.set_b([W3], L3, R3)
     finally:
         op.close()
+
+
+# ------------------------------------------------------------------------------ MPO chains (> 5 MPOs)
+
+CHAINS = [([2] * 6, [3, 4, 2, 5, 3, 4, 2], 9, 13), ([2, 3, 1, 2, 2, 1, 2], [1, 3, 2, 2, 4, 1, 3, 2], 17, 11),
+          ([2] * 8, [3] * 9, 24, 24)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128])
+@pytest.mark.parametrize("chain", CHAINS, ids=["6mpo", "7mpo-ragged", "8mpo"])
+def test_chain_matvec_and_per_mpo_shift(chain, dtype):
+    """Lists of more than 5 MPOs are applied one MPO at a time (contraction_tblis.cpp:307-420), against the
+    oracle's sequential restatement; set_shift adds (sigma_old - sigma_new)/N on each MPO's bottom-left block
+    (matvec_mpos.cpp:1439-1462) — the MPOs here are general (not lower-triangular), where that differs from a
+    shift of the merged MPO. FP32 matrix cores, a 3-vector block, the diagonal and the lowest eigenvalue of the
+    shifted operator ride along."""
+    ds, Ds, cL, cR = chain
+    cplx = dtype == np.complex128
+    print(f"[case] chain {len(ds)} MPOs cplx={cplx}", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(len(ds) * 10 + cplx)
+    f = (lambda *s: rng.standard_normal(s) + 1j * rng.standard_normal(s)) if cplx else (lambda *s: rng.standard_normal(s))
+    mpos = [np.asfortranarray(f(Ds[i], Ds[i + 1], ds[i], ds[i])) for i in range(len(ds))]
+    d = int(np.prod(ds))
+    th = np.asfortranarray(f(d, cL, cR))
+    L, R = np.asfortranarray(f(cL, cL, Ds[0])), np.asfortranarray(f(cR, cR, Ds[-1]))
+    op = G.MatVecMPOS(mpos, L, R, dtype=dtype)
+    try:
+        ref = O.matrix_vector_product_chain(th, mpos, L, R)
+        out = op.MultAx(th.reshape(-1, order="F")).reshape(th.shape, order="F")
+        assert rel(out, ref) <= TOL64, rel(out, ref)
+        sigma = 0.37
+        op.set_shift(sigma)
+        shifted = O.shift_mpos(mpos, 0.0, sigma)
+        ref_s = O.matrix_vector_product_chain(th, shifted, L, R)
+        X = np.asfortranarray(np.stack([th.reshape(-1, order="F")] + [f(d * cL * cR) for _ in range(2)], axis=1))
+        dx, dy = G.DeviceArray.from_numpy(X.astype(dtype)), G.DeviceArray(X.shape, dtype)
+        op.MultAx_device(dx, dy, nvec=3)
+        Y = dy.to_numpy()
+        assert rel(Y[:, 0].reshape(th.shape, order="F"), ref_s) <= TOL64
+        for j in (1, 2):
+            refj = O.matrix_vector_product_chain(X[:, j].reshape(th.shape, order="F"), shifted, L, R)
+            assert rel(Y[:, j].reshape(th.shape, order="F"), refj) <= TOL64
+        lo = np.complex64 if cplx else np.float32
+        dx32, dy32 = G.DeviceArray.from_numpy(X.astype(lo)), G.DeviceArray(X.shape, lo)
+        op.MultAx_device(dx32, dy32, nvec=3, fp32=True)
+        assert rel(dy32.to_numpy()[:, 0].reshape(th.shape, order="F"), ref_s) <= TOL32
+        for a in (dx, dy, dx32, dy32):
+            a.close()
+        # diag H[(s,a,b),(s,a,b)] = sum_{w,w'} L[a,a,w] W[w,w',s,s] R[b,b,w'] of the merged shifted MPO
+        Wm = O.merge_mpos(shifted)
+        ref_d = np.einsum("aw,wvs,bv->sab", np.einsum("aaw->aw", L), np.einsum("wvss->wvs", Wm),
+                          np.einsum("bbv->bv", R)).reshape(-1, order="F")
+        diag = op.get_diagonal().reshape(-1, order="F")
+        assert np.abs(diag - ref_d).max() <= 1e-12 * np.abs(ref_d).max() * 10
+    finally:
+        op.close()
+
+
+def test_chain_eigs_hermitian():
+    """Ground state of a Hermitian 6-MPO chain operator (each MPO block Hermitian with real coefficients, Hermitian
+    env blocks) through the Davidson solver on the chain path, against the dense lowest eigenvalue."""
+    rng = np.random.default_rng(21)
+    ds, D, chi = [2] * 6, 3, 6
+    mpos = []
+    for _ in ds:
+        W = np.zeros((D, D, 2, 2), dtype=np.complex128, order="F")
+        for a in range(D):
+            for b in range(D):
+                A = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+                W[a, b] = (A + A.conj().T) / 2
+        mpos.append(W)
+    from dmrg_amd.workloads import _herm_blocks
+    L, R = _herm_blocks(rng, chi, D, True), _herm_blocks(rng, chi, D, True)
+    op = G.MatVecMPOS(mpos, L, R, dtype=np.complex128)
+    try:
+        H = O.dense_heff(O.merge_mpos(mpos), L, R)
+        # The operator applied is H^T (out(s') = sum_s W[.., s, s'] theta(s)); same spectrum.
+        e_ref = np.linalg.eigvalsh(0.5 * (H + H.conj().T))[0]
+        w, _, info = op.eigs(nev=1, tol=1e-12, seed=3)
+        assert info["nconv"] == 1
+        assert abs(w[0] - e_ref) <= 1e-10 * np.abs(H).max() * H.shape[0] ** 0.5
+    finally:
+        op.close()

@@ -240,3 +240,62 @@ def test_entanglement_entropy_definition():
     s = np.array([np.sqrt(0.5), np.sqrt(0.5)])
     assert abs(O.entanglement_entropy(s) - np.log(2)) < 1e-15
     assert O.entanglement_entropy(np.array([1.0])) == 0.0
+
+
+def _random_chain(rng, ds, Ds, complex_=True):
+    f = (lambda *s: rng.standard_normal(s) + 1j * rng.standard_normal(s)) if complex_ else (lambda *s: rng.standard_normal(s))
+    return [f(Ds[i], Ds[i + 1], ds[i], ds[i]) for i in range(len(ds))]
+
+
+@pytest.mark.parametrize("ds,Ds", [([2] * 6, [3, 4, 2, 5, 3, 4, 2]), ([2, 3, 1, 2, 2, 1, 2], [1, 3, 2, 2, 4, 1, 3, 2])])
+def test_chain_matvec_equals_merged(ds, Ds):
+    """The MPO-chain path (one MPO at a time) is the same operator as the merged MPO (6 and 7 MPOs, uneven
+    physical and bond dims), and for tiny shapes it equals the literal dense H_eff of the merged MPO."""
+    rng = np.random.default_rng(11)
+    mpos = _random_chain(rng, ds, Ds)
+    d, cL, cR = int(np.prod(ds)), 3, 4
+    th = rng.standard_normal((d, cL, cR)) + 1j * rng.standard_normal((d, cL, cR))
+    L = rng.standard_normal((cL, cL, Ds[0])) + 1j * rng.standard_normal((cL, cL, Ds[0]))
+    R = rng.standard_normal((cR, cR, Ds[-1])) + 1j * rng.standard_normal((cR, cR, Ds[-1]))
+    W = O.merge_mpos(mpos)
+    out = O.matrix_vector_product_chain(th, mpos, L, R)
+    ref = O.matrix_vector_product(th, W, L, R)
+    assert np.linalg.norm(out - ref) <= 1e-12 * np.linalg.norm(ref)
+    H = O.dense_heff(W, L, R)
+    assert np.linalg.norm(out.reshape(-1, order="F") - H @ th.reshape(-1, order="F")) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_chain_shift_per_mpo():
+    """set_shift on a list of N MPOs adds (sigma_old - sigma_new)/N on each MPO's bottom-left block. For
+    lower-triangular MPOs (the ising chain) the full-chain Hamiltonian moves by -sigma I; for a list that is not
+    lower-triangular the per-MPO shift and a shift of the merged MPO give different operators."""
+    L, sigma = 6, 0.37
+    h = np.random.default_rng(2).uniform(-1, 1, L)
+    mpos = O.ising_tf_rf_mpos(L, 1.0, 1.0, h)
+    H = O.chain_dense_from_mpos(mpos)
+    Hs = O.chain_dense_from_mpos(O.shift_mpos(mpos, 0.0, sigma))
+    assert np.linalg.norm(H - O.ising_dense(L, 1.0, 1.0, h)) <= 1e-12 * np.linalg.norm(H)
+    assert np.linalg.norm(Hs - (H - sigma * np.eye(H.shape[0]))) <= 1e-12 * np.linalg.norm(H)
+    rng = np.random.default_rng(5)
+    gen = _random_chain(rng, [2] * 6, [3] * 7)
+    merged_shift = O.merge_mpos(gen).copy()
+    merged_shift[2, 0] -= sigma * np.eye(merged_shift.shape[2])
+    per_mpo = O.merge_mpos(O.shift_mpos(gen, 0.0, sigma))
+    assert np.linalg.norm(per_mpo - merged_shift) > 1e-3 * np.linalg.norm(merged_shift)
+
+
+def test_lbit_site_mpo_matches_kronecker_sum():
+    """The l-bit-style site MPO of the two-site workload (1-body h Z + g X, 2-body Z Z at range 1-3, 3-body Z Z Z
+    at ranges 2 and 3) reproduces, on a 7-site chain with the ising edges (row 7 on the left, column 0 on the right),
+    the Hamiltonian summed term by term from Pauli Kronecker products."""
+    from dmrg_amd.workloads import lbit_dense_hamiltonian, lbit_site_couplings, lbit_site_mpo
+
+    rng = np.random.default_rng(3)
+    cs = [lbit_site_couplings(rng) for _ in range(7)]
+    mpos = [lbit_site_mpo(c) for c in cs]
+    mpos[0] = mpos[0][7:8]
+    mpos[-1] = mpos[-1][:, 0:1]
+    H = O.chain_dense_from_mpos(mpos)
+    ref = lbit_dense_hamiltonian(cs)
+    assert np.linalg.norm(H - ref) <= 1e-12 * np.linalg.norm(ref)
+    assert all(abs(c["K1"]) > 0 and abs(c["K2"]) > 0 for c in cs)  # the 3-body terms are present
