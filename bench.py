@@ -63,13 +63,16 @@ def gpu_matvec_bench(chi, steps, warmup, fp32=False, legs_nrep=50):
             "legs_tflops": [8.0 * m / (t * 1e-3) / 1e12 for m, t in zip(macs, kernel_ms)], "flops": flops}
 
 
-def cpu_baseline(chi=512, budget_s=12.0, max_reps=200):
-    """Time the oracle (numpy restatement of the reference contraction) on the host cores."""
-    import numpy as np
-
-    from dmrg_amd.workloads import heff_flops, random_heff_problem
-    from oracle import heff as O  # cpu_baseline leg: the oracle is the thing timed here, as the CPU port
-
+def host_info():
+    """nproc, CPU model and the BLAS numpy runs on (with its thread count)."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
     threads, blas = None, "unknown"
     try:
         from threadpoolctl import threadpool_info
@@ -80,18 +83,46 @@ def cpu_baseline(chi=512, budget_s=12.0, max_reps=200):
             blas = f"{info[0].get('internal_api')} {info[0].get('version')}"
     except Exception:
         pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "blas": blas, "blas_threads": threads}
+
+
+def cpu_baseline(chi=512, budget_s=12.0, max_reps=200):
+    """Time the oracle (numpy restatement of the reference contraction, its GEMMs in the host BLAS) on the host
+    cores: all BLAS threads, then one thread, each on a bounded sample of about budget_s seconds."""
+    import numpy as np
+
+    from dmrg_amd.workloads import heff_flops, random_heff_problem
+    from oracle import heff as O  # cpu_baseline leg: the oracle is the thing timed here, as the CPU port
+
+    host = host_info()
     th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=1)
-    O.matrix_vector_product(th, W, L, R)  # warm
-    reps, t0 = 0, time.perf_counter()
-    while reps < max_reps and (time.perf_counter() - t0) < budget_s:
-        O.matrix_vector_product(th, W, L, R)
-        reps += 1
-    dt = time.perf_counter() - t0
     flops = heff_flops(2, chi, chi, 5, 5, True)
-    return {"value": round(flops * reps / dt / 1e12, 5), "unit": "TFLOP/s", "cores": threads or os.cpu_count(),
-            "kind": "port",
-            "sample": f"{reps} matvecs of the chi={chi} cx64 workload by the numpy oracle ({blas}, {threads} threads), "
-                      f"{dt:.1f} s"}
+
+    def timed(budget):
+        O.matrix_vector_product(th, W, L, R)  # warm
+        reps, t0 = 0, time.perf_counter()
+        while reps < max_reps and (time.perf_counter() - t0) < budget:
+            O.matrix_vector_product(th, W, L, R)
+            reps += 1
+        dt = time.perf_counter() - t0
+        return reps, dt, flops * reps / dt / 1e12
+
+    reps, dt, tf = timed(budget_s)
+    one = None
+    try:
+        from threadpoolctl import threadpool_limits
+
+        with threadpool_limits(limits=1, user_api="blas"):
+            r1, d1, t1 = timed(budget_s)
+        one = {"value": round(t1, 5), "unit": "TFLOP/s", "cores": 1,
+               "sample": f"{r1} matvecs in {d1:.1f} s with one BLAS thread"}
+    except Exception as e:  # threadpoolctl absent: no single-thread figure
+        one = {"error": str(e)}
+    cores = host["blas_threads"] or os.cpu_count()
+    return {"value": round(tf, 5), "unit": "TFLOP/s", "cores": cores, "kind": "port",
+            "sample": f"{reps} matvecs of the chi={chi} cx64 workload by the numpy oracle ({host['blas']}, "
+                      f"{cores} threads) in {dt:.1f} s",
+            "host": host, "single_thread": one}
 
 
 def load_traffic(name: str):
@@ -136,20 +167,53 @@ def section_h2_matvec(chi=512, steps=30):
     return out
 
 
-def section_eigs(chi=512, reps=6, tol=1e-10):
-    """Ground-state eigensolves of the isolated workload's H_eff (cold random starts), FP64 vs mixed
-    (FP32 matrix-core phase + FP64 refinement): solves/s, p50/p99 latency, |dE| between the modes."""
+def schmidt_values(theta):
+    """Schmidt values of a single-site theta (d, chiL, chiR) across its right bond: the singular values of the
+    (d chiL) x chiR matrix (the split of split.cpp:21-346), by the library's Jacobi SVD."""
     import numpy as np
 
     import dmrg_amd as G
-    from dmrg_amd.workloads import random_heff_problem
+
+    d, cL, cR = theta.shape
+    M = np.asfortranarray(theta.reshape(d * cL, cR, order="F"))
+    S = G.svd_truncate(M, 1e-300, 0, 3)["S"]
+    return S / np.linalg.norm(S)
+
+
+def schmidt_error(a, b):
+    """Largest difference of two Schmidt spectra relative to their largest value, and over the values above
+    1e-6 of it the largest relative difference."""
+    import numpy as np
+
+    k = min(len(a), len(b))
+    a, b = a[:k], b[:k]
+    big = a > 1e-6 * a[0]
+    return {"max_abs_over_s0": float(np.max(np.abs(a - b)) / a[0]),
+            "max_rel_above_1e-6": float(np.max(np.abs(a[big] - b[big]) / a[big]))}
+
+
+def mfma_fraction(f64, f32, seconds):
+    """Fraction of the matrix-core roofline reached by a solve: the time its FP64 and FP32 matvec FLOP would take
+    at the FP64 and FP32 matrix peaks, divided by the solve's time (FP64 alone: achieved / FP64 peak)."""
+    return (f64 / (PEAK_FP64_MFMA * 1e12) + f32 / (PEAK_FP32_MFMA * 1e12)) / seconds
+
+
+def section_eigs(chi=512, reps=100, tol=1e-10):
+    """Ground-state eigensolves of the isolated workload's H_eff (reps cold random starts per mode), FP64 vs mixed
+    (FP32 matrix-core phase + FP64 refinement): solves/s, p50/p99 latency over the reps, matrix-core roofline
+    fraction, |dE| between the modes, energy variance and Schmidt values of the solutions."""
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import heff_flops, random_heff_problem
 
     th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=1)
     op = G.MatVecMPOS([W], L, R, dtype=np.complex128)
     op.eigs(nev=1, tol=tol, seed=999)  # warm-up: first-use code loads and GEMM tuning
     op.eigs(nev=1, tol=tol, seed=999, mixed=True)
+    flop = heff_flops(2, chi, chi, 5, 5, True)
     res = {}
-    energies = {}
+    energies, first_vec = {}, {}
     n = op.rows()
     dv, dy = G.DeviceArray((n,), np.complex128), G.DeviceArray((n,), np.complex128)
 
@@ -173,23 +237,119 @@ def section_eigs(chi=512, reps=6, tol=1e-10):
             mv32.append(info["matvecs_fp32"])
             es.append(w[0])
             rel_r.append(info["rnorm"][0] / info["anorm"])
-            var.append(variance(v[:, 0]))
-        total = float(np.sum(lat))  # solve time only (the variance check is outside)
+            if k < 10:
+                var.append(variance(v[:, 0]))
+            if k == 0:
+                first_vec[mode] = v[:, 0].reshape(th.shape, order="F")
+        total = float(np.sum(lat))  # solve time only (the variance checks are outside)
         energies[mode] = np.array(es)
-        res[mode] = {"solves_per_s": round(reps / total, 3), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
-                     "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2), "mean_matvecs_fp64": float(np.mean(mv64)),
-                     "mean_matvecs_fp32": float(np.mean(mv32)), "max_rel_residual": float(np.max(rel_r)),
+        res[mode] = {"solves_per_s": round(reps / total, 3), "samples": reps,
+                     "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                     "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2), "max_ms": round(max(lat) * 1e3, 2),
+                     "mean_matvecs_fp64": float(np.mean(mv64)), "mean_matvecs_fp32": float(np.mean(mv32)),
+                     "roofline_frac": round(mfma_fraction(np.mean(mv64) * flop, np.mean(mv32) * flop, total / reps), 4),
+                     "max_rel_residual": float(np.max(rel_r)),
                      "median_variance": float(np.median(var)), "max_variance": float(np.max(var))}
     dv.close()
     dy.close()
     op.close()
     dE = float(np.max(np.abs(energies["mixed"] - energies["fp64"])))
     spread = float(np.max(energies["fp64"]) - np.min(energies["fp64"]))
-    return {"workload": f"isolated H_eff chi={chi} cx64 (seed 1), {reps} cold starts per mode, tol {tol:g}*|H|",
+    S64, Smx = schmidt_values(first_vec["fp64"]), schmidt_values(first_vec["mixed"])
+    return {"workload": f"isolated H_eff chi={chi} cx64 (seed 1), {reps} cold starts per mode, tol {tol:g}*|H|; "
+                        f"roofline_frac = (FP64 FLOP / FP64 peak + FP32 FLOP / FP32 peak) / solve time; variances of "
+                        f"the first 10 solutions",
             "fp64": res["fp64"], "mixed": res["mixed"], "max_abs_dE_mixed_vs_fp64": dE,
             "variance_ratio_mixed_vs_fp64": res["mixed"]["median_variance"] / max(res["fp64"]["median_variance"], 1e-300),
+            "schmidt_values_mixed_vs_fp64": schmidt_error(S64, Smx),
             "speedup_mixed_vs_fp64": round(res["mixed"]["solves_per_s"] / res["fp64"]["solves_per_s"], 3),
             "max_abs_dE_between_fp64_starts": spread, "E0": float(energies["fp64"][0])}
+
+
+def section_batched_chi512(nops=8, tol=1e-10):
+    """Batched ground states of nops operators of the chi=512 workload (seeds 1..nops): one after another in FP64,
+    batched in FP64 (grouped matvec launches, one synchronisation per step for all) and batched in mixed
+    precision (FP32 batch, then FP64 batch). Solves/s, latency, roofline fraction, |dE| and variance ratio."""
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import heff_flops, random_heff_problem
+
+    ops = []
+    for i in range(nops):
+        th, W, L, R = random_heff_problem(2, 512, 512, 5, 5, seed=1 + i)
+        ops.append(G.MatVecMPOS([W], L, R, dtype=np.complex128))
+    flop = heff_flops(2, 512, 512, 5, 5, True)
+    G.eigs_batched(ops[:2], tol=1e-4, seed=3)  # warm-up: plans of the grouped legs, code loads
+    G.eigs_batched(ops[:2], tol=1e-4, seed=3, mixed=True)
+    ops[0].eigs(nev=1, tol=1e-4, seed=3)
+    out = {"workload": f"{nops} ground states of the chi=512 cx64 workload (seeds 1-{nops}), tol {tol:g}*|H|, random starts"}
+    lat, e_seq, v_seq, mv = [], [], [], 0
+    t0 = time.perf_counter()
+    for op in ops:
+        t1 = time.perf_counter()
+        w, v, info = op.eigs(nev=1, tol=tol, seed=11)
+        lat.append(time.perf_counter() - t1)
+        e_seq.append(w[0])
+        v_seq.append(v[:, 0])
+        mv += info["matvecs_fp64"]
+    t_seq = time.perf_counter() - t0
+    out["single_fp64"] = {"seconds": round(t_seq, 3), "solves_per_s": round(nops / t_seq, 3),
+                          "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1), "max_ms": round(max(lat) * 1e3, 1),
+                          "matvecs_fp64": mv, "roofline_frac": round(mfma_fraction(mv * flop, 0, t_seq), 4)}
+
+    def variance(op, v):
+        v = v / np.linalg.norm(v)
+        hv = op.MultAx(v)
+        return float(np.linalg.norm(hv - np.vdot(v, hv).real * v) ** 2)
+
+    res = {}
+    for mode in ("fp64", "mixed"):
+        t0 = time.perf_counter()
+        r = G.eigs_batched(ops, tol=tol, seed=11, mixed=(mode == "mixed"))
+        dt = time.perf_counter() - t0
+        m64 = sum(x[2]["matvecs_fp64"] for x in r)
+        m32 = sum(x[2]["matvecs_fp32"] for x in r)
+        fin = sorted(x[2]["seconds"] for x in r)
+        res[mode] = r
+        out["batched_" + mode] = {"seconds": round(dt, 3), "solves_per_s": round(nops / dt, 3),
+                                  "all_converged": all(x[2]["nconv"] == 1 for x in r),
+                                  "matvecs_fp64": m64, "matvecs_fp32": m32,
+                                  "roofline_frac": round(mfma_fraction(m64 * flop, m32 * flop, dt), 4),
+                                  "max_abs_dE_vs_single": float(max(abs(x[0] - e) for x, e in zip(r, e_seq))),
+                                  "median_variance": float(np.median([variance(op, x[1]) for op, x in zip(ops, r)]))}
+    var64 = float(np.median([variance(op, v) for op, v in zip(ops, v_seq)]))
+    out["single_fp64"]["median_variance"] = var64
+    out["max_abs_dE_mixed_vs_fp64"] = float(max(abs(a[0] - b[0]) for a, b in zip(res["mixed"], res["fp64"])))
+    out["variance_ratio_mixed_vs_fp64"] = out["batched_mixed"]["median_variance"] / max(out["batched_fp64"]["median_variance"], 1e-300)
+    out["speedup_batched_mixed_vs_single_fp64"] = round(t_seq / out["batched_mixed"]["seconds"], 3)
+    out["speedup_batched_mixed_vs_batched_fp64"] = round(out["batched_fp64"]["seconds"] / out["batched_mixed"]["seconds"], 3)
+    out["schmidt_values_mixed_vs_fp64"] = schmidt_error(schmidt_values(res["fp64"][0][1].reshape(2, 512, 512, order="F")),
+                                                        schmidt_values(res["mixed"][0][1].reshape(2, 512, 512, order="F")))
+    for op in ops:
+        op.close()
+    return out
+
+
+def section_block_vectors(chi=512, reps=50):
+    """The eigensolver's HBM-bound block-vector kernels on n = 2 chi^2 complex vectors (the chi=512 problem) with
+    8 and 16 basis columns: achieved algorithmic GB/s (bytes the kernel must move / its duration, HIP events) and
+    the fraction of the 8 TB/s HBM peak."""
+    import ctypes as C
+
+    import dmrg_amd as G
+
+    n = 2 * chi * chi
+    out = {"workload": f"n = {n} complex128 vectors, {reps} launches per kernel"}
+    for m in (8, 16):
+        ms = (C.c_float * 3)()
+        by = (C.c_double * 3)()
+        G._lib.call("dmrg_block_vector_time", G.DMRG_CX64, n, m, reps, ms, by)
+        for i, name in enumerate(("projections_VHx", "update_y_minus_Vc", "ritz_vector_and_residual")):
+            gbs = by[i] / (ms[i] * 1e-3) / 1e9
+            out[f"{name}_m{m}"] = {"ms": round(ms[i], 4), "bytes": int(by[i]), "GB_per_s": round(gbs, 1),
+                                   "frac_of_hbm_peak": round(gbs / 8000.0, 3)}
+    return out
 
 
 def section_sweep(cpu=True):
@@ -233,7 +393,7 @@ def section_sweep(cpu=True):
     return out
 
 
-def section_serve(nreq=32, workers=16, load=0.7, seed=4, mixed=True):
+def section_serve(nreq=128, workers=16, load=0.7, seed=4, mixed=True, nreq_batched=32):
     """Serving stream of independent ising_tf_rf realisations: L in {16,20,24,32}, chi_max in {64,128,256},
     h_i ~ U[-w,w] with w in {0.5,2,6}, seeds 100... A first pass with every request queued at t = 0 measures
     the capacity (solves/s at saturation); the second pass offers Poisson arrivals at `load` x that capacity and
@@ -264,21 +424,26 @@ def section_serve(nreq=32, workers=16, load=0.7, seed=4, mixed=True):
     gflop = sum(r["gflop"] for r in res)
     # Same saturation pass with the requests' FP64 energy solves batched into one eigensolver (the mixed-precision
     # solves of large problems stay with their request).
-    res_b, cap_b = G.serve(requests(None), workers=workers, hbm_budget_bytes=128 << 30, batch_eigs=True, mixed=mixed)
-    batched = {"capacity_solves_per_s": round(cap_b["solves_per_s"], 3),
+    res_b, cap_b = G.serve(requests(None)[:nreq_batched], workers=workers, hbm_budget_bytes=128 << 30, batch_eigs=True,
+                           mixed=mixed)
+    _, cap_s = G.serve(requests(None)[:nreq_batched], workers=workers, hbm_budget_bytes=128 << 30, mixed=mixed)
+    batched = {"requests": nreq_batched, "capacity_solves_per_s": round(cap_b["solves_per_s"], 3),
+               "capacity_solves_per_s_unbatched": round(cap_s["solves_per_s"], 3),
                "p50_latency_s_at_saturation": round(cap_b["p50_latency_s"], 3),
-               "p50_latency_s_at_saturation_unbatched": round(cap["p50_latency_s"], 3),
+               "p50_latency_s_at_saturation_unbatched": round(cap_s["p50_latency_s"], 3),
                "batch_steps": cap_b["batch_steps"], "peak_batch": cap_b["peak_batch"],
                "mean_problems_per_step": round(sum(r["matvecs"] for r in res_b) / max(1, cap_b["batch_steps"]), 2),
                "step_ms": round(cap_b["batch_step_s"] / max(1, cap_b["batch_steps"]) * 1e3, 3),
                "rebuild_s": round(cap_b["batch_rebuild_s"], 2),
                "max_abs_dE_vs_unbatched": float(max(abs(a["energy"] - b["energy"]) for a, b in zip(res_b, res)))}
+    lat = [r["latency_s"] for r in res]
     return {"workload": f"{nreq} requests, L in {{16,20,24,32}}, chi_max in {{64,128,256}}, w in {{0.5,2,6}}, "
                         f"{workers} workers, {'mixed precision (n >= 131072)' if mixed else 'FP64'}; Poisson arrivals "
                         f"at {load:g} x the measured capacity",
             "capacity_solves_per_s": round(capacity, 3), "arrival_rate_per_s": round(rate, 3),
-            "solves_per_s": round(st["solves_per_s"], 3), "p50_latency_s": round(st["p50_latency_s"], 3),
-            "p99_latency_s": round(st["p99_latency_s"], 3), "mean_latency_s": round(st["mean_latency_s"], 3),
+            "solves_per_s": round(st["solves_per_s"], 3), "latency_samples": len(lat),
+            "p50_latency_s": round(st["p50_latency_s"], 3), "p99_latency_s": round(st["p99_latency_s"], 3),
+            "max_latency_s": round(max(lat), 3), "mean_latency_s": round(st["mean_latency_s"], 3),
             "completed": st["completed"], "failed": st["failed"], "makespan_s": round(st["makespan_s"], 2),
             "total_mfma_tflops": round(gflop / st["makespan_s"] / 1e3, 3), "total_gflop": round(gflop, 1),
             "peak_concurrency": st["peak_concurrency"], "peak_admitted_gb": round(st["peak_admitted_bytes"] / 2**30, 2),
@@ -597,7 +762,15 @@ def section_two_site(chi=1024, D=8, steps=10, tol=1e-10):
                      "rel_residual": info["rnorm"][0] / info["anorm"]}
         if mode == "fp64":
             vec = v[:, 0]
+        else:
+            vec_mixed = v[:, 0]
     sol["abs_dE_mixed_vs_fp64"] = abs(sol["mixed"]["E0"] - sol["fp64"]["E0"])
+    # Schmidt values of the two solutions across the middle bond (the (2 chi) x (2 chi) split matrix)
+    sv = {}
+    for mode, x in (("fp64", vec), ("mixed", vec_mixed)):
+        S = G.svd_truncate(theta_as_matrix(x.reshape((4, chi, chi), order="F")), 1e-300, 0, 3)["S"]
+        sv[mode] = S / np.linalg.norm(S)
+    sol["schmidt_values_mixed_vs_fp64"] = schmidt_error(sv["fp64"], sv["mixed"])
     out["eigs"] = sol
     op.close()
     M = theta_as_matrix(vec.reshape((4, chi, chi), order="F"))
@@ -778,6 +951,8 @@ def main():
     if not args.quick:
         out["h2_matvec"] = section_h2_matvec(args.chi)
         out["eigs"] = section_eigs(args.chi)
+        out["batched_eigs_chi512"] = section_batched_chi512()
+        out["block_vectors"] = section_block_vectors(args.chi)
         out["eigs_h2_folded"] = section_eigs_h2(args.chi)
         out["sweep"] = section_sweep(cpu=not args.no_cpu_baseline)
         out["serve"] = section_serve()

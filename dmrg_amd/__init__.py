@@ -246,10 +246,13 @@ def svd_truncate(A, trunc_lim: float = 1e-14, rank_max: int = 0, method: int = 3
 def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_max: int = 64, max_sweeps: int = 20,
                       two_site_sweeps: int = 10, var_tol: float = 1e-12, trunc_lim: float = 1e-14, eigs_tol: float = 1e-12,
                       mixed: bool = False, complex_: bool = False, seed: int = 0, mixed_min_n: int = 0,
-                      xdmrg: bool = False, energy_target: float = 0.0, eig_max_size: int = 0, stream=None) -> dict:
+                      xdmrg: bool = False, energy_target: float = 0.0, eig_max_size: int = 0, expand_mode: int = 0,
+                      stream=None) -> dict:
     """Finite DMRG ground state of the random transverse-field Ising chain on the GPU
     (fdmrg::run_algorithm with ising_tf_rf, fdmrg.cpp:170-376), or with xdmrg the eigenstate found by minimising
-    <(H - energy_target)^2> (xdmrg.cpp:297-341, folded spectrum). Returns energies, variance, entropies, timings."""
+    <(H - energy_target)^2> (xdmrg.cpp:297-341, folded spectrum). expand_mode: EnvExpandMode bits; 43 (H1 | H2 |
+    NSITE | FORWARD, the reference's default) grows the bonds by subspace expansion in single-site sweeps.
+    Returns energies, variance, entropies, timings."""
     h = np.ascontiguousarray(np.asarray(h_rand, dtype=np.float64))
     assert h.shape == (L,)
     cfg = _lib.FdmrgConfig()
@@ -261,6 +264,7 @@ def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_m
     cfg.mixed, cfg.complex_, cfg.seed, cfg.mixed_min_n = int(mixed), int(complex_), seed, mixed_min_n
     cfg.xdmrg, cfg.energy_target = int(xdmrg), float(energy_target)
     cfg.eig_max_size = int(eig_max_size)
+    cfg.expand_mode = int(expand_mode)
     res = _lib.FdmrgResult()
     call("dmrg_fdmrg_ising_tf_rf", C.byref(cfg), C.byref(res), stream)
     out = {k: getattr(res, k) for k, _ in res._fields_ if k not in ("entropies", "chi")}
@@ -527,12 +531,16 @@ class MatVecMPOS:
     def eigs(self, nev: int = 1, target: str = "SA", tol: float = 1e-12, x0=None, mixed: bool = False,
              precond: int = 0, ncv: int = 0, min_restart: int = 0, shift: float = 0.0,
              max_matvecs: int = 100000, seed: int = 12345, jcb: int = 128, inner_max=None, inner_tol=None,
-             stream=None):
+             projection: str = "RR", conv_test: str = "tol", stream=None):
         """Eigenpairs on the GPU (eig::solver::eigs with PRIMME, solver.cpp:491-509). Returns
         (eigenvalues, eigenvectors (n, nev) numpy, result dict). mixed: FP32 phase, then FP64 refinement whose
-        expansions come from at most inner_max FP32 correction-equation CG steps (None: library default)."""
+        expansions come from at most inner_max FP32 correction-equation CG steps (None: library default).
+        projection 'refined' (target CLOSEST): refined Ritz vectors; conv_test 'saturation': the convergence rule
+        of the reference's folded-spectrum convTestFun."""
         cfg = _lib.EigsConfig()
         load().dmrg_eigs_default_config(C.byref(cfg))
+        cfg.projection = {"RR": 0, "refined": 1}[projection]
+        cfg.conv_test = {"tol": 0, "saturation": 1}[conv_test]
         cfg.nev, cfg.ncv, cfg.min_restart = nev, ncv, min_restart
         cfg.target = {"SA": 0, "LA": 1, "CLOSEST": 2}[target]
         cfg.shift, cfg.tol, cfg.max_matvecs = shift, tol, max_matvecs
@@ -556,11 +564,15 @@ class MatVecMPOS:
 
     def eigs_folded_spectrum(self, h: "MatVecMPOS", x0, energy_shift: float, target: str = "CLOSEST",
                              shift: float = 0.0, tol: float = 1e-12, precond: int = 0, ncv: int = 0,
-                             max_matvecs: int = 100000, stream=None):
+                             max_matvecs: int = 100000, projection: str = "refined", jcb: int = 128,
+                             mixed: bool = False, stream=None):
         """Folded-spectrum executor on this operator (the MPO^2 of H - energy_shift on the variance envs), h = H
-        (optimize_folded_spectrum.cpp:235-342). Returns (energy, variance, eigenvector, result dict)."""
+        (optimize_folded_spectrum.cpp:235-342); by default with the refined projection of the reference's SM case
+        (:295-299). Returns (energy, variance, eigenvector, result dict)."""
         cfg = _lib.EigsConfig()
         load().dmrg_eigs_default_config(C.byref(cfg))
+        cfg.projection = {"RR": 0, "refined": 1}[projection]
+        cfg.jcb_block, cfg.mixed = jcb, int(mixed)
         cfg.nev, cfg.ncv, cfg.target = 1, ncv, {"SA": 0, "LA": 1, "CLOSEST": 2}[target]
         cfg.shift, cfg.tol, cfg.max_matvecs, cfg.precond = shift, tol, max_matvecs, int(precond)
         n = self.rows()

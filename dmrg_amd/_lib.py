@@ -81,6 +81,7 @@ SIGNATURES: dict[str, tuple] = {
     "dmrg_gemm_plan_override": (st, [C.c_int, C.c_int]),
     "dmrg_gemm_hybrid_override": (st, [C.c_int, C.c_int, C.c_int]),
     "dmrg_matvec_time_legs": (st, [vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_float), vp]),
+    "dmrg_block_vector_time": (st, [C.c_int, i64, C.c_int, C.c_int, C.POINTER(C.c_float), dp]),
     "dmrg_heff_dense": (st, [C.c_int, vp, i64, vp, i64p, vp, i64p, vp, i64p, C.c_int, vp]),
     "dmrg_matvec_get_matrix": (st, [vp, vp, i64, C.c_int, vp]),
     "dmrg_dense_eigh": (st, [C.c_int, i64, vp, i64, dp, C.c_int, vp]),
@@ -93,7 +94,7 @@ class EigsConfig(C.Structure):
     _fields_ = [("nev", C.c_int), ("ncv", C.c_int), ("min_restart", C.c_int), ("target", C.c_int),
                 ("shift", C.c_double), ("tol", C.c_double), ("max_matvecs", C.c_int64), ("precond", C.c_int),
                 ("mixed", C.c_int), ("max_time_s", C.c_double), ("seed", C.c_uint64), ("jcb_block", C.c_int64),
-                ("inner_max", C.c_int), ("inner_tol", C.c_double)]
+                ("inner_max", C.c_int), ("inner_tol", C.c_double), ("projection", C.c_int), ("conv_test", C.c_int)]
 
 
 class EigsResult(C.Structure):
@@ -110,14 +111,15 @@ class FdmrgConfig(C.Structure):
                 ("chi_max", C.c_int), ("max_sweeps", C.c_int), ("two_site_sweeps", C.c_int), ("var_tol", C.c_double),
                 ("trunc_lim", C.c_double), ("eigs_tol", C.c_double), ("eigs_ncv", C.c_int), ("mixed", C.c_int),
                 ("complex_", C.c_int), ("seed", C.c_uint64), ("mixed_min_n", C.c_int64), ("xdmrg", C.c_int),
-                ("energy_target", C.c_double), ("eig_max_size", C.c_int64)]
+                ("energy_target", C.c_double), ("eig_max_size", C.c_int64), ("expand_mode", C.c_int)]
 
 
 class FdmrgResult(C.Structure):
     _fields_ = [("energy", C.c_double), ("variance", C.c_double), ("entropies", C.c_double * (MAX_L + 1)),
                 ("chi", C.c_int64 * (MAX_L + 1)), ("sweeps", C.c_int), ("eigensolves", C.c_int), ("converged", C.c_int),
                 ("matvecs", C.c_int64), ("matvecs_fp32", C.c_int64), ("seconds", C.c_double), ("t_eigs", C.c_double),
-                ("t_env", C.c_double), ("t_svd", C.c_double), ("trunc_error_max", C.c_double), ("gflop", C.c_double)]
+                ("t_env", C.c_double), ("t_svd", C.c_double), ("trunc_error_max", C.c_double), ("gflop", C.c_double),
+                ("expansions", C.c_int), ("t_expand", C.c_double)]
 
 
 class Request(C.Structure):

@@ -298,46 +298,6 @@ __global__ __launch_bounds__(BT) void b_fold(const BProb<T> *pr, const double *p
     if(threadIdx.x == 0) P.io[dst + c] = res[0];
 }
 
-// Device memory recycled by size class (powers of two and 1.5 times them, from 4 KiB): blocks return to the pool
-// instead of hipFree, which would synchronise the whole device (and with it every other stream's work).
-struct DevPool {
-    std::map<size_t, std::vector<void *>> free_;
-    std::vector<void *>                   all_;
-    DevPool() = default;
-    DevPool(const DevPool &) = delete;
-    DevPool &operator=(const DevPool &) = delete;
-    ~DevPool() {
-        for(void *p : all_) (void) hipFree(p);
-    }
-    static size_t cls(size_t b) {
-        for(int k = 12; k < 62; ++k) {
-            const size_t p2 = size_t(1) << k;
-            if(p2 >= b) return p2;
-            if(p2 + p2 / 2 >= b) return p2 + p2 / 2;
-        }
-        return b;
-    }
-    void *get(size_t b, size_t &c) {
-        c       = cls(b);
-        auto &f = free_[c];
-        if(!f.empty()) {
-            void *p = f.back();
-            f.pop_back();
-            return p;
-        }
-        void *p = nullptr;
-        if(hipMalloc(&p, c) != hipSuccess) {
-            (void) hipGetLastError();
-            throw Error(DMRG_ERR_OOM, "batched eigs: device allocation of " + std::to_string(c) + " bytes failed");
-        }
-        all_.push_back(p);
-        return p;
-    }
-    void put(void *p, size_t c) {
-        if(p) free_[c].push_back(p);
-    }
-};
-
 template <typename T>
 struct Slot {
     MatVecOp       *op = nullptr;

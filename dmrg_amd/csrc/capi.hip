@@ -13,6 +13,7 @@
 #include "sweep.hpp"
 #include "dense.hpp"
 #include "svd.hpp"
+#include "vecops.hpp"
 
 #include <complex>
 #include <cstring>
@@ -122,6 +123,58 @@ dmrg::EigsParams batched_params(const dmrg_eigs_config *cfg) {
     p.seed        = cfg->seed;
     return p;
 }
+// Times the eigensolver's block-vector kernels on an n x (2m + 4) working set of its own, filled with random data.
+template <typename T>
+void time_block_vectors(int64_t n, int m, int nrep, float ms[3], double bytes[3]) {
+    const size_t es = sizeof(T);
+    DevBuf       buf, scr, dev;
+    buf.reserve(static_cast<size_t>(n) * (2 * m + 4) * es);
+    scr.reserve(static_cast<size_t>(vec_scratch_doubles()) * sizeof(double));
+    dev.reserve(sizeof(double) * (4 * VEC_MAX_M + 8));
+    T      *V = buf.as<T>(), *W = V + n * m, *x = W + n * m, *r = x + n, *y = r + n;
+    double *d = dev.as<double>();
+    hipStream_t s = nullptr;
+    hipEvent_t  e0 = nullptr, e1 = nullptr;
+    DMRG_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    try {
+        DMRG_HIP_CHECK(hipEventCreate(&e0));
+        DMRG_HIP_CHECK(hipEventCreate(&e1));
+        random_fill<T>(n * (2 * m + 4), V, 7, s);
+        vec_scratch_init(scr.as<double>(), s);
+        std::vector<double> h(2 * VEC_MAX_M, 1.0 / m);
+        DMRG_HIP_CHECK(hipMemcpyAsync(d, h.data(), sizeof(double) * 2 * m, hipMemcpyHostToDevice, s));
+        auto timed = [&](auto &&launch) {
+            launch(); // warm-up
+            DMRG_HIP_CHECK(hipEventRecord(e0, s));
+            for(int k = 0; k < nrep; ++k) launch();
+            DMRG_HIP_CHECK(hipEventRecord(e1, s));
+            DMRG_HIP_CHECK(hipEventSynchronize(e1));
+            float t = 0;
+            DMRG_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+            return t / nrep;
+        };
+        // projections c = V^H x: reads m + 1 vectors
+        ms[0]    = timed([&] { multidot<T>(n, m, V, n, x, d + 2 * VEC_MAX_M, scr.as<double>(), s); });
+        bytes[0] = static_cast<double>(m + 1) * n * es;
+        // update y = y - V c: reads m + 1, writes 1
+        ms[1]    = timed([&] { gemv_update<T>(n, m, V, n, d, -1.0, 1.0, y, s); });
+        bytes[1] = static_cast<double>(m + 2) * n * es;
+        // Ritz vector and residual x = V y, r = W y - theta x: reads 2m, writes 2
+        ms[2]    = timed([&] { ritz_residual<T>(n, m, V, W, n, d, 0.5, x, r, d + 2 * VEC_MAX_M, scr.as<double>(), s); });
+        bytes[2] = static_cast<double>(2 * m + 2) * n * es;
+    } catch(...) {
+        if(e0) (void) hipEventDestroy(e0);
+        if(e1) (void) hipEventDestroy(e1);
+        (void) hipStreamSynchronize(s);
+        (void) hipStreamDestroy(s);
+        throw;
+    }
+    (void) hipEventDestroy(e0);
+    (void) hipEventDestroy(e1);
+    DMRG_HIP_CHECK(hipStreamSynchronize(s));
+    DMRG_HIP_CHECK(hipStreamDestroy(s));
+}
+
 dmrg_eigs_result batched_result(const BatchEigsResult &r, double sec) {
     dmrg_eigs_result o{};
     o.eigval[0]    = r.eigval;
@@ -286,6 +339,27 @@ dmrg_status dmrg_env_build_block(dmrg_dtype dtype, char side, void *block_out, c
     });
 }
 
+namespace dmrg {
+namespace {
+// Real-cast branch of the expansion term (EnvBase.cpp:305-307): helpers on complex double arrays.
+__global__ void any_imag_kernel(int64_t n, const cpx<double> *x, int *flag) {
+    bool nz = false;
+    for(int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        nz |= x[i].im != 0.0;
+    if(nz) atomicOr(flag, 1);
+}
+__global__ void real_part_kernel(int64_t n, const cpx<double> *x, double *y) {
+    for(int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        y[i] = x[i].re;
+}
+__global__ void cx_from_real_kernel(int64_t n, const double *x, double alpha, cpx<double> *y) {
+    for(int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        y[i] = cpx<double>{alpha * x[i], 0.0};
+}
+dim3 ew_blocks(int64_t n) { return dim3(static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096)))); }
+} // namespace
+} // namespace dmrg
+
 dmrg_status dmrg_env_expansion_term(dmrg_dtype dtype, char side, void *P, const void *block, const int64_t block_dims[3],
                                     const void *mps, const int64_t mps_dims[3], const void *mpo, const int64_t mpo_dims[4],
                                     double alpha, void *stream) {
@@ -312,7 +386,33 @@ dmrg_status dmrg_env_expansion_term(dmrg_dtype dtype, char side, void *P, const 
             DMRG_HIP_CHECK(hipMemsetAsync(P, 0, static_cast<size_t>(nP) * dtype_size(dtype), s));
             return;
         }
-        void *ws = scratch_workspace(static_cast<size_t>(D.d * D.chiL * D.chiR * (side == 'L' ? D.DL : D.DR)) * dtype_size(dtype));
+        const int64_t nX = D.d * D.chiL * D.chiR * (side == 'L' ? D.DL : D.DR); // the contraction's intermediate
+        if(dtype == DMRG_CX64) {
+            // EnvBase.cpp:305-307: when mps and mpo are real, the term is contracted in real arithmetic with the
+            // real part of the env block and cast back to complex.
+            const int64_t nB = block_dims[0] * block_dims[1] * block_dims[2], nM = D.n(), nW = D.DL * D.DR * D.d * D.d;
+            double       *w0 = static_cast<double *>(scratch_workspace(static_cast<size_t>(2 + nB + nM + nW + nP + nX) * sizeof(double)));
+            int          *flag = reinterpret_cast<int *>(w0);
+            DMRG_HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+            hipLaunchKernelGGL(any_imag_kernel, ew_blocks(nM), dim3(256), 0, s, nM, static_cast<const cpx<double> *>(mps), flag);
+            hipLaunchKernelGGL(any_imag_kernel, ew_blocks(nW), dim3(256), 0, s, nW, static_cast<const cpx<double> *>(mpo), flag);
+            DMRG_HIP_CHECK(hipGetLastError());
+            int host_flag = 1;
+            DMRG_HIP_CHECK(hipMemcpyAsync(&host_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+            DMRG_HIP_CHECK(hipStreamSynchronize(s));
+            if(host_flag == 0) {
+                double *Br = w0 + 2, *Mr = Br + nB, *Wr = Mr + nM, *Pr = Wr + nW, *Xr = Pr + nP;
+                hipLaunchKernelGGL(real_part_kernel, ew_blocks(nB), dim3(256), 0, s, nB, static_cast<const cpx<double> *>(block), Br);
+                hipLaunchKernelGGL(real_part_kernel, ew_blocks(nM), dim3(256), 0, s, nM, static_cast<const cpx<double> *>(mps), Mr);
+                hipLaunchKernelGGL(real_part_kernel, ew_blocks(nW), dim3(256), 0, s, nW, static_cast<const cpx<double> *>(mpo), Wr);
+                DMRG_HIP_CHECK(hipGetLastError());
+                env_expansion_term(DMRG_FP64, side, D, Pr, Br, Mr, Wr, Xr, s);
+                hipLaunchKernelGGL(cx_from_real_kernel, ew_blocks(nP), dim3(256), 0, s, nP, Pr, alpha, static_cast<cpx<double> *>(P));
+                DMRG_HIP_CHECK(hipGetLastError());
+                return;
+            }
+        }
+        void *ws = scratch_workspace(static_cast<size_t>(nX) * dtype_size(dtype));
         env_expansion_term(dtype, side, D, P, block, mps, mpo, ws, s);
         if(alpha != 1.0) {
             switch(dtype) {
@@ -505,6 +605,8 @@ void dmrg_eigs_default_config(dmrg_eigs_config *cfg) {
     cfg->max_matvecs = 100000;
     cfg->precond     = 0;
     cfg->mixed       = 0;
+    cfg->projection  = 0;
+    cfg->conv_test   = 0;
     cfg->max_time_s  = 0.0;
     cfg->seed        = 12345;
     cfg->jcb_block   = 128;
@@ -535,6 +637,10 @@ dmrg_status dmrg_eigs(dmrg_matvec *opp, const dmrg_eigs_config *cfg, const void 
         DMRG_REQUIRE(cfg->inner_max >= 0 && cfg->inner_tol > 0 && cfg->inner_tol < 1, DMRG_ERR_ARG, "bad inner_max / inner_tol");
         p.inner_max = cfg->mixed ? cfg->inner_max : 0;
         p.inner_tol = cfg->inner_tol;
+        DMRG_REQUIRE(cfg->projection == 0 || cfg->projection == 1, DMRG_ERR_ARG, "projection: 0 Rayleigh-Ritz or 1 refined");
+        DMRG_REQUIRE(cfg->conv_test == 0 || cfg->conv_test == 1, DMRG_ERR_ARG, "conv_test: 0 or 1");
+        p.projection = cfg->projection;
+        p.conv_test  = cfg->conv_test;
         auto      s   = static_cast<hipStream_t>(stream);
         EigsStats st, s1;
         *res          = dmrg_eigs_result{};
@@ -761,6 +867,18 @@ dmrg_status dmrg_matvec_time_legs(dmrg_matvec *op, const void *dx, void *dy, int
     return guard([&] {
         DMRG_REQUIRE(op && dx && dy && ms_per_leg && nrep > 0, DMRG_ERR_ARG, "bad argument");
         reinterpret_cast<MatVecOp *>(op)->time_legs(dx, dy, nrep, fp32 != 0, ms_per_leg, static_cast<hipStream_t>(stream));
+    });
+}
+
+dmrg_status dmrg_block_vector_time(dmrg_dtype dtype, int64_t n, int m, int nrep, float ms[3], double bytes[3]) {
+    return guard([&] {
+        DMRG_REQUIRE(n > 0 && m >= 1 && m <= VEC_MAX_M && nrep > 0 && ms && bytes, DMRG_ERR_ARG, "bad argument");
+        switch(dtype) {
+            case DMRG_FP64: time_block_vectors<double>(n, m, nrep, ms, bytes); break;
+            case DMRG_CX64: time_block_vectors<cpx<double>>(n, m, nrep, ms, bytes); break;
+            case DMRG_FP32: time_block_vectors<float>(n, m, nrep, ms, bytes); break;
+            case DMRG_CX32: time_block_vectors<cpx<float>>(n, m, nrep, ms, bytes); break;
+        }
     });
 }
 

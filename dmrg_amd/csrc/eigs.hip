@@ -4,6 +4,7 @@
 #include "dense_eig.hpp"
 
 #include <algorithm>
+#include <deque>
 #include <chrono>
 #include <cmath>
 #include <limits>
@@ -19,7 +20,12 @@ constexpr int SB_NRM  = 2 * VEC_MAX_M;     // |t|^2 before / after orthogonalisa
 constexpr int SB_RITZ = 2 * VEC_MAX_M + 4; // |r|^2, |x|^2
 constexpr int SB_Y    = 2 * VEC_MAX_M + 8; // Ritz coefficients, 2 * VEC_MAX_M
 constexpr int SB_FLAG = 4 * VEC_MAX_M + 8; // second-pass flag of the DGKS orthogonalisation (int)
-constexpr int SB_SIZE = 4 * VEC_MAX_M + 10;
+// Refined projection: QR of Z = W - tau V, column by column (two Gram-Schmidt passes and the norm).
+constexpr int SB_QC1  = 4 * VEC_MAX_M + 10; // coefficients of pass 1, 2 * VEC_MAX_M
+constexpr int SB_QC2  = 6 * VEC_MAX_M + 10; // coefficients of pass 2, 2 * VEC_MAX_M
+constexpr int SB_QN   = 8 * VEC_MAX_M + 10; // |z|^2 after both passes
+constexpr int SB_TAU  = 8 * VEC_MAX_M + 12; // -tau as a complex number (2 doubles)
+constexpr int SB_SIZE = 8 * VEC_MAX_M + 14;
 
 // ------------------------------------------------------------------------------------------- correction solver
 constexpr int CG_CHECK = 4; // host look at the CG state every CG_CHECK steps
@@ -29,12 +35,15 @@ CorrectionSolver<T32>::CorrectionSolver(MatVecOp &o, hipStream_t st) : op(&o), s
     scratch.reserve(static_cast<size_t>(vec_scratch_doubles()) * sizeof(double));
     vec_scratch_init(scratch.as<double>(), s);
     dev.reserve((8 + CG_STATE) * sizeof(double));
-    DMRG_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host), CG_STATE * sizeof(double), hipHostMallocDefault));
+    host = static_cast<double *>(pinned_get(CG_STATE * sizeof(double)));
 }
 
 template <typename T32>
 CorrectionSolver<T32>::~CorrectionSolver() {
-    if(host) (void) hipHostFree(host);
+    if(host) {
+        (void) hipStreamSynchronize(s); // copies into host may still be queued
+        pinned_put(host, CG_STATE * sizeof(double));
+    }
 }
 
 template <typename T32>
@@ -79,12 +88,15 @@ Davidson<T>::Davidson(MatVecOp &o, hipStream_t st) : op(&o), s(st), n(o.D.n()) {
     scratch.reserve(static_cast<size_t>(vec_scratch_doubles()) * sizeof(double));
     vec_scratch_init(scratch.as<double>(), s);
     small.reserve(SB_SIZE * sizeof(double));
-    DMRG_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host), SB_SIZE * sizeof(double), hipHostMallocDefault));
+    host = static_cast<double *>(pinned_get(SB_SIZE * sizeof(double)));
 }
 
 template <typename T>
 Davidson<T>::~Davidson() {
-    if(host) (void) hipHostFree(host);
+    if(host) {
+        (void) hipStreamSynchronize(s); // copies into host may still be queued
+        pinned_put(host, SB_SIZE * sizeof(double));
+    }
 }
 
 template <typename T>
@@ -124,6 +136,28 @@ void Davidson<T>::expand_from(const T *src, T *t, int m, double *dnrm_pre, doubl
     scale_by_inv_norm<T>(n, t, dnrm_post, s);
 }
 
+// Appends column m to the QR factorisation of Z = W - shift V: z = W[:, m] - shift V[:, m], two classical
+// Gram-Schmidt passes against Qr[:, 0:m] (coefficients to SB_QC1 / SB_QC2), Qr[:, m] = z / |z| (|z|^2 to SB_QN).
+// The host reads the coefficients at the iteration's synchronisation (column m of R = c1 + c2, R(m, m) = |z|).
+template <typename T>
+void Davidson<T>::qr_append(int m, double shift) {
+    double *sm = small.as<double>(), *scr = scratch.as<double>();
+    T      *z  = Qr.as<T>() + static_cast<int64_t>(m) * n;
+    host[SB_TAU] = -shift, host[SB_TAU + 1] = 0.0; // pinned; no copy from the device ever covers SB_TAU
+    DMRG_HIP_CHECK(hipMemcpyAsync(sm + SB_TAU, host + SB_TAU, 2 * sizeof(double), hipMemcpyHostToDevice, s));
+    DMRG_HIP_CHECK(hipMemcpyAsync(z, W.as<T>() + static_cast<int64_t>(m) * n, static_cast<size_t>(n) * sizeof(T),
+                                  hipMemcpyDeviceToDevice, s));
+    gemv_update<T>(n, 1, V.as<T>() + static_cast<int64_t>(m) * n, n, sm + SB_TAU, 1.0, 1.0, z, s);
+    if(m > 0) {
+        multidot<T>(n, m, Qr.as<T>(), n, z, sm + SB_QC1, scr, s);
+        gemv_update<T>(n, m, Qr.as<T>(), n, sm + SB_QC1, -1.0, 1.0, z, s);
+        multidot<T>(n, m, Qr.as<T>(), n, z, sm + SB_QC2, scr, s);
+        gemv_update<T>(n, m, Qr.as<T>(), n, sm + SB_QC2, -1.0, 1.0, z, s);
+    }
+    multidot<T>(n, 1, z, n, z, sm + SB_QN, scr, s);
+    scale_by_inv_norm<T>(n, z, sm + SB_QN, s);
+}
+
 template <typename T>
 void Davidson<T>::transform_basis(T *B, int m, const std::vector<cx> &Q, int k) {
     // B[:, 0:k] <- B[:, 0:m] Q  (Q: m x k, column-major)
@@ -144,6 +178,62 @@ void Davidson<T>::transform_basis(T *B, int m, const std::vector<cx> &Q, int k) 
     DMRG_HIP_CHECK(hipMemcpyAsync(B, tmp.p, static_cast<size_t>(n) * k * sizeof(T), hipMemcpyDeviceToDevice, s));
     // The host copy of q must outlive the async upload.
     DMRG_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// Right singular vectors of an m x m complex matrix R (column-major) by the one-sided (Hestenes) Jacobi method on
+// its columns, which keeps the small singular values to high relative accuracy (the refined projection needs the
+// smallest one of W - tau V, far below eps |A| relative to the largest when tau is near an eigenvalue). Returns
+// the singular values ascending and the matching right singular vectors as the columns of Vr.
+static void small_svd_right(int m, std::vector<std::complex<double>> A, std::vector<double> &sv,
+                            std::vector<std::complex<double>> &Vr) {
+    using cx = std::complex<double>;
+    Vr.assign(static_cast<size_t>(m) * m, cx(0.0, 0.0));
+    for(int i = 0; i < m; ++i) Vr[i + static_cast<size_t>(m) * i] = 1.0;
+    auto col = [&](std::vector<cx> &M, int j) { return M.data() + static_cast<size_t>(m) * j; };
+    const double tol = std::numeric_limits<double>::epsilon() * std::sqrt(static_cast<double>(m));
+    for(int sweep = 0; sweep < 60; ++sweep) {
+        bool rotated = false;
+        for(int p = 0; p < m - 1; ++p)
+            for(int q = p + 1; q < m; ++q) {
+                cx    *ap = col(A, p), *aq = col(A, q);
+                double a = 0, b = 0;
+                cx     c(0.0, 0.0);
+                for(int i = 0; i < m; ++i) a += std::norm(ap[i]), b += std::norm(aq[i]), c += std::conj(ap[i]) * aq[i];
+                const double ac = std::abs(c);
+                if(ac <= tol * std::sqrt(a * b) || ac == 0.0) continue;
+                rotated = true;
+                // Rotation that orthogonalises columns p and q: [ap aq] <- [ap aq] [cs, -sn conj(e); sn e, cs]
+                const cx     e   = c / ac;
+                const double zeta = (b - a) / (2.0 * ac);
+                const double t    = std::copysign(1.0, zeta) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+                const double cs   = 1.0 / std::sqrt(1.0 + t * t), sn = cs * t;
+                for(cx *M : {ap, col(Vr, p)}) {
+                    cx *Mq = M == ap ? aq : col(Vr, q);
+                    for(int i = 0; i < m; ++i) {
+                        const cx x = M[i], y = Mq[i];
+                        M[i]       = cs * x - sn * std::conj(e) * y;
+                        Mq[i]      = sn * e * x + cs * y;
+                    }
+                }
+            }
+        if(!rotated) break;
+    }
+    std::vector<double> nrm(m);
+    for(int j = 0; j < m; ++j) {
+        double a = 0;
+        for(int i = 0; i < m; ++i) a += std::norm(col(A, j)[i]);
+        nrm[j] = std::sqrt(a);
+    }
+    std::vector<int> ord(m);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return nrm[x] < nrm[y]; });
+    std::vector<cx> V2(static_cast<size_t>(m) * m);
+    sv.resize(m);
+    for(int j = 0; j < m; ++j) {
+        sv[j] = nrm[ord[j]];
+        for(int i = 0; i < m; ++i) V2[i + static_cast<size_t>(m) * j] = col(Vr, ord[j])[i];
+    }
+    Vr.swap(V2);
 }
 
 static void sort_by_target(const EigsParams &p, const std::vector<double> &w, std::vector<int> &ord) {
@@ -182,6 +272,24 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
     T      *Vp = V.as<T>(), *Wp = W.as<T>(), *r = rbuf.as<T>();
     double *sm = small.as<double>();
     bool pc_built = false;
+    const bool refined = p.projection == 1 && p.target == TARGET_CLOSEST;
+    std::vector<cx> Rr; // refined: upper-triangular factor of W - shift V = Qr Rr (mmax x mmax)
+    if(refined) {
+        Qr.reserve(vbytes * mmax);
+        Rr.assign(static_cast<size_t>(mmax) * mmax, cx(0.0, 0.0));
+    }
+    auto Rat = [&](int i, int j) -> cx & { return Rr[i + static_cast<size_t>(mmax) * j]; };
+    constexpr int QR_SPAN = SB_TAU - SB_QC1; // the doubles of SB_QC1 .. SB_QN
+    // Column j of Rr from the host copy of the QR exchange region (after the synchronisation that brought it).
+    auto qr_accept = [&](int j) {
+        for(int i = 0; i < j; ++i)
+            Rat(i, j) = cx(host[SB_QC1 + 2 * i] + host[SB_QC2 + 2 * i], host[SB_QC1 + 2 * i + 1] + host[SB_QC2 + 2 * i + 1]);
+        Rat(j, j) = cx(std::sqrt(std::max(0.0, host[SB_QN])), 0.0);
+        for(int i = j + 1; i < mmax; ++i) Rat(i, j) = cx(0.0, 0.0);
+    };
+    auto qr_fetch = [&]() {
+        DMRG_HIP_CHECK(hipMemcpyAsync(host + SB_QC1, sm + SB_QC1, QR_SPAN * sizeof(double), hipMemcpyDeviceToHost, s));
+    };
     if(p.precond == 1) {
         // Diagonal of H_eff in the operator's FP64 dtype; the kernel converts on the fly.
         diag.reserve(static_cast<size_t>(n) * dtype_size(op->dtype));
@@ -215,6 +323,13 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
         DMRG_HIP_CHECK(hipStreamSynchronize(s));
         for(int i = 0; i < m; ++i) Hat(i, j) = cx(host[SB_COL + 2 * i], host[SB_COL + 2 * i + 1]);
     }
+    if(refined)
+        for(int j = 0; j < m; ++j) {
+            qr_append(j, p.shift);
+            qr_fetch();
+            DMRG_HIP_CHECK(hipStreamSynchronize(s));
+            qr_accept(j);
+        }
 
     std::vector<double> w(mmax);
     std::vector<cx>     Hm, Z;
@@ -251,6 +366,41 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
         if(!hermitian_eig(m, Hm.data(), m, w.data(), Z.data())) throw Error(DMRG_ERR_NOT_CONVERGED, "Rayleigh-Ritz eigensolver failed");
         sort_by_target(p, w, ord);
         for(int i = 0; i < m; ++i) st.anorm = std::max(st.anorm, std::fabs(w[i]));
+        if(refined) {
+            // Refined vectors: right singular vectors of Rr (ascending singular values, the best first), each
+            // with its Rayleigh quotient y^H H y as the value.
+            std::vector<cx> R(static_cast<size_t>(m) * m), Y;
+            for(int j = 0; j < m; ++j)
+                for(int i = 0; i < m; ++i) R[i + static_cast<size_t>(m) * j] = Rat(i, j);
+            std::vector<double> sv;
+            small_svd_right(m, R, sv, Y);
+            Z = Y;
+            for(int j = 0; j < m; ++j) {
+                cx q(0.0, 0.0);
+                for(int b = 0; b < m; ++b)
+                    for(int a = 0; a < m; ++a)
+                        q += std::conj(Y[a + static_cast<size_t>(m) * j]) * Hm[a + static_cast<size_t>(m) * b] * Y[b + static_cast<size_t>(m) * j];
+                w[j] = q.real();
+            }
+            std::iota(ord.begin(), ord.end(), 0);
+        }
+    };
+    // Convergence of the pair with residual norm rq and value ev (q: its index among the wanted pairs).
+    std::deque<double> recent_ev, recent_rn;
+    auto converged = [&](double rq, double ev, int q) {
+        const double ptol = p.tol * std::max(st.anorm, 1e-300);
+        if(p.conv_test != 1) return rq <= ptol;
+        // folded_spectrum::convTestFun (optimize_folded_spectrum.cpp:41-93), on the leading pair's history
+        if(q == 0) {
+            recent_ev.push_back(ev);
+            recent_rn.push_back(rq);
+            while(recent_ev.size() > 500) recent_ev.pop_front();
+            while(recent_rn.size() > 500) recent_rn.pop_front();
+        }
+        const double dev = recent_ev.size() >= 250 ? recent_ev.back() - recent_ev.front() : -1.0; // sum of differences
+        const double drn = recent_rn.size() >= 250 ? recent_rn.back() - recent_rn.front() : -1.0;
+        const bool   saturated = dev > -p.tol || drn > -p.tol;
+        return (rq < ptol && saturated) || rq < p.tol;
     };
     // FP32 correction solves (FP64 solves of one lowest or largest pair without preconditioner).
     constexpr bool fp64     = std::is_same_v<typename vtraits<T>::R, double>;
@@ -300,6 +450,7 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
         for(int i = 0; i <= mm; ++i) Hat(i, mm) = cx(host[SB_COL + 2 * i], host[SB_COL + 2 * i + 1]);
         for(int i = 0; i < mm; ++i) Hat(mm, i) = std::conj(Hat(i, mm));
         Hat(mm, mm) = cx(Hat(mm, mm).real(), 0.0);
+        if(refined) qr_accept(mm);
         ++mm;
     };
 
@@ -328,10 +479,12 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
             matvec(t, Wp + static_cast<int64_t>(m) * n, 1);
             st.matvecs += 1;
             multidot<T>(n, m + 1, Vp, n, Wp + static_cast<int64_t>(m) * n, sm + SB_COL, scratch.as<double>(), s);
+            if(refined) qr_append(m, p.shift);
             DMRG_HIP_CHECK(hipMemcpyAsync(host, sm, (SB_RITZ + 2) * sizeof(double), hipMemcpyDeviceToHost, s));
+            if(refined) qr_fetch();
             DMRG_HIP_CHECK(hipStreamSynchronize(s));
             rn[0]    = ritz_norm();
-            st.nconv = rn[0] <= p.tol * std::max(st.anorm, 1e-300) ? 1 : 0;
+            st.nconv = converged(rn[0], w[ord[0]], 0) ? 1 : 0;
             if(st.nconv == 1) break; // converged: the expansion is dropped (m unchanged)
             if(rn[0] < 0.9 * best_r) {
                 best_r     = rn[0];
@@ -349,7 +502,7 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
         int pstar = -1;
         for(int q = 0; q < nev && q < m; ++q) {
             rn[q] = ritz(ord[q], m, out + q * n);
-            if(rn[q] > p.tol * std::max(st.anorm, 1e-300)) {
+            if(!converged(rn[q], w[ord[q]], q)) {
                 pstar = q;
                 break;
             }
@@ -424,6 +577,34 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
             std::fill(H.begin(), H.end(), cx(0.0, 0.0));
             for(int j = 0; j < k; ++j)
                 for(int i = 0; i < k; ++i) Hat(i, j) = Hn[i + static_cast<size_t>(k) * j];
+            if(refined) {
+                // (W - shift V) Q = Qr (Rr Q): QR of the small m x k matrix B = Rr Q (modified Gram-Schmidt, twice),
+                // then Qr <- Qr Qs and Rr <- Rs.
+                std::vector<cx> B(static_cast<size_t>(m) * k, cx(0.0, 0.0)), Qs(static_cast<size_t>(m) * k, cx(0.0, 0.0));
+                std::vector<cx> Rs(static_cast<size_t>(k) * k, cx(0.0, 0.0));
+                for(int j = 0; j < k; ++j)
+                    for(int l = 0; l < m; ++l)
+                        for(int i = 0; i <= l; ++i) B[i + static_cast<size_t>(m) * j] += Rat(i, l) * Q[l + static_cast<size_t>(m) * j];
+                for(int j = 0; j < k; ++j) {
+                    std::vector<cx> c(B.begin() + static_cast<size_t>(m) * j, B.begin() + static_cast<size_t>(m) * (j + 1));
+                    for(int pass = 0; pass < 2; ++pass)
+                        for(int i = 0; i < j; ++i) {
+                            cx d(0.0, 0.0);
+                            for(int l = 0; l < m; ++l) d += std::conj(Qs[l + static_cast<size_t>(m) * i]) * c[l];
+                            for(int l = 0; l < m; ++l) c[l] -= d * Qs[l + static_cast<size_t>(m) * i];
+                            Rs[i + static_cast<size_t>(k) * j] += d;
+                        }
+                    double nr = 0;
+                    for(auto &v : c) nr += std::norm(v);
+                    nr = std::sqrt(nr);
+                    Rs[j + static_cast<size_t>(k) * j] = nr;
+                    for(int l = 0; l < m; ++l) Qs[l + static_cast<size_t>(m) * j] = nr > 0 ? c[l] / nr : cx(0.0, 0.0);
+                }
+                transform_basis(Qr.as<T>(), m, Qs, k);
+                std::fill(Rr.begin(), Rr.end(), cx(0.0, 0.0));
+                for(int j = 0; j < k; ++j)
+                    for(int i = 0; i <= j; ++i) Rat(i, j) = Rs[i + static_cast<size_t>(k) * j];
+            }
             m = k;
             ++st.restarts;
             yprev.clear();
@@ -448,7 +629,9 @@ EigsStats Davidson<T>::solve(const EigsParams &p, const T *init, int k0, T *out,
         matvec(t, Wp + static_cast<int64_t>(m) * n, 1);
         st.matvecs += 1;
         multidot<T>(n, m + 1, Vp, n, Wp + static_cast<int64_t>(m) * n, sm + SB_COL, scratch.as<double>(), s);
+        if(refined) qr_append(m, p.shift);
         DMRG_HIP_CHECK(hipMemcpyAsync(host, sm, (SB_NRM + 4) * sizeof(double), hipMemcpyDeviceToHost, s));
+        if(refined) qr_fetch();
         DMRG_HIP_CHECK(hipStreamSynchronize(s));
         accept_expansion(m);
         ++st.iters;

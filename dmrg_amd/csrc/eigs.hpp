@@ -34,6 +34,15 @@ struct EigsParams {
     // (Jacobi-Davidson), stopped early at a residual reduction of inner_tol (0: plain Davidson expansion).
     int    inner_max = 0;
     double inner_tol = 1e-2;
+    // 1 (with TARGET_CLOSEST): refined projection (primme_proj_refined, the reference's choice for folded-spectrum
+    // solves, optimize_folded_spectrum.cpp:295-299): the pair is x = V y with y the right singular vector of the
+    // smallest singular value of the triangular factor of W - shift V (kept as a QR factorisation column by
+    // column), and its value the Rayleigh quotient y^H H y. 0: Rayleigh-Ritz.
+    int projection = 0;
+    // 1: the convergence test of folded_spectrum::convTestFun (optimize_folded_spectrum.cpp:41-93): converged when
+    // |r| < tol |A| and the last up to 500 Ritz values or residual norms have stopped decreasing (their change
+    // over at least 250 iterations is above -tol), or when |r| < tol. 0: |r| <= tol |A|.
+    int conv_test = 0;
 };
 
 struct EigsStats {
@@ -111,7 +120,7 @@ class Davidson {
     MatVecOp   *op;
     hipStream_t s;
     int64_t     n;
-    DevBuf      V, W, tmp, rbuf, scratch, small, Ydev, diag, pcw;
+    DevBuf      V, W, tmp, rbuf, scratch, small, Ydev, diag, pcw, Qr; // Qr: orthonormal factor of W - shift V (refined)
     std::unique_ptr<CorrectionSolver<typename lower_precision<T>::type>> inner;
     double     *host = nullptr; // pinned
     void        matvec(const T *x, T *y, int nvec);
@@ -119,6 +128,7 @@ class Davidson {
     void        expand_from(const T *src, T *t, int m, double *dnrm_pre, double *dnrm_post);
     void        project_column(int m, std::vector<std::complex<double>> &H, int ldh, int col);
     void        transform_basis(T *B, int m, const std::vector<std::complex<double>> &Q, int k);
+    void        qr_append(int m, double shift); // Qr[:, m] from W[:, m] - shift V[:, m]; coefficients to SB_QC*
 };
 
 // Mixed precision: FP32 matrix-core Davidson to a loose tolerance, then FP64 Davidson seeded with the

@@ -8,6 +8,93 @@ namespace dmrg {
 
 using cx = std::complex<double>;
 
+// ------------------------------------------------------------------------------------------ memory pools
+
+DevPool::~DevPool() {
+    for(auto &b : all_) (void) hipFree(b.first);
+}
+
+size_t DevPool::cls(size_t b) {
+    for(int k = 12; k < 62; ++k) {
+        const size_t p2 = size_t(1) << k;
+        if(p2 >= b) return p2;
+        if(p2 + p2 / 2 >= b) return p2 + p2 / 2;
+    }
+    return b;
+}
+
+void *DevPool::get(size_t b, size_t &c) {
+    c = cls(b);
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto                        it = free_.find(c);
+        if(it != free_.end() && !it->second.empty()) {
+            void *p = it->second.back();
+            it->second.pop_back();
+            return p;
+        }
+    }
+    void *p = nullptr;
+    if(hipMalloc(&p, c) != hipSuccess) {
+        (void) hipGetLastError();
+        throw Error(DMRG_ERR_OOM, "device allocation of " + std::to_string(c) + " bytes failed");
+    }
+    std::lock_guard<std::mutex> lk(mu_);
+    all_.emplace_back(p, c);
+    return p;
+}
+
+void DevPool::put(void *p, size_t c) {
+    if(!p) return;
+    std::lock_guard<std::mutex> lk(mu_);
+    free_[c].push_back(p);
+}
+
+size_t DevPool::held_bytes() const {
+    std::lock_guard<std::mutex> lk(mu_);
+    size_t                      t = 0;
+    for(const auto &b : all_) t += b.second;
+    return t;
+}
+
+std::shared_ptr<DevPool> &thread_dev_pool() {
+    static thread_local std::shared_ptr<DevPool> pool;
+    return pool;
+}
+
+DevPoolScope::DevPoolScope() : prev_(thread_dev_pool()) { thread_dev_pool() = std::make_shared<DevPool>(); }
+DevPoolScope::~DevPoolScope() { thread_dev_pool() = std::move(prev_); }
+
+namespace {
+struct PinnedCache {
+    std::vector<std::pair<void *, size_t>> free_;
+    ~PinnedCache() {
+        for(auto &b : free_) (void) hipHostFree(b.first);
+    }
+};
+PinnedCache &pinned_cache() {
+    static thread_local PinnedCache c;
+    return c;
+}
+} // namespace
+
+void *pinned_get(size_t bytes) {
+    auto &f = pinned_cache().free_;
+    for(size_t i = 0; i < f.size(); ++i)
+        if(f[i].second >= bytes) {
+            void *p = f[i].first;
+            f.erase(f.begin() + static_cast<std::ptrdiff_t>(i));
+            return p;
+        }
+    void *p = nullptr;
+    DMRG_HIP_CHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    return p;
+}
+
+void pinned_put(void *p, size_t bytes) {
+    if(p) pinned_cache().free_.emplace_back(p, bytes);
+}
+
 std::vector<cx> merge_mpos(int nmpo, const cx *const *mpos, const int64_t *dims, int64_t out[4]) {
     DMRG_REQUIRE(nmpo >= 1 && mpos && dims, DMRG_ERR_ARG, "need at least one MPO");
     for(int i = 0; i < nmpo; ++i) {
@@ -130,9 +217,38 @@ std::vector<ChainMpo> MatVecOp::chain_view(bool fp32) const {
 }
 
 void MatVecOp::upload_mpo() {
-    upload_as<double>(mpo, mpo_host.data(), static_cast<int64_t>(mpo_host.size()), dtype);
-    for(size_t i = 0; i < chain_host.size(); ++i)
-        upload_as<double>(chain_dev[i], chain_host[i].data(), static_cast<int64_t>(chain_host[i].size()), dtype);
+    if(upload_stream) {
+        // Stream-ordered upload through a pinned staging block: the host never waits for the copy, only (rarely)
+        // for the previous upload's copy to have left the staging block before it is overwritten.
+        const size_t es    = dtype_size(dtype);
+        size_t       total = mpo_host.size();
+        for(const auto &c : chain_host) total += c.size();
+        total *= es;
+        if(staging_ev) DMRG_HIP_CHECK(hipEventSynchronize(staging_ev));
+        else DMRG_HIP_CHECK(hipEventCreateWithFlags(&staging_ev, hipEventDisableTiming));
+        if(staging_bytes < total) {
+            if(staging) pinned_put(staging, staging_bytes);
+            staging       = pinned_get(total);
+            staging_bytes = total;
+        }
+        char *st = static_cast<char *>(staging);
+        auto  put = [&](DevBuf &dst, const std::vector<cx> &src) {
+            const size_t bytes = src.size() * es;
+            if(dtype == DMRG_CX64) std::memcpy(st, src.data(), bytes);
+            else
+                for(size_t i = 0; i < src.size(); ++i) reinterpret_cast<double *>(st)[i] = src[i].real();
+            dst.reserve(bytes);
+            DMRG_HIP_CHECK(hipMemcpyAsync(dst.p, st, bytes, hipMemcpyHostToDevice, upload_stream));
+            st += bytes;
+        };
+        put(mpo, mpo_host);
+        for(size_t i = 0; i < chain_host.size(); ++i) put(chain_dev[i], chain_host[i]);
+        DMRG_HIP_CHECK(hipEventRecord(staging_ev, upload_stream));
+    } else {
+        upload_as<double>(mpo, mpo_host.data(), static_cast<int64_t>(mpo_host.size()), dtype);
+        for(size_t i = 0; i < chain_host.size(); ++i)
+            upload_as<double>(chain_dev[i], chain_host[i].data(), static_cast<int64_t>(chain_host[i].size()), dtype);
+    }
     have32     = false;
     diag_valid = false;
     pc_ready   = false;

@@ -4,10 +4,48 @@
 
 #include <algorithm>
 #include <complex>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 namespace dmrg {
+
+// Device memory recycled by size class (powers of two and 1.5 times them, from 4 KiB): blocks go back to the pool
+// instead of to hipFree, which would synchronise the whole device (and with it every other stream's work). A
+// block handed out again is reused by work that is ordered after the work of its previous holder (the same
+// stream, or after a host synchronisation); every pool below is used that way.
+struct DevPool {
+    DevPool() = default;
+    DevPool(const DevPool &)            = delete;
+    DevPool &operator=(const DevPool &) = delete;
+    ~DevPool();
+    static size_t cls(size_t b);
+    void         *get(size_t b, size_t &c); // a block of class c >= b
+    void          put(void *p, size_t c);
+    size_t        held_bytes() const;
+
+  private:
+    mutable std::mutex                    mu_;
+    std::map<size_t, std::vector<void *>> free_;
+    std::vector<std::pair<void *, size_t>> all_;
+};
+
+// The pool of the calling thread (null: none). While a DevPoolScope lives on a thread, DevBufs that grow on it take
+// their blocks from a pool of that thread, and a block goes back to the pool it came from when its DevBuf grows
+// again or dies; the pool frees its memory when the scope has ended and the last DevBuf holding one of its blocks
+// is gone. For threads that issue all their device work on one stream (the request loop's workers): their
+// buffers then grow and die without any hipFree, hence without device-wide synchronisation.
+std::shared_ptr<DevPool> &thread_dev_pool();
+struct DevPoolScope {
+    DevPoolScope();
+    ~DevPoolScope();
+    DevPoolScope(const DevPoolScope &)            = delete;
+    DevPoolScope &operator=(const DevPoolScope &) = delete;
+
+  private:
+    std::shared_ptr<DevPool> prev_;
+};
 
 // Owning device buffer (grow-only).
 struct DevBuf {
@@ -16,24 +54,44 @@ struct DevBuf {
     DevBuf()     = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr, o.bytes = 0; }
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), pool_(std::move(o.pool_)), cls_(o.cls_) { o.p = nullptr, o.bytes = 0; }
     DevBuf &operator=(DevBuf &&o) noexcept {
         std::swap(p, o.p);
         std::swap(bytes, o.bytes);
+        std::swap(pool_, o.pool_);
+        std::swap(cls_, o.cls_);
         return *this;
     }
     ~DevBuf() { release(); }
     void release() {
-        if(p) (void) hipFree(p);
-        p = nullptr, bytes = 0;
+        if(p) {
+            if(pool_) pool_->put(p, cls_);
+            else (void) hipFree(p);
+        }
+        p = nullptr, bytes = 0, cls_ = 0;
+        pool_.reset();
     }
     // Grow-only; growth is geometric (x1.5) so that buffers that follow a growing bond dimension are
-    // reallocated O(log chi) times (hipFree synchronises the device).
+    // reallocated O(log chi) times. Without a thread pool the old block is freed after a device synchronisation
+    // (queued kernels may still read it; hipFree synchronises the device anyway); with one it goes back to its
+    // pool, to be reused by later work of the same stream.
     void *reserve(size_t b) {
         if(b > bytes) {
             const size_t nb = std::max(b, bytes + bytes / 2);
-            // Queued kernels may still read the old buffer.
-            if(p) (void) hipDeviceSynchronize();
+            if(std::shared_ptr<DevPool> tp = thread_dev_pool()) {
+                size_t c  = 0;
+                void  *np = tp->get(nb, c);
+                if(p) {
+                    if(pool_) pool_->put(p, cls_);
+                    else {
+                        (void) hipDeviceSynchronize();
+                        (void) hipFree(p);
+                    }
+                }
+                p = np, bytes = c, cls_ = c, pool_ = std::move(tp);
+                return p;
+            }
+            if(p && !pool_) (void) hipDeviceSynchronize();
             release();
             if(hipMalloc(&p, nb) != hipSuccess) {
                 (void) hipGetLastError();
@@ -48,7 +106,16 @@ struct DevBuf {
     T *as() const {
         return static_cast<T *>(p);
     }
+
+  private:
+    std::shared_ptr<DevPool> pool_; // the pool the block came from (null: hipMalloc)
+    size_t                   cls_ = 0;
 };
+
+// Pinned host blocks cached per thread (hipHostFree synchronises the device): get returns a block of at least
+// bytes, put hands it back to the calling thread's cache.
+void *pinned_get(size_t bytes);
+void  pinned_put(void *p, size_t bytes);
 
 // Merge a chain of MPOs (cx64, each (DL_i, DR_i, d_i, d_i)) into one (DL_0, DR_last, prod d, prod d) with the
 // first site's physical index fastest (matvec_mpos.cpp:104-119: contract {1},{0}, shuffle {0,3,1,4,2,5}).
@@ -72,6 +139,12 @@ struct MatVecOp {
     std::vector<std::vector<std::complex<double>>> chain_host; // per MPO, cx64, shift applied
     std::vector<int64_t>               chain_dims;            // 4 per MPO
     std::vector<DevBuf>                chain_dev, chain_dev32;
+    // When set, MPO uploads (rebind, set_shift) are enqueued on this stream from a pinned staging block instead of
+    // blocking copies; the operator must then be used on this stream.
+    hipStream_t                        upload_stream = nullptr;
+    void                              *staging       = nullptr;
+    size_t                             staging_bytes = 0;
+    hipEvent_t                         staging_ev    = nullptr;
     std::complex<double>               sigma{0.0, 0.0};
     bool                               ready_shift = false;
     int64_t                            num_mv      = 0;

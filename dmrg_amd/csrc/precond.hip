@@ -31,6 +31,11 @@ constexpr size_t BLOCK_PC_MAX_BYTES = size_t(16) << 30;
 
 MatVecOp::~MatVecOp() {
     if(blas) (void) rocblas_destroy_handle(static_cast<rocblas_handle>(blas));
+    if(staging_ev) {
+        (void) hipEventSynchronize(staging_ev);
+        (void) hipEventDestroy(staging_ev);
+    }
+    if(staging) pinned_put(staging, staging_bytes);
 }
 
 namespace {

@@ -68,7 +68,10 @@ void serve(const dmrg_serve_config &cfg, const dmrg_request *req, int nreq, dmrg
     auto                    now_s   = [&] { return std::chrono::duration<double>(clock::now() - t0).count(); };
 
     auto worker = [&](int wid) {
-        hipStream_t s = nullptr;
+        // All of a worker's device work goes to its stream, so its buffers are recycled in a pool of its own:
+        // no hipFree (a device-wide synchronisation that would stall every other worker) while requests run.
+        DevPoolScope pool_scope;
+        hipStream_t  s = nullptr;
         if(hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
         while(true) {
             int idx = -1;

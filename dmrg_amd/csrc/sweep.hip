@@ -12,6 +12,7 @@
 #include "dense.hpp"
 
 #include "eigs.hpp"
+#include "expand.hpp"
 #include "svd.hpp"
 
 #include <chrono>
@@ -95,7 +96,7 @@ struct Fdmrg {
     bool                                      folded = false; // xDMRG: solves on (H - E_tgt)^2
     dmrg_fdmrg_result                        *res;
     EigService                               *svc = nullptr; // shared batcher for the iterative FP64 energy solves
-    double t_eigs = 0, t_env = 0, t_svd = 0;
+    double t_eigs = 0, t_env = 0, t_svd = 0, t_expand = 0;
     double env_flop = 0; // algorithmic FLOP of the env updates
 
     Fdmrg(const dmrg_fdmrg_config &c, hipStream_t st, dmrg_fdmrg_result *r) : cfg(c), s(st), L(c.L), svd(st), res(r) {}
@@ -230,8 +231,12 @@ struct Fdmrg {
         const int64_t dl[3] = {chi[i], chi[i], Wds[i][0]};
         const int64_t dr[3] = {chi[i + ns], chi[i + ns], Wds[i + ns - 1][1]};
         const void   *el = (sq ? EL2 : EL)[i].p, *er = (sq ? ER2 : ER)[i + ns - 1].p;
-        if(!h) h = std::make_unique<MatVecOp>(DT, ns, mp.data(), md.data(), el, dl, er, dr, true);
-        else h->rebind(ns, mp.data(), md.data(), el, dl, er, dr);
+        if(!h) {
+            h                = std::make_unique<MatVecOp>(DT, ns, mp.data(), md.data(), el, dl, er, dr, true);
+            h->upload_stream = s; // later rebinds upload their MPO on the sweep's stream
+        } else {
+            h->rebind(ns, mp.data(), md.data(), el, dl, er, dr);
+        }
         return *h;
     }
 
@@ -367,9 +372,19 @@ struct Fdmrg {
         theta.reserve(static_cast<size_t>(n) * es());
         merge_two(i, theta.as<T>());
         const Solve sol = solve(i, 2, theta.as<T>());
+        split_two(i, right, theta.as<T>());
+        return sol.E;
+    }
+
+    // Truncated split of the two-site tensor th of sites (i, i+1) (mps::merge_multisite_mps, split.cpp:21-346):
+    // centre_right puts the singular values on site i+1 (A_i = U) and rebuilds the left envs of site i+1,
+    // otherwise on site i (B_{i+1} = Vt) and rebuilds the right envs of site i.
+    void split_two(int i, bool right, const T *th) {
+        const int64_t a = chi[i], b = chi[i + 2];
+        const int64_t n = d * d * a * b;
         // X[(s1, a), (s2, b)] = theta[s1 + d s2, a, b]
         theta2.reserve(static_cast<size_t>(n) * es());
-        remap2d(DT, d * a, d * b, theta.p, fused(1, d * d, d), fused(d, d * d * a, d), theta2.p, plain(1), plain(d * a), nullptr, 0, s);
+        remap2d(DT, d * a, d * b, th, fused(1, d * d, d), fused(d, d * d * a, d), theta2.p, plain(1), plain(d * a), nullptr, 0, s);
         const auto          t0 = clock::now();
         std::vector<double> S;
         svd.compute<T>(static_cast<int>(d * a), static_cast<int>(d * b), theta2.as<T>(), S, SVD_JACOBI);
@@ -399,11 +414,32 @@ struct Fdmrg {
         t_svd += std::chrono::duration<double>(clock::now() - t0).count();
         if(right) env_left(i);
         else env_right(i + 1);
-        return sol.E;
+    }
+
+    // Subspace expansion before the single-site step at site c (env.cpp:1285-1380 with NSITE | FORWARD): the
+    // active site and the next one in the sweep direction form a two-site block; its optimally mixed block
+    // (the combination of theta, H theta and (H - E_s)^2 theta of lowest K3 value, env.cpp:96-239; H1 / H2 bits of
+    // the mode) replaces it, split back with the bond limit and the truncation limit and the centre left on c.
+    void expand_step(int c, bool right) {
+        const int i = right ? c : c - 1;
+        if(i < 0 || i + 1 >= L) return;
+        const int64_t n = d * d * chi[i] * chi[i + 2];
+        theta.reserve(static_cast<size_t>(n) * es());
+        merge_two(i, theta.as<T>());
+        MatVecOp &H1 = bind_op(hop, i, 2, false);
+        MatVecOp &H2 = bind_op(h2op, i, 2, true);
+        xbuf.reserve(static_cast<size_t>(n) * es());
+        MixedBlockResult mr;
+        const auto       t0 = clock::now();
+        optimally_mixed_block(H1, H2, theta.p, cfg.expand_mode & 3, xbuf.p, mr, s);
+        t_expand += std::chrono::duration<double>(clock::now() - t0).count();
+        ++res->expansions;
+        split_two(i, !right, xbuf.as<T>());
     }
 
     // Single-site update of site c, then move the centre one site (right or left).
     std::pair<double, double> one_site(int c, bool right) {
+        if(cfg.expand_mode != 0) expand_step(c, right);
         const int64_t a = chi[c], b = chi[c + 1];
         const int64_t n = d * a * b;
         theta.reserve(static_cast<size_t>(n) * es());
@@ -483,10 +519,18 @@ struct Fdmrg {
         eshift = folded ? cfg.energy_target / L : 0.0;
         build_mpos(eshift);
         init_state();
+        // expand_mode: EnvExpandMode bits (config/enums.h:224-234). With NSITE | FORWARD the bonds grow by
+        // subspace expansion in the single-site sweeps (the reference's default); otherwise two-site sweeps grow them.
+        const bool expand = cfg.expand_mode != 0;
+        if(expand) {
+            DMRG_REQUIRE((cfg.expand_mode & ~(1 | 2 | 8 | 32)) == 0 && (cfg.expand_mode & 8) && (cfg.expand_mode & 32) &&
+                             (cfg.expand_mode & 3),
+                         DMRG_ERR_UNSUPPORTED, "expand_mode: H1 and/or H2 with NSITE | FORWARD (SSITE and BACKWARD are not built)");
+        }
         // Two-site sweeps: grow bonds and converge the energy.
         double E = 0, Eprev = 1e300;
         int    sweeps = 0;
-        const int two_max = cfg.two_site_sweeps > 0 ? cfg.two_site_sweeps : 10;
+        const int two_max = expand ? 0 : (cfg.two_site_sweeps > 0 ? cfg.two_site_sweeps : 10);
         for(int sw = 0; sw < two_max; ++sw) {
             for(int i = 0; i + 1 < L; ++i) E = two_site(i, true);
             for(int i = L - 2; i >= 0; --i) E = two_site(i, false);
@@ -520,6 +564,13 @@ struct Fdmrg {
             }
             ++sweeps;
             if(var < cfg.var_tol) break;
+            if(expand && !folded && std::fabs(E / L - eshift) > 1e-10 * std::max(1.0, std::fabs(E / L))) {
+                // The bonds grew from a product state: move the H^2 shift to the current energy (the centre is on
+                // site 0, every other site right-normalised) so that the variance needs no large cancellation.
+                eshift = E / L;
+                build_mpos(eshift);
+                for(int i = L - 1; i >= 1; --i) env_right(i);
+            }
         }
         res->energy    = E;
         res->variance  = var;
@@ -534,6 +585,7 @@ struct Fdmrg {
         res->t_eigs  = t_eigs;
         res->t_env   = t_env;
         res->t_svd   = t_svd;
+        res->t_expand = t_expand;
         res->seconds = std::chrono::duration<double>(clock::now() - t0).count();
     }
 };

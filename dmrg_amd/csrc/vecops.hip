@@ -383,7 +383,7 @@ void ritz_residual(int64_t n, int m, const T *V, const T *W, int64_t ld, const d
 // ------------------------------------------------------------------------------------------- elementwise
 template <typename T>
 __global__ void scale_inv_norm_kernel(int64_t n, T *x, const double *nrm2) {
-    const double a = 1.0 / sqrt(nrm2[0]);
+    const double a = nrm2[0] > 0.0 ? 1.0 / sqrt(nrm2[0]) : 0.0; // a zero vector stays zero (breakdown tests see it)
     for(int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         double re, im;
         ld(x[i], re, im);

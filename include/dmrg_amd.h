@@ -92,7 +92,9 @@ dmrg_status dmrg_env_build_block(dmrg_dtype dtype, char side, void *block_out, c
  * variance block with the MPO^2), P (d, chiL, chiR*DR), P[s',a',b + chiR wR] = sum block[a,a',w] mps[s,a,b]
  * W[w,wR,s,s']. side 'R': block (chiR, chiR, DR), P (d, chiL*DL, chiR) holding, at storage offset
  * s' + d(b' + chiR(a + chiL wL)), sum block[b,b',w] mps[s,a,b] W[wL,w,s,s'] (the reference's shuffle {3,0,1,2}
- * followed by a plain reshape). alpha = 0 gives zeros; a NaN alpha is DMRG_ERR_ARG (EnvBase.cpp:358). */
+ * followed by a plain reshape). alpha = 0 gives zeros; a NaN alpha is DMRG_ERR_ARG (EnvBase.cpp:358). With
+ * DMRG_CX64 data whose mps and mpo have no imaginary part, the term is computed in real arithmetic from the real
+ * part of the block and returned as complex, as the reference does (EnvBase.cpp:305-307). */
 dmrg_status dmrg_env_expansion_term(dmrg_dtype dtype, char side, void *P, const void *block, const int64_t block_dims[3],
                                     const void *mps, const int64_t mps_dims[3], const void *mpo, const int64_t mpo_dims[4],
                                     double alpha, void *stream);
@@ -203,6 +205,13 @@ typedef struct dmrg_eigs_config {
                              many FP32 conjugate-gradient steps on the Jacobi-Davidson correction equation
                              (default 20; 0: plain FP64 Davidson refinement) */
     double   inner_tol;   /* those steps stop at this reduction of the correction residual (default 1e-2) */
+    int      projection;  /* 0: Rayleigh-Ritz; 1 (with DMRG_TARGET_CLOSEST): refined projection, the reference's
+                             primme_proj_refined for folded-spectrum solves (optimize_folded_spectrum.cpp:295-299):
+                             x = V y with y the right singular vector of the smallest singular value of the
+                             triangular factor of W - shift V, its value the Rayleigh quotient */
+    int      conv_test;   /* 0: |r| <= tol |A|; 1: folded_spectrum::convTestFun (optimize_folded_spectrum.cpp:41-93):
+                             |r| < tol |A| and the Ritz values or residual norms of the last up to 500 iterations
+                             (at least 250) no longer decreasing by more than tol, or |r| < tol */
 } dmrg_eigs_config;
 
 typedef struct dmrg_eigs_result {
@@ -312,6 +321,13 @@ typedef struct dmrg_fdmrg_config {
                                       reference's OptSolver::EIG for problem_size <= eig_max_size (fdmrg.cpp:307,
                                       AlgorithmFinite.cpp:318; the reference's default is 4096); 0 (the default):
                                       the iterative solver at every size, which is faster on the GPU */
+    int           expand_mode;     /* EnvExpandMode bits (config/enums.h:224-234; dmrg_env_expand_mode, settings.h:228):
+                                      0 (NONE): two-site sweeps grow the bonds, then single-site sweeps; H1 (1) and/or
+                                      H2 (2) with NSITE (8) | FORWARD (32), e.g. the reference's DEFAULT 43: single-site
+                                      sweeps from the product state, each step preceded by subspace expansion of the active
+                                      site and the next one in the sweep direction (env.cpp:1285-1380, the optimally mixed
+                                      block of env.cpp:96-239, split with chi_max and trunc_lim). SSITE and BACKWARD:
+                                      DMRG_ERR_UNSUPPORTED */
 } dmrg_fdmrg_config;
 
 typedef struct dmrg_fdmrg_result {
@@ -328,6 +344,8 @@ typedef struct dmrg_fdmrg_result {
     double  t_eigs, t_env, t_svd;
     double  trunc_error_max;                 /* max over bonds of |S[discarded]|_2 (svd.cpp:257-280) */
     double  gflop;                           /* algorithmic matrix-core work: matvecs (FP64 and FP32) and env updates */
+    int     expansions;                      /* subspace expansions (expand_mode != 0) */
+    double  t_expand;                        /* seconds in the optimally mixed blocks of the expansions */
 } dmrg_fdmrg_result;
 
 void        dmrg_fdmrg_default_config(dmrg_fdmrg_config *cfg);
@@ -445,6 +463,12 @@ dmrg_status dmrg_gemm_hybrid_override(int cfg, int nsplit, int ksplit2);
  * matvec_mpos.h:174-179, time the whole call). */
 dmrg_status dmrg_matvec_time_legs(dmrg_matvec *op, const void *dx, void *dy, int nrep, int fp32, float ms[4],
                                   void *stream);
+/* Profiling of the eigensolver's HBM-bound block-vector kernels on a working set of n-element vectors with m basis
+ * columns (allocated and filled by the call): average ms per launch (HIP events) and algorithmic bytes per launch
+ * of [0] the projections V^H x (m + 1 vectors read), [1] the update y - V c (m + 1 read, 1 written), [2] the Ritz
+ * vector and residual V y, W y - theta V y (2m read, 2 written). No counterpart in the reference (PRIMME's
+ * orthogonalisation and restarts run inside its own BLAS calls). */
+dmrg_status dmrg_block_vector_time(dmrg_dtype dtype, int64_t n, int m, int nrep, float ms[3], double bytes[3]);
 
 /* ------------------------------------------------------------ dense path, edges, truncated SVD */
 

@@ -40,6 +40,11 @@ def expansion_term(side: str, block, mps, mpo, alpha: float):
     side 'R': block (chiR, chiR, DR); the reference's contraction gives R2[b', a, wL, s'], shuffles it to
     [s', b', a, wL] and reshapes that column-major buffer to (d, chiL * DL, chiR) without reordering.
     """
+    if np.iscomplexobj(mps) or np.iscomplexobj(mpo) or np.iscomplexobj(block):
+        # EnvBase.cpp:305-307: complex storage whose mps and mpo are real is contracted in real arithmetic with the
+        # real part of the (complex) env block, and cast back to complex.
+        if not np.any(np.imag(mps)) and not np.any(np.imag(mpo)):
+            return expansion_term(side, np.real(block), np.real(mps), np.real(mpo), alpha).astype(np.complex128)
     d, cL, cR = mps.shape
     if side == "L":
         if block.shape[0] != cL:

@@ -83,3 +83,23 @@ def test_optimally_mixed_block_matches_oracle(chi, mode, cplx):
     finally:
         h1.close()
         h2.close()
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+def test_expansion_term_real_cast_branch(side):
+    """Complex storage with real mps and mpo (EnvBase.cpp:305-307): the term uses the real part of the complex env
+    block only (against the oracle's restatement, and visibly different from the full complex contraction); with a
+    complex mps the full complex contraction is used."""
+    print(f"[case] expansion real-cast side={side}", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(12)
+    d, cL, cR, DL, DR = 2, 24, 20, 5, 5
+    mps = rand(rng, False, d, cL, cR).astype(np.complex128)
+    mpo = rand(rng, False, DL, DR, d, d).astype(np.complex128)
+    block = rand(rng, True, cL, cL, DL) if side == "L" else rand(rng, True, cR, cR, DR)
+    P = G.env_expansion_term(side, block, mps, mpo, 0.5)
+    ref = X.expansion_term(side, block, mps, mpo, 0.5)
+    assert rel(P, ref) <= 1e-12 and not np.any(P.imag)
+    full = X.expansion_term_loops(side, block, mps, mpo, 0.5)
+    assert rel(P, full) > 1e-3
+    mps_c = mps + 1e-3j * rng.standard_normal(mps.shape)
+    assert rel(G.env_expansion_term(side, block, mps_c, mpo, 0.5), X.expansion_term_loops(side, block, mps_c, mpo, 0.5)) <= 1e-12

@@ -61,3 +61,26 @@ def test_sweep_dense_branch_matches_iterative(cplx):
     assert rde["variance"] < 1e-12 and rde["matvecs"] < rit["matvecs"]
     np.testing.assert_allclose(rde["entropies"], rit["entropies"], atol=1e-8)
     np.testing.assert_allclose(rde["entropies"], O.ising_free_fermion_entropies(1.0, 1.0, h), atol=1e-8)
+
+
+@pytest.mark.parametrize("L,chi,cplx,mode", [(8, 16, False, 43), (8, 16, True, 43), (16, 64, False, 43), (12, 32, False, 41),
+                                             (12, 32, False, 42)])
+def test_sweep_subspace_expansion(L, chi, cplx, mode):
+    """Single-site sweeps whose bonds grow only by the reference's default subspace expansion (expand_mode H1 | H2 |
+    NSITE | FORWARD = 43; also H1 or H2 alone), from the product state: the free-fermion energy and entropies (exact
+    when chi >= 2^(L/2), truncated otherwise) and variance below 1e-12. Unsupported modes are refused."""
+    import sys
+
+    print(f"[case] sweep expansion L={L} chi={chi} cplx={cplx} mode={mode}", file=sys.stderr, flush=True)
+    h = ising_fields(L, 1.0, seed=L + 1)
+    r = G.fdmrg_ising_tf_rf(L, h, chi_max=chi, complex_=cplx, expand_mode=mode, max_sweeps=40)
+    e0 = O.ising_free_fermion_ground_energy(1.0, 1.0, h)
+    exact = chi >= 2 ** (L // 2)
+    assert r["expansions"] > 0 and r["converged"] == 1, r
+    assert abs(r["energy"] - e0) <= (1e-10 if exact else 1e-9) * abs(e0), (r["energy"], e0, r["sweeps"])
+    assert r["variance"] < 1e-12
+    np.testing.assert_allclose(r["entropies"], O.ising_free_fermion_entropies(1.0, 1.0, h), atol=1e-8 if exact else 1e-6)
+    print(f"[info] sweeps {r['sweeps']} expansions {r['expansions']} t_expand {r['t_expand']:.3f}s total {r['seconds']:.2f}s",
+          file=sys.stderr, flush=True)
+    with pytest.raises(G.DmrgError):
+        G.fdmrg_ising_tf_rf(L, h, chi_max=chi, expand_mode=1 | 2 | 4)  # SSITE

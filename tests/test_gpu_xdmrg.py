@@ -66,3 +66,91 @@ def test_xdmrg_finds_eigenstate_near_target(frac, cplx):
     k = int(np.argmin(np.abs(w - Et)))
     assert abs(r["variance"]) <= 1e-8, r
     assert abs(r["energy"] - w[k]) <= 1e-9 * np.abs(w).max(), (r["energy"], w[k], w[max(0, k - 2):k + 3])
+
+
+def interior_shift(w, k, frac=0.3):
+    """A shift between levels k and k + 1 of the ascending spectrum w, closer to level k."""
+    return w[k] + frac * (w[k + 1] - w[k])
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_refined_projection_interior_dense_check(cplx, mixed):
+    """Refined projection (the reference's primme_proj_refined for closest_abs) on a reduced isolated workload
+    (chi = 12, n = 288) whose dense spectrum is known: CLOSEST to a shift in the middle of the spectrum on H itself
+    returns the level closest to the shift, and the folded operator (H - sigma)^2 at CLOSEST 0 returns the same
+    level (its energy <x|H|x> within 1e-9 |H|, residual within the tolerance)."""
+    from dmrg_amd.workloads import folded_problem, random_heff_problem
+
+    print(f"[case] refined interior cplx={cplx} mixed={mixed}", file=sys.stderr, flush=True)
+    dt = np.complex128 if cplx else np.float64
+    th, W, L, R = random_heff_problem(2, 12, 12, 5, 5, seed=33, complex_=cplx)
+    H = O.dense_heff(W.astype(dt), L.astype(dt), R.astype(dt))
+    w = np.linalg.eigvalsh(H)
+    k = len(w) // 2
+    sigma = interior_shift(w, k)
+    anorm = np.abs(w).max()
+    op = G.MatVecMPOS([W], L, R, dtype=dt)
+    try:
+        lam, v, info = op.eigs(nev=1, target="CLOSEST", shift=sigma, tol=1e-12, projection="refined", mixed=mixed,
+                               max_matvecs=20000)
+        assert info["nconv"] == 1, info
+        assert abs(lam[0] - w[k]) <= 1e-9 * anorm, (lam[0], w[k], w[k - 1:k + 2])
+        x = v[:, 0] / np.linalg.norm(v[:, 0])
+        assert np.linalg.norm(H @ x - lam[0] * x) <= 1e-10 * anorm
+    finally:
+        op.close()
+    if cplx:
+        Wt, Lt, Rt = folded_problem(th, W, L, R, sigma)
+        A = G.MatVecMPOS([Wt], Lt, Rt, dtype=np.complex128)
+        try:
+            mu, v, info = A.eigs(nev=1, target="CLOSEST", shift=0.0, tol=1e-13, projection="refined", mixed=mixed,
+                                 max_matvecs=50000)
+            x = v[:, 0] / np.linalg.norm(v[:, 0])
+            E = np.vdot(x, H @ x).real
+            assert info["nconv"] == 1, info
+            assert abs(E - w[k]) <= 1e-9 * anorm, (E, w[k])
+            assert abs(mu[0] - (w[k] - sigma) ** 2) <= 1e-12 * anorm ** 2
+        finally:
+            A.close()
+
+
+def test_folded_interior_chi256_mixed_vs_fp64():
+    """The H^2 workload at chi = 256 with the shift inside the spectrum (between its 3rd and 4th lowest levels,
+    closer to the 3rd, found by a 4-pair solve): the folded operator (H - sigma)^2 at CLOSEST 0 with the refined
+    projection, in FP64 and in mixed precision. Both return the 3rd level (|E - lambda_3| <= 1e-9 |H|), agree with
+    each other (|dE| <= 1e-10 |H|), and their residuals through the oracle's matvec of the folded operator are
+    within 1e-10 |A|."""
+    from dmrg_amd.workloads import folded_problem, random_heff_problem
+
+    print("[case] folded interior chi=256", file=sys.stderr, flush=True)
+    th, W, L, R = random_heff_problem(2, 256, 256, 5, 5, seed=2)
+    h = G.MatVecMPOS([W], L, R, dtype=np.complex128)
+    try:
+        w4, _, i4 = h.eigs(nev=4, tol=1e-13, seed=5)
+        anorm = i4["anorm"]
+        sigma = interior_shift(np.sort(w4), 2)
+        Wt, Lt, Rt = folded_problem(th, W, L, R, sigma)
+        A = G.MatVecMPOS([Wt], Lt, Rt, dtype=np.complex128)
+        res = {}
+        try:
+            for mixed in (False, True):
+                mu, v, info = A.eigs(nev=1, target="CLOSEST", shift=0.0, tol=1e-11, projection="refined",
+                                     mixed=mixed, max_matvecs=60000, seed=9)
+                x = v[:, 0] / np.linalg.norm(v[:, 0])
+                th_x = x.reshape(th.shape, order="F")
+                Ax = O.matrix_vector_product(th_x, Wt, Lt, Rt).reshape(-1, order="F")
+                Hx = O.matrix_vector_product(th_x, W, L, R).reshape(-1, order="F")
+                E = np.vdot(x, Hx).real
+                resid = np.linalg.norm(Ax - np.vdot(x, Ax).real * x)
+                print(f"[info] mixed={mixed} matvecs {info['matvecs_fp64']}+{info['matvecs_fp32']} E={E:.12f} "
+                      f"l3={np.sort(w4)[2]:.12f} resid/|A|={resid / info['anorm']:.2e}", file=sys.stderr, flush=True)
+                assert info["nconv"] == 1, info
+                assert resid <= 1e-10 * info["anorm"]
+                assert abs(E - np.sort(w4)[2]) <= 1e-9 * anorm, (E, np.sort(w4))
+                res[mixed] = E
+            assert abs(res[True] - res[False]) <= 1e-10 * anorm
+        finally:
+            A.close()
+    finally:
+        h.close()

@@ -122,3 +122,21 @@ def test_optimally_mixed_block_properties(mode, cplx):
     # the mixed block lies in span{psi, H1 psi, H2 psi}
     coef, *_ = np.linalg.lstsq(Q, mixed.reshape(-1, order="F"), rcond=None)
     assert np.linalg.norm(Q @ coef - mixed.reshape(-1, order="F")) <= 1e-10
+
+
+def test_expansion_term_real_cast_branch():
+    """EnvBase.cpp:305-307: with real mps and mpo in complex storage the term uses only the real part of a complex
+    env block (real arithmetic, then cast to complex); with a complex mps the full complex contraction is used."""
+    rng = np.random.default_rng(4)
+    d, cL, cR, DL, DR = 2, 5, 4, 3, 2
+    block = rng.standard_normal((cL, cL, DL)) + 1j * rng.standard_normal((cL, cL, DL))
+    mps = rng.standard_normal((d, cL, cR)).astype(np.complex128)
+    mpo = rng.standard_normal((DL, DR, d, d)).astype(np.complex128)
+    P = X.expansion_term("L", block, mps, mpo, 0.7)
+    assert P.dtype == np.complex128 and not np.any(P.imag)
+    assert np.allclose(P, X.expansion_term_loops("L", block.real, mps.real, mpo.real, 0.7), atol=1e-13)
+    full = X.expansion_term_loops("L", block, mps, mpo, 0.7)
+    assert np.linalg.norm(P - full) > 1e-3 * np.linalg.norm(full)  # the imaginary part of the block is dropped
+    mps_c = mps + 1e-3j * rng.standard_normal(mps.shape)
+    assert np.allclose(X.expansion_term("L", block, mps_c, mpo, 0.7), X.expansion_term_loops("L", block, mps_c, mpo, 0.7),
+                       atol=1e-12)
