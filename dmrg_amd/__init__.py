@@ -504,8 +504,9 @@ class MatVecMPOS:
         call("dmrg_matvec_set_jcb_block_size", self.h, size)
 
     def set_factorization(self, factorization: str) -> None:
-        """MatVecMPOS::factorization of the block-Jacobi blocks: 'LLT', 'LDLT', 'LU' (default) or 'QR'."""
-        call("dmrg_matvec_set_factorization", self.h, {"LLT": 1, "LDLT": 2, "LU": 3, "QR": 4}[factorization])
+        """MatVecMPOS::factorization of the block-Jacobi blocks (an eig::Factorization name): 'LLT' keeps the
+        reference's sign rule, every other name applies the exact inverse of each block ('LU' is the default)."""
+        call("dmrg_matvec_set_factorization", self.h, {"NONE": 0, "LLT": 1, "LDLT": 2, "LU": 3, "QR": 4, "ILUT": 5, "ILDLT": 6}[factorization])
 
     def MultPc(self, x: np.ndarray, shifts) -> np.ndarray:
         """Preconditioner callback on host vectors (MultPc, matvec_mpos.cpp:1333-1361): column i is

@@ -85,9 +85,9 @@ __global__ void sign_blocks(int64_t nb, int64_t b, const int *sign, E *inv) {
 
 void MatVecOp::set_factorization(int f) {
     // eig::Factorization: NONE 0, LLT 1, LDLT 2, LU 3, QR 4, ILUT 5, ILDLT 6 (enums.h:30).
-    DMRG_REQUIRE(f >= 1 && f <= 4, DMRG_ERR_UNSUPPORTED,
-                 "block-Jacobi factorization " + std::to_string(f) + ": LLT (1), LDLT (2), LU (3) or QR (4)");
-    const int g = f == 1 ? 1 : 3; // LDLT and QR give the LU inverse
+    DMRG_REQUIRE(f >= 0 && f <= 6, DMRG_ERR_UNSUPPORTED,
+                 "block-Jacobi factorization " + std::to_string(f) + ": an eig::Factorization value 0..6");
+    const int g = f == 1 ? 1 : 3; // every other choice applies the exact (LU) inverse
     if(g != pc_fact) pc_ready = false;
     pc_fact = g;
 }

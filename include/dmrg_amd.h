@@ -169,9 +169,12 @@ dmrg_status dmrg_matvec_get_diagonal(dmrg_matvec *op, void *ddiag, void *stream)
  * built at the next dmrg_matvec_precond call with its first shift, and again after set_shift or a size change). */
 dmrg_status dmrg_matvec_set_jcb_block_size(dmrg_matvec *op, int64_t size);
 /* The MatVecMPOS::factorization of the block-Jacobi blocks (matvec_mpos.cpp:1143-1240), as eig::Factorization
- * values: 1 LLT, 2 LDLT, 3 LU (the default), 4 QR; others DMRG_ERR_UNSUPPORTED. LDLT, LU and QR solve every
- * nonsingular block exactly and share the LU inverse; LLT keeps the reference's sign rule: a block with
- * Re B(0,0) <= 0 whose negation is positive definite is applied as -B^-1, every other block as B^-1. */
+ * values 0 NONE, 1 LLT, 2 LDLT, 3 LU (the default), 4 QR, 5 ILUT, 6 ILDLT; others DMRG_ERR_UNSUPPORTED. LLT keeps
+ * the reference's sign rule: a block with Re B(0,0) <= 0 whose negation is positive definite is applied as -B^-1,
+ * every other block as B^-1. Every other value applies the exact inverse B^-1 of each nonsingular block: LDLT, LU
+ * and QR solve with it; NONE falls back to it (the reference's ColPivHouseholderQR inverse); ILUT and ILDLT, which
+ * the reference applies as incomplete factorizations with iterative block solves to 1e-3, get the exact inverse
+ * instead. The choice matters only for block sizes > 1. */
 dmrg_status dmrg_matvec_set_factorization(dmrg_matvec *op, int factorization);
 dmrg_status dmrg_matvec_precond(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec,
                                 const double *shifts);

@@ -165,10 +165,20 @@ class MatVecMPOS {
     const EnumValue &get_form() const { return form_; }
     const EnumValue &get_side() const { return side_; }
 
-    // MultBx (matvec_mpos.cpp:986-1019): the B operator; mps_out is left as it is when there is none.
-    void MultBx(T *mps_in, T *mps_out) { check(dmrg_matvec_multbx(op_, mps_in, rows(), mps_out, rows(), 1), "MultBx"); }
+    // MultBx (matvec_mpos.cpp:986-1019): the B operator; mps_out is left as it is when there is none. Like
+    // MultAx it counts into num_mv and t_multAx.
+    void MultBx(T *mps_in, T *mps_out) {
+        const auto t0 = std::chrono::steady_clock::now();
+        check(dmrg_matvec_multbx(op_, mps_in, rows(), mps_out, rows(), 1), "MultBx");
+        t_multAx->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        num_mv = static_cast<long>(dmrg_matvec_num_mv(op_));
+    }
     void MultBx(void *x, int *ldx, void *y, int *ldy, int *blockSize, primme_params *, int *err) {
-        *err = dmrg_matvec_multbx(op_, x, *ldx, y, *ldy, *blockSize) == DMRG_OK ? 0 : -1;
+        const auto        t0 = std::chrono::steady_clock::now();
+        const dmrg_status s  = dmrg_matvec_multbx(op_, x, *ldx, y, *ldy, *blockSize);
+        t_multAx->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        num_mv = static_cast<long>(dmrg_matvec_num_mv(op_));
+        *err = (s == DMRG_OK) ? 0 : -1;
     }
 
     // Jacobi preconditioner, PRIMME applyPreconditioner callback (matvec_mpos.cpp:1333-1361 with

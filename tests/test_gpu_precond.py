@@ -140,14 +140,14 @@ def test_block_jacobi_llt_sign_rule(kind, dtype):
         flips = sum(1 for off in range(0, n, jcb) if not np.allclose(ref[off:off + jcb], np.linalg.solve(
             H[off:off + jcb, off:off + jcb] - shift * np.eye(min(jcb, n - off)), X[off:off + jcb])))
         assert flips == {"indefinite": 0, "negative": n // jcb, "half-negative": cR // 2}[kind]
-        for fact in ("LU", "LDLT", "QR"):
+        for fact in ("LU", "LDLT", "QR", "NONE", "ILUT"):
             op.set_factorization(fact)
             Y2 = op.MultPc(X, [shift])
             ref2 = np.concatenate([np.linalg.solve(H[o:o + jcb, o:o + jcb] - shift * np.eye(min(jcb, n - o)), X[o:o + jcb])
                                    for o in range(0, n, jcb)])
             assert rel(Y2[:, 0], ref2[:, 0]) <= 1e-10
         with pytest.raises(G.DmrgError) as e:
-            G._lib.call("dmrg_matvec_set_factorization", op.h, 5)  # ILUT is not provided
+            G._lib.call("dmrg_matvec_set_factorization", op.h, 7)  # not an eig::Factorization value
         assert e.value.status == 7
     finally:
         op.close()
