@@ -354,7 +354,8 @@ def section_block_vectors(chi=512, reps=50):
 
 def section_sweep(cpu=True):
     """Full fDMRG sweep of ising_tf_rf: L=16, J=1, g=1, h_i ~ U[-1,1] (seed 0), chi_max=64, variance < 1e-12,
-    real FP64, on the GPU; the oracle's CPU sweep of the same algorithm timed on the host cores."""
+    real FP64, on the GPU, with two-site growth (expand_mode 0, the algorithm of the oracle's CPU sweep, which is
+    timed beside it on the host cores) and with the reference's default subspace expansion (expand_mode 43)."""
     import numpy as np
 
     import dmrg_amd as G
@@ -363,17 +364,25 @@ def section_sweep(cpu=True):
 
     L = 16
     h = ising_fields(L, 1.0, seed=0)
-    G.fdmrg_ising_tf_rf(L, h, chi_max=64)  # warm-up (code loads, tuning)
+    G.fdmrg_ising_tf_rf(L, h, chi_max=64, expand_mode=0)  # warm-up (code loads, tuning)
+    G.fdmrg_ising_tf_rf(L, h, chi_max=64, expand_mode=43)
     t0 = time.perf_counter()
-    r = G.fdmrg_ising_tf_rf(L, h, chi_max=64)
+    r = G.fdmrg_ising_tf_rf(L, h, chi_max=64, expand_mode=0)
     gpu_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rx = G.fdmrg_ising_tf_rf(L, h, chi_max=64, expand_mode=43)
+    gpu_x = time.perf_counter() - t0
     e0 = O.ising_free_fermion_ground_energy(1.0, 1.0, h)
     S0 = O.ising_free_fermion_entropies(1.0, 1.0, h)
     out = {"workload": "ising_tf_rf L=16 J=1 g=1 h~U[-1,1] seed 0, chi_max=64, single-site to var<1e-12, FP64 real",
            "gpu_s": round(gpu_s, 3), "energy": r["energy"], "exact_energy": e0, "abs_dE_vs_exact": abs(r["energy"] - e0),
            "variance": r["variance"], "max_entropy_err_vs_exact": float(np.max(np.abs(r["entropies"] - S0))),
            "sweeps": r["sweeps"], "eigensolves": r["eigensolves"], "t_eigs_s": round(r["t_eigs"], 3),
-           "t_env_s": round(r["t_env"], 3), "t_svd_s": round(r["t_svd"], 3)}
+           "t_env_s": round(r["t_env"], 3), "t_svd_s": round(r["t_svd"], 3),
+           "subspace_expansion_43": {"gpu_s": round(gpu_x, 3), "abs_dE_vs_exact": abs(rx["energy"] - e0),
+                                     "variance": rx["variance"], "sweeps": rx["sweeps"], "expansions": rx["expansions"],
+                                     "t_expand_s": round(rx["t_expand"], 3),
+                                     "max_entropy_err_vs_exact": float(np.max(np.abs(rx["entropies"] - S0)))}}
     if cpu:
         from oracle import sweep as S  # cpu_baseline leg: the oracle sweep is the thing timed here
 
@@ -438,7 +447,8 @@ def section_serve(nreq=128, workers=16, load=0.7, seed=4, mixed=True, nreq_batch
                "max_abs_dE_vs_unbatched": float(max(abs(a["energy"] - b["energy"]) for a, b in zip(res_b, res)))}
     lat = [r["latency_s"] for r in res]
     return {"workload": f"{nreq} requests, L in {{16,20,24,32}}, chi_max in {{64,128,256}}, w in {{0.5,2,6}}, "
-                        f"{workers} workers, {'mixed precision (n >= 131072)' if mixed else 'FP64'}; Poisson arrivals "
+                        f"{workers} workers, two-site growth (expand_mode 0), "
+                        f"{'mixed precision (n >= 131072)' if mixed else 'FP64'}; Poisson arrivals "
                         f"at {load:g} x the measured capacity",
             "capacity_solves_per_s": round(capacity, 3), "arrival_rate_per_s": round(rate, 3),
             "solves_per_s": round(st["solves_per_s"], 3), "latency_samples": len(lat),
@@ -869,6 +879,8 @@ def main():
                     help="one 256 MiB device-to-device copy (scales the HBM byte counters of a profiler run), then exit")
     ap.add_argument("--profile-eigs", choices=("fp64", "mixed"),
                     help="one warm-up and 3 timed chi=512 eigensolves in this mode (for profiler passes), then exit")
+    ap.add_argument("--profile-block-vectors", action="store_true",
+                    help="only the eigensolver's block-vector kernels at the chi=512 size (for profiler passes), then exit")
     ap.add_argument("--profile-svd", type=int, metavar="N",
                     help="3 Jacobi SVDs of a random N x N complex matrix (for profiler passes), then exit")
     args = ap.parse_args()
@@ -883,6 +895,9 @@ def main():
         return
     if args.profile_eigs:
         print(json.dumps(profile_eigs(args.profile_eigs, args.chi)))
+        return
+    if args.profile_block_vectors:
+        print(json.dumps(section_block_vectors(args.chi)))
         return
     if args.profile_svd:
         print(json.dumps(profile_svd(args.profile_svd)))

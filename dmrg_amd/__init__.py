@@ -246,12 +246,13 @@ def svd_truncate(A, trunc_lim: float = 1e-14, rank_max: int = 0, method: int = 3
 def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_max: int = 64, max_sweeps: int = 20,
                       two_site_sweeps: int = 10, var_tol: float = 1e-12, trunc_lim: float = 1e-14, eigs_tol: float = 1e-12,
                       mixed: bool = False, complex_: bool = False, seed: int = 0, mixed_min_n: int = 0,
-                      xdmrg: bool = False, energy_target: float = 0.0, eig_max_size: int = 0, expand_mode: int = 0,
+                      xdmrg: bool = False, energy_target: float = 0.0, eig_max_size: int = 0, expand_mode=None,
                       stream=None) -> dict:
     """Finite DMRG ground state of the random transverse-field Ising chain on the GPU
     (fdmrg::run_algorithm with ising_tf_rf, fdmrg.cpp:170-376), or with xdmrg the eigenstate found by minimising
-    <(H - energy_target)^2> (xdmrg.cpp:297-341, folded spectrum). expand_mode: EnvExpandMode bits; 43 (H1 | H2 |
-    NSITE | FORWARD, the reference's default) grows the bonds by subspace expansion in single-site sweeps.
+    <(H - energy_target)^2> (xdmrg.cpp:297-341, folded spectrum). expand_mode: EnvExpandMode bits; None keeps the
+    library default 43 (H1 | H2 | NSITE | FORWARD, the reference's default: the bonds grow by subspace expansion in
+    single-site sweeps); 0 grows them with two-site sweeps first.
     Returns energies, variance, entropies, timings."""
     h = np.ascontiguousarray(np.asarray(h_rand, dtype=np.float64))
     assert h.shape == (L,)
@@ -264,7 +265,8 @@ def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_m
     cfg.mixed, cfg.complex_, cfg.seed, cfg.mixed_min_n = int(mixed), int(complex_), seed, mixed_min_n
     cfg.xdmrg, cfg.energy_target = int(xdmrg), float(energy_target)
     cfg.eig_max_size = int(eig_max_size)
-    cfg.expand_mode = int(expand_mode)
+    if expand_mode is not None:
+        cfg.expand_mode = int(expand_mode)
     res = _lib.FdmrgResult()
     call("dmrg_fdmrg_ising_tf_rf", C.byref(cfg), C.byref(res), stream)
     out = {k: getattr(res, k) for k, _ in res._fields_ if k not in ("entropies", "chi")}
@@ -275,10 +277,11 @@ def fdmrg_ising_tf_rf(L: int, h_rand, J: float = 1.0, h_tran: float = 1.0, chi_m
 
 def serve(requests, workers: int = 4, hbm_budget_bytes: int = 64 << 30, mixed: bool = False, complex_: bool = False,
           var_tol: float = 1e-12, eigs_tol: float = 1e-12, max_sweeps: int = 20, two_site_sweeps: int = 10,
-          J: float = 1.0, h_tran: float = 1.0, batch_eigs: bool = False):
+          J: float = 1.0, h_tran: float = 1.0, batch_eigs: bool = False, expand_mode: int = 0):
     """Run independent ising_tf_rf ground-state requests as they arrive. requests: list of dicts with keys
     L, chi_max, h_rand, arrival_s, seed. With batch_eigs the requests' FP64 energy solves run together in one
-    batched eigensolver (grouped matvec launches). Returns (per-request results, stats)."""
+    batched eigensolver (grouped matvec launches). expand_mode: the sweeps' EnvExpandMode (0: two-site growth;
+    43: the reference's subspace expansion). Returns (per-request results, stats)."""
     n = len(requests)
     fields = [np.ascontiguousarray(np.asarray(r["h_rand"], dtype=np.float64)) for r in requests]
     arr = (_lib.Request * max(n, 1))()
@@ -287,7 +290,7 @@ def serve(requests, workers: int = 4, hbm_budget_bytes: int = 64 << 30, mixed: b
         arr[i].h_rand = fields[i].ctypes.data_as(C.POINTER(C.c_double))
         arr[i].arrival_s, arr[i].seed = float(r.get("arrival_s", 0.0)), int(r.get("seed", 0))
     cfg = _lib.ServeConfig(workers, hbm_budget_bytes, int(mixed), int(complex_), var_tol, eigs_tol, max_sweeps,
-                           two_site_sweeps, J, h_tran, int(batch_eigs))
+                           two_site_sweeps, J, h_tran, int(batch_eigs), int(expand_mode))
     out = (_lib.RequestResult * max(n, 1))()
     st = _lib.ServeStats()
     call("dmrg_serve", C.byref(cfg), arr, n, out, C.byref(st))

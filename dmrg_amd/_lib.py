@@ -137,7 +137,7 @@ class RequestResult(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("workers", C.c_int), ("hbm_budget_bytes", C.c_int64), ("mixed", C.c_int), ("complex_", C.c_int),
                 ("var_tol", C.c_double), ("eigs_tol", C.c_double), ("max_sweeps", C.c_int), ("two_site_sweeps", C.c_int),
-                ("J", C.c_double), ("h_tran", C.c_double), ("batch_eigs", C.c_int)]
+                ("J", C.c_double), ("h_tran", C.c_double), ("batch_eigs", C.c_int), ("expand_mode", C.c_int)]
 
 
 class ServeStats(C.Structure):

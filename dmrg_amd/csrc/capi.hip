@@ -814,6 +814,7 @@ void dmrg_fdmrg_default_config(dmrg_fdmrg_config *cfg) {
     cfg->trunc_lim       = 1e-14;
     cfg->eigs_tol        = 1e-12;
     cfg->seed            = 0;
+    cfg->expand_mode     = 1 | 2 | 8 | 32; // H1 | H2 | NSITE | FORWARD, dmrg_env_expand_mode's default (settings.h:228)
 }
 
 dmrg_status dmrg_fdmrg_ising_tf_rf(const dmrg_fdmrg_config *cfg, dmrg_fdmrg_result *res, void *stream) {

@@ -124,6 +124,7 @@ void serve(const dmrg_serve_config &cfg, const dmrg_request *req, int nreq, dmrg
             c.mixed           = cfg.mixed;
             c.complex_        = cfg.complex_;
             c.seed            = req[idx].seed;
+            c.expand_mode     = cfg.expand_mode;
             try {
                 dmrg_fdmrg_result fr;
                 fdmrg_ising(c, fr, s, svc.get());

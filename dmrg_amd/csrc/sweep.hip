@@ -437,6 +437,34 @@ struct Fdmrg {
         split_two(i, !right, xbuf.as<T>());
     }
 
+    // Moves the centre from site c (tensor th, shape (d, chi[c], chi[c+1])) to site c + 1: th as (d a) x b =
+    // U S Vt, A_c = U, M_{c+1} <- (S Vt) M_{c+1}; records the Schmidt values' entropy at bond c + 1.
+    void move_right(int c, T *th) {
+        const int64_t       a = chi[c], b = chi[c + 1];
+        std::vector<double> S;
+        svd.compute<T>(static_cast<int>(d * a), static_cast<int>(b), th, S, SVD_JACOBI);
+        const int k = svd.k();
+        sdev.reserve(S.size() * sizeof(double));
+        DMRG_HIP_CHECK(hipMemcpyAsync(sdev.p, S.data(), k * sizeof(double), hipMemcpyHostToDevice, s));
+        M[c].reserve(static_cast<size_t>(d * a * k) * es());
+        DMRG_HIP_CHECK(hipMemcpyAsync(M[c].p, svd.U(), static_cast<size_t>(d * a * k) * es(), hipMemcpyDeviceToDevice, s));
+        theta2.reserve(static_cast<size_t>(k * b) * es());
+        remap2d(DT, k, b, svd.Vt(), plain(1), plain(k), theta2.p, plain(1), plain(k), sdev.as<double>(), 1, s); // S Vt
+        const int64_t e = chi[c + 2];
+        tmp.reserve(static_cast<size_t>(d * k * e) * es());
+        GemmDesc g;
+        g.dtype = DT;
+        g.M = k, g.N = static_cast<int32_t>(d * e), g.K = static_cast<int32_t>(b), g.batch = 1;
+        g.A = theta2.p, g.am = plain(1), g.ak = plain(k);
+        g.B = M[c + 1].p, g.bk = plain(d), g.bn = fused(1, d * b, d);
+        g.C = tmp.p, g.cm = plain(d), g.cn = fused(1, d * k, d);
+        gemm(g, s);
+        M[c + 1].reserve(static_cast<size_t>(d * k * e) * es());
+        DMRG_HIP_CHECK(hipMemcpyAsync(M[c + 1].p, tmp.p, static_cast<size_t>(d * k * e) * es(), hipMemcpyDeviceToDevice, s));
+        chi[c + 1]     = k;
+        entropy[c + 1] = entropy_of(S, k);
+    }
+
     // Single-site update of site c, then move the centre one site (right or left).
     std::pair<double, double> one_site(int c, bool right) {
         if(cfg.expand_mode != 0) expand_step(c, right);
@@ -453,28 +481,7 @@ struct Fdmrg {
         const auto   t0  = clock::now();
         std::vector<double> S;
         if(right && c + 1 < L) {
-            // theta as (d a) x b: A_c = U, M_{c+1} <- (S Vt) M_{c+1}
-            svd.compute<T>(static_cast<int>(d * a), static_cast<int>(b), theta.as<T>(), S, SVD_JACOBI);
-            const int k = svd.k();
-            sdev.reserve(S.size() * sizeof(double));
-            DMRG_HIP_CHECK(hipMemcpyAsync(sdev.p, S.data(), k * sizeof(double), hipMemcpyHostToDevice, s));
-            M[c].reserve(static_cast<size_t>(d * a * k) * es());
-            DMRG_HIP_CHECK(hipMemcpyAsync(M[c].p, svd.U(), static_cast<size_t>(d * a * k) * es(), hipMemcpyDeviceToDevice, s));
-            theta2.reserve(static_cast<size_t>(k * b) * es());
-            remap2d(DT, k, b, svd.Vt(), plain(1), plain(k), theta2.p, plain(1), plain(k), sdev.as<double>(), 1, s); // S Vt
-            const int64_t e = chi[c + 2];
-            tmp.reserve(static_cast<size_t>(d * k * e) * es());
-            GemmDesc g;
-            g.dtype = DT;
-            g.M = k, g.N = static_cast<int32_t>(d * e), g.K = static_cast<int32_t>(b), g.batch = 1;
-            g.A = theta2.p, g.am = plain(1), g.ak = plain(k);
-            g.B = M[c + 1].p, g.bk = plain(d), g.bn = fused(1, d * b, d);
-            g.C = tmp.p, g.cm = plain(d), g.cn = fused(1, d * k, d);
-            gemm(g, s);
-            M[c + 1].reserve(static_cast<size_t>(d * k * e) * es());
-            DMRG_HIP_CHECK(hipMemcpyAsync(M[c + 1].p, tmp.p, static_cast<size_t>(d * k * e) * es(), hipMemcpyDeviceToDevice, s));
-            chi[c + 1]     = k;
-            entropy[c + 1] = entropy_of(S, k);
+            move_right(c, theta.as<T>());
             t_svd += std::chrono::duration<double>(clock::now() - t0).count();
             env_left(c);
         } else if(!right && c > 0) {
@@ -571,6 +578,15 @@ struct Fdmrg {
                 build_mpos(eshift);
                 for(int i = L - 1; i >= 1; --i) env_right(i);
             }
+        }
+        if(expand) {
+            // In the expansion sweeps a bond's Schmidt values are recorded when the centre crosses it, and every
+            // later expansion and solve of the half-sweep still changes the state. One pass of the centre from
+            // site 0 (where the left-moving half-sweep ended) to the right end, without solves, records the
+            // Schmidt values of the final state at every bond; the state itself is unchanged.
+            const auto t1 = clock::now();
+            for(int c = 0; c + 1 < L; ++c) move_right(c, M[c].as<T>());
+            t_svd += std::chrono::duration<double>(clock::now() - t1).count();
         }
         res->energy    = E;
         res->variance  = var;

@@ -326,7 +326,8 @@ typedef struct dmrg_fdmrg_config {
                                       the iterative solver at every size, which is faster on the GPU */
     int           expand_mode;     /* EnvExpandMode bits (config/enums.h:224-234; dmrg_env_expand_mode, settings.h:228):
                                       0 (NONE): two-site sweeps grow the bonds, then single-site sweeps; H1 (1) and/or
-                                      H2 (2) with NSITE (8) | FORWARD (32), e.g. the reference's DEFAULT 43: single-site
+                                      H2 (2) with NSITE (8) | FORWARD (32), e.g. the reference's default 43 (also
+                                      dmrg_fdmrg_default_config's): single-site
                                       sweeps from the product state, each step preceded by subspace expansion of the active
                                       site and the next one in the sweep direction (env.cpp:1285-1380, the optimally mixed
                                       block of env.cpp:96-239, split with chi_max and trunc_lim). SSITE and BACKWARD:
@@ -389,6 +390,8 @@ typedef struct dmrg_serve_config {
     double  J, h_tran;
     int     batch_eigs; /* 1: the requests' iterative FP64 energy solves run together in one batch (grouped matvec
                            launches, one synchronisation per Davidson step for all of them); 0: each on its own stream */
+    int     expand_mode; /* the requests' dmrg_fdmrg_config.expand_mode: 0 (the zero default here) grows the bonds with
+                            two-site sweeps; 43 is the reference's subspace expansion */
 } dmrg_serve_config;
 
 typedef struct dmrg_serve_stats {

@@ -14,10 +14,12 @@ from oracle import heff as O
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("expand", [0, None])
 @pytest.mark.parametrize("L,chi,cplx", [(6, 8, False), (8, 16, False), (8, 16, True)])
-def test_sweep_exact_bond_dimension(L, chi, cplx):
+def test_sweep_exact_bond_dimension(L, chi, cplx, expand):
+    """Two-site growth (expand_mode 0) and the default subspace expansion (43) reach the exact ground state."""
     h = ising_fields(L, 1.0, seed=L)
-    r = G.fdmrg_ising_tf_rf(L, h, chi_max=chi, complex_=cplx)
+    r = G.fdmrg_ising_tf_rf(L, h, chi_max=chi, complex_=cplx, expand_mode=expand)
     e0 = O.ising_free_fermion_ground_energy(1.0, 1.0, h)
     assert abs(r["energy"] - e0) <= 1e-10 * abs(e0), (r["energy"], e0, r)
     assert r["converged"] == 1 and r["variance"] < 1e-12
@@ -25,7 +27,8 @@ def test_sweep_exact_bond_dimension(L, chi, cplx):
 
 
 def test_sweep_l16_chi64_seed0():
-    """L=16, J=1, g=1, h_i ~ U[-1,1] (seed 0), chi_max=64, single-site sweeps to variance < 1e-12, FP64 real."""
+    """L=16, J=1, g=1, h_i ~ U[-1,1] (seed 0), chi_max=64, single-site sweeps to variance < 1e-12, FP64 real, with
+    the default growth (the reference's subspace expansion)."""
     L = 16
     h = ising_fields(L, 1.0, seed=0)
     r = G.fdmrg_ising_tf_rf(L, h, chi_max=64)
@@ -80,7 +83,6 @@ def test_sweep_subspace_expansion(L, chi, cplx, mode):
     assert abs(r["energy"] - e0) <= (1e-10 if exact else 1e-9) * abs(e0), (r["energy"], e0, r["sweeps"])
     assert r["variance"] < 1e-12
     np.testing.assert_allclose(r["entropies"], O.ising_free_fermion_entropies(1.0, 1.0, h), atol=1e-8 if exact else 1e-6)
-    print(f"[info] sweeps {r['sweeps']} expansions {r['expansions']} t_expand {r['t_expand']:.3f}s total {r['seconds']:.2f}s",
-          file=sys.stderr, flush=True)
+    print(f"[info] sweeps {r['sweeps']} expansions {r['expansions']} t_expand {r['t_expand']:.3f}s total {r['seconds']:.2f}s")
     with pytest.raises(G.DmrgError):
         G.fdmrg_ising_tf_rf(L, h, chi_max=chi, expand_mode=1 | 2 | 4)  # SSITE
