@@ -52,6 +52,8 @@ SIGNATURES: dict[str, tuple] = {
     "dmrg_event_record": (st, [vp, vp]),
     "dmrg_event_elapsed_ms": (st, [C.POINTER(C.c_float), vp, vp]),
     "dmrg_matrix_vector_product": (st, [C.c_int, vp, vp, i64p, vp, i64p, vp, i64p, vp, i64p, vp]),
+    "dmrg_matrix_vector_product_noalloc": (st, [C.c_int, vp, vp, i64p, vp, i64p, vp, i64p, vp, i64p, vp]),
+    "dmrg_matrix_vector_product_noalloc_max_bytes": (i64, []),
     "dmrg_env_build_block": (st, [C.c_int, C.c_char, vp, vp, i64p, vp, i64p, vp, i64p, vp]),
     "dmrg_env_expansion_term": (st, [C.c_int, C.c_char, vp, vp, i64p, vp, i64p, vp, i64p, C.c_double, vp]),
     "dmrg_optimally_mixed_block": (st, [vp, vp, vp, C.c_int, vp, C.POINTER(C.c_double), C.POINTER(C.c_double), vp]),

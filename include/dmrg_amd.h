@@ -77,6 +77,21 @@ dmrg_status dmrg_matrix_vector_product(dmrg_dtype dtype, void *res, const void *
                                        const int64_t envL_dims[3], const void *envR, const int64_t envR_dims[3],
                                        void *stream);
 
+/* Allocation-free form of dmrg_matrix_vector_product: same arguments, result and dimension checks. No counterpart
+ * in the reference; it is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The two intermediates (d*chiL*chiR*(DL+DR) elements) live in a
+ * fixed device array of DMRG_NOALLOC_SCRATCH_BYTES per GPU (held from the first call on that GPU until the process
+ * ends); larger problems are refused with DMRG_ERR_ARG. No leg is split over K and GEMM plans are never timed (a
+ * plan tuned earlier for the same shape is used, else the size heuristic's). The call allocates and frees
+ * nothing. Calls that overlap on one GPU (other streams or host threads) share the array and race. */
+#define DMRG_NOALLOC_SCRATCH_BYTES (128LL << 20)
+dmrg_status dmrg_matrix_vector_product_noalloc(dmrg_dtype dtype, void *res, const void *mps, const int64_t mps_dims[3],
+                                               const void *mpo, const int64_t mpo_dims[4], const void *envL,
+                                               const int64_t envL_dims[3], const void *envR, const int64_t envR_dims[3],
+                                               void *stream);
+/* DMRG_NOALLOC_SCRATCH_BYTES as compiled into the library. */
+int64_t     dmrg_matrix_vector_product_noalloc_max_bytes(void);
+
 /* Environment update: side 'L': block[b,b',w'] = sum theta[s,a,b] other[a,a',w] W[w,w',s,s'] conj(theta[s',a',b']);
  *                     side 'R': block[a,a',w]  = sum theta[s,a,b] other[b,b',w'] W[w,w',s,s'] conj(theta[s',a',b']).
  * Replaces EnvBase::build_block(otherblock, mps, mpo) (EnvBase.cpp:67-151); the reference is always cx64
