@@ -843,6 +843,34 @@ def profile_eigs(mode, chi=512, tol=1e-10):
     return {"profile_eigs": mode, "chi": chi, "solves": out}
 
 
+def vendor_gemm_rates(chi=512):
+    """The vendor BLAS (torch.matmul on the GPU, i.e. hipBLASLt/rocBLAS) on plain column-major GEMMs of the
+    matvec legs' sizes at chi and chi/2 (leg 1: M = d chi, N = chi D, K = chi; leg 3: M = d chi, N = chi, K = chi D;
+    d = 2, D = 5) and on 4096^3, for comparison with the library's own GEMM legs. FLOP: 8 per complex MAC."""
+    import torch
+
+    out = {}
+    shapes = [(2 * chi, 5 * chi, chi), (2 * chi, chi, 5 * chi), (chi, 5 * chi // 2, chi // 2), (4096, 4096, 4096)]
+    for M, N, K in shapes:
+        for dt in (torch.complex128, torch.complex64):
+            a = torch.randn(M, K, dtype=dt, device="cuda:0")
+            b = torch.randn(K, N, dtype=dt, device="cuda:0")
+            for _ in range(5):
+                c = a @ b
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = 50 if M < 4096 else 10
+            e0.record()
+            for _ in range(reps):
+                c = a @ b
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            out[f"{M}x{N}x{K}_{str(dt).split('.')[-1]}"] = {"ms": round(ms, 4), "tflops": round(8 * M * N * K / ms / 1e9, 1)}
+            del a, b, c
+    return {"vendor_gemm": out, "torch": torch.__version__}
+
+
 def profile_svd(n, seed=5):
     """Warm-up plus 3 Jacobi SVDs (method 3) of a random n x n complex matrix resident on the device."""
     import numpy as np
@@ -879,6 +907,8 @@ def main():
                     help="one 256 MiB device-to-device copy (scales the HBM byte counters of a profiler run), then exit")
     ap.add_argument("--profile-eigs", choices=("fp64", "mixed"),
                     help="one warm-up and 3 timed chi=512 eigensolves in this mode (for profiler passes), then exit")
+    ap.add_argument("--vendor-gemm", action="store_true",
+                    help="only the vendor BLAS (torch.matmul) on GEMMs of the matvec legs' sizes, then exit")
     ap.add_argument("--profile-block-vectors", action="store_true",
                     help="only the eigensolver's block-vector kernels at the chi=512 size (for profiler passes), then exit")
     ap.add_argument("--profile-svd", type=int, metavar="N",
@@ -895,6 +925,9 @@ def main():
         return
     if args.profile_eigs:
         print(json.dumps(profile_eigs(args.profile_eigs, args.chi)))
+        return
+    if args.vendor_gemm:
+        print(json.dumps(vendor_gemm_rates(args.chi)))
         return
     if args.profile_block_vectors:
         print(json.dumps(section_block_vectors(args.chi)))

@@ -748,12 +748,45 @@ struct MixedEngine {
     }
 };
 
-template <typename T, typename Eng = Engine<T>>
+// Problems at least this large (n = d chi^2 >= 2^17: chi >= 256 at d = 2) fill the GPU with their own matvecs:
+// dmrg_eigs_batched solves them one after another with the single-problem solver (in mixed precision with its
+// FP32 correction-equation expansions) rather than in the lockstep engine, whose grouped launches pay off only
+// for problems that leave most of the GPU idle.
+constexpr int64_t SOLO_MIN_N = 131072;
+
+// One problem alone: Davidson in FP64, or the mixed solver when T32 is not void.
+template <typename T, typename T32>
+BatchEigsResult solve_alone(MatVecOp &op, const EigsParams &p, const void *x0, void *evec, hipStream_t s) {
+    BatchEigsResult r;
+    double          ev[DMRG_EIGS_MAX_NEV] = {}, rn[DMRG_EIGS_MAX_NEV] = {};
+    EigsStats       st;
+    if constexpr(std::is_void_v<T32>) {
+        Davidson<T> d(op, s);
+        st = d.solve(p, static_cast<const T *>(x0), x0 ? 1 : 0, static_cast<T *>(evec), ev, rn);
+    } else {
+        EigsStats s1;
+        st             = eigs_mixed<T, T32>(op, s, p, static_cast<const T *>(x0), static_cast<T *>(evec), ev, rn, &s1);
+        r.matvecs_fp32 = s1.matvecs + st.inner_matvecs;
+        r.iters        = s1.iters;
+    }
+    r.eigval  = ev[0];
+    r.rnorm   = rn[0];
+    r.anorm   = st.anorm;
+    r.nconv   = st.nconv;
+    r.iters  += st.iters;
+    r.matvecs = st.matvecs;
+    return r;
+}
+
+template <typename T, typename Eng = Engine<T>, typename T32 = void>
 void batched_impl(MatVecOp *const *ops, int k, const EigsParams &p, const void *const *x0s, void *const *evecs,
                   std::vector<BatchEigsResult> &res, hipStream_t s) {
-    Eng E(s, p);
-    for(int q = 0; q < k; ++q) E.add(ops[q], x0s ? x0s[q] : nullptr, evecs[q], q);
     res.assign(k, BatchEigsResult{});
+    for(int q = 0; q < k; ++q)
+        if(ops[q]->D.n() >= SOLO_MIN_N) res[q] = solve_alone<T, T32>(*ops[q], p, x0s ? x0s[q] : nullptr, evecs[q], s);
+    Eng E(s, p);
+    for(int q = 0; q < k; ++q)
+        if(ops[q]->D.n() < SOLO_MIN_N) E.add(ops[q], x0s ? x0s[q] : nullptr, evecs[q], q);
     while(E.nactive > 0)
         for(const auto &d : E.step()) res[d.id] = d.res;
 }
@@ -958,10 +991,10 @@ void eigs_batched(MatVecOp *const *ops, int k, const EigsParams &p, const void *
     const dmrg_dtype t = ops[0]->dtype;
     for(int q = 0; q < k; ++q) DMRG_REQUIRE(ops[q] && ops[q]->dtype == t && evecs[q], DMRG_ERR_ARG, "batched eigs: one dtype, outputs");
     if(t == DMRG_FP64) {
-        if(mixed) batched_impl<double, MixedEngine<double, float>>(ops, k, p, x0s, evecs, res, s);
+        if(mixed) batched_impl<double, MixedEngine<double, float>, float>(ops, k, p, x0s, evecs, res, s);
         else batched_impl<double>(ops, k, p, x0s, evecs, res, s);
     } else if(t == DMRG_CX64) {
-        if(mixed) batched_impl<cpx<double>, MixedEngine<cpx<double>, cpx<float>>>(ops, k, p, x0s, evecs, res, s);
+        if(mixed) batched_impl<cpx<double>, MixedEngine<cpx<double>, cpx<float>>, cpx<float>>(ops, k, p, x0s, evecs, res, s);
         else batched_impl<cpx<double>>(ops, k, p, x0s, evecs, res, s);
     } else {
         throw Error(DMRG_ERR_ARG, "batched eigs: FP64 or CX64 operators");

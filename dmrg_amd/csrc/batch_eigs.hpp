@@ -23,6 +23,8 @@ struct BatchEigsResult {
 // p.target SA or LA, p.precond 0. Converged problems leave the batch; the grouped launches are rebuilt when
 // the set changes.
 // mixed: FP32 phase (FP32 vectors and matrix-core matvecs) to 5e-6 |A| or stagnation, then FP64 to p.tol.
+// Problems with n >= 2^17 fill the GPU alone: they are solved one after another with the single-problem solver
+// (eigs_mixed in mixed precision, with p.inner_max correction-equation steps), the rest in the lockstep batch.
 void eigs_batched(MatVecOp *const *ops, int k, const EigsParams &p, const void *const *x0s, void *const *evecs,
                   std::vector<BatchEigsResult> &res, hipStream_t s, bool mixed = false);
 

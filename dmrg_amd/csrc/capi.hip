@@ -121,6 +121,10 @@ dmrg::EigsParams batched_params(const dmrg_eigs_config *cfg) {
     p.tol         = std::max(cfg->tol, std::numeric_limits<double>::epsilon());
     p.max_matvecs = cfg->max_matvecs > 0 ? cfg->max_matvecs : 100000;
     p.seed        = cfg->seed;
+    // Used by the problems that dmrg_eigs_batched solves alone in mixed precision.
+    DMRG_REQUIRE(cfg->inner_max >= 0 && cfg->inner_tol > 0 && cfg->inner_tol < 1, DMRG_ERR_ARG, "bad inner_max / inner_tol");
+    p.inner_max = cfg->mixed ? cfg->inner_max : 0;
+    p.inner_tol = cfg->inner_tol;
     return p;
 }
 // Times the eigensolver's block-vector kernels on an n x (2m + 4) working set of its own, filled with random data.

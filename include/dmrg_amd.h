@@ -284,7 +284,10 @@ dmrg_status dmrg_eigs_folded_spectrum(dmrg_matvec *h, dmrg_matvec *h2, const dmr
  * seed as in dmrg_eigs. cfg->mixed = 1: each problem first runs in an FP32 batch (FP32 vectors, grouped FP32
  * matrix-core matvecs, double reductions) to 5e-6 |A| or until its residual stops dropping by 10% over 25 steps,
  * then its FP32 Ritz vector starts it in the FP64 batch, which converges it to tol; results report both matvec
- * counts. x0s: NULL or k device start vectors (entries may be NULL); evecs: k device outputs. */
+ * counts. Problems with n >= 131072 fill the GPU with their own matvecs: they are solved first, one after
+ * another, by the single-problem solver of dmrg_eigs (with cfg->mixed, its FP32 phase and cfg->inner_max
+ * correction-equation steps), and only the smaller ones run in the lockstep batch. x0s: NULL or k device start
+ * vectors (entries may be NULL); evecs: k device outputs. */
 dmrg_status dmrg_eigs_batched(dmrg_matvec *const *ops, int k, const dmrg_eigs_config *cfg, const void *const *x0s,
                               void *const *evecs, dmrg_eigs_result *results, void *stream);
 

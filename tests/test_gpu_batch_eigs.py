@@ -211,3 +211,26 @@ def test_serve_eigs_mixed_alone_vs_mix():
     finally:
         for op in ops:
             op.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batched_large_problems_solved_alone(mixed):
+    """A batch with problems of n >= 2^17 (chi = 256): those are solved one after another by the single-problem
+    solver, the small ones in the lockstep batch; every eigenvalue equals the single FP64 solver's within
+    1e-10 |A|, and in mixed precision the large problems report FP32 matvecs."""
+    print(f"[case] batched large alone mixed={mixed}", file=sys.stderr, flush=True)
+    shapes = [(2, 256, 256, 5, 5), (2, 16, 16, 5, 5), (2, 256, 256, 5, 5), (2, 24, 20, 5, 5)]
+    probs, ops = build(True, shapes, seed=80)
+    try:
+        out = G.eigs_batched(ops, tol=1e-10, mixed=mixed)
+        for op, (lam, vec, info) in zip(ops, out):
+            w, _, i1 = op.eigs(nev=1, tol=1e-10)
+            assert info["nconv"] == 1 and i1["nconv"] == 1, info
+            assert abs(lam - w[0]) <= 1e-10 * i1["anorm"], (lam, w[0])
+            if mixed:
+                assert info["matvecs_fp32"] > 0, info
+            if op.rows() >= 131072 and mixed:
+                assert info["matvecs_fp64"] < 30, info  # correction-equation expansions: few FP64 matvecs
+    finally:
+        for op in ops:
+            op.close()
