@@ -525,6 +525,23 @@ class MatVecMPOS:
     def set_shift(self, shift: complex) -> None:
         call("dmrg_matvec_set_shift", self.h, float(np.real(shift)), float(np.imag(shift)))
 
+    def FactorOP(self, factorization: str = "LU") -> None:
+        """MatVecMPOS::FactorOP (matvec_mpos.cpp:887-913): factor the dense H_eff (with its shift) on the device for
+        shift-invert; an eig::Factorization name: 'LLT' (Cholesky), 'LU' or 'LDLT' (pivoted LU), 'NONE'."""
+        call("dmrg_matvec_factor_op", self.h, {"NONE": 0, "LLT": 1, "LDLT": 2, "LU": 3, "QR": 4}[factorization], None)
+
+    def isReadyFactorOp(self) -> bool:
+        return bool(load().dmrg_matvec_factor_ready(self.h))
+
+    def MultOPv(self, x: np.ndarray) -> np.ndarray:
+        """y = (H_eff)^-1 x for host vectors (columns), with the factor of FactorOP (matvec_mpos.cpp:936-975)."""
+        x = np.asfortranarray(np.asarray(x, dtype=self.dtype))
+        x2 = x.reshape(self.rows(), -1, order="F")
+        y = np.empty_like(x2, order="F")
+        call("dmrg_matvec_mult_opv", self.h, x2.ctypes.data_as(C.c_void_p), x2.shape[0], y.ctypes.data_as(C.c_void_p),
+             y.shape[0], x2.shape[1])
+        return y.reshape(x.shape, order="F")
+
     def get_diagonal(self) -> np.ndarray:
         out = DeviceArray(self.shape_mps, self.dtype)
         call("dmrg_matvec_get_diagonal", self.h, out.ptr, None)

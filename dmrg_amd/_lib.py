@@ -75,6 +75,10 @@ SIGNATURES: dict[str, tuple] = {
     "dmrg_matvec_get_diagonal": (st, [vp, vp, vp]),
     "dmrg_matvec_precond": (st, [vp, vp, i64, vp, i64, C.c_int, dp]),
     "dmrg_matvec_set_jcb_block_size": (st, [vp, i64]),
+    "dmrg_matvec_factor_op": (st, [vp, C.c_int, vp]),
+    "dmrg_matvec_factor_ready": (C.c_int, [vp]),
+    "dmrg_matvec_mult_opv": (st, [vp, vp, i64, vp, i64, C.c_int]),
+    "dmrg_matvec_num_op": (i64, [vp]),
     "dmrg_matvec_set_factorization": (st, [vp, C.c_int]),
     "dmrg_host_heev": (st, [C.c_int, dp, dp, dp]),
     "dmrg_autotune_enable": (st, [C.c_int]),

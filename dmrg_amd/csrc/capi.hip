@@ -590,6 +590,21 @@ dmrg_status dmrg_matvec_set_factorization(dmrg_matvec *op, int factorization) {
     });
 }
 
+dmrg_status dmrg_matvec_factor_op(dmrg_matvec *op, int factorization, void *stream) {
+    return guard([&] {
+        DMRG_REQUIRE(op, DMRG_ERR_ARG, "null operator");
+        reinterpret_cast<MatVecOp *>(op)->factor_op(factorization, static_cast<hipStream_t>(stream));
+    });
+}
+int dmrg_matvec_factor_ready(const dmrg_matvec *op) { return op && reinterpret_cast<const MatVecOp *>(op)->op_ready ? 1 : 0; }
+dmrg_status dmrg_matvec_mult_opv(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec) {
+    return guard([&] {
+        DMRG_REQUIRE(op && (nvec == 0 || (x && y)) && nvec >= 0, DMRG_ERR_ARG, "bad argument");
+        reinterpret_cast<MatVecOp *>(op)->mult_opv_host(x, ldx, y, ldy, nvec);
+    });
+}
+int64_t dmrg_matvec_num_op(const dmrg_matvec *op) { return op ? reinterpret_cast<const MatVecOp *>(op)->num_op : -1; }
+
 dmrg_status dmrg_matvec_precond(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec,
                                 const double *shifts) {
     return guard([&] {
@@ -862,7 +877,7 @@ dmrg_status dmrg_gemm_hybrid_override(int cfg, int nsplit, int ksplit2) {
 }
 dmrg_status dmrg_gemm_plan_override(int cfg, int ksplit) {
     return guard([&] {
-        DMRG_REQUIRE(cfg <= static_cast<int>(TileCfg::SMALLK) && ksplit >= 0 && ksplit <= 8, DMRG_ERR_ARG, "bad plan override");
+        DMRG_REQUIRE(cfg <= static_cast<int>(TileCfg::T128x64W) && ksplit >= 0 && ksplit <= 8, DMRG_ERR_ARG, "bad plan override");
         set_plan_override(cfg, ksplit);
     });
 }

@@ -332,6 +332,7 @@ void MatVecOp::set_shift(cx shift) {
     sigma = shift;
     upload_mpo();
     ready_shift = true;
+    op_ready    = false; // a factorization of the unshifted operator no longer applies
 }
 
 void MatVecOp::diagonal(void *out, hipStream_t s) {

@@ -190,6 +190,16 @@ struct MatVecOp {
     void    block_pc_build(double shift, hipStream_t s);
     // y = P x for one vector in the operator's dtype (device buffers of n elements).
     void    block_pc_apply(const void *x, void *y, hipStream_t s);
+    // Shift-invert (MatVecMPOS::FactorOP / MultOPv, matvec_mpos.cpp:887-975): the dense H_eff of the operator as it
+    // stands (with its shift), factored on the device; op_fact is the eig::Factorization applied (0 NONE: marked
+    // ready without a factorization, as the reference does), op_ready false until factor_op and after set_shift.
+    bool    op_ready = false;
+    int     op_fact  = 0;
+    int64_t num_op   = 0;
+    DevBuf  opA, oppiv, opinfo, opB_rhs;
+    void    factor_op(int f, hipStream_t s);
+    // y[:, i] = (H_eff)^-1 x[:, i] for nvec host vectors with the factorization of factor_op.
+    void    mult_opv_host(const void *x, int64_t ldx, void *y, int64_t ldy, int nvec);
     // Second operator for MultBx (mpos_B, envL_B, envR_B; matvec_mpos.h:47), null when none was given.
     std::unique_ptr<MatVecOp> opB;
 

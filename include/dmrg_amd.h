@@ -191,6 +191,19 @@ dmrg_status dmrg_matvec_set_jcb_block_size(dmrg_matvec *op, int64_t size);
  * the reference applies as incomplete factorizations with iterative block solves to 1e-3, get the exact inverse
  * instead. The choice matters only for block sizes > 1. */
 dmrg_status dmrg_matvec_set_factorization(dmrg_matvec *op, int factorization);
+/* MatVecMPOS::FactorOP (matvec_mpos.cpp:887-913), for shift-invert: forms the dense H_eff of the operator as it
+ * stands (with the shift of dmrg_matvec_set_shift), in the orientation whose product with x is the matvec, and
+ * factors it on the device: factorization 1 (LLT) with Cholesky, 3 (LU) and 2 (LDLT) with pivoted LU (an exact
+ * solve with the same matrix); 0 (NONE) marks the operator ready without a factor, as the reference does; other
+ * values DMRG_ERR_UNSUPPORTED. A matrix that is not positive definite (LLT) or is singular (LU) is DMRG_ERR_ARG.
+ * n^2 elements of device memory, at most 8 GiB (DMRG_ERR_OOM beyond). set_shift afterwards clears it. */
+dmrg_status dmrg_matvec_factor_op(dmrg_matvec *op, int factorization, void *stream);
+/* MatVecMPOS::isReadyFactorOp. */
+int         dmrg_matvec_factor_ready(const dmrg_matvec *op);
+/* MatVecMPOS::MultOPv (matvec_mpos.cpp:936-975): y_i = (H_eff)^-1 x_i for nvec host vectors (columns at strides
+ * ldx / ldy) with the factor of dmrg_matvec_factor_op; counted in dmrg_matvec_num_op. */
+dmrg_status dmrg_matvec_mult_opv(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec);
+int64_t     dmrg_matvec_num_op(const dmrg_matvec *op);
 dmrg_status dmrg_matvec_precond(dmrg_matvec *op, const void *x, int64_t ldx, void *y, int64_t ldy, int nvec,
                                 const double *shifts);
 
@@ -473,8 +486,9 @@ dmrg_status dmrg_autotune_enable(int enable);
 dmrg_status dmrg_autotune_reset(void);
 int         dmrg_autotune_entries(void);
 /* Testing hook: force every GEMM to tile configuration cfg (0..5: 64x64, 32x64, 64x32, 16x128, 128x64, 32x32;
- * 6: the column-streaming kernel for tiny K, which falls back to 64x64 beyond K = 32 or M = 64) with ksplit K
- * slices; cfg < 0 restores the tuned/heuristic choice. */
+ * 6: the column-streaming kernel for tiny K, which falls back to 64x64 beyond K = 32 or M = 64; 7, 8: 64x64 and
+ * 128x64 built from 32x32x2 MFMAs for FP32 and CX32, the same as 0 and 4 for FP64) with ksplit K slices; cfg < 0
+ * restores the tuned/heuristic choice. */
 dmrg_status dmrg_gemm_plan_override(int cfg, int ksplit);
 /* Testing hook: force every GEMM to a hybrid plan of configuration cfg (0..2): output columns [0, nsplit) in one
  * launch without K slicing, the rest in a second launch with ksplit2 K slices (GEMMs without split-K scratch,

@@ -19,8 +19,10 @@
 //   eig::solver_arpack<MatrixType> (solver_arpack.cpp:77-244, 320-380): set_mode, set_side, get_form, get_side,
 //     rows, MultAx(T*, T*) (also as the member pointer &MatrixType::MultAx) and the fields num_mv, num_op.
 // Block-Jacobi MultPc (jcbMaxBlockSize > 1) runs on the GPU with the factorization chosen by the field
-// factorization (LLT, or LU for every other choice; matvec_mpos.cpp:1086-1331). Shift-invert (FactorOP /
-// MultOPv) is not provided: isReadyFactorOp() is false, which keeps PRIMME and ARPACK on MultAx.
+// factorization (LLT, or LU for every other choice; matvec_mpos.cpp:1086-1331). Shift-invert: FactorOP factors
+// the dense H_eff (with its shift) on the GPU with the same field (LLT: Cholesky; LU and LDLT: pivoted LU; NONE:
+// ready without a factor, as the reference) and MultOPv solves with it (matvec_mpos.cpp:887-975), for PRIMME's
+// matrixMatvec and for ARPACK's shift-invert mode.
 #pragma once
 #include "../dmrg_amd.h"
 
@@ -88,14 +90,15 @@ class MatVecMPOS {
   public:
     using Scalar                            = T;
     static constexpr bool can_shift         = true;
-    static constexpr bool can_shift_invert  = false;
+    static constexpr bool can_shift_invert  = true;
     static constexpr bool can_precondition  = true;
     EnumValue             preconditioner;   // eig::Preconditioner: NONE (0) or JACOBI
-    EnumValue             factorization;    // eig::Factorization of the block-Jacobi blocks: LLT (1), else LU
+    EnumValue             factorization;    // eig::Factorization of the block-Jacobi blocks and of FactorOP
     long                  num_mv = 0, num_op = 0, num_pc = 0; // matvec_mpos.h:139-141 (num_mv: vectors applied)
     std::unique_ptr<Timer> t_multAx  = std::make_unique<Timer>();
     std::unique_ptr<Timer> t_multOPv = std::make_unique<Timer>();
     std::unique_ptr<Timer> t_multPc  = std::make_unique<Timer>();
+    std::unique_ptr<Timer> t_factorOP = std::make_unique<Timer>();
 
     // mpos: host cx64 MPO tensors (DL, DR, d, d); envs: host cx64 blocks (chi, chi, D), as EnvBase stores them.
     MatVecMPOS(const std::vector<const std::complex<double> *> &mpos, const std::vector<std::array<long, 4>> &mpo_dims,
@@ -233,9 +236,30 @@ class MatVecMPOS {
         ready_shift_ = true;
     }
     bool isReadyShift() const { return ready_shift_; }
-    bool isReadyFactorOp() const { return false; }
-    void FactorOP() { throw error(DMRG_ERR_UNSUPPORTED, "MatVecMPOS: shift-invert is not provided"); }
-    void MultOPv(void *, int *, void *, int *, int *, primme_params *, int *err) { *err = -1; }
+    bool isReadyFactorOp() const { return dmrg_matvec_factor_ready(op_) != 0; }
+    // FactorOP (matvec_mpos.cpp:887-913): once, with the factorization field, on the operator as set_shift left it.
+    void FactorOP() {
+        if(isReadyFactorOp()) return;
+        const auto t0 = std::chrono::steady_clock::now();
+        check(dmrg_matvec_factor_op(op_, factorization.value, nullptr), "FactorOP");
+        t_factorOP->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    // MultOPv, PRIMME matrixMatvec form (matvec_mpos.cpp:936-975): y_i = (H_eff - shift)^-1 x_i; errors (no
+    // factor, factorization NONE) set *err.
+    void MultOPv(void *x, int *ldx, void *y, int *ldy, int *blockSize, primme_params *, int *err) {
+        const auto        t0 = std::chrono::steady_clock::now();
+        const dmrg_status s  = dmrg_matvec_mult_opv(op_, x, *ldx, y, *ldy, *blockSize);
+        t_multOPv->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        num_op = static_cast<long>(dmrg_matvec_num_op(op_));
+        *err   = (s == DMRG_OK) ? 0 : -1;
+    }
+    // One vector of rows() elements, the form ARPACK's shift-invert reverse communication applies.
+    void MultOPv(T *mps_in, T *mps_out) {
+        const auto t0 = std::chrono::steady_clock::now();
+        check(dmrg_matvec_mult_opv(op_, mps_in, rows(), mps_out, rows(), 1), "MultOPv");
+        t_multOPv->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        num_op = static_cast<long>(dmrg_matvec_num_op(op_));
+    }
 
     dmrg_matvec *handle() { return op_; }
 

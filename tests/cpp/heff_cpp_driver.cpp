@@ -22,7 +22,7 @@ enum class Preconditioner { NONE, JACOBI };
 template <typename M>
 static int solver_requirements(M &matrix, void *x, int ld, void *y, int nvec, double *shifts) {
     using Scalar = typename M::Scalar;
-    static_assert(M::can_shift && M::can_precondition && !M::can_shift_invert, "operator capabilities");
+    static_assert(M::can_shift && M::can_precondition && M::can_shift_invert, "operator capabilities");
     static_assert(sizeof(Scalar) == 8 || sizeof(Scalar) == 16, "fp64 or cx64");
     if(matrix.isReadyFactorOp()) return 7;
     if(matrix.isReadyShift()) return 8;

@@ -304,8 +304,10 @@ static int run(FILE *in, FILE *out, const int64_t h[7]) {
     const int64_t n = d * cL * cR;
     std::vector<cx> mpo(DL * DR * d * d), eL(cL * cL * DL), eR(cR * cR * DR);
     std::vector<T>  x(n), ypc(n, T(0));
+    double          sig[2]; // shift-invert shifts: inside the spectrum (LU) and below it (LLT)
     if(fread(mpo.data(), sizeof(cx), mpo.size(), in) != mpo.size() || fread(eL.data(), sizeof(cx), eL.size(), in) != eL.size() ||
-       fread(eR.data(), sizeof(cx), eR.size(), in) != eR.size() || fread(x.data(), sizeof(T), x.size(), in) != x.size())
+       fread(eR.data(), sizeof(cx), eR.size(), in) != eR.size() || fread(x.data(), sizeof(T), x.size(), in) != x.size() ||
+       fread(sig, sizeof(double), 2, in) != 2)
         return 2;
     using M = dmrg_amd::MatVecMPOS<T>;
     eig::settings cfg;
@@ -334,6 +336,29 @@ static int run(FILE *in, FILE *out, const int64_t h[7]) {
     fwrite(e, sizeof(double), 3, out);
     fwrite(k, sizeof(int64_t), 6, out);
     fwrite(ypc.data(), sizeof(T), ypc.size(), out);
+
+    // Shift-invert, as eigs_primme sets it up (shift, FactorOP, matrixMatvec = MultOPv): the stand-in's Lanczos
+    // finds the lowest eigenvalue of (H - sigma)^-1 with an LU factor of H - sigma for a shift inside the spectrum.
+    M             op3({mpo.data()}, {{DL, DR, d, d}}, eL.data(), {cL, cL, DL}, eR.data(), {cR, cR, DR});
+    eig::settings cfg3;
+    cfg3.sigma        = cx(sig[0], 0.0);
+    cfg3.shift_invert = eig::Shinv::ON;
+    op3.factorization = eig::Factorization::LU;
+    if(op3.isReadyFactorOp()) return 6;
+    const auto rsi = eigs_primme_like(op3, cfg3);
+    if(!op3.isReadyFactorOp() || rsi.num_op != rsi.num_mv || op3.t_factorOP->get_time() <= 0.0) return 7;
+    // ARPACK's shift-invert reverse communication with MultOPv(T*, T*), an LLT factor of H - sigma below the
+    // spectrum; the member-pointer solver still applies MultAx, i.e. H - sigma.
+    M op4({mpo.data()}, {{DL, DR, d, d}}, eL.data(), {cL, cL, DL}, eR.data(), {cR, cR, DR});
+    op4.set_shift(cx(sig[1], 0.0));
+    op4.factorization = eig::Factorization::LLT;
+    op4.FactorOP();
+    solver_arpack_like<M> arp4{op4, cfg};
+    arp4.eigs();
+    const double  esi[3] = {rsi.energy, arp4.energy_member, arp4.energy_rc};
+    const int64_t ksi[2] = {rsi.num_op, arp4.num_op};
+    fwrite(esi, sizeof(double), 3, out);
+    fwrite(ksi, sizeof(int64_t), 2, out);
     return 0;
 }
 

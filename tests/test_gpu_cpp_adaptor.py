@@ -125,24 +125,40 @@ def test_cpp_adaptor_in_solver_templates(tmp_path, cplx):
     assert SOLVER_BIN.exists(), "build first (__graft_entry__.build())"
     d, cL, cR, DL, DR = 2, 12, 10, 3, 3
     th, W, L, R = random_heff_problem(d, cL, cR, DL, DR, seed=8, complex_=cplx)
+    # Identity blocks at the MPO's edge channels (as on a chain), so that set_shift(s) shifts H_eff by -s I.
+    L, R = L.copy(order="F"), R.copy(order="F")
+    L[:, :, -1], R[:, :, 0] = np.eye(cL), np.eye(cR)
     n = d * cL * cR
     dt = np.complex128 if cplx else np.float64
     x = np.random.default_rng(3).standard_normal(n) + (1j * np.random.default_rng(4).standard_normal(n) if cplx else 0)
+    H = O.dense_heff(W.astype(dt), L.astype(dt), R.astype(dt))
+    w = np.linalg.eigvalsh(0.5 * (H + H.conj().T))
+    lev = n // 2
+    sig_in = w[lev] + 0.3 * (w[lev + 1] - w[lev])  # inside the spectrum, nearer level lev
+    sig_lo = w[0] - 1.0                      # below it: H - sig_lo is positive definite
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(inp, "wb") as f:
         np.array([int(cplx), d, cL, cR, DL, DR, 64], dtype=np.int64).tofile(f)
         for a in (W, L, R):
             np.asfortranarray(a.astype(np.complex128)).ravel(order="F").tofile(f)
         x.astype(dt).tofile(f)
+        np.array([sig_in, sig_lo], dtype=np.float64).tofile(f)
     r = subprocess.run([str(SOLVER_BIN), str(inp), str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, (r.returncode, r.stderr)
     raw = np.fromfile(out, dtype=np.uint8)
     E = np.frombuffer(raw[:24].tobytes(), dtype=np.float64)
     k = np.frombuffer(raw[24:72].tobytes(), dtype=np.int64)
-    ypc = np.frombuffer(raw[72:72 + n * np.dtype(dt).itemsize].tobytes(), dtype=dt)
-    H = O.dense_heff(W.astype(dt), L.astype(dt), R.astype(dt))
-    w = np.linalg.eigvalsh(0.5 * (H + H.conj().T))
+    off = 72 + n * np.dtype(dt).itemsize
+    ypc = np.frombuffer(raw[72:off].tobytes(), dtype=dt)
+    esi = np.frombuffer(raw[off:off + 24].tobytes(), dtype=np.float64)
+    ksi = np.frombuffer(raw[off + 24:off + 40].tobytes(), dtype=np.int64)
     scale = np.abs(w).max()
+    # Shift-invert (FactorOP / MultOPv): the lowest eigenvalue of (H - s)^-1 is 1 / (w[lev] - s) for the level just
+    # below an interior shift, and 1 / (w[-1] - s) below the spectrum; MultAx of the shifted operator gives w[0] - s.
+    assert abs(1.0 / esi[0] + sig_in - w[lev]) <= 1e-9 * scale, (esi[0], w[lev - 1:lev + 2], sig_in)
+    assert abs(esi[1] - (w[0] - sig_lo)) <= 1e-9 * scale, (esi[1], w[0] - sig_lo)
+    assert abs(1.0 / esi[2] + sig_lo - w[-1]) <= 1e-9 * scale, (esi[2], w[-1])
+    assert ksi[0] > 0 and ksi[1] > 0, ksi
     for e in E:
         assert abs(e - w[0]) <= 1e-9 * scale, (E, w[0])
     mv_stub, mv_field, mv_arpack, jcb, steps_primme, steps_arpack = (int(v) for v in k)
