@@ -101,10 +101,7 @@ struct GemmDesc {
 // Tile configuration id (see gemm.hip for the table).
 // SMALLK: a column-streaming kernel for GEMMs whose K (and M) are tiny, such as the MPO leg of the matvec
 // (K = d * D); it is HBM-bound and skips the tile machinery. GEMMs outside its limits fall back to T64x64.
-// T64x64W / T128x64W: the 64x64 and 128x64 tiles built from 32x32x2 MFMAs (FP32 and CX32; for FP64 they equal
-// T64x64 / T128x64).
-enum class TileCfg : int { T64x64 = 0, T32x64 = 1, T64x32 = 2, T16x128 = 3, T128x64 = 4, T32x32 = 5, SMALLK = 6,
-                           T64x64W = 7, T128x64W = 8, AUTO = -1 };
+enum class TileCfg : int { T64x64 = 0, T32x64 = 1, T64x32 = 2, T16x128 = 3, T128x64 = 4, T32x32 = 5, SMALLK = 6, AUTO = -1 };
 constexpr int SMALLK_MAX_K = 32;
 constexpr int SMALLK_MAX_M = 64;
 

@@ -46,31 +46,6 @@ struct mfma_t<float> {
     using acc                                = v4f;
     __device__ static inline acc op(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 };
-typedef float v16f __attribute__((ext_vector_type(16)));
-
-// MFMA block shapes used by the tiles: MB x MB outputs and a K depth of 64 / MB per instruction. Operand lanes:
-// lane l supplies row (or column) l % MB at k = l / MB. Result layout (measured on gfx950 with in-bounds probes):
-// lane l, register v holds column l % MB of row row(l / MB, v).
-template <typename R, int MB>
-struct mfma_sh;
-template <>
-struct mfma_sh<double, 16> : mfma_t<double> {
-    static constexpr int NREG = 4;
-    __device__ static constexpr int row(int lr, int v) { return lr + 4 * v; }
-};
-template <>
-struct mfma_sh<float, 16> : mfma_t<float> {
-    static constexpr int NREG = 4;
-    __device__ static constexpr int row(int lr, int v) { return 4 * lr + v; }
-};
-// 32x32x2: half the instructions of 16x16x4 for the same work (each keeps the FP32 pipe twice as long).
-template <>
-struct mfma_sh<float, 32> {
-    using acc                  = v16f;
-    static constexpr int NREG = 16;
-    __device__ static inline acc op(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0); }
-    __device__ static constexpr int row(int lr, int v) { return 8 * (v / 4) + 4 * lr + (v % 4); }
-};
 
 template <typename R, bool CPLX>
 using elem_t = std::conditional_t<CPLX, cpx<R>, R>;
@@ -78,8 +53,8 @@ using elem_t = std::conditional_t<CPLX, cpx<R>, R>;
 // LDS row padding (in elements) that makes the MFMA fragment reads conflict free:
 // a fragment read puts 16 consecutive elements of one row on each group of 16 lanes;
 // the groups that the LDS serves in one pass must land on disjoint bank ranges.
-constexpr int lds_pad(int row_elems, int elem_bytes, int seg = 16) {
-    const int seg_bytes = seg * elem_bytes;                // bytes one group of seg lanes reads
+constexpr int lds_pad(int row_elems, int elem_bytes) {
+    const int seg_bytes = 16 * elem_bytes;                 // bytes one 16-lane group reads
     const int groups    = seg_bytes >= 256 ? 1 : 256 / seg_bytes; // groups per 256-byte pass
     for(int p = 0; p <= 64; ++p) {
         const int row_bytes = (row_elems + p) * elem_bytes;
@@ -101,9 +76,7 @@ constexpr int lds_pad(int row_elems, int elem_bytes, int seg = 16) {
 // along K: consecutive lanes write consecutive rows, so the row pitch must spread them over the banks).
 // 16-byte elements: one pad of 1 serves both. The K-direction pads were chosen with a bank model of
 // 64 banks x 4 bytes (and 32 x 4) over the fragment reads and the tile writes.
-constexpr int pad_rfast(int row_elems, int elem_bytes, int seg = 16) {
-    return elem_bytes >= 16 ? 1 : lds_pad(row_elems, elem_bytes, seg);
-}
+constexpr int pad_rfast(int row_elems, int elem_bytes) { return elem_bytes >= 16 ? 1 : lds_pad(row_elems, elem_bytes); }
 constexpr int pad_kfast(int row_elems, int elem_bytes) { return (elem_bytes == 4 && row_elems >= 64) ? 9 : 1; }
 constexpr int imax(int a, int b) { return a > b ? a : b; }
 
@@ -252,22 +225,21 @@ struct TileLoader {
 };
 
 // One output tile of g: bx is the tile's position in the launch (before the XCD remap), split the K slice
-// and b the batch index. MB: the MFMA block (16: 16x16x4; 32: 32x32x2, FP32 only).
-template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN, bool SWAP, bool ALIGNED, int MB = 16>
+// and b the batch index.
+template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN, bool SWAP, bool ALIGNED>
 __device__ __forceinline__ void gemm_body(const GemmDesc &g, int bx, int split, int b) {
     using E               = elem_t<R, CPLX>;
-    using MF              = mfma_sh<R, MB>;
+    using MF              = mfma_t<R>;
     using acc_t           = typename MF::acc;
-    constexpr int KS      = 64 / MB; // K depth of one MFMA
     constexpr int NT      = 64 * WM * WN;
-    constexpr int TM      = BM / (MB * WM);
-    constexpr int TN      = BN / (MB * WN);
+    constexpr int TM      = BM / (16 * WM);
+    constexpr int TN      = BN / (16 * WN);
     constexpr int EB      = static_cast<int>(sizeof(E));
-    constexpr int LDA     = BM + imax(pad_rfast(BM, EB, MB), pad_kfast(BM, EB)); // allocated row length of A's tile
-    constexpr int LDB     = BN + imax(pad_rfast(BN, EB, MB), pad_kfast(BN, EB));
+    constexpr int LDA     = BM + imax(pad_rfast(BM, EB), pad_kfast(BM, EB)); // allocated row length of A's tile
+    constexpr int LDB     = BN + imax(pad_rfast(BN, EB), pad_kfast(BN, EB));
     constexpr int NACC    = CPLX ? 3 : 1; // complex: sum ar*br, sum ai*bi, sum (ar+ai)*(br+bi)
     static_assert(TM >= 1 && TN >= 1, "wave tile too small");
-    static_assert(BK % KS == 0, "BK must be a multiple of the MFMA k");
+    static_assert(BK % 4 == 0, "BK must be a multiple of the MFMA k");
 
     __shared__ E As[2][BK * LDA];
     __shared__ E Bs[2][BK * LDB];
@@ -306,8 +278,8 @@ __device__ __forceinline__ void gemm_body(const GemmDesc &g, int bx, int split, 
     const int kend  = min(g.K, kbeg + g.kchunk);
 
     // Row pitches: A is loaded along M when g.a_fast_m, B along N when !g.b_fast_k.
-    const int pa = g.a_fast_m ? BM + pad_rfast(BM, EB, MB) : BM + pad_kfast(BM, EB);
-    const int pb = !g.b_fast_k ? BN + pad_rfast(BN, EB, MB) : BN + pad_kfast(BN, EB);
+    const int pa = g.a_fast_m ? BM + pad_rfast(BM, EB) : BM + pad_kfast(BM, EB);
+    const int pb = !g.b_fast_k ? BN + pad_rfast(BN, EB) : BN + pad_kfast(BN, EB);
     TileLoader<E, BM, BK, NT, ALIGNED> la;
     TileLoader<E, BN, BK, NT, ALIGNED> lb;
     la.init(static_cast<const E *>(g.A) + b * g.a_batch, g.am, g.ak, m0, g.M, kbeg, g.a_fast_m, tid);
@@ -320,11 +292,11 @@ __device__ __forceinline__ void gemm_body(const GemmDesc &g, int bx, int split, 
 #pragma unroll
         for(int i = 0; i < TM; ++i)
 #pragma unroll
-            for(int j = 0; j < TN; ++j) acc[c][i][j] = acc_t{};
+            for(int j = 0; j < TN; ++j) acc[c][i][j] = acc_t{0, 0, 0, 0};
 
     const int numK = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
-    const int lrow = lane / MB; // k row within the KS-deep MFMA step
-    const int lcol = lane % MB; // m (or n) within an MB-wide block
+    const int lrow = lane >> 4; // k row within the 4-deep MFMA step
+    const int lcol = lane & 15; // m (or n) within a 16-wide block
     if(numK > 0) {
         la.load(ra, kbeg, kend);
         lb.load(rb, kbeg, kend);
@@ -339,13 +311,13 @@ __device__ __forceinline__ void gemm_body(const GemmDesc &g, int bx, int split, 
             lb.load(rb, kbeg + (kt + 1) * BK, kend);
         }
 #pragma unroll
-        for(int ks = 0; ks < BK / KS; ++ks) {
-            const int kr = ks * KS + lrow;
+        for(int ks = 0; ks < BK / 4; ++ks) {
+            const int kr = ks * 4 + lrow;
             E         fa[TM], fb[TN];
 #pragma unroll
-            for(int i = 0; i < TM; ++i) fa[i] = As[buf][kr * pa + (wm * TM + i) * MB + lcol];
+            for(int i = 0; i < TM; ++i) fa[i] = As[buf][kr * pa + (wm * TM + i) * 16 + lcol];
 #pragma unroll
-            for(int j = 0; j < TN; ++j) fb[j] = Bs[buf][kr * pb + (wn * TN + j) * MB + lcol];
+            for(int j = 0; j < TN; ++j) fb[j] = Bs[buf][kr * pb + (wn * TN + j) * 16 + lcol];
             R sa[CPLX ? TM : 1], sb[CPLX ? TN : 1];
             if constexpr(CPLX) {
 #pragma unroll
@@ -381,8 +353,9 @@ __device__ __forceinline__ void gemm_body(const GemmDesc &g, int bx, int split, 
         __syncthreads();
     }
 
-    // Epilogue. Accumulator layout: lane l, register v holds column l % MB of row MF::row(l / MB, v). Without
-    // swap rows run along M; with swap they run along N.
+    // Epilogue. 16x16 accumulator layout (measured on gfx950): lane l, register v holds column l%16 of
+    // row 4*(l/16)+v for v_mfma_f32_16x16x4_f32 and of row (l/16)+4*v for v_mfma_f64_16x16x4_f64.
+    // Without swap rows run along M; with swap they run along N.
     // Split-K partials go to g.part as plain column-major M x N slabs, one per (batch, split).
     E *C = static_cast<E *>(g.C) + b * g.c_batch;
     E *P = g.ksplit > 1 ? static_cast<E *>(g.part) + (static_cast<int64_t>(b) * g.ksplit + split) * g.M * g.N : nullptr;
@@ -391,15 +364,15 @@ __device__ __forceinline__ void gemm_body(const GemmDesc &g, int bx, int split, 
 #pragma unroll
         for(int j = 0; j < TN; ++j)
 #pragma unroll
-            for(int v = 0; v < MF::NREG; ++v) {
-                const int row = MF::row(lrow, v);
+            for(int v = 0; v < 4; ++v) {
+                const int row = std::is_same_v<R, double> ? (lrow + 4 * v) : (4 * lrow + v);
                 int       gm, gn;
                 if constexpr(!SWAP) {
-                    gm = m0 + (wm * TM + i) * MB + row;
-                    gn = n0 + (wn * TN + j) * MB + lcol;
+                    gm = m0 + (wm * TM + i) * 16 + row;
+                    gn = n0 + (wn * TN + j) * 16 + lcol;
                 } else {
-                    gm = m0 + (wm * TM + i) * MB + lcol;
-                    gn = n0 + (wn * TN + j) * MB + row;
+                    gm = m0 + (wm * TM + i) * 16 + lcol;
+                    gn = n0 + (wn * TN + j) * 16 + row;
                 }
                 if(gm < g.M && gn < g.N) {
                     E val;
@@ -427,9 +400,9 @@ __device__ __forceinline__ void gemm_body(const GemmDesc &g, int bx, int split, 
             }
 }
 
-template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN, bool SWAP, bool ALIGNED, int MB = 16>
+template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN, bool SWAP, bool ALIGNED>
 __global__ __launch_bounds__(64 * WM *WN) void gemm_kernel(const GemmDesc g) {
-    gemm_body<R, CPLX, BM, BN, BK, WM, WN, SWAP, ALIGNED, MB>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+    gemm_body<R, CPLX, BM, BN, BK, WM, WN, SWAP, ALIGNED>(g, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Grouped GEMM: one launch for several independent GEMMs of one tile configuration. Workgroup x belongs to
@@ -547,8 +520,6 @@ static CfgShape cfg_shape(TileCfg c) {
         case TileCfg::T16x128: return {16, 128};
         case TileCfg::T128x64: return {128, 64};
         case TileCfg::T32x32: return {32, 32};
-        case TileCfg::T64x64W: return {64, 64};
-        case TileCfg::T128x64W: return {128, 64};
         default: return {64, 64};
     }
 }
@@ -610,7 +581,7 @@ static bool small_tile_offsets(const Idx2 &r, int rt, const Idx2 &k, int bk, int
            k.s1 >= 0;
 }
 
-template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN, int MB = 16>
+template <typename R, bool CPLX, int BM, int BN, int BK, int WM, int WN>
 static void launch_cfg(const GemmDesc &g, bool swap, hipStream_t s) {
     const int  tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
     const dim3 grid(tiles, g.ksplit, g.batch);
@@ -619,11 +590,11 @@ static void launch_cfg(const GemmDesc &g, bool swap, hipStream_t s) {
                          (g.kchunk % BK) == 0 && small_tile_offsets(g.am, BM, g.ak, BK, static_cast<int>(sizeof(elem_t<R, CPLX>))) &&
                          small_tile_offsets(g.bn, BN, g.bk, BK, static_cast<int>(sizeof(elem_t<R, CPLX>)));
     if(aligned) {
-        if(swap) hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, true, true, MB>), grid, block, 0, s, g);
-        else hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, false, true, MB>), grid, block, 0, s, g);
+        if(swap) hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, true, true>), grid, block, 0, s, g);
+        else hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, false, true>), grid, block, 0, s, g);
     } else {
-        if(swap) hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, true, false, MB>), grid, block, 0, s, g);
-        else hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, false, false, MB>), grid, block, 0, s, g);
+        if(swap) hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, true, false>), grid, block, 0, s, g);
+        else hipLaunchKernelGGL((gemm_kernel<R, CPLX, BM, BN, BK, WM, WN, false, false>), grid, block, 0, s, g);
     }
     if(g.ev_after_main) DMRG_HIP_CHECK(hipEventRecord(static_cast<hipEvent_t>(g.ev_after_main), s));
     if(g.ksplit > 1) {
@@ -641,10 +612,7 @@ static void launch_typed(const GemmDesc &g, TileCfg cfg, bool swap, hipStream_t 
     constexpr int E  = static_cast<int>(sizeof(elem_t<R, CPLX>));
     constexpr int BK = E == 4 ? 32 : 16;
     constexpr int BH = E == 16 ? 8 : BK;
-    constexpr int MW = std::is_same_v<R, float> ? 32 : 16; // the 32x32x2 MFMA exists for FP32 only
     switch(cfg) {
-        case TileCfg::T64x64W: launch_cfg<R, CPLX, 64, 64, BK, 2, 2, MW>(g, swap, s); break;
-        case TileCfg::T128x64W: launch_cfg<R, CPLX, 128, 64, BH, 2, 2, MW>(g, swap, s); break;
         case TileCfg::T64x64: launch_cfg<R, CPLX, 64, 64, BK, 2, 2>(g, swap, s); break;
         case TileCfg::T32x64: launch_cfg<R, CPLX, 32, 64, BK, 2, 2>(g, swap, s); break;
         case TileCfg::T64x32: launch_cfg<R, CPLX, 64, 32, BK, 2, 2>(g, swap, s); break;
@@ -910,9 +878,7 @@ static int resident_blocks(dmrg_dtype dt, TileCfg t) {
 
 static std::vector<LaunchPlan> candidates(const GemmDesc &g) {
     std::vector<LaunchPlan> c;
-    const bool           single = g.dtype == DMRG_FP32 || g.dtype == DMRG_CX32; // 32x32x2 MFMA tiles
-    std::vector<TileCfg> cfgs   = {TileCfg::T64x64, TileCfg::T32x64, TileCfg::T64x32, TileCfg::T32x32, TileCfg::T128x64, TileCfg::T16x128};
-    if(single) cfgs.insert(cfgs.end(), {TileCfg::T64x64W, TileCfg::T128x64W});
+    const TileCfg cfgs[] = {TileCfg::T64x64, TileCfg::T32x64, TileCfg::T64x32, TileCfg::T32x32, TileCfg::T128x64, TileCfg::T16x128};
     if(g.K <= SMALLK_MAX_K && g.M <= SMALLK_MAX_M) c.push_back({TileCfg::SMALLK, 1});
     // Hybrid plans: whole waves of tiles without K slicing, the remaining tile columns sliced over K.
     if(g.part && g.batch == 1 && g.K >= 128) {
@@ -933,7 +899,7 @@ static std::vector<LaunchPlan> candidates(const GemmDesc &g) {
     }
     for(TileCfg t : cfgs) {
         if(t == TileCfg::T16x128 && g.M > 32) continue;
-        if((t == TileCfg::T128x64 || t == TileCfg::T128x64W) && g.M < 128) continue;
+        if(t == TileCfg::T128x64 && g.M < 128) continue;
         for(int ks : {1, 2, 4, 8}) {
             if(ks > 1 && (g.K < 256 * ks || !g.part || g.part_capacity < static_cast<int64_t>(g.M) * g.N * g.batch * ks)) continue;
             c.push_back({t, ks});
