@@ -86,43 +86,47 @@ def host_info():
     return {"nproc": os.cpu_count(), "cpu_model": model, "blas": blas, "blas_threads": threads}
 
 
-def cpu_baseline(chi=512, budget_s=12.0, max_reps=200):
-    """Time the oracle (numpy restatement of the reference contraction, its GEMMs in the host BLAS) on the host
-    cores: all BLAS threads, then one thread, each on a bounded sample of about budget_s seconds."""
-    import numpy as np
-
+def cpu_baseline(chi=512, budget_s=10.0, max_reps=200):
+    """Time the oracle's C restatement of the reference contraction (oracle/heff_c.c: the same three
+    contractions, packed AVX2 GEMMs, OpenMP) on the host cores: all cores, then one thread; then the numpy
+    restatement (its GEMMs in the host BLAS) for comparison. Each sample is bounded to about budget_s seconds."""
     from dmrg_amd.workloads import heff_flops, random_heff_problem
     from oracle import heff as O  # cpu_baseline leg: the oracle is the thing timed here, as the CPU port
+    from oracle import heff_c as OC
 
     host = host_info()
     th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=1)
     flops = heff_flops(2, chi, chi, 5, 5, True)
 
-    def timed(budget):
-        O.matrix_vector_product(th, W, L, R)  # warm
+    def timed(fn, budget):
+        fn()  # warm (and, for the C port, its workspace)
         reps, t0 = 0, time.perf_counter()
         while reps < max_reps and (time.perf_counter() - t0) < budget:
-            O.matrix_vector_product(th, W, L, R)
+            fn()
             reps += 1
         dt = time.perf_counter() - t0
         return reps, dt, flops * reps / dt / 1e12
 
-    reps, dt, tf = timed(budget_s)
-    one = None
-    try:
-        from threadpoolctl import threadpool_limits
+    def numpy_port(budget):
+        r, dt, tf = timed(lambda: O.matrix_vector_product(th, W, L, R), budget)
+        return {"value": round(tf, 5), "unit": "TFLOP/s", "cores": host["blas_threads"] or os.cpu_count(),
+                "sample": f"{r} matvecs by the numpy oracle ({host['blas']}) in {dt:.1f} s"}
 
-        with threadpool_limits(limits=1, user_api="blas"):
-            r1, d1, t1 = timed(budget_s)
-        one = {"value": round(t1, 5), "unit": "TFLOP/s", "cores": 1,
-               "sample": f"{r1} matvecs in {d1:.1f} s with one BLAS thread"}
-    except Exception as e:  # threadpoolctl absent: no single-thread figure
-        one = {"error": str(e)}
-    cores = host["blas_threads"] or os.cpu_count()
-    return {"value": round(tf, 5), "unit": "TFLOP/s", "cores": cores, "kind": "port",
-            "sample": f"{reps} matvecs of the chi={chi} cx64 workload by the numpy oracle ({host['blas']}, "
-                      f"{cores} threads) in {dt:.1f} s",
-            "host": host, "single_thread": one}
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or os.cpu_count() or 1
+    if not OC.host_supported():  # no AVX2/FMA: the numpy restatement is the baseline
+        out = numpy_port(budget_s)
+        out.update(kind="port", host=host, note="host CPU lacks AVX2/FMA: C restatement not run")
+        return out
+    reps, dt, tf = timed(lambda: OC.matrix_vector_product(th, W, L, R, threads=threads), budget_s)
+    r1, d1, t1 = timed(lambda: OC.matrix_vector_product(th, W, L, R, threads=1), 0.8 * budget_s)
+    out = {"value": round(tf, 5), "unit": "TFLOP/s", "cores": threads, "kind": "port",
+           "sample": f"{reps} matvecs of the chi={chi} cx64 workload by the oracle's C restatement "
+                     f"(oracle/heff_c.c, OpenMP, {threads} threads) in {dt:.1f} s",
+           "host": host,
+           "single_thread": {"value": round(t1, 5), "unit": "TFLOP/s", "cores": 1,
+                             "sample": f"{r1} matvecs in {d1:.1f} s on one thread"}}
+    out["numpy_blas"] = numpy_port(0.8 * budget_s)
+    return out
 
 
 def load_traffic(name: str):
