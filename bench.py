@@ -126,6 +126,21 @@ def cpu_baseline(chi=512, budget_s=10.0, max_reps=200):
            "single_thread": {"value": round(t1, 5), "unit": "TFLOP/s", "cores": 1,
                              "sample": f"{r1} matvecs in {d1:.1f} s on one thread"}}
     out["numpy_blas"] = numpy_port(0.8 * budget_s)
+    # The H^2 matvec shape (D^2 = 25 squared MPO and env blocks, seed 2): the C restatement on all cores.
+    from dmrg_amd.workloads import squared_problem
+
+    th2, W1, L1, R1 = random_heff_problem(2, chi, chi, 5, 5, seed=2)
+    W2, L2, R2 = squared_problem(th2, W1, L1, R1)
+    del W1, L1, R1
+    f2 = heff_flops(2, chi, chi, 25, 25, True)
+    OC.matrix_vector_product(th2, W2, L2, R2, threads=threads)  # warm
+    r2, t0 = 0, time.perf_counter()
+    while r2 < max_reps and (time.perf_counter() - t0) < 0.5 * budget_s:
+        OC.matrix_vector_product(th2, W2, L2, R2, threads=threads)
+        r2 += 1
+    d2 = time.perf_counter() - t0
+    out["h2_matvec"] = {"value": round(f2 * r2 / d2 / 1e12, 5), "unit": "TFLOP/s", "cores": threads,
+                        "sample": f"{r2} H^2 matvecs (chi={chi}, D^2=25) by the C restatement in {d2:.1f} s"}
     return out
 
 
