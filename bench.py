@@ -63,6 +63,41 @@ def gpu_matvec_bench(chi, steps, warmup, fp32=False, legs_nrep=50):
             "legs_tflops": [8.0 * m / (t * 1e-3) / 1e12 for m, t in zip(macs, kernel_ms)], "flops": flops}
 
 
+def multivector_rates(chis=(128, 256), nvecs=(4, 16), steps=100):
+    """Matvec rate at small chi when one call applies H_eff to a block of nvec vectors (MultAx's blockSize,
+    matvec_mpos.cpp:976-984; the library runs the block as one batched GEMM per leg instead of a loop)."""
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd.workloads import heff_flops, random_heff_problem
+
+    out = {}
+    for chi in chis:
+        th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=1)
+        op = G.MatVecMPOS([W], L, R, dtype=np.complex128)
+        n = op.rows()
+        rng = np.random.default_rng(chi)
+        for nv in nvecs:
+            X = (rng.standard_normal((n, nv)) + 1j * rng.standard_normal((n, nv))).astype(np.complex128)
+            dx = G.DeviceArray.from_numpy(np.asfortranarray(X))
+            dy = G.DeviceArray((n, nv), np.complex128)
+            for _ in range(5):
+                op.MultAx_device(dx, dy, nvec=nv)
+            G._lib.call("dmrg_synchronize")
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                op.MultAx_device(dx, dy, nvec=nv)
+            G._lib.call("dmrg_synchronize")
+            dt = (time.perf_counter() - t0) / steps
+            tf = heff_flops(2, chi, chi, 5, 5, True) * nv / dt / 1e12
+            out[f"chi{chi}_nvec{nv}"] = {"ms_per_call": round(dt * 1e3, 4), "tflops": round(tf, 3),
+                                         "frac_of_fp64_peak": round(tf / PEAK_FP64_MFMA, 4)}
+            dx.close()
+            dy.close()
+        op.close()
+    return out
+
+
 def host_info():
     """nproc, CPU model and the BLAS numpy runs on (with its thread count)."""
     model = "unknown"
@@ -1013,6 +1048,7 @@ def main():
         "fp32_mfma": {"tflops": round(r32["tflops"], 4), "ms_per_step": round(r32["ms_per_step"], 5),
                       "frac_of_fp32_peak": round(r32["tflops"] / PEAK_FP32_MFMA, 4)},
         "chi_sweep_tflops": sweep,
+        "chi_sweep_multivector": multivector_rates(),
         "host_buffer_matvec": host_buffer_matvec(args.chi),
     }
     if not args.quick:

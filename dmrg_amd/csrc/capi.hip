@@ -877,7 +877,7 @@ dmrg_status dmrg_gemm_hybrid_override(int cfg, int nsplit, int ksplit2) {
 }
 dmrg_status dmrg_gemm_plan_override(int cfg, int ksplit) {
     return guard([&] {
-        DMRG_REQUIRE(cfg <= static_cast<int>(TileCfg::SMALLK) && ksplit >= 0 && ksplit <= 8, DMRG_ERR_ARG, "bad plan override");
+        DMRG_REQUIRE(cfg <= static_cast<int>(TileCfg::T32x32K) && ksplit >= 0 && ksplit <= 8, DMRG_ERR_ARG, "bad plan override");
         set_plan_override(cfg, ksplit);
     });
 }

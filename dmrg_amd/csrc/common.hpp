@@ -101,7 +101,9 @@ struct GemmDesc {
 // Tile configuration id (see gemm.hip for the table).
 // SMALLK: a column-streaming kernel for GEMMs whose K (and M) are tiny, such as the MPO leg of the matvec
 // (K = d * D); it is HBM-bound and skips the tile machinery. GEMMs outside its limits fall back to T64x64.
-enum class TileCfg : int { T64x64 = 0, T32x64 = 1, T64x32 = 2, T16x128 = 3, T128x64 = 4, T32x32 = 5, SMALLK = 6, AUTO = -1 };
+// T32x32K: 32x32 tiles with one deep K stage per step (128 elements of 4 bytes, 64 of 8 or 16 bytes) for GEMMs too
+// small to fill the GPU, where a K step's MFMA work is shorter than a global load's latency.
+enum class TileCfg : int { T64x64 = 0, T32x64 = 1, T64x32 = 2, T16x128 = 3, T128x64 = 4, T32x32 = 5, SMALLK = 6, T32x32K = 7, AUTO = -1 };
 constexpr int SMALLK_MAX_K = 32;
 constexpr int SMALLK_MAX_M = 64;
 

@@ -520,6 +520,7 @@ static CfgShape cfg_shape(TileCfg c) {
         case TileCfg::T16x128: return {16, 128};
         case TileCfg::T128x64: return {128, 64};
         case TileCfg::T32x32: return {32, 32};
+        case TileCfg::T32x32K: return {32, 32};
         default: return {64, 64};
     }
 }
@@ -612,6 +613,7 @@ static void launch_typed(const GemmDesc &g, TileCfg cfg, bool swap, hipStream_t 
     constexpr int E  = static_cast<int>(sizeof(elem_t<R, CPLX>));
     constexpr int BK = E == 4 ? 32 : 16;
     constexpr int BH = E == 16 ? 8 : BK;
+    constexpr int BD = E == 4 ? 128 : 64; // deep K stage of T32x32K (LDS: at most 132 KiB of the 160 KiB)
     switch(cfg) {
         case TileCfg::T64x64: launch_cfg<R, CPLX, 64, 64, BK, 2, 2>(g, swap, s); break;
         case TileCfg::T32x64: launch_cfg<R, CPLX, 32, 64, BK, 2, 2>(g, swap, s); break;
@@ -619,6 +621,7 @@ static void launch_typed(const GemmDesc &g, TileCfg cfg, bool swap, hipStream_t 
         case TileCfg::T16x128: launch_cfg<R, CPLX, 16, 128, BH, 1, 4>(g, swap, s); break;
         case TileCfg::T128x64: launch_cfg<R, CPLX, 128, 64, BH, 2, 2>(g, swap, s); break;
         case TileCfg::T32x32: launch_cfg<R, CPLX, 32, 32, BK, 2, 2>(g, swap, s); break;
+        case TileCfg::T32x32K: launch_cfg<R, CPLX, 32, 32, BD, 2, 2>(g, swap, s); break;
         case TileCfg::SMALLK: {
             if(g.K > SMALLK_MAX_K || g.M > SMALLK_MAX_M) { // outside the kernel's limits
                 launch_cfg<R, CPLX, 64, 64, BK, 2, 2>(g, swap, s);
@@ -658,6 +661,9 @@ static std::string tune_key(const GemmDesc &g) {
 
 static void launch_plan_single(GemmDesc g, const LaunchPlan &p0, hipStream_t stream);
 
+// K depth of one LDS stage rounded up to 32 (every configuration's BK divides 32, except T32x32K's deep stage).
+static int deep_bk(TileCfg c, dmrg_dtype t) { return c == TileCfg::T32x32K ? (dtype_size(t) == 4 ? 128 : 64) : 32; }
+
 // Columns [n0, n0 + nn) of g as a GEMM of their own (valid when the B and C column maps are separable at n0).
 static GemmDesc column_slice(const GemmDesc &g, int n0, int nn) {
     GemmDesc h     = g;
@@ -688,9 +694,10 @@ static void launch_plan_single(GemmDesc g, const LaunchPlan &p0, hipStream_t str
         if(!g.part || g.part_capacity < need) p.ksplit = 1;
     }
     g.ksplit = 1;
-    g.kchunk = (g.K + 31) / 32 * 32; // K extent per slice, a multiple of every BK
+    const int kr = deep_bk(p.cfg, g.dtype); // K extent per slice: a multiple of the configuration's BK
+    g.kchunk = (g.K + kr - 1) / kr * kr;
     if(p.ksplit > 1) {
-        const int kc = (((g.K + p.ksplit - 1) / p.ksplit) + 31) / 32 * 32;
+        const int kc = (((g.K + p.ksplit - 1) / p.ksplit) + kr - 1) / kr * kr;
         g.kchunk     = kc;
         g.ksplit     = (g.K + kc - 1) / kc;
     }
@@ -903,6 +910,15 @@ static std::vector<LaunchPlan> candidates(const GemmDesc &g) {
         for(int ks : {1, 2, 4, 8}) {
             if(ks > 1 && (g.K < 256 * ks || !g.part || g.part_capacity < static_cast<int64_t>(g.M) * g.N * g.batch * ks)) continue;
             c.push_back({t, ks});
+        }
+    }
+    // Deep-K 32x32 tiles for grids that fill at most two workgroups per CU; K slices down to one stage each.
+    const int64_t tiles32 = static_cast<int64_t>((g.M + 31) / 32) * ((g.N + 31) / 32) * g.batch;
+    if(tiles32 <= 2 * device_cu_count()) {
+        const int bd = deep_bk(TileCfg::T32x32K, g.dtype);
+        for(int ks : {1, 2, 4, 8}) {
+            if(ks > 1 && (g.K < bd * ks || !g.part || g.part_capacity < static_cast<int64_t>(g.M) * g.N * g.batch * ks)) continue;
+            c.push_back({TileCfg::T32x32K, ks});
         }
     }
     return c;

@@ -486,8 +486,9 @@ dmrg_status dmrg_autotune_enable(int enable);
 dmrg_status dmrg_autotune_reset(void);
 int         dmrg_autotune_entries(void);
 /* Testing hook: force every GEMM to tile configuration cfg (0..5: 64x64, 32x64, 64x32, 16x128, 128x64, 32x32;
- * 6: the column-streaming kernel for tiny K, which falls back to 64x64 beyond K = 32 or M = 64) with ksplit K
- * slices; cfg < 0 restores the tuned/heuristic choice. */
+ * 6: the column-streaming kernel for tiny K, which falls back to 64x64 beyond K = 32 or M = 64; 7: 32x32 with a
+ * deep K stage of 128 (4-byte elements) or 64 elements) with ksplit K slices; cfg < 0 restores the
+ * tuned/heuristic choice. */
 dmrg_status dmrg_gemm_plan_override(int cfg, int ksplit);
 /* Testing hook: force every GEMM to a hybrid plan of configuration cfg (0..2): output columns [0, nsplit) in one
  * launch without K slicing, the rest in a second launch with ksplit2 K slices (GEMMs without split-K scratch,

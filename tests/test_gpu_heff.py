@@ -213,7 +213,7 @@ def plan_override():
     _lib.call("dmrg_gemm_plan_override", -1, 0)
 
 
-@pytest.mark.parametrize("cfg", range(7), ids=["64x64", "32x64", "64x32", "16x128", "128x64", "32x32", "smallk"])
+@pytest.mark.parametrize("cfg", range(8), ids=["64x64", "32x64", "64x32", "16x128", "128x64", "32x32", "smallk", "32x32k"])
 @pytest.mark.parametrize("ks", [1, 3, 8])
 def test_every_tile_config_matvec_and_env(cfg, ks, plan_override):
     """Each tile configuration, with and without K slicing, on aligned (chi=64) and ragged shapes."""
