@@ -66,7 +66,8 @@ static void tile4x4(long kc, const double *restrict ap, const double *restrict b
 /* C[m,n] = sum_k A[m,k] B[k,n] (complex), every operand read and written through index maps. Work is split
  * over (MC x NC) output blocks; each block packs its own panels and owns its part of C. */
 static void zgemm(long M, long N, long K, const double *A, imap am, imap ak, const double *B, imap bk, imap bn,
-                      double *C, imap cm, imap cn) {
+                  int conjb, double *C, imap cm, imap cn) {
+    const double sb = conjb ? -1.0 : 1.0; /* B's imaginary parts are read times sb (conj(B) when conjb) */
     const long mb = (M + MC - 1) / MC, nb = (N + NC - 1) / NC;
 #pragma omp parallel
     {
@@ -111,7 +112,7 @@ static void zgemm(long M, long N, long K, const double *A, imap am, imap ak, con
                             if(j < nc) {
                                 const double *src = B + 2 * (okb[k] + on[j]);
                                 dst[c] = src[0];
-                                dst[NR + c] = src[1];
+                                dst[NR + c] = sb * src[1];
                             } else {
                                 dst[c] = 0.0;
                                 dst[NR + c] = 0.0;
@@ -212,21 +213,91 @@ int oracle_heff_matvec(const double *theta, const double *mpo, const double *env
     if(order) {
         /* t1[s, b, a', w] = sum_a theta[s,a,b] L[a,a',w]   (mps.contract(envL, idx({1},{0}))) */
         zgemm(d * chiR, chiL * DL, chiL, theta, imap2(d, 1, d * chiL), imap1(chiL, d), envL, imap1(chiL, 1),
-              imap1(chiL * DL, chiL), t1, imap1(d * chiR, 1), imap1(chiL * DL, d * chiR));
+              imap1(chiL * DL, chiL), 0, t1, imap1(d * chiR, 1), imap1(chiL * DL, d * chiR));
         /* t2[b, s', a', w'] = sum_{w,s} t1[s, b, a', w] W[w,w',s,s']   (.contract(mpo, idx({3,0},{0,2}))) */
         mpo_leg(t1, mpo, t2, d, chiR, chiL, DL, DR, DL, DR, 1);
         /* res[s',a',b'] = sum_{b,w'} t2[b, s', a', w'] R[b,b',w']   (.contract(envR, idx({0,2},{0,2})), shuffled) */
         zgemm(d * chiL, chiR, chiR * DR, t2, imap1(d * chiL, chiR), imap2(chiR, 1, chiR * d * chiL), envR,
-              imap2(chiR, 1, chiR * chiR), imap1(chiR, chiR), res, imap1(d * chiL, 1), imap1(chiR, d * chiL));
+              imap2(chiR, 1, chiR * chiR), imap1(chiR, chiR), 0, res, imap1(d * chiL, 1), imap1(chiR, d * chiL));
     } else {
         /* t1[s, a, b', w'] = sum_b theta[s,a,b] R[b,b',w']   (mps.contract(envR, idx({2},{0}))) */
         zgemm(d * chiL, chiR * DR, chiR, theta, imap1(d * chiL, 1), imap1(chiR, d * chiL), envR, imap1(chiR, 1),
-              imap1(chiR * DR, chiR), t1, imap1(d * chiL, 1), imap1(chiR * DR, d * chiL));
+              imap1(chiR * DR, chiR), 0, t1, imap1(d * chiL, 1), imap1(chiR * DR, d * chiL));
         /* t2[a, s', b', w] = sum_{w',s} t1[s, a, b', w'] W[w,w',s,s']   (.contract(mpo, idx({3,0},{1,2}))) */
         mpo_leg(t1, mpo, t2, d, chiL, chiR, DR, DL, DL, DR, 0);
         /* res[s',a',b'] = sum_{a,w} t2[a, s', b', w] L[a,a',w]   (.contract(envL, idx({0,2},{0,2})), shuffled) */
         zgemm(d * chiR, chiL, chiL * DL, t2, imap1(d * chiR, chiL), imap2(chiL, 1, chiL * d * chiR), envL,
-              imap2(chiL, 1, chiL * chiL), imap1(chiL, chiL), res, imap2(d, 1, d * chiL), imap1(chiL, d));
+              imap2(chiL, 1, chiL * chiL), imap1(chiL, chiL), 0, res, imap2(d, 1, d * chiL), imap1(chiL, d));
+    }
+    pthread_mutex_unlock(&ws_lock);
+    return 0;
+}
+
+/* The MPO contraction of the environment update: out[x + nx*(o + no*(wo + Dout*s))] = sum_{wi, s'} W(., ., s, s')
+ * in[x + nx*(wi + Din*(s' + d*o))], with W's contracted bond the left one (wl = 1, left block) or the right one
+ * (wl = 0, right block); W's third index s stays free and its fourth s' meets the conjugated site tensor. */
+static void env_mpo_leg(const double *in, const double *W, double *out, long d, long nx, long no, long Din, long Dout,
+                        long DL, long DR, int wl) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for(long s = 0; s < d; ++s)
+        for(long wo = 0; wo < Dout; ++wo)
+            for(long o = 0; o < no; ++o) {
+                double *y = out + 2 * nx * (o + no * (wo + Dout * s));
+                memset(y, 0, sizeof(double) * 2 * nx);
+                for(long sp = 0; sp < d; ++sp)
+                    for(long wi = 0; wi < Din; ++wi) {
+                        const long w = wl ? wi + DL * (wo + DR * (s + d * sp)) : wo + DL * (wi + DR * (s + d * sp));
+                        const double cr = W[2 * w], ci = W[2 * w + 1];
+                        if(cr == 0.0 && ci == 0.0) continue;
+                        const double *v = in + 2 * nx * (wi + Din * (sp + d * o));
+                        for(long x = 0; x < nx; ++x) {
+                            const double vr = v[2 * x], vi = v[2 * x + 1];
+                            y[2 * x] += cr * vr - ci * vi;
+                            y[2 * x + 1] += cr * vi + ci * vr;
+                        }
+                    }
+            }
+}
+
+/* New environment block of EnvBase::build_block (EnvBase.cpp:67-151) in its contraction order.
+ * side 0 (left): out[b, b', w'] (chiR, chiR, DR) from other = L (chiL, chiL, DL):
+ *   t1[a, w, s', b'] = sum_a' L[a, a', w] conj(mps[s', a', b'])        (.contract(mps.conjugate(), idx({1},{1})))
+ *   t2[a, b', w', s] = sum_{w, s'} t1 W[w, w', s, s']                  (.contract(mpo, idx({1,2},{0,3})))
+ *   out[b, b', w']   = sum_{s, a} mps[s, a, b] t2[a, b', w', s]        (mps.contract(., idx({0,1},{3,0})))
+ * side 1 (right): out[a, a', w] (chiL, chiL, DL) from other = R (chiR, chiR, DR), symmetric over b.
+ * The intermediates keep their first index contiguous (Eigen's output order). Returns 0, 1 on bad sizes, 2 on
+ * allocation failure. */
+int oracle_env_build_block(int side, const double *other, const double *mps, const double *mpo, double *out, long d,
+                           long chiL, long chiR, long DL, long DR, int nthreads) {
+    if(d <= 0 || chiL <= 0 || chiR <= 0 || DL <= 0 || DR <= 0 || (side != 0 && side != 1)) return 1;
+    if(nthreads > 0) omp_set_num_threads(nthreads);
+    const long n1 = side == 0 ? chiL * DL * d * chiR : chiR * DR * d * chiL;
+    const long n2 = side == 0 ? chiL * chiR * DR * d : chiR * chiL * DL * d;
+    pthread_mutex_lock(&ws_lock);
+    double *t1 = workspace((size_t)(2 * (n1 + n2)));
+    if(!t1) {
+        pthread_mutex_unlock(&ws_lock);
+        return 2;
+    }
+    double *t2 = t1 + 2 * n1;
+    if(side == 0) {
+        /* t1[a + chiL*(w + DL*(s' + d*b'))]: M = (a, w), K = a', N = (s', b') */
+        zgemm(chiL * DL, d * chiR, chiL, other, imap2(chiL, 1, chiL * chiL), imap1(chiL, chiL), mps, imap1(chiL, d),
+              imap2(d, 1, d * chiL), 1, t1, imap1(chiL * DL, 1), imap1(d * chiR, chiL * DL));
+        /* t2[a + chiL*(b' + chiR*(w' + DR*s))] */
+        env_mpo_leg(t1, mpo, t2, d, chiL, chiR, DL, DR, DL, DR, 1);
+        /* out[b + chiR*(b' + chiR*w')]: M = b, K = (s, a), N = (b', w') */
+        zgemm(chiR, chiR * DR, d * chiL, mps, imap1(chiR, d * chiL), imap1(d * chiL, 1), t2, imap2(d, chiL * chiR * DR, 1),
+              imap1(chiR * DR, chiL), 0, out, imap1(chiR, 1), imap1(chiR * DR, chiR));
+    } else {
+        /* t1[b + chiR*(w' + DR*(s' + d*a'))]: M = (b, w'), K = b', N = (s', a') */
+        zgemm(chiR * DR, d * chiL, chiR, other, imap2(chiR, 1, chiR * chiR), imap1(chiR, chiR), mps, imap1(chiR, d * chiL),
+              imap1(d * chiL, 1), 1, t1, imap1(chiR * DR, 1), imap1(d * chiL, chiR * DR));
+        /* t2[b + chiR*(a' + chiL*(w + DL*s))] */
+        env_mpo_leg(t1, mpo, t2, d, chiR, chiL, DR, DL, DL, DR, 0);
+        /* out[a + chiL*(a' + chiL*w)]: M = a, K = (s, b), N = (a', w) */
+        zgemm(chiL, chiL * DL, d * chiR, mps, imap1(chiL, d), imap2(d, 1, d * chiL), t2, imap2(d, chiR * chiL * DL, 1),
+              imap1(chiL * DL, chiR), 0, out, imap1(chiL, 1), imap1(chiL * DL, chiL));
     }
     pthread_mutex_unlock(&ws_lock);
     return 0;

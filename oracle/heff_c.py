@@ -1,5 +1,5 @@
 """Oracle (test infrastructure only): ctypes binding of the C restatement of the H_eff matvec
-(heff_c.c, following contraction_tblis.cpp:213-292).
+(heff_c.c, following contraction_tblis.cpp:213-292) and of the environment update (EnvBase.cpp:67-151).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module.
 """
@@ -35,6 +35,8 @@ def load():
         p = ctypes.c_void_p
         lib.oracle_heff_matvec.argtypes = [p, p, p, p, p] + [ctypes.c_long] * 5 + [ctypes.c_int, ctypes.c_int]
         lib.oracle_heff_matvec.restype = ctypes.c_int
+        lib.oracle_env_build_block.argtypes = [ctypes.c_int] + [p] * 4 + [ctypes.c_long] * 5 + [ctypes.c_int]
+        lib.oracle_env_build_block.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -60,4 +62,28 @@ def matrix_vector_product(mps, mpo, envL, envR, order: str | None = None, thread
                                 d, chiL, chiR, DL, DR, o, int(threads))
     if rc != 0:
         raise RuntimeError(f"oracle_heff_matvec failed with code {rc}")
+    return out
+
+
+def env_build_block(side: str, other, mps, mpo, threads: int = 0):
+    """EnvBase::build_block in C: the new left (side 'L', from the block L on the left of the site) or right
+    ('R') environment block, complex128, Eigen column-major."""
+    lib = load()
+    th, W, B = (_fortran_c128(a) for a in (mps, mpo, other))
+    d, chiL, chiR = th.shape
+    DL, DR = W.shape[0], W.shape[1]
+    if side == "L":
+        if d != W.shape[2] or chiL != B.shape[0] or DL != B.shape[2] or B.shape[0] != B.shape[1]:
+            raise ValueError("dimension mismatch")
+        out = np.zeros((chiR, chiR, DR), dtype=np.complex128, order="F")
+    elif side == "R":
+        if d != W.shape[2] or chiR != B.shape[0] or DR != B.shape[2] or B.shape[0] != B.shape[1]:
+            raise ValueError("dimension mismatch")
+        out = np.zeros((chiL, chiL, DL), dtype=np.complex128, order="F")
+    else:
+        raise ValueError("side must be L or R")
+    rc = lib.oracle_env_build_block(0 if side == "L" else 1, B.ctypes.data, th.ctypes.data, W.ctypes.data,
+                                    out.ctypes.data, d, chiL, chiR, DL, DR, int(threads))
+    if rc != 0:
+        raise RuntimeError(f"oracle_env_build_block failed with code {rc}")
     return out

@@ -1,5 +1,6 @@
-"""CPU tests that pin the oracle's C restatement of the matvec (oracle/heff_c.c) against brute force and
-against the numpy restatement (oracle/heff.py), on both contraction branches.
+"""CPU tests that pin the oracle's C restatement of the matvec and of the environment update
+(oracle/heff_c.c) against brute force and against the numpy restatement (oracle/heff.py), on both contraction
+branches and both sides.
 
 The inputs are general (non-Hermitian, complex, ragged sizes) so that a transposed operand or a swapped
 index in one of the three contractions fails the brute-force check.
@@ -85,3 +86,35 @@ def test_c_matvec_rejects_mismatch():
     th, W, L, R = general_problem(rng, 2, 4, 5, 2, 3)
     with pytest.raises(ValueError):
         OC.matrix_vector_product(th, W, R, R)
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("shape", [(2, 3, 4, 2, 3), (4, 5, 2, 3, 1), (1, 1, 1, 1, 1), (3, 6, 7, 2, 4)])
+def test_c_env_literal_sum(side, shape):
+    """Every element of the new block as the explicit sum over (s, s', a, a', w) of EnvBase.cpp:109-112 (L) or
+    over (s, s', b, b', w') of :146-149 (R)."""
+    d, cL, cR, DL, DR = shape
+    rng = np.random.default_rng(sum(shape))
+    th, W, L, R = general_problem(rng, d, cL, cR, DL, DR)
+    if side == "L":
+        # new[b, b2, w2] = sum L[a, a2, w] conj(th[s2, a2, b2]) W[w, w2, s, s2] th[s, a, b]
+        ref = np.einsum("xyw,tyc,wvst,sxb->bcv", L, th.conj(), W, th)
+        out = OC.env_build_block("L", L, th, W)
+    else:
+        # new[a, a2, w] = sum R[b, b2, w2] conj(th[s2, a2, b2]) W[w, w2, s, s2] th[s, a, b]
+        ref = np.einsum("byv,tcy,wvst,sab->acw", R, th.conj(), W, th)
+        out = OC.env_build_block("R", R, th, W)
+    assert out.shape == ref.shape
+    np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("shape", [(2, 130, 129, 5, 5), (2, 67, 200, 3, 4), (4, 96, 80, 8, 8)])
+def test_c_env_matches_numpy_oracle(side, shape):
+    d, cL, cR, DL, DR = shape
+    rng = np.random.default_rng(cL * 3 + cR)
+    th, W, L, R = general_problem(rng, d, cL, cR, DL, DR)
+    other = L if side == "L" else R
+    ref = O.env_build_block(side, other, th, W)
+    out = OC.env_build_block(side, other, th, W, threads=4)
+    assert np.linalg.norm(out - ref) <= 1e-13 * np.linalg.norm(ref)
