@@ -176,6 +176,15 @@ def cpu_baseline(chi=512, budget_s=10.0, max_reps=200):
     d2 = time.perf_counter() - t0
     out["h2_matvec"] = {"value": round(f2 * r2 / d2 / 1e12, 5), "unit": "TFLOP/s", "cores": threads,
                         "sample": f"{r2} H^2 matvecs (chi={chi}, D^2=25) by the C restatement in {d2:.1f} s"}
+    # Left environment update of the same site (D = 5 H blocks), EnvBase::build_block in C on all cores.
+    OC.env_build_block("L", L, th, W, threads=threads)  # warm
+    r3, t0 = 0, time.perf_counter()
+    while r3 < max_reps and (time.perf_counter() - t0) < 0.3 * budget_s:
+        OC.env_build_block("L", L, th, W, threads=threads)
+        r3 += 1
+    d3 = time.perf_counter() - t0
+    out["env_update"] = {"value": round(flops * r3 / d3 / 1e12, 5), "unit": "TFLOP/s", "cores": threads,
+                         "sample": f"{r3} left env updates (chi={chi}, D=5) by the C restatement in {d3:.1f} s"}
     return out
 
 
@@ -187,6 +196,49 @@ def load_traffic(name: str):
         except Exception:
             return None
     return None
+
+
+def section_env_update(chi=512, steps=30):
+    """Environment updates (EnvBase::build_block) of the chi=512 site, left and right, with the H blocks (D=5) and
+    the variance blocks (D^2=25), device-resident; each call writes a preallocated block."""
+    import ctypes
+
+    import numpy as np
+
+    import dmrg_amd as G
+    from dmrg_amd import _dims
+    from dmrg_amd.workloads import heff_flops, random_heff_problem, squared_problem
+
+    th, W, L, R = random_heff_problem(2, chi, chi, 5, 5, seed=2)
+    W2, L2, R2 = squared_problem(th, W, L, R)
+    dth = G.DeviceArray.from_numpy(th)
+    out = {}
+    for tag, (Wx, Lx, Rx) in (("D5", (W, L, R)), ("D25", (W2, L2, R2))):
+        D = Wx.shape[0]
+        dW = G.DeviceArray.from_numpy(Wx)
+        for side, other in (("L", Lx), ("R", Rx)):
+            dother = G.DeviceArray.from_numpy(other)
+            dout = G.DeviceArray((chi, chi, D), np.complex128)
+            args = (dth.code, ctypes.c_char(side.encode()), dout.ptr, dother.ptr, _dims(dother.shape), dth.ptr,
+                    _dims(dth.shape), dW.ptr, _dims(dW.shape), None)
+            for _ in range(3):
+                G._lib.call("dmrg_env_build_block", *args)
+            G._lib.call("dmrg_synchronize")
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                G._lib.call("dmrg_env_build_block", *args)
+            G._lib.call("dmrg_synchronize")
+            dt = (time.perf_counter() - t0) / steps
+            fl = heff_flops(2, chi, chi, D, D, True)
+            out[f"{tag}_{side}"] = {"ms": round(dt * 1e3, 4), "tflops": round(fl / dt / 1e12, 3),
+                                    "frac_of_fp64_peak": round(fl / dt / 1e12 / PEAK_FP64_MFMA, 4)}
+            dother.close()
+            dout.close()
+        dW.close()
+    dth.close()
+    out["workload"] = (f"theta (2,{chi},{chi}) cx64 seed 2, H blocks ({chi},{chi},5) and variance blocks "
+                       f"({chi},{chi},25); FLOP = 8 x (d chi^3 D + chi^2 D^2 d^2 + d chi^3 D) per update")
+    return out
 
 
 def section_h2_matvec(chi=512, steps=30):
@@ -1053,6 +1105,7 @@ def main():
     }
     if not args.quick:
         out["h2_matvec"] = section_h2_matvec(args.chi)
+        out["env_update"] = section_env_update(args.chi)
         out["eigs"] = section_eigs(args.chi)
         out["batched_eigs_chi512"] = section_batched_chi512()
         out["block_vectors"] = section_block_vectors(args.chi)
