@@ -231,7 +231,7 @@ def test_every_tile_config_matvec_and_env(cfg, ks, plan_override):
 
 
 @pytest.mark.parametrize("cfg", [0, 1, 2], ids=["64x64", "32x64", "64x32"])
-@pytest.mark.parametrize("nsplit,ks2", [(64, 2), (128, 4), (192, 8)])
+@pytest.mark.parametrize("nsplit,ks2", [(64, 2), (128, 4), (192, 8), (96, 3)])
 def test_hybrid_plans(cfg, nsplit, ks2):
     """Hybrid plans: columns [0, nsplit) in one launch, the tail sliced over K, on aligned and ragged shapes."""
     _lib.call("dmrg_gemm_hybrid_override", cfg, nsplit, ks2)
