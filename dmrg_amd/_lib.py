@@ -83,6 +83,7 @@ SIGNATURES: dict[str, tuple] = {
     "dmrg_host_heev": (st, [C.c_int, dp, dp, dp]),
     "dmrg_autotune_enable": (st, [C.c_int]),
     "dmrg_autotune_reset": (st, []),
+    "dmrg_autotune_min_macs": (st, [i64]),
     "dmrg_autotune_entries": (C.c_int, []),
     "dmrg_gemm_plan_override": (st, [C.c_int, C.c_int]),
     "dmrg_gemm_hybrid_override": (st, [C.c_int, C.c_int, C.c_int]),

@@ -867,6 +867,12 @@ dmrg_status dmrg_autotune_enable(int enable) {
 dmrg_status dmrg_autotune_reset(void) {
     return guard([&] { clear_tuning(); });
 }
+dmrg_status dmrg_autotune_min_macs(int64_t macs) {
+    return guard([&] {
+        DMRG_REQUIRE(macs >= 0, DMRG_ERR_ARG, "autotune threshold must be >= 0");
+        set_tune_min_macs(macs);
+    });
+}
 int         dmrg_autotune_entries(void) { return tuned_entries(); }
 dmrg_status dmrg_gemm_hybrid_override(int cfg, int nsplit, int ksplit2) {
     return guard([&] {

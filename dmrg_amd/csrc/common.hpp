@@ -160,6 +160,7 @@ void                     grouped_free(std::vector<GroupedGemm> &gg);
 int  device_cu_count();
 // Autotuner controls: on/off, a forced plan for every GEMM (cfg < 0 clears it), cache reset and size.
 void set_autotune(int on);
+void set_tune_min_macs(int64_t macs);
 void set_plan_override(int cfg, int ksplit, int nsplit = 0, int ksplit2 = 1);
 void clear_tuning();
 int  tuned_entries();

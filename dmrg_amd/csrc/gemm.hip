@@ -569,17 +569,18 @@ static LaunchPlan heuristic_plan(const GemmDesc &g) {
     if(g.M <= 16) p.cfg = TileCfg::T16x128;
     else if(g.M >= 48 && g.N >= 48) p.cfg = TileCfg::T64x64;
     else p.cfg = g.M < g.N ? TileCfg::T32x64 : TileCfg::T64x32;
+    // Fewer 64x64 tiles than CUs: 32x32 tiles, four times as many.
+    if(p.cfg == TileCfg::T64x64 && nblk(64, 64) < cus) p.cfg = TileCfg::T32x32;
     const CfgShape sh    = cfg_shape(p.cfg);
     const int64_t  tiles = nblk(sh.BM, sh.BN);
-    // Small grids with a long K: slice K so that about two workgroups per CU are in flight.
-    if(tiles < 2 * cus && g.K >= 512) {
-        int64_t ks = std::min<int64_t>((2 * cus + tiles - 1) / tiles, g.K / 256);
-        ks         = std::min<int64_t>(ks, 8);
+    // Small grids with a long K: slice K (slices of at least 128) so that about two workgroups per CU are in
+    // flight. These are the plans the tuner picks for the chi <= 256 matvec legs.
+    if(tiles < 2 * cus && g.K >= 256 && p.cfg != TileCfg::T16x128) {
+        int64_t ks = std::min<int64_t>({(2 * cus + tiles - 1) / tiles, g.K / 128, 8});
         if(g.part && ks > 1) ks = std::min<int64_t>(ks, g.part_capacity / (static_cast<int64_t>(g.M) * g.N * g.batch));
         else ks = 1;
         p.ksplit = static_cast<int>(std::max<int64_t>(1, ks));
     }
-    if(p.ksplit == 1 && tiles < cus && g.M > 16 && p.cfg == TileCfg::T64x64) p.cfg = TileCfg::T32x32;
     return p;
 }
 
@@ -665,6 +666,9 @@ static int fast_score(const Idx2 &x) { return x.s0 == 1 ? static_cast<int>(std::
 static std::mutex                        g_tune_mu;
 static std::map<std::string, LaunchPlan> g_tuned;
 static int                               g_autotune     = 1;
+// GEMMs below this many (complex or real) multiply-adds, and with fewer than a sixteenth as many bytes of operands,
+// take the size heuristic instead of being tuned.
+static int64_t                           g_tune_min_macs = int64_t(1) << 29;
 static int                               g_override_cfg = -1, g_override_ks = 0;
 static int                               g_override_nsplit = 0, g_override_ks2 = 1;
 
@@ -1013,6 +1017,7 @@ static void append_tune_cache_locked(const std::string &key, const LaunchPlan &p
 }
 
 void set_autotune(int on) { g_autotune = on; }
+void set_tune_min_macs(int64_t macs) { g_tune_min_macs = macs; }
 int  autotune_enabled() { return g_autotune; }
 void set_plan_override(int cfg, int ks, int nsplit, int ks2) {
     g_override_cfg    = cfg;
@@ -1052,7 +1057,14 @@ void gemm(const GemmDesc &gin, hipStream_t stream, TileCfg cfg, int ksplit) {
         // add up) and while the stream is being captured into a graph.
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         DMRG_HIP_CHECK(hipStreamIsCapturing(stream, &cs));
-        const bool can_tune = g_autotune && !g.accumulate && cs == hipStreamCaptureStatusNone && !g_tls_no_tune;
+        // Only large GEMMs are timed (at least g_tune_min_macs multiply-adds, or g_tune_min_macs / 16 bytes of
+        // operands for the memory-bound ones): a sweep or a request stream meets thousands of small shapes once
+        // each, whose tuning (about 180 timed, synchronised launches) costs more than any plan saves.
+        const double macs  = static_cast<double>(g.M) * g.N * std::max(g.K, 1) * g.batch;
+        const double bytes = (static_cast<double>(g.M) * g.K + static_cast<double>(g.K) * g.N + static_cast<double>(g.M) * g.N) *
+                             g.batch * static_cast<double>(dtype_size(g.dtype));
+        const bool   large = macs >= static_cast<double>(g_tune_min_macs) || bytes * 16.0 >= static_cast<double>(g_tune_min_macs);
+        const bool   can_tune = g_autotune && !g.accumulate && cs == hipStreamCaptureStatusNone && !g_tls_no_tune && large;
         const std::string key = tune_key(g);
         bool              found = false;
         {

@@ -484,6 +484,11 @@ dmrg_status dmrg_host_heev(int m, const double *A, double *w, double *Z);
  * signature is seen on a stream that is not being captured, and cached for the process. */
 dmrg_status dmrg_autotune_enable(int enable);
 dmrg_status dmrg_autotune_reset(void);
+/* Only GEMMs of at least macs multiply-adds (M N K batch; default 2^29: the chi = 512 matvec legs) or of at least
+ * macs / 16 bytes of operands (the chi = 512 MPO leg) are tuned; smaller ones take a size heuristic, since a sweep
+ * or a request stream meets thousands of small shapes once each and their tuning costs more than it saves.
+ * 0 tunes every GEMM. */
+dmrg_status dmrg_autotune_min_macs(int64_t macs);
 int         dmrg_autotune_entries(void);
 /* Testing hook: force every GEMM to tile configuration cfg (0..5: 64x64, 32x64, 64x32, 16x128, 128x64, 32x32;
  * 6: the column-streaming kernel for tiny K, which falls back to 64x64 beyond K = 32 or M = 64; 7: 32x32 with a

@@ -251,14 +251,27 @@ def test_hybrid_plans(cfg, nsplit, ks2):
 
 def test_autotuner_caches_plans():
     _lib.call("dmrg_autotune_reset")
+    _lib.call("dmrg_autotune_min_macs", 0)  # tune these small legs too
+    try:
+        th, W, L, R = random_heff_problem(2, 96, 96, 5, 5, seed=9)
+        a = G.matrix_vector_product(th, W, L, R)
+        n = _lib.load().dmrg_autotune_entries()
+        assert n >= 3
+        b = G.matrix_vector_product(th, W, L, R)
+        assert _lib.load().dmrg_autotune_entries() == n
+        np.testing.assert_array_equal(a, b)  # same plan -> bitwise identical
+        assert rel(a, O.matrix_vector_product(th, W, L, R)) <= TOL64
+    finally:
+        _lib.call("dmrg_autotune_min_macs", 1 << 29)
+
+
+def test_small_gemms_are_not_tuned():
+    """Below the threshold (default 2^29 multiply-adds) the size heuristic plans the legs and nothing is tuned."""
+    _lib.call("dmrg_autotune_reset")
     th, W, L, R = random_heff_problem(2, 96, 96, 5, 5, seed=9)
-    a = G.matrix_vector_product(th, W, L, R)
-    n = _lib.load().dmrg_autotune_entries()
-    assert n >= 3
-    b = G.matrix_vector_product(th, W, L, R)
-    assert _lib.load().dmrg_autotune_entries() == n
-    np.testing.assert_array_equal(a, b)  # same plan -> bitwise identical
-    assert rel(a, O.matrix_vector_product(th, W, L, R)) <= TOL64
+    out = G.matrix_vector_product(th, W, L, R)
+    assert _lib.load().dmrg_autotune_entries() == 0
+    assert rel(out, O.matrix_vector_product(th, W, L, R)) <= TOL64
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128])
