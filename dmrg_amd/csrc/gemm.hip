@@ -229,13 +229,14 @@ struct TileLoader {
     }
 
     // Writes the tile into LDS laid out [k][r] with a row pitch of `pitch` elements.
-    __device__ __forceinline__ void store(E *lds, int pitch, const E (&reg)[L], int tid, uint32_t imsign) const {
+    // The registers are updated in place (conjugation), so that each element goes to LDS without a copy.
+    __device__ __forceinline__ void store(E *lds, int pitch, E (&reg)[L], int tid, uint32_t imsign) const {
 #pragma unroll
         for(int e = 0; e < L; ++e)
             if(L * NT == RT * BK || tid + e * NT < RT * BK) {
-                E v = reg[e];
-                if constexpr(!ALIGNED) v = lval[e] ? v : zero_elem<E>();
-                lds[kk[e] * pitch + rr[e]] = conj_by_mask(v, imsign);
+                if constexpr(!ALIGNED) reg[e] = lval[e] ? reg[e] : zero_elem<E>();
+                reg[e]                     = conj_by_mask(reg[e], imsign);
+                lds[kk[e] * pitch + rr[e]] = reg[e];
             }
     }
 };
