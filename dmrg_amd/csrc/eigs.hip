@@ -129,8 +129,7 @@ void Davidson<T>::expand_from(const T *src, T *t, int m, double *dnrm_pre, doubl
     multidot_x<T>(n, m, V.as<T>(), n, src, sm + SB_COL, dnrm_pre, nullptr, scr, s);
     gemv_update_norm<T>(n, m, V.as<T>(), n, sm + SB_COL, -1.0, 1.0, src, t, dnrm_post, nullptr, scr, s);
     if(m > 0) {
-        dgks_flag(dnrm_pre, dnrm_post, 0.5, flag, s);
-        multidot_x<T>(n, m, V.as<T>(), n, t, sm + SB_COL, nullptr, flag, scr, s);
+        multidot_dgks<T>(n, m, V.as<T>(), n, t, sm + SB_COL, dnrm_pre, dnrm_post, 0.5, flag, scr, s);
         gemv_update_norm<T>(n, m, V.as<T>(), n, sm + SB_COL, -1.0, 1.0, t, t, dnrm_post, flag, scr, s);
     }
     scale_by_inv_norm<T>(n, t, dnrm_post, s);

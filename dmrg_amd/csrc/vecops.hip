@@ -135,12 +135,29 @@ __device__ inline bool rs_owner(int lane) {
     return (lane & (64 / NV - 1)) == 0;
 }
 
+// DGKS test evaluated by the projection kernel of a second orthogonalisation pass itself: the pass runs when
+// post[0] < eta2 * pre[0] (|t|^2 fell by more than that factor in the first pass); block 0 records the decision
+// in flag[0] for the kernels of the pass that follow. pre and post are not written during the kernel, so every
+// block reaches the same decision.
+struct DgksTest {
+    const double *pre  = nullptr;
+    const double *post = nullptr;
+    double        eta2 = 0.0;
+    int          *flag = nullptr;
+};
+
 // Columns j < m are V[:, j]; column m (when with_self) is x itself, so x^H x comes out of the same pass.
 template <typename T>
 __global__ __launch_bounds__(VT_THREADS) void multidot_partial(int64_t n, int m, int mc, const T *V, int64_t ldv, const T *x,
                                                                const int *skip, double *partial, unsigned *counter, double *out,
-                                                               int m_out, double *self_out) {
-    if(skip && *skip == 0) return;
+                                                               int m_out, double *self_out, DgksTest dg) {
+    if(dg.flag) {
+        const bool run = dg.post[0] < dg.eta2 * dg.pre[0];
+        if(blockIdx.x == 0 && threadIdx.x == 0) dg.flag[0] = run ? 1 : 0;
+        if(!run) return;
+    } else if(skip && *skip == 0) {
+        return;
+    }
     constexpr int     NV = 2 * MD_CH;
     __shared__ double red[NV][VT_THREADS / 64];
     const int         lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -210,11 +227,30 @@ void multidot(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_o
     const int g = md_grid(n);
     if(g <= FUSE_MAX_BLOCKS && 2 * m <= FUSE_MAX_VALUES) {
         hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, m, V, ldv, x, nullptr, scratch,
-                           scratch_counter(scratch), d_out, -1, nullptr);
+                           scratch_counter(scratch), d_out, -1, nullptr, DgksTest{});
     } else {
         hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, m, V, ldv, x, nullptr, scratch, nullptr,
-                           nullptr, -1, nullptr);
+                           nullptr, -1, nullptr, DgksTest{});
         hipLaunchKernelGGL(fold_columns, dim3(m), dim3(VT_THREADS), 0, s, g, m, scratch, d_out, -1, nullptr, nullptr);
+    }
+    DMRG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+static void multidot_x_impl(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_out, double *d_self, const int *skip,
+                            const DgksTest &dg, double *scratch, hipStream_t s) {
+    const int mc = m + (d_self ? 1 : 0);
+    if(mc <= 0) return;
+    DMRG_REQUIRE(mc <= VEC_MAX_M, DMRG_ERR_ARG, "multidot: too many columns");
+    const int  g     = md_grid(n);
+    const int *fskip = dg.flag ? dg.flag : skip; // the fold follows the pass's decision
+    if(g <= FUSE_MAX_BLOCKS && 2 * mc <= FUSE_MAX_VALUES) {
+        hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, mc, V, ldv, x, skip, scratch,
+                           scratch_counter(scratch), d_out, d_self ? m : -1, d_self, dg);
+    } else {
+        hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, mc, V, ldv, x, skip, scratch, nullptr,
+                           nullptr, -1, nullptr, dg);
+        hipLaunchKernelGGL(fold_columns, dim3(mc), dim3(VT_THREADS), 0, s, g, mc, scratch, d_out, d_self ? m : -1, d_self, fskip);
     }
     DMRG_HIP_CHECK(hipGetLastError());
 }
@@ -222,19 +258,18 @@ void multidot(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_o
 template <typename T>
 void multidot_x(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_out, double *d_self, const int *skip,
                 double *scratch, hipStream_t s) {
-    const int mc = m + (d_self ? 1 : 0);
-    if(mc <= 0) return;
-    DMRG_REQUIRE(mc <= VEC_MAX_M, DMRG_ERR_ARG, "multidot: too many columns");
-    const int g = md_grid(n);
-    if(g <= FUSE_MAX_BLOCKS && 2 * mc <= FUSE_MAX_VALUES) {
-        hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, mc, V, ldv, x, skip, scratch,
-                           scratch_counter(scratch), d_out, d_self ? m : -1, d_self);
-    } else {
-        hipLaunchKernelGGL(multidot_partial<T>, dim3(g), dim3(VT_THREADS), 0, s, n, m, mc, V, ldv, x, skip, scratch, nullptr,
-                           nullptr, -1, nullptr);
-        hipLaunchKernelGGL(fold_columns, dim3(mc), dim3(VT_THREADS), 0, s, g, mc, scratch, d_out, d_self ? m : -1, d_self, skip);
-    }
-    DMRG_HIP_CHECK(hipGetLastError());
+    multidot_x_impl<T>(n, m, V, ldv, x, d_out, d_self, skip, DgksTest{}, scratch, s);
+}
+
+template <typename T>
+void multidot_dgks(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_out, const double *d_pre, const double *d_post,
+                   double eta2, int *d_flag, double *scratch, hipStream_t s) {
+    DgksTest dg;
+    dg.pre  = d_pre;
+    dg.post = d_post;
+    dg.eta2 = eta2;
+    dg.flag = d_flag;
+    multidot_x_impl<T>(n, m, V, ldv, x, d_out, nullptr, nullptr, dg, scratch, s);
 }
 
 // ------------------------------------------------------------------------------------------- gemv
@@ -294,12 +329,6 @@ __global__ __launch_bounds__(VT_THREADS) void gemv_update_norm_kernel(int64_t n,
     if(counter) fold_in_last_block(partial, 2, out, -1, nullptr, counter);
 }
 
-// flag = 1 when |t|^2 after a projection fell below eta2 times its value before (the DGKS test for a
-// second orthogonalisation pass), else 0.
-__global__ void dgks_kernel(const double *pre, const double *post, double eta2, int *flag) {
-    if(threadIdx.x == 0 && blockIdx.x == 0) flag[0] = (post[0] < eta2 * pre[0]) ? 1 : 0;
-}
-
 static int ew_grid(int64_t n) { return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(8192, (n + VT_THREADS - 1) / VT_THREADS))); }
 
 template <typename T>
@@ -322,11 +351,6 @@ void gemv_update_norm(int64_t n, int m, const T *V, int64_t ldv, const double *d
                            nullptr, nullptr);
         hipLaunchKernelGGL(fold_columns, dim3(1), dim3(VT_THREADS), 0, s, g, 1, scratch, d_nrm2, -1, nullptr, skip);
     }
-    DMRG_HIP_CHECK(hipGetLastError());
-}
-
-void dgks_flag(const double *d_pre, const double *d_post, double eta2, int *d_flag, hipStream_t s) {
-    hipLaunchKernelGGL(dgks_kernel, dim3(1), dim3(64), 0, s, d_pre, d_post, eta2, d_flag);
     DMRG_HIP_CHECK(hipGetLastError());
 }
 
@@ -612,6 +636,8 @@ void cg_step(int64_t n, const T *x, T *p, const T *q, const double *d_dots, T *t
     template void random_fill<T>(int64_t, T *, uint64_t, hipStream_t);                                               \
     template void multidot_x<T>(int64_t, int, const T *, int64_t, const T *, double *, double *, const int *, double *,  \
                                 hipStream_t);                                                                          \
+    template void multidot_dgks<T>(int64_t, int, const T *, int64_t, const T *, double *, const double *, const double *, \
+                                   double, int *, double *, hipStream_t);                                              \
     template void gemv_update_norm<T>(int64_t, int, const T *, int64_t, const double *, double, double, const T *, T *,  \
                                       double *, const int *, double *, hipStream_t);
 DMRG_VEC_INST(double)

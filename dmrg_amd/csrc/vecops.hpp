@@ -55,8 +55,12 @@ void multidot_x(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d
 template <typename T>
 void gemv_update_norm(int64_t n, int m, const T *V, int64_t ldv, const double *d_h, double alpha, double beta, const T *x, T *y,
                       double *d_nrm2, const int *skip, double *scratch, hipStream_t s);
-// d_flag[0] = (d_post[0] < eta2 * d_pre[0]): whether a second orthogonalisation pass is needed (DGKS).
-void dgks_flag(const double *d_pre, const double *d_post, double eta2, int *d_flag, hipStream_t s);
+// The projection of a second orthogonalisation pass with its DGKS test folded in: when d_post[0] < eta2 *
+// d_pre[0], out[j] = sum_i conj(V[i, j]) x[i] (j < m) as in multidot_x; otherwise out is left as it is. Either way
+// d_flag[0] receives the decision, for the kernels of the pass that follow (their skip argument).
+template <typename T>
+void multidot_dgks(int64_t n, int m, const T *V, int64_t ldv, const T *x, double *d_out, const double *d_pre, const double *d_post,
+                   double eta2, int *d_flag, double *scratch, hipStream_t s);
 // x = beta x + alpha V h   (h: complex double, m entries, device).
 template <typename T>
 void gemv_update(int64_t n, int m, const T *V, int64_t ldv, const double *d_h, double alpha, double beta, T *x, hipStream_t s);
